@@ -1,0 +1,455 @@
+// C ABI of libgpufsm_hip.so (declared in include/gpufsm_hip.h).
+//
+// Owns the device-resident automaton handles, per-call device memory (stream-ordered
+// hipMallocAsync, so concurrent calls on distinct streams never share scratch), technique
+// resolution, HIP-event timing of the host forms, and the conversion of exceptions into
+// status codes plus a thread-local message.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gpufsm_hip.h"
+#include "internal.hpp"
+#include "layout.hpp"
+
+using namespace gfsm;
+
+struct gfsm_nfa {
+    int32_t n = 0, start = 0;
+    int64_t nnz = 0, nne = 0;
+    bool start_accepts = false;
+    NfaWorklistDev wl{};
+    bool chain_ok = false;
+    NfaTileDev tile{};
+    int32_t max_chain = 0;
+    std::vector<void *> blocks;
+    int64_t device_bytes = 0;
+};
+
+struct gfsm_dfa {
+    int32_t n = 0, start = 0;
+    bool start_accepts = false;
+    DfaDev dev{};
+    bool lds_ok = false;
+    std::vector<void *> blocks;
+    int64_t device_bytes = 0;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+template <typename F>
+int guarded(F &&f) {
+    try {
+        f();
+        return GFSM_OK;
+    } catch (const Error &e) {
+        return fail(e.code, e.what());
+    } catch (const std::bad_alloc &) {
+        return fail(GFSM_ENOMEM, "host allocation failed");
+    } catch (const std::exception &e) {
+        return fail(GFSM_EDEVICE, e.what());
+    }
+}
+
+template <typename T>
+T *upload(std::vector<void *> &blocks, int64_t &bytes, const T *src, size_t count) {
+    const size_t sz = (count ? count : 1) * sizeof(T);
+    void *p = nullptr;
+    if (hipMalloc(&p, sz) != hipSuccess) throw Error(GFSM_ENOMEM, "hipMalloc of automaton failed");
+    blocks.push_back(p);
+    bytes += static_cast<int64_t>(sz);
+    if (count) GFSM_HIP(hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice));
+    return static_cast<T *>(p);
+}
+
+void free_blocks(std::vector<void *> &blocks) {
+    for (void *p : blocks) (void)hipFree(p);
+    blocks.clear();
+}
+
+// Stream-ordered device buffer released on the same stream.
+struct StreamBuf {
+    void *p = nullptr;
+    hipStream_t s = nullptr;
+    StreamBuf(size_t bytes, hipStream_t stream) : s(stream) {
+        if (hipMallocAsync(&p, bytes ? bytes : 1, stream) != hipSuccess)
+            throw Error(GFSM_ENOMEM, "hipMallocAsync of " + std::to_string(bytes) + " bytes failed");
+    }
+    ~StreamBuf() {
+        if (p) (void)hipFreeAsync(p, s);
+    }
+    StreamBuf(const StreamBuf &) = delete;
+    StreamBuf &operator=(const StreamBuf &) = delete;
+};
+
+hipStream_t host_stream() {
+    thread_local hipStream_t s = nullptr;
+    if (!s) GFSM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return s;
+}
+
+void check_batch_args(const void *data, const int64_t *offsets, int32_t count,
+                      const int32_t *flags, const int32_t *lens) {
+    if (count < 0) throw Error(GFSM_EINVAL, "num_strings must be >= 0");
+    if (count > 0 && (!offsets || !flags || !lens)) throw Error(GFSM_EINVAL, "null buffer");
+    (void)data;
+}
+
+int32_t resolve_nfa(const gfsm_nfa *h, int32_t technique) {
+    switch (technique) {
+        case GFSM_NFA_AUTO:
+            return h->chain_ok ? GFSM_NFA_BITPACKED : GFSM_NFA_WORKLIST;
+        case GFSM_NFA_WORKLIST:
+            return technique;
+        case GFSM_NFA_BITPACKED:
+            if (!h->chain_ok)
+                throw Error(GFSM_EUNSUPPORTED,
+                            "bitpacked needs a chain-structured NFA (no epsilon edges, at most "
+                            "one predecessor and successor per non-start state, homogeneous "
+                            "labels, chains of at most 4096 states)");
+            return technique;
+        default:
+            throw Error(GFSM_EINVAL, "unknown NFA technique " + std::to_string(technique));
+    }
+}
+
+int32_t resolve_dfa(const gfsm_dfa *h, int32_t technique) {
+    switch (technique) {
+        case GFSM_DFA_AUTO:
+            return h->lds_ok ? GFSM_DFA_LDS : GFSM_DFA_GATHER;
+        case GFSM_DFA_GATHER:
+            return technique;
+        case GFSM_DFA_LDS:
+            if (!h->lds_ok)
+                throw Error(GFSM_EUNSUPPORTED, "lds needs a DFA of at most 320 states");
+            return technique;
+        default:
+            throw Error(GFSM_EINVAL, "unknown DFA technique " + std::to_string(technique));
+    }
+}
+
+void nfa_enqueue(gfsm_nfa *h, int32_t tech, const uint8_t *d_data, const int64_t *d_off,
+                 int32_t count, int32_t *d_flags, int32_t *d_lens, hipStream_t s) {
+    if (count == 0) return;
+    if (h->start_accepts) {
+        launch_fill_result(d_flags, d_lens, count, 1, 0, s);
+        return;
+    }
+    if (tech == GFSM_NFA_BITPACKED) {
+        StreamBuf best(static_cast<size_t>(count) * sizeof(int32_t), s);
+        launch_nfa_bitpacked(h->tile, d_data, d_off, count, d_flags, d_lens,
+                             static_cast<int32_t *>(best.p), s);
+        return;
+    }
+    int32_t grid = 1;
+    int64_t scratch_elems = 0;
+    nfa_worklist_geometry(h->wl, count, &grid, &scratch_elems);
+    if (scratch_elems > 0) {
+        StreamBuf scratch(static_cast<size_t>(scratch_elems) * sizeof(int32_t), s);
+        launch_nfa_worklist(h->wl, d_data, d_off, count, d_flags, d_lens,
+                            static_cast<int32_t *>(scratch.p), grid, s);
+    } else {
+        launch_nfa_worklist(h->wl, d_data, d_off, count, d_flags, d_lens, nullptr, grid, s);
+    }
+}
+
+void dfa_enqueue(gfsm_dfa *h, int32_t tech, const uint8_t *d_data, const int64_t *d_off,
+                 int32_t count, int32_t *d_flags, int32_t *d_lens, hipStream_t s) {
+    if (count == 0) return;
+    if (h->start_accepts) {
+        launch_fill_result(d_flags, d_lens, count, 1, 0, s);
+        return;
+    }
+    if (tech == GFSM_DFA_LDS)
+        launch_dfa_lds(h->dev, d_data, d_off, count, d_flags, d_lens, s);
+    else
+        launch_dfa_gather(h->dev, d_data, d_off, count, d_flags, d_lens, s);
+}
+
+// Host form shared by NFA and DFA: stage the batch, time the launch with events, copy back.
+template <typename Enqueue>
+void run_host(const uint8_t *data, const int64_t *offsets, int32_t count, int32_t *flags,
+              int32_t *lens, float *kernel_ms, Enqueue &&enqueue) {
+    if (kernel_ms) *kernel_ms = 0.0f;
+    if (count == 0) return;
+    const int64_t total = offsets[count] - offsets[0];
+    if (offsets[0] != 0) throw Error(GFSM_EINVAL, "offsets[0] must be 0");
+    for (int32_t i = 0; i < count; ++i)
+        if (offsets[i + 1] < offsets[i]) throw Error(GFSM_EINVAL, "offsets are not non-decreasing");
+    if (total > 0 && !data) throw Error(GFSM_EINVAL, "null data");
+    hipStream_t s = host_stream();
+    StreamBuf d_data(static_cast<size_t>(total), s);
+    StreamBuf d_off(static_cast<size_t>(count + 1) * sizeof(int64_t), s);
+    StreamBuf d_out(static_cast<size_t>(count) * 2 * sizeof(int32_t), s);
+    if (total > 0)
+        GFSM_HIP(hipMemcpyAsync(d_data.p, data, static_cast<size_t>(total), hipMemcpyHostToDevice, s));
+    GFSM_HIP(hipMemcpyAsync(d_off.p, offsets, static_cast<size_t>(count + 1) * sizeof(int64_t),
+                            hipMemcpyHostToDevice, s));
+    int32_t *d_flags = static_cast<int32_t *>(d_out.p);
+    int32_t *d_lens = d_flags + count;
+    hipEvent_t e0, e1;
+    GFSM_HIP(hipEventCreate(&e0));
+    GFSM_HIP(hipEventCreate(&e1));
+    try {
+        GFSM_HIP(hipEventRecord(e0, s));
+        enqueue(static_cast<const uint8_t *>(d_data.p), static_cast<const int64_t *>(d_off.p),
+                d_flags, d_lens, s);
+        GFSM_HIP(hipEventRecord(e1, s));
+        GFSM_HIP(hipMemcpyAsync(flags, d_flags, static_cast<size_t>(count) * sizeof(int32_t),
+                                hipMemcpyDeviceToHost, s));
+        GFSM_HIP(hipMemcpyAsync(lens, d_lens, static_cast<size_t>(count) * sizeof(int32_t),
+                                hipMemcpyDeviceToHost, s));
+        GFSM_HIP(hipStreamSynchronize(s));
+        float ms = 0.0f;
+        GFSM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (kernel_ms) *kernel_ms = ms;
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
+
+}  // namespace
+
+extern "C" {
+
+int gfsm_abi_version(void) { return GFSM_ABI_VERSION; }
+
+const char *gfsm_last_error(void) { return g_last_error.c_str(); }
+
+int gfsm_device_arch(int32_t ordinal, char *buf, int32_t buflen) {
+    return guarded([&] {
+        if (!buf || buflen <= 0) throw Error(GFSM_EINVAL, "null buffer");
+        hipDeviceProp_t prop;
+        GFSM_HIP(hipGetDeviceProperties(&prop, ordinal));
+        std::strncpy(buf, prop.gcnArchName, static_cast<size_t>(buflen) - 1);
+        buf[buflen - 1] = '\0';
+    });
+}
+
+int gfsm_nfa_create(int32_t num_states, int32_t start_state, const uint8_t *accept,
+                    const int32_t *sym_row_ptr, const int32_t *sym_targets,
+                    const int32_t *sym_symbols, const int32_t *eps_row_ptr,
+                    const int32_t *eps_targets, gfsm_nfa **out) {
+    return guarded([&] {
+        if (!out) throw Error(GFSM_EINVAL, "null out");
+        *out = nullptr;
+        NfaInput a{num_states, start_state, accept, sym_row_ptr, sym_targets, sym_symbols,
+                   eps_row_ptr, eps_targets};
+        validate_nfa(a);
+        WorklistHost w = build_worklist(a);
+        TileHost t = build_tiles(a);
+        auto *h = new gfsm_nfa();
+        try {
+            h->n = num_states;
+            h->start = start_state;
+            h->nnz = sym_row_ptr[num_states];
+            h->nne = eps_row_ptr[num_states];
+            h->start_accepts = w.start_accepts;
+            auto &B = h->blocks;
+            auto &by = h->device_bytes;
+            h->wl.row = upload(B, by, w.row.data(), w.row.size());
+            h->wl.edge = upload(B, by, w.edge.data(), w.edge.size());
+            h->wl.bigidx = upload(B, by, w.bigidx.data(), w.bigidx.size());
+            h->wl.bigoff = upload(B, by, w.bigoff.data(), w.bigoff.size());
+            h->wl.erow = upload(B, by, w.erow.data(), w.erow.size());
+            h->wl.etgt = upload(B, by, w.etgt.data(), w.etgt.size());
+            h->wl.accw = upload(B, by, w.accw.data(), w.accw.size());
+            h->wl.start_list = upload(B, by, w.start_list.data(), w.start_list.size());
+            h->wl.n = num_states;
+            h->wl.nwords = w.nwords;
+            h->wl.k0 = static_cast<int32_t>(w.start_list.size());
+            h->wl.has_eps = w.has_eps ? 1 : 0;
+            h->chain_ok = t.ok;
+            if (t.ok) {
+                h->tile.mask = upload(B, by, t.mask.data(), t.mask.size());
+                h->tile.first = upload(B, by, t.first.data(), t.first.size());
+                h->tile.hassucc = upload(B, by, t.hassucc.data(), t.hassucc.size());
+                h->tile.acc = upload(B, by, t.acc.data(), t.acc.size());
+                for (int i = 0; i < 4; ++i) h->tile.selfmask[i] = t.selfmask[i];
+                h->tile.tiles = t.tiles;
+                h->tile.start_always = t.start_always ? 1 : 0;
+                h->max_chain = t.max_chain;
+            }
+        } catch (...) {
+            free_blocks(h->blocks);
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int gfsm_nfa_destroy(gfsm_nfa *nfa) {
+    return guarded([&] {
+        if (!nfa) return;
+        free_blocks(nfa->blocks);
+        delete nfa;
+    });
+}
+
+int gfsm_nfa_query(const gfsm_nfa *nfa, gfsm_nfa_props *out) {
+    return guarded([&] {
+        if (!nfa || !out) throw Error(GFSM_EINVAL, "null argument");
+        std::memset(out, 0, sizeof(*out));
+        out->num_states = nfa->n;
+        out->start_state = nfa->start;
+        out->num_sym_edges = nfa->nnz;
+        out->num_eps_edges = nfa->nne;
+        out->start_accepts = nfa->start_accepts;
+        out->chain_ok = nfa->chain_ok;
+        out->num_tiles = nfa->chain_ok ? nfa->tile.tiles : 0;
+        out->start_always = nfa->chain_ok ? nfa->tile.start_always : 0;
+        out->max_chain = nfa->max_chain;
+        out->device_bytes = nfa->device_bytes;
+    });
+}
+
+int gfsm_nfa_analyze(int32_t num_states, int32_t start_state, const uint8_t *accept,
+                     const int32_t *sym_row_ptr, const int32_t *sym_targets,
+                     const int32_t *sym_symbols, const int32_t *eps_row_ptr,
+                     const int32_t *eps_targets, gfsm_nfa_props *out) {
+    return guarded([&] {
+        if (!out) throw Error(GFSM_EINVAL, "null out");
+        NfaInput a{num_states, start_state, accept, sym_row_ptr, sym_targets, sym_symbols,
+                   eps_row_ptr, eps_targets};
+        validate_nfa(a);
+        WorklistHost w = build_worklist(a);
+        TileHost t = build_tiles(a);
+        std::memset(out, 0, sizeof(*out));
+        out->num_states = num_states;
+        out->start_state = start_state;
+        out->num_sym_edges = sym_row_ptr[num_states];
+        out->num_eps_edges = eps_row_ptr[num_states];
+        out->start_accepts = w.start_accepts;
+        out->chain_ok = t.ok;
+        out->num_tiles = t.ok ? t.tiles : 0;
+        out->start_always = t.ok ? t.start_always : 0;
+        out->max_chain = t.max_chain;
+        int64_t bytes = 0;
+        auto add = [&](size_t count, size_t elem) {
+            bytes += static_cast<int64_t>((count ? count : 1) * elem);
+        };
+        add(w.row.size(), 4); add(w.edge.size(), 4); add(w.bigidx.size(), 4);
+        add(w.bigoff.size(), 4); add(w.erow.size(), 4); add(w.etgt.size(), 4);
+        add(w.accw.size(), 8); add(w.start_list.size(), 4);
+        if (t.ok) {
+            add(t.mask.size(), 8); add(t.first.size(), 8); add(t.hassucc.size(), 8);
+            add(t.acc.size(), 8);
+        }
+        out->device_bytes = bytes;
+    });
+}
+
+int gfsm_nfa_run_host(gfsm_nfa *nfa, int32_t technique, const uint8_t *data,
+                      const int64_t *offsets, int32_t num_strings, int32_t *flags,
+                      int32_t *lens, float *kernel_ms) {
+    return guarded([&] {
+        if (!nfa) throw Error(GFSM_EINVAL, "null handle");
+        check_batch_args(data, offsets, num_strings, flags, lens);
+        const int32_t tech = resolve_nfa(nfa, technique);
+        run_host(data, offsets, num_strings, flags, lens, kernel_ms,
+                 [&](const uint8_t *dd, const int64_t *doff, int32_t *df, int32_t *dl,
+                     hipStream_t s) { nfa_enqueue(nfa, tech, dd, doff, num_strings, df, dl, s); });
+    });
+}
+
+int gfsm_nfa_run_device(gfsm_nfa *nfa, int32_t technique, const uint8_t *d_data,
+                        const int64_t *d_offsets, int32_t num_strings, int32_t *d_flags,
+                        int32_t *d_lens, void *stream) {
+    return guarded([&] {
+        if (!nfa) throw Error(GFSM_EINVAL, "null handle");
+        check_batch_args(d_data, d_offsets, num_strings, d_flags, d_lens);
+        const int32_t tech = resolve_nfa(nfa, technique);
+        nfa_enqueue(nfa, tech, d_data, d_offsets, num_strings, d_flags, d_lens,
+                    static_cast<hipStream_t>(stream));
+    });
+}
+
+int gfsm_dfa_create(int32_t num_states, int32_t start_state, const uint8_t *accept,
+                    const int32_t *trans, gfsm_dfa **out) {
+    return guarded([&] {
+        if (!out) throw Error(GFSM_EINVAL, "null out");
+        *out = nullptr;
+        DfaHost t = build_dfa(num_states, start_state, accept, trans);
+        auto *h = new gfsm_dfa();
+        try {
+            h->n = num_states;
+            h->start = start_state;
+            h->start_accepts = t.start_accepts;
+            if (t.wide)
+                h->dev.table = upload(h->blocks, h->device_bytes, t.t32.data(), t.t32.size());
+            else
+                h->dev.table = upload(h->blocks, h->device_bytes, t.t16.data(), t.t16.size());
+            h->dev.n = num_states;
+            h->dev.start = start_state;
+            h->dev.wide = t.wide ? 1 : 0;
+            h->lds_ok = !t.wide && dfa_lds_bytes(num_states) <= 160 * 1024;
+        } catch (...) {
+            free_blocks(h->blocks);
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int gfsm_dfa_destroy(gfsm_dfa *dfa) {
+    return guarded([&] {
+        if (!dfa) return;
+        free_blocks(dfa->blocks);
+        delete dfa;
+    });
+}
+
+int gfsm_dfa_query(const gfsm_dfa *dfa, gfsm_dfa_props *out) {
+    return guarded([&] {
+        if (!dfa || !out) throw Error(GFSM_EINVAL, "null argument");
+        std::memset(out, 0, sizeof(*out));
+        out->num_states = dfa->n;
+        out->start_state = dfa->start;
+        out->start_accepts = dfa->start_accepts;
+        out->entry_bytes = dfa->dev.wide ? 4 : 2;
+        out->lds_ok = dfa->lds_ok;
+        out->device_bytes = dfa->device_bytes;
+    });
+}
+
+int gfsm_dfa_run_host(gfsm_dfa *dfa, int32_t technique, const uint8_t *data,
+                      const int64_t *offsets, int32_t num_strings, int32_t *flags,
+                      int32_t *lens, float *kernel_ms) {
+    return guarded([&] {
+        if (!dfa) throw Error(GFSM_EINVAL, "null handle");
+        check_batch_args(data, offsets, num_strings, flags, lens);
+        const int32_t tech = resolve_dfa(dfa, technique);
+        run_host(data, offsets, num_strings, flags, lens, kernel_ms,
+                 [&](const uint8_t *dd, const int64_t *doff, int32_t *df, int32_t *dl,
+                     hipStream_t s) { dfa_enqueue(dfa, tech, dd, doff, num_strings, df, dl, s); });
+    });
+}
+
+int gfsm_dfa_run_device(gfsm_dfa *dfa, int32_t technique, const uint8_t *d_data,
+                        const int64_t *d_offsets, int32_t num_strings, int32_t *d_flags,
+                        int32_t *d_lens, void *stream) {
+    return guarded([&] {
+        if (!dfa) throw Error(GFSM_EINVAL, "null handle");
+        check_batch_args(d_data, d_offsets, num_strings, d_flags, d_lens);
+        const int32_t tech = resolve_dfa(dfa, technique);
+        dfa_enqueue(dfa, tech, d_data, d_offsets, num_strings, d_flags, d_lens,
+                    static_cast<hipStream_t>(stream));
+    });
+}
+
+}  // extern "C"

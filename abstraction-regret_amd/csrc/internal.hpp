@@ -1,0 +1,100 @@
+// Internal declarations shared by the translation units of libgpufsm_hip.so.
+//
+// Layout of the device-resident automata, the error plumbing used by the C ABI, and the
+// host-side launchers each kernel translation unit exports (kernels are launched from the
+// translation unit that defines them; no relocatable device code).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace gfsm {
+
+constexpr int kAnySymbol = 256;          // wildcard label (upstream ANY_SYMBOL)
+constexpr int kAlphabet = 256;           // DFA row stride (upstream ALPHABET)
+constexpr int kLabelBits = 9;            // packed edge = target << 9 | label (label <= 256)
+constexpr int kMaxPackedStates = 1 << 23;
+constexpr int kBigRow = 16;              // rows longer than this get a per-label index
+constexpr int kBigStride = 258;          // per-label index: 257 label starts + row end
+constexpr int kTileStates = 4096;        // bitpacked tile: 64 lanes x 64-bit words
+constexpr int kTileWords = 64;
+
+// Thrown inside the library, converted to a status code at the C boundary.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define GFSM_HIP(call)                                                                    \
+    do {                                                                                  \
+        hipError_t err_ = (call);                                                         \
+        if (err_ != hipSuccess)                                                           \
+            throw ::gfsm::Error(4, std::string(#call) + " failed at " + __FILE__ + ":" +  \
+                                       std::to_string(__LINE__) + ": " +                  \
+                                       hipGetErrorString(err_));                          \
+    } while (0)
+
+// ----- NFA ------------------------------------------------------------------------------
+
+// Worklist layout: label-sorted CSR rows with packed edges, and a per-label index for
+// the few states whose rows are long (e.g. the start state of a pattern union).
+struct NfaWorklistDev {
+    const int32_t *row;        // [n + 1]
+    const uint32_t *edge;      // [nnz] target << 9 | label, sorted by label in each row
+    const int32_t *bigidx;     // [n] index into bigoff, or -1
+    const int32_t *bigoff;     // [nbig * 258] absolute edge offset of the first label >= c
+    const int32_t *erow;       // [n + 1]
+    const int32_t *etgt;       // [nnz_eps]
+    const uint64_t *accw;      // [nwords] accept bitset
+    const int32_t *start_list; // [k0] start closure
+    int32_t n, nwords, k0, has_eps;
+};
+
+// Bitpacked chain layout: states renumbered into tiles of 4096 so that every chain is
+// contiguous inside one tile and the successor of bit i is bit i + 1.
+struct NfaTileDev {
+    const uint64_t *mask;      // [tiles][256][64] bit set <=> byte in the state's class
+    const uint64_t *first;     // [tiles][64] successors of the start state
+    const uint64_t *hassucc;   // [tiles][64] the state has a chain successor (bit i -> i+1)
+    const uint64_t *acc;       // [tiles][64] accepting states
+    uint64_t selfmask[4];      // bytes on which the start state stays active
+    int32_t tiles;
+    int32_t start_always;      // self-loop on every byte
+};
+
+// ----- DFA ------------------------------------------------------------------------------
+
+// Table entries carry the accept bit of the successor in their top bit, so one load per
+// byte gives both the next state and the latch test.
+struct DfaDev {
+    const void *table;         // uint16 [n * 256] or uint32 [n * 256]
+    int32_t n, start, wide;    // wide: uint32 entries
+};
+
+// ----- launchers (asynchronous on `stream`) ---------------------------------------------
+
+void launch_fill_result(int32_t *flags, int32_t *lens, int32_t count, int32_t flag,
+                        int32_t len, hipStream_t stream);
+
+void launch_dfa_gather(const DfaDev &d, const uint8_t *data, const int64_t *off, int32_t count,
+                       int32_t *flags, int32_t *lens, hipStream_t stream);
+void launch_dfa_lds(const DfaDev &d, const uint8_t *data, const int64_t *off, int32_t count,
+                    int32_t *flags, int32_t *lens, hipStream_t stream);
+size_t dfa_lds_bytes(int32_t n);
+
+void launch_nfa_worklist(const NfaWorklistDev &d, const uint8_t *data, const int64_t *off,
+                         int32_t count, int32_t *flags, int32_t *lens, int32_t *scratch,
+                         int32_t grid, hipStream_t stream);
+// Blocks and global list scratch (int32 elements) the worklist launch needs.
+void nfa_worklist_geometry(const NfaWorklistDev &d, int32_t count, int32_t *grid,
+                           int64_t *scratch_elems);
+
+void launch_nfa_bitpacked(const NfaTileDev &d, const uint8_t *data, const int64_t *off,
+                          int32_t count, int32_t *flags, int32_t *lens, int32_t *best,
+                          hipStream_t stream);
+
+}  // namespace gfsm

@@ -1,0 +1,251 @@
+// Host-side preprocessing of upstream CSR automata into the device layouts (see layout.hpp).
+#include "layout.hpp"
+
+#include <algorithm>
+#include <numeric>
+#include <string>
+
+#include "internal.hpp"
+
+namespace gfsm {
+
+namespace {
+
+[[noreturn]] void bad(const std::string &m) { throw Error(1, m); }
+
+struct Bits256 {
+    uint64_t w[4] = {0, 0, 0, 0};
+    void add_label(int32_t lab) {
+        if (lab == kAnySymbol) {
+            w[0] = w[1] = w[2] = w[3] = ~0ull;
+        } else {
+            w[lab >> 6] |= 1ull << (lab & 63);
+        }
+    }
+    bool any() const { return (w[0] | w[1] | w[2] | w[3]) != 0; }
+    bool operator==(const Bits256 &o) const {
+        return w[0] == o.w[0] && w[1] == o.w[1] && w[2] == o.w[2] && w[3] == o.w[3];
+    }
+    bool has(int b) const { return (w[b >> 6] >> (b & 63)) & 1ull; }
+};
+
+}  // namespace
+
+void validate_nfa(const NfaInput &a) {
+    if (a.n < 1) bad("num_states must be >= 1");
+    if (a.n >= kMaxPackedStates)
+        bad("num_states " + std::to_string(a.n) + " exceeds the supported maximum " +
+            std::to_string(kMaxPackedStates - 1));
+    if (a.start < 0 || a.start >= a.n) bad("start_state out of range");
+    if (!a.accept || !a.srp || !a.erp) bad("null array");
+    if (a.srp[0] != 0 || a.erp[0] != 0) bad("row pointers must start at 0");
+    for (int32_t s = 0; s < a.n; ++s) {
+        if (a.srp[s + 1] < a.srp[s]) bad("sym_row_ptr is not non-decreasing");
+        if (a.erp[s + 1] < a.erp[s]) bad("eps_row_ptr is not non-decreasing");
+    }
+    const int32_t nnz = a.srp[a.n], nne = a.erp[a.n];
+    if (nnz > 0 && (!a.st || !a.ss)) bad("null symbol-edge array");
+    if (nne > 0 && !a.et) bad("null epsilon-edge array");
+    for (int32_t k = 0; k < nnz; ++k) {
+        if (a.st[k] < 0 || a.st[k] >= a.n) bad("sym_targets entry out of range");
+        if (a.ss[k] < 0 || a.ss[k] > kAnySymbol) bad("sym_symbols entry outside 0..256");
+    }
+    for (int32_t k = 0; k < nne; ++k)
+        if (a.et[k] < 0 || a.et[k] >= a.n) bad("eps_targets entry out of range");
+}
+
+WorklistHost build_worklist(const NfaInput &a) {
+    WorklistHost h;
+    const int32_t n = a.n;
+    const int32_t nnz = a.srp[n];
+    h.nwords = (n + 63) / 64;
+    h.row.assign(a.srp, a.srp + n + 1);
+    h.edge.resize(static_cast<size_t>(nnz));
+    h.bigidx.assign(static_cast<size_t>(n), -1);
+    int32_t count[kAnySymbol + 2];
+    for (int32_t s = 0; s < n; ++s) {
+        const int32_t lo = a.srp[s], hi = a.srp[s + 1];
+        std::fill(count, count + kAnySymbol + 2, 0);
+        for (int32_t k = lo; k < hi; ++k) count[a.ss[k] + 1]++;
+        for (int c = 0; c <= kAnySymbol; ++c) count[c + 1] += count[c];
+        if (hi - lo > kBigRow) {
+            h.bigidx[s] = static_cast<int32_t>(h.bigoff.size() / kBigStride);
+            for (int c = 0; c <= kAnySymbol; ++c) h.bigoff.push_back(lo + count[c]);
+            h.bigoff.push_back(hi);
+        }
+        for (int32_t k = lo; k < hi; ++k) {
+            const int32_t pos = lo + count[a.ss[k]]++;
+            h.edge[pos] = (static_cast<uint32_t>(a.st[k]) << kLabelBits) |
+                          static_cast<uint32_t>(a.ss[k]);
+        }
+    }
+    h.erow.assign(a.erp, a.erp + n + 1);
+    h.etgt.assign(a.et, a.et + a.erp[n]);
+    h.has_eps = a.erp[n] > 0;
+    h.accw.assign(static_cast<size_t>(h.nwords), 0ull);
+    for (int32_t s = 0; s < n; ++s)
+        if (a.accept[s]) h.accw[s >> 6] |= 1ull << (s & 63);
+    // start closure: depth-first over epsilon edges, seeds included
+    std::vector<uint8_t> seen(static_cast<size_t>(n), 0);
+    std::vector<int32_t> stack{a.start};
+    seen[a.start] = 1;
+    h.start_list.push_back(a.start);
+    while (!stack.empty()) {
+        const int32_t s = stack.back();
+        stack.pop_back();
+        for (int32_t k = a.erp[s]; k < a.erp[s + 1]; ++k) {
+            const int32_t t = a.et[k];
+            if (!seen[t]) {
+                seen[t] = 1;
+                h.start_list.push_back(t);
+                stack.push_back(t);
+            }
+        }
+    }
+    for (int32_t s : h.start_list)
+        if (a.accept[s]) h.start_accepts = true;
+    return h;
+}
+
+TileHost build_tiles(const NfaInput &a) {
+    TileHost h;
+    const int32_t n = a.n, s0 = a.start;
+    if (a.erp[n] > 0) {
+        h.why = "epsilon edges";
+        return h;
+    }
+    std::vector<int32_t> pred(static_cast<size_t>(n), -1), succ(static_cast<size_t>(n), -1);
+    std::vector<Bits256> from_start(static_cast<size_t>(n)), from_pred(static_cast<size_t>(n));
+    Bits256 self;
+    for (int32_t s = 0; s < n; ++s) {
+        for (int32_t k = a.srp[s]; k < a.srp[s + 1]; ++k) {
+            const int32_t t = a.st[k], lab = a.ss[k];
+            if (t == s0) {
+                if (s != s0) {
+                    h.why = "edge into the start state";
+                    return h;
+                }
+                self.add_label(lab);
+                continue;
+            }
+            if (s == s0) {
+                from_start[t].add_label(lab);
+                continue;
+            }
+            if (succ[s] != -1 && succ[s] != t) {
+                h.why = "a state with two successors";
+                return h;
+            }
+            succ[s] = t;
+            if (pred[t] != -1 && pred[t] != s) {
+                h.why = "a state with two predecessors";
+                return h;
+            }
+            pred[t] = s;
+            from_pred[t].add_label(lab);
+        }
+    }
+    for (int32_t t = 0; t < n; ++t) {
+        if (t == s0) continue;
+        if (pred[t] != -1 && from_start[t].any() && !(from_start[t] == from_pred[t])) {
+            h.why = "inhomogeneous labels";
+            return h;
+        }
+    }
+    // chains: heads have no non-start predecessor; every non-start state must be on one
+    std::vector<std::vector<int32_t>> chains;
+    std::vector<uint8_t> placed(static_cast<size_t>(n), 0);
+    size_t covered = 0;
+    for (int32_t t = 0; t < n; ++t) {
+        if (t == s0 || pred[t] != -1) continue;
+        std::vector<int32_t> c;
+        for (int32_t u = t; u != -1; u = succ[u]) {
+            if (placed[u]) {
+                h.why = "a cycle";
+                return h;
+            }
+            placed[u] = 1;
+            c.push_back(u);
+            if (static_cast<int32_t>(c.size()) > kTileStates) {
+                h.why = "a chain longer than one tile";
+                return h;
+            }
+        }
+        covered += c.size();
+        chains.push_back(std::move(c));
+    }
+    if (covered != static_cast<size_t>(n - 1)) {
+        h.why = "a cycle";
+        return h;
+    }
+    // first-fit decreasing into 4096-state tiles
+    std::vector<size_t> order(chains.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t x, size_t y) { return chains[x].size() > chains[y].size(); });
+    std::vector<int32_t> fill;
+    std::vector<std::vector<size_t>> bins;
+    for (size_t ci : order) {
+        const int32_t len = static_cast<int32_t>(chains[ci].size());
+        h.max_chain = std::max(h.max_chain, len);
+        size_t b = 0;
+        while (b < fill.size() && fill[b] + len > kTileStates) ++b;
+        if (b == fill.size()) {
+            fill.push_back(0);
+            bins.emplace_back();
+        }
+        bins[b].push_back(ci);
+        fill[b] += len;
+    }
+    h.tiles = std::max<int32_t>(1, static_cast<int32_t>(bins.size()));
+    const size_t T = static_cast<size_t>(h.tiles);
+    h.mask.assign(T * 256 * kTileWords, 0ull);
+    h.first.assign(T * kTileWords, 0ull);
+    h.hassucc.assign(T * kTileWords, 0ull);
+    h.acc.assign(T * kTileWords, 0ull);
+    h.slot_of_state.assign(static_cast<size_t>(n), -1);
+    for (size_t b = 0; b < bins.size(); ++b) {
+        int32_t slot = 0;
+        for (size_t ci : bins[b]) {
+            for (int32_t u : chains[ci]) {
+                const size_t w = b * kTileWords + (slot >> 6);
+                const uint64_t bit = 1ull << (slot & 63);
+                h.slot_of_state[u] = static_cast<int32_t>(b) * kTileStates + slot;
+                const Bits256 &cls = pred[u] != -1 ? from_pred[u] : from_start[u];
+                for (int c = 0; c < 256; ++c)
+                    if (cls.has(c)) h.mask[(b * 256 + c) * kTileWords + (slot >> 6)] |= bit;
+                if (from_start[u].any()) h.first[w] |= bit;
+                if (succ[u] != -1) h.hassucc[w] |= bit;
+                if (a.accept[u]) h.acc[w] |= bit;
+                ++slot;
+            }
+        }
+    }
+    for (int i = 0; i < 4; ++i) h.selfmask[i] = self.w[i];
+    h.start_always = (self.w[0] & self.w[1] & self.w[2] & self.w[3]) == ~0ull;
+    h.ok = true;
+    return h;
+}
+
+DfaHost build_dfa(int32_t n, int32_t start, const uint8_t *accept, const int32_t *trans) {
+    if (n < 1) bad("num_states must be >= 1");
+    if (start < 0 || start >= n) bad("start_state out of range");
+    if (!accept || !trans) bad("null array");
+    if (static_cast<int64_t>(n) * kAlphabet >= (1ll << 31)) bad("DFA table too large");
+    DfaHost h;
+    const size_t total = static_cast<size_t>(n) * kAlphabet;
+    h.wide = n > 0x7fff;
+    if (h.wide) h.t32.resize(total); else h.t16.resize(total);
+    for (size_t k = 0; k < total; ++k) {
+        const int32_t t = trans[k];
+        if (t < 0 || t >= n) bad("trans entry out of range at index " + std::to_string(k));
+        if (h.wide)
+            h.t32[k] = static_cast<uint32_t>(t) | (accept[t] ? 0x80000000u : 0u);
+        else
+            h.t16[k] = static_cast<uint16_t>(t | (accept[t] ? 0x8000 : 0));
+    }
+    h.start_accepts = accept[start] != 0;
+    return h;
+}
+
+}  // namespace gfsm

@@ -1,0 +1,155 @@
+"""ctypes binding of ``libgpufsm_hip.so`` (C ABI declared in ``include/gpufsm_hip.h``).
+
+This module is the only place that knows the library may be absent: :func:`load` raises
+:class:`HIPUnavailable` with the reason, and every caller fails loudly with it. There is no
+fallback to any other implementation.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+LIB_NAME = "libgpufsm_hip.so"
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+
+# status codes and technique ids (gpufsm_hip.h)
+OK, EINVAL, ENOMEM, EUNSUPPORTED, EDEVICE = 0, 1, 2, 3, 4
+NFA_AUTO, NFA_WORKLIST, NFA_BITPACKED = 0, 1, 2
+DFA_AUTO, DFA_GATHER, DFA_LDS = 0, 1, 2
+
+NFA_TECHNIQUES = {"auto": NFA_AUTO, "worklist": NFA_WORKLIST, "bitpacked": NFA_BITPACKED}
+DFA_TECHNIQUES = {"auto": DFA_AUTO, "gather": DFA_GATHER, "lds": DFA_LDS}
+
+# Every symbol the header declares; the CPU test suite checks the library exports them all.
+EXPORTS = (
+    "gfsm_abi_version",
+    "gfsm_last_error",
+    "gfsm_device_arch",
+    "gfsm_nfa_create",
+    "gfsm_nfa_destroy",
+    "gfsm_nfa_query",
+    "gfsm_nfa_analyze",
+    "gfsm_nfa_run_host",
+    "gfsm_nfa_run_device",
+    "gfsm_dfa_create",
+    "gfsm_dfa_destroy",
+    "gfsm_dfa_query",
+    "gfsm_dfa_run_host",
+    "gfsm_dfa_run_device",
+)
+
+
+class HIPUnavailable(RuntimeError):
+    """The HIP library could not be loaded; the message says why."""
+
+
+class HIPError(RuntimeError):
+    """A C-ABI call returned a non-zero status; ``code`` is the status."""
+
+    def __init__(self, code: int, message: str) -> None:
+        super().__init__(message)
+        self.code = code
+
+
+class NfaProps(ctypes.Structure):
+    _fields_ = [
+        ("num_states", ctypes.c_int32),
+        ("start_state", ctypes.c_int32),
+        ("num_sym_edges", ctypes.c_int64),
+        ("num_eps_edges", ctypes.c_int64),
+        ("start_accepts", ctypes.c_int32),
+        ("chain_ok", ctypes.c_int32),
+        ("num_tiles", ctypes.c_int32),
+        ("start_always", ctypes.c_int32),
+        ("max_chain", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+        ("device_bytes", ctypes.c_int64),
+    ]
+
+
+class DfaProps(ctypes.Structure):
+    _fields_ = [
+        ("num_states", ctypes.c_int32),
+        ("start_state", ctypes.c_int32),
+        ("start_accepts", ctypes.c_int32),
+        ("entry_bytes", ctypes.c_int32),
+        ("lds_ok", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+        ("device_bytes", ctypes.c_int64),
+    ]
+
+
+_c = ctypes
+_vp = _c.c_void_p
+_i32 = _c.c_int32
+_i32p = _c.POINTER(_c.c_int32)
+_i64p = _c.POINTER(_c.c_int64)
+_u8p = _c.POINTER(_c.c_uint8)
+_fp = _c.POINTER(_c.c_float)
+
+_SIGNATURES = {
+    "gfsm_abi_version": (_c.c_int, []),
+    "gfsm_last_error": (_c.c_char_p, []),
+    "gfsm_device_arch": (_c.c_int, [_i32, _c.c_char_p, _i32]),
+    "gfsm_nfa_create": (_c.c_int, [_i32, _i32, _u8p, _i32p, _i32p, _i32p, _i32p, _i32p,
+                                   _c.POINTER(_vp)]),
+    "gfsm_nfa_destroy": (_c.c_int, [_vp]),
+    "gfsm_nfa_query": (_c.c_int, [_vp, _c.POINTER(NfaProps)]),
+    "gfsm_nfa_analyze": (_c.c_int, [_i32, _i32, _u8p, _i32p, _i32p, _i32p, _i32p, _i32p,
+                                    _c.POINTER(NfaProps)]),
+    "gfsm_nfa_run_host": (_c.c_int, [_vp, _i32, _u8p, _i64p, _i32, _i32p, _i32p, _fp]),
+    "gfsm_nfa_run_device": (_c.c_int, [_vp, _i32, _vp, _vp, _i32, _vp, _vp, _vp]),
+    "gfsm_dfa_create": (_c.c_int, [_i32, _i32, _u8p, _i32p, _c.POINTER(_vp)]),
+    "gfsm_dfa_destroy": (_c.c_int, [_vp]),
+    "gfsm_dfa_query": (_c.c_int, [_vp, _c.POINTER(DfaProps)]),
+    "gfsm_dfa_run_host": (_c.c_int, [_vp, _i32, _u8p, _i64p, _i32, _i32p, _i32p, _fp]),
+    "gfsm_dfa_run_device": (_c.c_int, [_vp, _i32, _vp, _vp, _i32, _vp, _vp, _vp]),
+}
+
+_lib: ctypes.CDLL | None = None
+_load_error: str | None = None
+
+
+def load() -> ctypes.CDLL:
+    """Load the library once; raise :class:`HIPUnavailable` naming the cause otherwise."""
+    global _lib, _load_error
+    if _lib is not None:
+        return _lib
+    if _load_error is not None:
+        raise HIPUnavailable(_load_error)
+    if not os.path.exists(LIB_PATH):
+        _load_error = (
+            f"{LIB_NAME} is not built ({LIB_PATH} missing); run "
+            f"`python abstraction-regret_amd/csrc/build.py` or __graft_entry__.build()"
+        )
+        raise HIPUnavailable(_load_error)
+    try:
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+    except (OSError, AttributeError) as exc:
+        _load_error = f"{LIB_NAME} failed to load: {type(exc).__name__}: {exc}"
+        raise HIPUnavailable(_load_error) from exc
+    _lib = lib
+    return lib
+
+
+def check(rc: int) -> None:
+    """Raise :class:`HIPError` with the library's message for a non-zero status."""
+    if rc != OK:
+        msg = load().gfsm_last_error()
+        raise HIPError(rc, msg.decode() if msg else f"status {rc}")
+
+
+def device_arch(ordinal: int = 0) -> str:
+    buf = ctypes.create_string_buffer(64)
+    check(load().gfsm_device_arch(ordinal, buf, 64))
+    return buf.value.decode()
+
+
+def ptr(a, ctype):
+    """ctypes pointer to a contiguous numpy array's data."""
+    return a.ctypes.data_as(ctypes.POINTER(ctype))

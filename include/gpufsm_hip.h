@@ -1,0 +1,144 @@
+/*
+ * gpufsm_hip.h — C ABI of the MI355X (gfx950) HIP backend of gpufsm.
+ *
+ * Plain C types only (pointers, sizes, int status codes). Loaded from Python with ctypes
+ * (gpufsm/backends/hip/_lib.py); any other FFI can bind the same symbols.
+ *
+ * What this replaces in the upstream project: the pybind11 entry points of the CUDA
+ * extension, declared in `src/gpufsm/backends/cuda/native/include/api.hpp:23-104` and
+ * bound in `src/gpufsm/backends/cuda/native/bindings.cu:7-39`. Those take numpy CSR arrays
+ * and an int32 input batch, copy everything to the device on every call, run one kernel and
+ * return `(flags, lens, kernel_ms)`. Here the automaton is uploaded once into an opaque
+ * handle that stays resident in HBM, inputs are bytes with int64 offsets, and each run
+ * either takes host buffers (synchronous, the drop-in form) or device buffers on a caller
+ * stream (asynchronous, the resident form).
+ *
+ * Conventions
+ *   - Every function returns GFSM_OK (0) or a positive GFSM_E* code; the message of the
+ *     last failure on the calling thread is gfsm_last_error().
+ *   - The caller owns every buffer it passes; the library never keeps a pointer to a
+ *     caller buffer after the call returns (host forms) or after the stream reaches the
+ *     end of the enqueued work (device forms). The library owns device memory of handles.
+ *   - Results per input: flags[i] in {0,1} (accepted), lens[i] = match length
+ *     (latch-first-match: bytes consumed up to and including the first byte after which an
+ *     accepting state is active; 0 if the start closure accepts or nothing accepts).
+ *   - Device forms are asynchronous on `stream` (a hipStream_t passed as void*; NULL means
+ *     the null stream). Calls on one handle from several host threads must use distinct
+ *     streams; handles are read-only after creation, so concurrent runs are safe.
+ */
+#ifndef GPUFSM_HIP_H
+#define GPUFSM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GFSM_ABI_VERSION 1
+
+/* status codes */
+#define GFSM_OK 0
+#define GFSM_EINVAL 1        /* bad argument (null pointer, out-of-range index, ...) */
+#define GFSM_ENOMEM 2        /* device or host allocation failed */
+#define GFSM_EUNSUPPORTED 3  /* technique cannot run this automaton / size */
+#define GFSM_EDEVICE 4       /* a HIP runtime call failed */
+
+/* NFA techniques (upstream names in parentheses are the CUDA kernels each one replaces) */
+#define GFSM_NFA_AUTO 0       /* pick from the automaton's structure */
+#define GFSM_NFA_WORKLIST 1   /* one wave64 per input, compacted active list + LDS dedup
+                                 bitmap (worklist_warp / worklist_shared / worklist_compact) */
+#define GFSM_NFA_BITPACKED 2  /* bit-parallel chain step, 4096-state tiles, class masks in
+                                 LDS (bitpacked / multistream); chain-structured NFAs only */
+
+/* DFA techniques */
+#define GFSM_DFA_AUTO 0
+#define GFSM_DFA_GATHER 1     /* one lane per input, table in global memory (dfa_kernel) */
+#define GFSM_DFA_LDS 2        /* one lane per input, uint16 table staged in LDS */
+
+typedef struct gfsm_nfa gfsm_nfa;
+typedef struct gfsm_dfa gfsm_dfa;
+
+typedef struct {
+    int32_t num_states;
+    int32_t start_state;
+    int64_t num_sym_edges;
+    int64_t num_eps_edges;
+    int32_t start_accepts;    /* start closure holds an accepting state */
+    int32_t chain_ok;         /* GFSM_NFA_BITPACKED applies */
+    int32_t num_tiles;        /* 4096-state tiles of the bitpacked layout (0 if !chain_ok) */
+    int32_t start_always;     /* start state has a wildcard self-loop */
+    int32_t max_chain;        /* longest chain (bytes a match can span) if chain_ok */
+    int32_t pad;
+    int64_t device_bytes;     /* HBM held by the handle */
+} gfsm_nfa_props;
+
+typedef struct {
+    int32_t num_states;
+    int32_t start_state;
+    int32_t start_accepts;
+    int32_t entry_bytes;      /* 2 (uint16 table) or 4 (int32 table) */
+    int32_t lds_ok;           /* GFSM_DFA_LDS applies */
+    int32_t pad;
+    int64_t device_bytes;
+} gfsm_dfa_props;
+
+/* Library */
+int gfsm_abi_version(void);
+const char *gfsm_last_error(void);
+/* Name of device `ordinal`'s architecture (e.g. "gfx950") into buf. */
+int gfsm_device_arch(int32_t ordinal, char *buf, int32_t buflen);
+
+/* NFA handles. Replaces the per-call CSR upload of every run_* in api.hpp:39-97
+ * (dev_copy of sym_row_ptr/sym_targets/sym_symbols/eps_row_ptr/eps_targets/accept). The
+ * arrays follow `src/gpufsm/core/nfa.py:43-65`: labels 0..255 or 256 (= ANY_SYMBOL). */
+int gfsm_nfa_create(int32_t num_states, int32_t start_state, const uint8_t *accept,
+                    const int32_t *sym_row_ptr, const int32_t *sym_targets,
+                    const int32_t *sym_symbols, const int32_t *eps_row_ptr,
+                    const int32_t *eps_targets, gfsm_nfa **out);
+int gfsm_nfa_destroy(gfsm_nfa *nfa);
+int gfsm_nfa_query(const gfsm_nfa *nfa, gfsm_nfa_props *out);
+/* Structural analysis on the host only (no device needed, nothing uploaded): fills the
+ * same properties gfsm_nfa_query reports (device_bytes = what a handle would hold). No
+ * upstream counterpart; technique selection uses it before committing device memory. */
+int gfsm_nfa_analyze(int32_t num_states, int32_t start_state, const uint8_t *accept,
+                     const int32_t *sym_row_ptr, const int32_t *sym_targets,
+                     const int32_t *sym_symbols, const int32_t *eps_row_ptr,
+                     const int32_t *eps_targets, gfsm_nfa_props *out);
+
+/* NFA batch, host buffers, synchronous. Replaces run_worklist / run_worklist_warp /
+ * run_multistream / ... (api.hpp:47-97): same inputs and outputs, bytes instead of int32
+ * symbols and int64 offsets[num_strings + 1]. kernel_ms (nullable) is the device time of
+ * the launch(es), measured with HIP events as the upstream entry points do. */
+int gfsm_nfa_run_host(gfsm_nfa *nfa, int32_t technique, const uint8_t *data,
+                      const int64_t *offsets, int32_t num_strings, int32_t *flags,
+                      int32_t *lens, float *kernel_ms);
+
+/* NFA batch, device buffers, asynchronous on `stream`. No upstream counterpart (upstream
+ * always copies from the host); it is the resident form the batched and served paths use. */
+int gfsm_nfa_run_device(gfsm_nfa *nfa, int32_t technique, const uint8_t *d_data,
+                        const int64_t *d_offsets, int32_t num_strings, int32_t *d_flags,
+                        int32_t *d_lens, void *stream);
+
+/* DFA handles. Replaces the per-call trans/accept upload of run_dfa (api.hpp:101-104);
+ * layout of `src/gpufsm/core/dfa.py:25-37` (trans[s * 256 + c]). */
+int gfsm_dfa_create(int32_t num_states, int32_t start_state, const uint8_t *accept,
+                    const int32_t *trans, gfsm_dfa **out);
+int gfsm_dfa_destroy(gfsm_dfa *dfa);
+int gfsm_dfa_query(const gfsm_dfa *dfa, gfsm_dfa_props *out);
+
+/* DFA batch, host buffers, synchronous. Replaces run_dfa (`native/dfa.cu:32-71`). */
+int gfsm_dfa_run_host(gfsm_dfa *dfa, int32_t technique, const uint8_t *data,
+                      const int64_t *offsets, int32_t num_strings, int32_t *flags,
+                      int32_t *lens, float *kernel_ms);
+
+/* DFA batch, device buffers, asynchronous on `stream` (resident form). */
+int gfsm_dfa_run_device(gfsm_dfa *dfa, int32_t technique, const uint8_t *d_data,
+                        const int64_t *d_offsets, int32_t num_strings, int32_t *d_flags,
+                        int32_t *d_lens, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPUFSM_HIP_H */

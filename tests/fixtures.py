@@ -1,0 +1,85 @@
+"""Small hand-built automata with labelled inputs and (de)serialisation of the golden corpus.
+
+The two example automata restate `src/gpufsm/examples.py:15-65` (their edges and their
+labelled inputs); the JSON layout is the one of the vendored ``tests/golden/golden.json``
+(`tests/generate_golden.py:50-91`).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from gpufsm.core.dfa import DFA
+from gpufsm.core.nfa import NFA, NFABuilder
+
+
+def ab_star_c_plus_d():
+    """Roughly ``(a b* c+)+ d`` with an epsilon back edge."""
+    b = NFABuilder()
+    s0, s1, s2, s3 = b.add_state(), b.add_state(), b.add_state(), b.add_state(accept=True)
+    b.set_start(s0)
+    b.add_transition(s0, "a", s1)
+    b.add_transition(s1, "b", s1)
+    b.add_transition(s1, "c", s2)
+    b.add_epsilon(s2, s0)
+    b.add_transition(s2, "c", s2)
+    b.add_transition(s2, "d", s3)
+    labelled = [(b"abcd", True), (b"acd", True), (b"abcabcabcd", True), (b"acacd", True),
+                (b"cd", False), (b"d", False), (b"", False), (b"abc", False)]
+    return b.build(), labelled
+
+
+def a_star_b_c_opt_d():
+    """``a* b (c|epsilon) d``."""
+    b = NFABuilder()
+    s0, s1, s2, s3 = b.add_state(), b.add_state(), b.add_state(), b.add_state(accept=True)
+    b.set_start(s0)
+    b.add_transition(s0, "a", s0)
+    b.add_transition(s0, "b", s1)
+    b.add_transition(s1, "c", s2)
+    b.add_epsilon(s1, s2)
+    b.add_transition(s2, "d", s3)
+    labelled = [(b"abd", True), (b"bcd", True), (b"bd", True), (b"aaaaaaabd", True),
+                (b"aaabcd", True), (b"ac", False), (b"bb", False), (b"", False)]
+    return b.build(), labelled
+
+
+EXAMPLES = {"ab_star_c_plus_d": ab_star_c_plus_d, "a_star_b_c_opt_d": a_star_b_c_opt_d}
+
+
+def nfa_from_json(d: dict) -> NFA:
+    return NFA(
+        num_states=int(d["num_states"]),
+        start_state=int(d["start_state"]),
+        accept=np.array(d["accept"], dtype=bool),
+        sym_row_ptr=np.array(d["sym_row_ptr"], dtype=np.int32),
+        sym_targets=np.array(d["sym_targets"], dtype=np.int32),
+        sym_symbols=np.array(d["sym_symbols"], dtype=np.int32),
+        eps_row_ptr=np.array(d["eps_row_ptr"], dtype=np.int32),
+        eps_targets=np.array(d["eps_targets"], dtype=np.int32),
+    )
+
+
+def dfa_from_json(d: dict) -> DFA:
+    return DFA(
+        num_states=int(d["num_states"]),
+        start_state=int(d["start_state"]),
+        accept=np.array(d["accept"], dtype=bool),
+        trans=np.array(d["trans"], dtype=np.int32),
+    )
+
+
+def random_nfa(rng, n: int, alphabet: bytes, *, max_sym: int = 2, max_eps: int = 1,
+               p_accept: float = 0.2, p_any: float = 0.1) -> NFA:
+    """Small random NFA with wildcard and epsilon edges (test generator)."""
+    b = NFABuilder()
+    for _ in range(n):
+        b.add_state(accept=rng.random() < p_accept)
+    b.set_start(rng.randrange(n))
+    for s in range(n):
+        for _ in range(rng.randint(0, max_sym)):
+            sym = 256 if rng.random() < p_any else rng.choice(alphabet)
+            b.add_transition(s, sym, rng.randrange(n))
+        for _ in range(rng.randint(0, max_eps)):
+            b.add_epsilon(s, rng.randrange(n))
+    return b.build()

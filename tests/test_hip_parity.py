@@ -1,0 +1,208 @@
+"""HIP backend parity with the CPU oracle: every verdict bit-exact.
+
+Run on an MI355X (``-m gpu``). The HIP backend must be available there: a missing or
+broken library fails these tests instead of skipping them. Cases mirror the upstream GPU
+suites (`tests/test_gpu_backends.py:36-137`, `tests/test_dfa_gpu.py:34-63`) and add the
+golden corpus, ragged batches, unaligned offsets and the pattern-union layouts.
+"""
+
+from __future__ import annotations
+
+import random
+
+import numpy as np
+import pytest
+
+from fixtures import EXAMPLES, dfa_from_json, nfa_from_json, random_nfa
+from gpufsm import Backend, is_available, run, run_batch
+from gpufsm import workloads as W
+from gpufsm.core.dfa import random_dfa
+from gpufsm.core.nfa import NFABuilder
+from gpufsm.core.packing import pack_bytes
+from gpufsm.core.registry import unavailable_reason
+from gpufsm.backends.hip.handles import DeviceDFA, DeviceNFA
+from gpufsm.backends.hip import _lib
+from oracle import native
+from oracle.reference import simulate, simulate_dfa
+
+pytestmark = pytest.mark.gpu
+
+NFA_TECHS = ["auto", "worklist"]
+DFA_TECHS = ["auto", "gather", "lds"]
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _require_hip():
+    assert is_available(Backend.HIP), f"HIP backend unavailable: {unavailable_reason(Backend.HIP)}"
+
+
+def _verdicts(results):
+    return [(r.accepted, r.match_len) for r in results]
+
+
+def _oracle_nfa(nfa, batch):
+    data, off = pack_bytes(batch)
+    f, n, _ = native.nfa_batch(nfa, data, off)
+    return [(bool(a), int(b)) for a, b in zip(f, n)]
+
+
+def _oracle_dfa(dfa, batch):
+    data, off = pack_bytes(batch)
+    f, n, _ = native.dfa_batch(dfa, data, off)
+    return [(bool(a), int(b)) for a, b in zip(f, n)]
+
+
+def test_device_is_gfx950():
+    assert _lib.device_arch(0).startswith("gfx950")
+
+
+@pytest.mark.parametrize("tech", NFA_TECHS)
+def test_golden_nfa_corpus(golden, tech):
+    for case in golden["nfa_cases"]:
+        nfa = nfa_from_json(case["nfa"])
+        data = bytes.fromhex(case["input_hex"])
+        res = run(nfa, data, backend="hip", technique=tech)
+        assert (res.accepted, res.match_len) == (case["accepted"], case["match_len"]), case["id"]
+
+
+@pytest.mark.parametrize("tech", ["auto", "gather"])
+def test_golden_dfa_corpus(golden, tech):
+    for case in golden["dfa_cases"]:
+        dfa = dfa_from_json(case["dfa"])
+        data = bytes.fromhex(case["input_hex"])
+        res = run(dfa, data, backend="hip", technique=tech)
+        assert (res.accepted, res.match_len) == (case["accepted"], case["match_len"]), case["id"]
+
+
+def test_golden_dfa_corpus_lds(golden):
+    """All golden DFAs have at most 8 states, so the LDS table applies to every one."""
+    for case in golden["dfa_cases"]:
+        dfa = dfa_from_json(case["dfa"])
+        data = bytes.fromhex(case["input_hex"])
+        res = run(dfa, data, backend="hip", technique="lds")
+        assert (res.accepted, res.match_len) == (case["accepted"], case["match_len"]), case["id"]
+
+
+@pytest.mark.parametrize("tech", NFA_TECHS)
+def test_examples(tech):
+    for name, make in EXAMPLES.items():
+        nfa, labelled = make()
+        batch = [d for d, _ in labelled]
+        got = _verdicts(run_batch(nfa, batch, backend="hip", technique=tech))
+        assert got == [simulate(nfa, d) for d in batch], name
+
+
+@pytest.mark.parametrize("tech", NFA_TECHS)
+def test_fuzz_batches(tech):
+    rng = random.Random(11)
+    for trial in range(30):
+        nfa = random_nfa(rng, rng.randint(1, 64), b"abcd", max_sym=3, max_eps=2, p_any=0.05)
+        batch = [bytes(rng.choice(b"abcd") for _ in range(rng.randint(0, 24)))
+                 for _ in range(rng.randint(1, 40))]
+        got = _verdicts(run_batch(nfa, batch, backend="hip", technique=tech))
+        assert got == [simulate(nfa, d) for d in batch], trial
+
+
+def test_multiword_and_large_automata():
+    rng = random.Random(23)
+    for n in (65, 130, 300, 1000, 5000, 20000):
+        nfa = random_nfa(rng, n, b"abcde", max_sym=3, max_eps=1, p_accept=0.01, p_any=0.02)
+        batch = [bytes(rng.choice(b"abcdef") for _ in range(rng.randint(0, 200)))
+                 for _ in range(50)]
+        got = _verdicts(run_batch(nfa, batch, backend="hip", technique="worklist"))
+        assert got == _oracle_nfa(nfa, batch), n
+
+
+def test_edge_cases():
+    b = NFABuilder()
+    b.set_start(b.add_state(accept=True))
+    acc = b.build()
+    assert _verdicts(run_batch(acc, [b"", b"xyz"], backend="hip")) == [(True, 0), (True, 0)]
+    assert run_batch(acc, [], backend="hip") == []
+    nfa, _ = EXAMPLES["a_star_b_c_opt_d"]()
+    assert _verdicts(run_batch(nfa, [b"", b"", b""], backend="hip")) == [(False, 0)] * 3
+    dfa = random_dfa(8, accept_prob=0.3, seed=2)
+    assert run_batch(dfa, [], backend="hip") == []
+    assert _verdicts(run_batch(dfa, [b""], backend="hip")) == [simulate_dfa(dfa, b"")]
+
+
+@pytest.mark.parametrize("n", [16, 256, 300, 4096, 40000])
+@pytest.mark.parametrize("tech", DFA_TECHS)
+def test_random_dfa_ragged_unaligned(n, tech):
+    dfa = random_dfa(n, accept_prob=0.02, seed=n)
+    if tech == "lds" and n > 320:
+        with pytest.raises(ValueError):
+            run_batch(dfa, [b"x"], backend="hip", technique=tech)
+        return
+    rng = random.Random(n)
+    batch = [bytes(rng.randrange(256) for _ in range(rng.randint(0, 300))) for _ in range(257)]
+    got = _verdicts(run_batch(dfa, batch, backend="hip", technique=tech))
+    assert got == _oracle_dfa(dfa, batch)
+
+
+@pytest.mark.parametrize("tech", DFA_TECHS)
+def test_dfa_non_latching_long_inputs(tech):
+    """No accepting state: every byte is walked, across 16-byte vector boundaries."""
+    dfa = random_dfa(256, accept_prob=0.0, seed=5)
+    dfa.accept[17] = True  # a rare accepting state so some inputs latch late
+    rng = np.random.default_rng(5)
+    lens = rng.integers(0, 5000, size=300)
+    data = rng.integers(0, 256, size=int(lens.sum()), dtype=np.uint8)
+    off = np.zeros(lens.size + 1, dtype=np.int64)
+    np.cumsum(lens, out=off[1:])
+    with DeviceDFA(dfa) as dev:
+        f, n, _ = dev.run_host(data, off, _lib.DFA_TECHNIQUES[tech])
+    wf, wn, _ = native.dfa_batch(dfa, data, off)
+    np.testing.assert_array_equal(f, wf)
+    np.testing.assert_array_equal(n, wn)
+
+
+@pytest.mark.parametrize("tech", ["auto", "bitpacked", "worklist"])
+def test_pattern_union_ragged(tech):
+    pu, (data, off) = W.workload_ragged(count=300, num_patterns=300, seed=3)
+    # shorten to keep the oracle quick
+    lens = np.minimum(np.diff(off), 3000)
+    data2 = np.concatenate([data[off[i]:off[i] + lens[i]] for i in range(lens.size)])
+    off2 = np.zeros(lens.size + 1, dtype=np.int64)
+    np.cumsum(lens, out=off2[1:])
+    with DeviceNFA(pu.nfa) as dev:
+        assert dev.props.chain_ok == 1
+        f, n, _ = dev.run_host(data2, off2, _lib.NFA_TECHNIQUES[tech])
+    wf, wn, _ = native.nfa_batch(pu.nfa, data2, off2)
+    assert wf.any() and not wf.all()
+    np.testing.assert_array_equal(f, wf)
+    np.testing.assert_array_equal(n, wn)
+
+
+def test_pattern_union_many_tiles_non_latching_prefix():
+    """65536-state union: 16 tiles; a few accepting states planted so some inputs latch."""
+    pu, (data, off) = W.workload_divergent(count=16, length=1500, seed=4, accepting=False)
+    last = np.cumsum(pu.lengths)  # state id of the last position of each pattern
+    pu.nfa.accept[last[:400]] = True
+    with DeviceNFA(pu.nfa) as dev:
+        assert dev.props.chain_ok == 1 and dev.props.num_tiles >= 16
+        f, n, _ = dev.run_host(data, off, _lib.NFA_BITPACKED)
+    wf, wn, _ = native.nfa_batch(pu.nfa, data, off)
+    np.testing.assert_array_equal(f, wf)
+    np.testing.assert_array_equal(n, wn)
+
+
+def test_pattern_union_latching_bitpacked_vs_worklist():
+    pu, (data, off) = W.workload_divergent(count=64, length=4096, seed=6)
+    with DeviceNFA(pu.nfa) as dev:
+        fb, nb, _ = dev.run_host(data, off, _lib.NFA_BITPACKED)
+        fw, nw, _ = dev.run_host(data, off, _lib.NFA_WORKLIST)
+    wf, wn, _ = native.nfa_batch(pu.nfa, data, off)
+    np.testing.assert_array_equal(fb, wf)
+    np.testing.assert_array_equal(nb, wn)
+    np.testing.assert_array_equal(fw, wf)
+    np.testing.assert_array_equal(nw, wn)
+
+
+def test_plain_nfa_config():
+    nfa, (data, off) = W.workload_plain()
+    with DeviceNFA(nfa) as dev:
+        f, n, _ = dev.run_host(data, off, _lib.NFA_AUTO)
+    wf, wn, _ = native.nfa_batch(nfa, data, off)
+    np.testing.assert_array_equal(f, wf)
+    np.testing.assert_array_equal(n, wn)
