@@ -122,11 +122,18 @@ int32_t resolve_nfa(const gfsm_nfa *h, int32_t technique) {
     }
 }
 
-int32_t resolve_dfa(const gfsm_dfa *h, int32_t technique) {
+int32_t resolve_dfa(const gfsm_dfa *h, int32_t technique, int32_t count, int64_t total) {
     switch (technique) {
-        case GFSM_DFA_AUTO:
+        case GFSM_DFA_AUTO: {
+            // Segment the batch when inputs span several segments: one lane per input
+            // would serialise each of them into one long dependent chain.
+            const DfaSplitPlan plan = dfa_split_plan(h->dev, total, count);
+            const int64_t avg = count > 0 ? total / count : 0;
+            if (avg >= 4 * plan.seg_bytes) return GFSM_DFA_SPLIT;
             return h->lds_ok ? GFSM_DFA_LDS : GFSM_DFA_GATHER;
+        }
         case GFSM_DFA_GATHER:
+        case GFSM_DFA_SPLIT:
             return technique;
         case GFSM_DFA_LDS:
             if (!h->lds_ok)
@@ -163,10 +170,17 @@ void nfa_enqueue(gfsm_nfa *h, int32_t tech, const uint8_t *d_data, const int64_t
 }
 
 void dfa_enqueue(gfsm_dfa *h, int32_t tech, const uint8_t *d_data, const int64_t *d_off,
-                 int32_t count, int32_t *d_flags, int32_t *d_lens, hipStream_t s) {
+                 int32_t count, int64_t total, int32_t *d_flags, int32_t *d_lens,
+                 hipStream_t s) {
     if (count == 0) return;
     if (h->start_accepts) {
         launch_fill_result(d_flags, d_lens, count, 1, 0, s);
+        return;
+    }
+    if (tech == GFSM_DFA_SPLIT) {
+        const DfaSplitPlan plan = dfa_split_plan(h->dev, total, count);
+        StreamBuf scratch(plan.scratch_bytes, s);
+        launch_dfa_split(h->dev, plan, d_data, d_off, count, total, d_flags, d_lens, scratch.p, s);
         return;
     }
     if (tech == GFSM_DFA_LDS)
@@ -202,7 +216,7 @@ void run_host(const uint8_t *data, const int64_t *offsets, int32_t count, int32_
     try {
         GFSM_HIP(hipEventRecord(e0, s));
         enqueue(static_cast<const uint8_t *>(d_data.p), static_cast<const int64_t *>(d_off.p),
-                d_flags, d_lens, s);
+                total, d_flags, d_lens, s);
         GFSM_HIP(hipEventRecord(e1, s));
         GFSM_HIP(hipMemcpyAsync(flags, d_flags, static_cast<size_t>(count) * sizeof(int32_t),
                                 hipMemcpyDeviceToHost, s));
@@ -361,17 +375,18 @@ int gfsm_nfa_run_host(gfsm_nfa *nfa, int32_t technique, const uint8_t *data,
         check_batch_args(data, offsets, num_strings, flags, lens);
         const int32_t tech = resolve_nfa(nfa, technique);
         run_host(data, offsets, num_strings, flags, lens, kernel_ms,
-                 [&](const uint8_t *dd, const int64_t *doff, int32_t *df, int32_t *dl,
+                 [&](const uint8_t *dd, const int64_t *doff, int64_t, int32_t *df, int32_t *dl,
                      hipStream_t s) { nfa_enqueue(nfa, tech, dd, doff, num_strings, df, dl, s); });
     });
 }
 
 int gfsm_nfa_run_device(gfsm_nfa *nfa, int32_t technique, const uint8_t *d_data,
-                        const int64_t *d_offsets, int32_t num_strings, int32_t *d_flags,
-                        int32_t *d_lens, void *stream) {
+                        const int64_t *d_offsets, int32_t num_strings, int64_t total_bytes,
+                        int32_t *d_flags, int32_t *d_lens, void *stream) {
     return guarded([&] {
         if (!nfa) throw Error(GFSM_EINVAL, "null handle");
         check_batch_args(d_data, d_offsets, num_strings, d_flags, d_lens);
+        if (total_bytes < 0) throw Error(GFSM_EINVAL, "total_bytes must be >= 0");
         const int32_t tech = resolve_nfa(nfa, technique);
         nfa_enqueue(nfa, tech, d_data, d_offsets, num_strings, d_flags, d_lens,
                     static_cast<hipStream_t>(stream));
@@ -394,8 +409,10 @@ int gfsm_dfa_create(int32_t num_states, int32_t start_state, const uint8_t *acce
             else
                 h->dev.table = upload(h->blocks, h->device_bytes, t.t16.data(), t.t16.size());
             h->dev.n = num_states;
-            h->dev.start = start_state;
             h->dev.wide = t.wide ? 1 : 0;
+            h->dev.start_entry = t.start_entry;
+            h->dev.accept_from = t.accept_from;
+            h->dev.col_bytes = t.col_bytes;
             h->lds_ok = !t.wide && dfa_lds_bytes(num_states) <= 160 * 1024;
         } catch (...) {
             free_blocks(h->blocks);
@@ -433,21 +450,25 @@ int gfsm_dfa_run_host(gfsm_dfa *dfa, int32_t technique, const uint8_t *data,
     return guarded([&] {
         if (!dfa) throw Error(GFSM_EINVAL, "null handle");
         check_batch_args(data, offsets, num_strings, flags, lens);
-        const int32_t tech = resolve_dfa(dfa, technique);
+        const int64_t total = num_strings > 0 ? offsets[num_strings] : 0;
+        const int32_t tech = resolve_dfa(dfa, technique, num_strings, total);
         run_host(data, offsets, num_strings, flags, lens, kernel_ms,
-                 [&](const uint8_t *dd, const int64_t *doff, int32_t *df, int32_t *dl,
-                     hipStream_t s) { dfa_enqueue(dfa, tech, dd, doff, num_strings, df, dl, s); });
+                 [&](const uint8_t *dd, const int64_t *doff, int64_t tot, int32_t *df,
+                     int32_t *dl, hipStream_t s) {
+                     dfa_enqueue(dfa, tech, dd, doff, num_strings, tot, df, dl, s);
+                 });
     });
 }
 
 int gfsm_dfa_run_device(gfsm_dfa *dfa, int32_t technique, const uint8_t *d_data,
-                        const int64_t *d_offsets, int32_t num_strings, int32_t *d_flags,
-                        int32_t *d_lens, void *stream) {
+                        const int64_t *d_offsets, int32_t num_strings, int64_t total_bytes,
+                        int32_t *d_flags, int32_t *d_lens, void *stream) {
     return guarded([&] {
         if (!dfa) throw Error(GFSM_EINVAL, "null handle");
         check_batch_args(d_data, d_offsets, num_strings, d_flags, d_lens);
-        const int32_t tech = resolve_dfa(dfa, technique);
-        dfa_enqueue(dfa, tech, d_data, d_offsets, num_strings, d_flags, d_lens,
+        if (total_bytes < 0) throw Error(GFSM_EINVAL, "total_bytes must be >= 0");
+        const int32_t tech = resolve_dfa(dfa, technique, num_strings, total_bytes);
+        dfa_enqueue(dfa, tech, d_data, d_offsets, num_strings, total_bytes, d_flags, d_lens,
                     static_cast<hipStream_t>(stream));
     });
 }

@@ -68,11 +68,12 @@ struct NfaTileDev {
 
 // ----- DFA ------------------------------------------------------------------------------
 
-// Table entries carry the accept bit of the successor in their top bit, so one load per
-// byte gives both the next state and the latch test.
+// Table by byte, entries = successor's byte offset in a column (see layout.hpp): the step
+// is `e = *(T *)(table + byte * col_bytes + e)`, and e >= accept_from means accepting.
 struct DfaDev {
-    const void *table;         // uint16 [n * 256] or uint32 [n * 256]
-    int32_t n, start, wide;    // wide: uint32 entries
+    const void *table;         // uint16 [256][n] or uint32 [256][n]
+    int32_t n, wide;           // wide: uint32 entries
+    uint32_t start_entry, accept_from, col_bytes;
 };
 
 // ----- launchers (asynchronous on `stream`) ---------------------------------------------
@@ -85,6 +86,19 @@ void launch_dfa_gather(const DfaDev &d, const uint8_t *data, const int64_t *off,
 void launch_dfa_lds(const DfaDev &d, const uint8_t *data, const int64_t *off, int32_t count,
                     int32_t *flags, int32_t *lens, hipStream_t stream);
 size_t dfa_lds_bytes(int32_t n);
+
+// Split DFA walk (exact, in-input parallel): geometry and scratch of one launch.
+struct DfaSplitPlan {
+    int64_t seg_bytes = 0;     // segment length C (multiple of the checkpoint interval)
+    int32_t ckpt_shift = 0;    // log2 of the checkpoint interval R
+    int64_t segments = 0;      // ceil(total_bytes / C)
+    size_t scratch_bytes = 0;  // device scratch the launch needs
+};
+DfaSplitPlan dfa_split_plan(const DfaDev &d, int64_t total_bytes, int32_t count);
+void launch_dfa_split(const DfaDev &d, const DfaSplitPlan &plan, const uint8_t *data,
+                      const int64_t *off, int32_t count, int64_t total_bytes, int32_t *flags,
+                      int32_t *lens, void *scratch, hipStream_t stream);
+int32_t device_cu_count();
 
 void launch_nfa_worklist(const NfaWorklistDev &d, const uint8_t *data, const int64_t *off,
                          int32_t count, int32_t *flags, int32_t *lens, int32_t *scratch,

@@ -231,20 +231,35 @@ DfaHost build_dfa(int32_t n, int32_t start, const uint8_t *accept, const int32_t
     if (n < 1) bad("num_states must be >= 1");
     if (start < 0 || start >= n) bad("start_state out of range");
     if (!accept || !trans) bad("null array");
-    if (static_cast<int64_t>(n) * kAlphabet >= (1ll << 31)) bad("DFA table too large");
-    DfaHost h;
+    if (static_cast<int64_t>(n) * kAlphabet * 4 >= (1ll << 31)) bad("DFA table too large");
     const size_t total = static_cast<size_t>(n) * kAlphabet;
+    for (size_t k = 0; k < total; ++k)
+        if (trans[k] < 0 || trans[k] >= n)
+            bad("trans entry out of range at index " + std::to_string(k));
+    DfaHost h;
     h.wide = n > 0x7fff;
-    if (h.wide) h.t32.resize(total); else h.t16.resize(total);
-    for (size_t k = 0; k < total; ++k) {
-        const int32_t t = trans[k];
-        if (t < 0 || t >= n) bad("trans entry out of range at index " + std::to_string(k));
-        if (h.wide)
-            h.t32[k] = static_cast<uint32_t>(t) | (accept[t] ? 0x80000000u : 0u);
-        else
-            h.t16[k] = static_cast<uint16_t>(t | (accept[t] ? 0x8000 : 0));
-    }
+    const uint32_t esz = h.wide ? 4u : 2u;
+    // non-accepting states first, accepting states last (original order kept in each)
+    std::vector<uint32_t> id(static_cast<size_t>(n));
+    uint32_t next = 0;
+    for (int32_t s = 0; s < n; ++s)
+        if (!accept[s]) id[s] = next++;
+    const uint32_t first_accepting = next;
+    for (int32_t s = 0; s < n; ++s)
+        if (accept[s]) id[s] = next++;
+    h.accept_from = first_accepting * esz;
+    h.start_entry = id[start] * esz;
+    h.col_bytes = static_cast<uint32_t>(n) * esz;
     h.start_accepts = accept[start] != 0;
+    if (h.wide) h.t32.resize(total); else h.t16.resize(total);
+    for (int32_t s = 0; s < n; ++s) {
+        const uint32_t row = id[s];
+        for (int c = 0; c < kAlphabet; ++c) {
+            const uint32_t e = id[trans[static_cast<size_t>(s) * kAlphabet + c]] * esz;
+            const size_t at = static_cast<size_t>(c) * n + row;
+            if (h.wide) h.t32[at] = e; else h.t16[at] = static_cast<uint16_t>(e);
+        }
+    }
     return h;
 }
 

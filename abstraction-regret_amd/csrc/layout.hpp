@@ -41,12 +41,18 @@ void validate_nfa(const NfaInput &a);
 WorklistHost build_worklist(const NfaInput &a);
 TileHost build_tiles(const NfaInput &a);
 
-// DFA: entries = successor | accept(successor) << (15 or 31).
+// DFA device layout. States are renumbered so that the accepting ones come last; the
+// table is stored by byte (column c holds the successor of every state on byte c) and each
+// entry is the successor's byte offset inside a column (state * entry size). A step is then
+// one multiply-add and one load, and "the successor accepts" is "entry >= accept_from".
 struct DfaHost {
-    std::vector<uint16_t> t16;
-    std::vector<uint32_t> t32;
+    std::vector<uint16_t> t16;          // [256][n] when n <= 32767
+    std::vector<uint32_t> t32;          // [256][n] otherwise
     bool wide = false;
     bool start_accepts = false;
+    uint32_t start_entry = 0;           // renumbered start state, scaled
+    uint32_t accept_from = 0;           // scaled id of the first accepting state
+    uint32_t col_bytes = 0;             // n * entry size
 };
 DfaHost build_dfa(int32_t n, int32_t start, const uint8_t *accept, const int32_t *trans);
 

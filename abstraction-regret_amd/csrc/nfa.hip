@@ -174,7 +174,8 @@ void launch_nfa_worklist(const NfaWorklistDev &d, const uint8_t *data, const int
     const size_t bytes = wl_lds_bytes(d, lds_lists);
     if (bytes > 64 * 1024) {
         GFSM_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&nfa_worklist_kernel),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     static_cast<int>(bytes)));
     }
     nfa_worklist_kernel<<<grid, 64, bytes, stream>>>(d, data, off, count, flags, lens, scratch,
                                                      lds_lists ? 1 : 0);
@@ -307,7 +308,8 @@ void launch_nfa_bitpacked(const NfaTileDev &d, const uint8_t *data, const int64_
     const size_t bytes = kBpLdsWords * sizeof(uint64_t);
     if (!attr_set) {
         GFSM_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&nfa_bitpacked_kernel),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     static_cast<int>(bytes)));
         attr_set = true;
     }
     bp_init_kernel<<<(count + 255) / 256, 256, 0, stream>>>(best, count);

@@ -16,10 +16,10 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 # status codes and technique ids (gpufsm_hip.h)
 OK, EINVAL, ENOMEM, EUNSUPPORTED, EDEVICE = 0, 1, 2, 3, 4
 NFA_AUTO, NFA_WORKLIST, NFA_BITPACKED = 0, 1, 2
-DFA_AUTO, DFA_GATHER, DFA_LDS = 0, 1, 2
+DFA_AUTO, DFA_GATHER, DFA_LDS, DFA_SPLIT = 0, 1, 2, 3
 
 NFA_TECHNIQUES = {"auto": NFA_AUTO, "worklist": NFA_WORKLIST, "bitpacked": NFA_BITPACKED}
-DFA_TECHNIQUES = {"auto": DFA_AUTO, "gather": DFA_GATHER, "lds": DFA_LDS}
+DFA_TECHNIQUES = {"auto": DFA_AUTO, "gather": DFA_GATHER, "lds": DFA_LDS, "split": DFA_SPLIT}
 
 # Every symbol the header declares; the CPU test suite checks the library exports them all.
 EXPORTS = (
@@ -99,12 +99,12 @@ _SIGNATURES = {
     "gfsm_nfa_analyze": (_c.c_int, [_i32, _i32, _u8p, _i32p, _i32p, _i32p, _i32p, _i32p,
                                     _c.POINTER(NfaProps)]),
     "gfsm_nfa_run_host": (_c.c_int, [_vp, _i32, _u8p, _i64p, _i32, _i32p, _i32p, _fp]),
-    "gfsm_nfa_run_device": (_c.c_int, [_vp, _i32, _vp, _vp, _i32, _vp, _vp, _vp]),
+    "gfsm_nfa_run_device": (_c.c_int, [_vp, _i32, _vp, _vp, _i32, _c.c_int64, _vp, _vp, _vp]),
     "gfsm_dfa_create": (_c.c_int, [_i32, _i32, _u8p, _i32p, _c.POINTER(_vp)]),
     "gfsm_dfa_destroy": (_c.c_int, [_vp]),
     "gfsm_dfa_query": (_c.c_int, [_vp, _c.POINTER(DfaProps)]),
     "gfsm_dfa_run_host": (_c.c_int, [_vp, _i32, _u8p, _i64p, _i32, _i32p, _i32p, _fp]),
-    "gfsm_dfa_run_device": (_c.c_int, [_vp, _i32, _vp, _vp, _i32, _vp, _vp, _vp]),
+    "gfsm_dfa_run_device": (_c.c_int, [_vp, _i32, _vp, _vp, _i32, _c.c_int64, _vp, _vp, _vp]),
 }
 
 _lib: ctypes.CDLL | None = None

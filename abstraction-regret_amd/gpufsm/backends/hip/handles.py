@@ -105,11 +105,14 @@ class DeviceNFA(_Handle):
             count, ptr(flags, ctypes.c_int32), ptr(lens, ctypes.c_int32), ctypes.byref(ms)))
         return flags[:count], lens[:count], float(ms.value)
 
-    def run_device(self, d_data: int, d_offsets: int, count: int, d_flags: int, d_lens: int,
-                   stream: int = 0, technique: int = _lib.NFA_AUTO) -> None:
+    def run_device(self, d_data: int, d_offsets: int, count: int, total_bytes: int,
+                   d_flags: int, d_lens: int, stream: int = 0,
+                   technique: int = _lib.NFA_AUTO) -> None:
+        """Enqueue on ``stream``; ``total_bytes`` must equal ``offsets[count]``."""
         check(_lib.load().gfsm_nfa_run_device(
             self.handle, technique, ctypes.c_void_p(d_data), ctypes.c_void_p(d_offsets), count,
-            ctypes.c_void_p(d_flags), ctypes.c_void_p(d_lens), ctypes.c_void_p(stream)))
+            int(total_bytes), ctypes.c_void_p(d_flags), ctypes.c_void_p(d_lens),
+            ctypes.c_void_p(stream)))
 
 
 class DeviceDFA(_Handle):
@@ -142,8 +145,11 @@ class DeviceDFA(_Handle):
             count, ptr(flags, ctypes.c_int32), ptr(lens, ctypes.c_int32), ctypes.byref(ms)))
         return flags[:count], lens[:count], float(ms.value)
 
-    def run_device(self, d_data: int, d_offsets: int, count: int, d_flags: int, d_lens: int,
-                   stream: int = 0, technique: int = _lib.DFA_AUTO) -> None:
+    def run_device(self, d_data: int, d_offsets: int, count: int, total_bytes: int,
+                   d_flags: int, d_lens: int, stream: int = 0,
+                   technique: int = _lib.DFA_AUTO) -> None:
+        """Enqueue on ``stream``; ``total_bytes`` must equal ``offsets[count]``."""
         check(_lib.load().gfsm_dfa_run_device(
             self.handle, technique, ctypes.c_void_p(d_data), ctypes.c_void_p(d_offsets), count,
-            ctypes.c_void_p(d_flags), ctypes.c_void_p(d_lens), ctypes.c_void_p(stream)))
+            int(total_bytes), ctypes.c_void_p(d_flags), ctypes.c_void_p(d_lens),
+            ctypes.c_void_p(stream)))

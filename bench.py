@@ -66,13 +66,14 @@ class DeviceBatch:
 
     def __init__(self, torch, data: np.ndarray, offsets: np.ndarray) -> None:
         self.count = int(offsets.size) - 1
+        self.total = int(offsets[-1])
         self.data = torch.from_numpy(data if data.size else np.zeros(1, np.uint8)).to("cuda")
         self.offsets = torch.from_numpy(offsets).to("cuda")
         self.flags = torch.empty(max(self.count, 1), dtype=torch.int32, device="cuda")
         self.lens = torch.empty(max(self.count, 1), dtype=torch.int32, device="cuda")
 
     def run(self, dev, technique: int, stream: int) -> None:
-        dev.run_device(self.data.data_ptr(), self.offsets.data_ptr(), self.count,
+        dev.run_device(self.data.data_ptr(), self.offsets.data_ptr(), self.count, self.total,
                        self.flags.data_ptr(), self.lens.data_ptr(), stream, technique)
 
     def results(self):
@@ -98,93 +99,123 @@ def _time_steps(torch, fn, steps: int, warmup: int):
     return wall, [s.elapsed_time(e) for s, e in zip(starts, ends)]
 
 
-def bench_dfa(args, torch) -> dict:
-    from gpufsm import run_batch, workloads as W
+KERNELS = {
+    "gather": "dfa_gather_kernel",
+    "lds": "dfa_lds_kernel",
+    "split": "dfa_split_walk_kernel",
+}
+
+
+def _measure_dfa(torch, batch, dfa, technique: str, data, offsets, steps: int, warmup: int,
+                 what: str) -> dict:
+    """Gate one DFA/technique on the full batch against the C oracle, then time it."""
     from gpufsm.backends.hip import _lib
     from gpufsm.backends.hip.handles import DeviceDFA
 
     from oracle import gate, native
 
-    n, count, length = args.states, args.count, args.length
-    tech = _lib.DFA_TECHNIQUES[args.technique]
+    tech = _lib.DFA_TECHNIQUES[technique]
     stream = torch.cuda.current_stream().cuda_stream
-    _log(f"workload dfa n={n} count={count} length={length} technique={args.technique}")
-    data, offsets = W.uniform_batch(count, length, seed=0)
-    batch = DeviceBatch(torch, data, offsets)
-    extras: dict = {}
-
-    # ---- latching variant as configured (10% accepting) ----
-    dfa_l = W.dense_dfa(n, 0.10, seed=0)
-    with DeviceDFA(dfa_l) as dev:
+    with DeviceDFA(dfa) as dev:
         batch.run(dev, tech, stream)
         torch.cuda.synchronize()
         f, m = batch.results()
-        wf, wm, _ = native.dfa_batch(dfa_l, data, offsets)
-        gate.require_arrays(f, m, wf, wm, f"hip dfa {args.technique} (latching)")
-        examined = int(np.where(f != 0, m, np.diff(offsets)).sum())
-        wall, ks = _time_steps(torch, lambda: batch.run(dev, tech, stream), args.steps, args.warmup)
-        extras["latching"] = {
-            "accepted": int(f.sum()),
-            "examined_bytes": examined,
-            "credited_bytes": int(offsets[-1]),
-            "ms_per_step": 1e3 * wall / args.steps,
-            "examined_GBps": examined / (wall / args.steps) / 1e9,
-            "credited_GBps": int(offsets[-1]) / (wall / args.steps) / 1e9,
-        }
-    _log(f"latching: {extras['latching']}")
+        wf, wm, wex = native.dfa_batch(dfa, data, offsets)
+        gate.require_arrays(f, m, wf, wm, f"hip dfa {technique} ({what})")
+        wall, ks = _time_steps(torch, lambda: batch.run(dev, tech, stream), steps, warmup)
+        entry_bytes = dev.props.entry_bytes
+    examined = int(wex.sum())
+    credited = int(offsets[-1])
+    step_s = wall / steps
+    out = {
+        "workload": what,
+        "technique": technique,
+        "accepted": int(f.sum()),
+        "examined_bytes": examined,
+        "credited_bytes": credited,
+        "ms_per_step": 1e3 * step_s,
+        "kernel_ms": float(np.mean(ks)),
+        "examined_GBps": examined / step_s / 1e9,
+        "credited_GBps": credited / step_s / 1e9,
+        "table_bytes": dfa.num_states * 256 * entry_bytes,
+    }
+    _log(f"{what} [{technique}]: {out['examined_GBps']:.1f} GB/s examined, "
+         f"{out['ms_per_step']:.3f} ms/step")
+    return out
 
-    # ---- non-latching control: the headline ----
+
+def bench_dfa(args, torch) -> dict:
+    from gpufsm import run_batch, workloads as W
+
+    from oracle import native
+
+    n, count, length = args.states, args.count, args.length
+    _log(f"workload dfa n={n} count={count} length={length} technique={args.technique}")
+    data, offsets = W.uniform_batch(count, length, seed=0)
+    batch = DeviceBatch(torch, data, offsets)
+    extras: dict = {"variants": []}
+
+    # headline: non-latching control of the chosen table size
     dfa = W.dense_dfa(n, 0.0, seed=0)
-    dev = DeviceDFA(dfa)
-    batch.run(dev, tech, stream)
-    torch.cuda.synchronize()
-    f, m = batch.results()
+    head = _measure_dfa(torch, batch, dfa, args.technique, data, offsets, args.steps,
+                        args.warmup, f"dfa{n} non-latching")
+    # the other variants of the same workload (not the headline)
+    for other_n in (256, 4096):
+        for latching in (False, True):
+            if other_n == n and not latching:
+                continue
+            techs = ["auto"] if latching else ["auto", "gather", "split"]
+            for t in techs:
+                d = W.dense_dfa(other_n, 0.10 if latching else 0.0, seed=0)
+                extras["variants"].append(_measure_dfa(
+                    torch, batch, d, t, data, offsets, max(2, args.steps // 2), 1,
+                    f"dfa{other_n} {'latching 10%' if latching else 'non-latching'}"))
+    _log("oracle gate passed on every variant (full batch)")
+
+    # CPU baseline: the C oracle over all host cores, full batch, headline automaton
     threads = _threads()
     t0 = time.perf_counter()
     reps = 0
     while True:
-        wf, wm, wex = native.dfa_batch(dfa, data, offsets, threads=threads)
+        native.dfa_batch(dfa, data, offsets, threads=threads)
         reps += 1
         if time.perf_counter() - t0 > 2.0 or reps >= 8:
             break
     cpu_s = (time.perf_counter() - t0) / reps
-    gate.require_arrays(f, m, wf, wm, f"hip dfa {args.technique} (non-latching)")
-    _log("oracle gate passed (full batch, both variants)")
-    examined = int(wex.sum())
-    wall, ks = _time_steps(torch, lambda: batch.run(dev, tech, stream), args.steps, args.warmup)
-    step_s = wall / args.steps
-    kernel_ms = float(np.mean(ks))
-    table_bytes = n * 256 * (2 if dev.props.entry_bytes == 2 else 4)
-    compulsory = table_bytes + 8 * (count + 1) + 8 * count
-    achieved = (examined + compulsory) / (kernel_ms * 1e-3) / 1e9
-    value = examined / step_s / 1e9
 
-    # ---- plain per-call drop-in (gpufsm.run_batch: executor built per call) ----
+    # plain per-call drop-in (gpufsm.run_batch builds an executor, copies, launches, copies back)
     inputs = [data[offsets[i]:offsets[i + 1]].tobytes() for i in range(count)]
     run_batch(dfa, inputs[:4], backend="hip", technique=args.technique)
     t0 = time.perf_counter()
     res = run_batch(dfa, inputs, backend="hip", technique=args.technique)
     plain_s = time.perf_counter() - t0
-    assert [r.match_len for r in res[:8]] == [int(x) for x in wm[:8]]
-    extras["plain_per_call"] = {"seconds": plain_s, "examined_GBps": examined / plain_s / 1e9,
+    assert not any(r.accepted for r in res)
+    extras["plain_per_call"] = {"seconds": plain_s,
+                                "examined_GBps": head["examined_bytes"] / plain_s / 1e9,
                                 "kernel_ms": res[0].kernel_ms}
-    dev.close()
-    kernel_name = {
-        "gather": "dfa_gather_kernel", "lds": "dfa_lds_kernel",
-    }.get(args.technique, "dfa_gather_kernel" if n > 320 else "dfa_lds_kernel")
+
+    examined = head["examined_bytes"]
+    compulsory = head["table_bytes"] + 8 * (count + 1) + 8 * count
+    kernel_ms = head["kernel_ms"]
+    achieved = (examined + compulsory) / (kernel_ms * 1e-3) / 1e9
+    tech = args.technique
+    if tech == "auto":
+        tech = "split" if length >= 4 * (1024 if n <= 320 else max(1024, 4 * n)) else (
+            "lds" if n <= 320 else "gather")
+    kernel_name = KERNELS.get(tech, tech)
     workload_name = f"dfa{n}_{count}x{length}_nonlatching"
     return {
         "metric": "input GB/s (examined bytes), DFA walk, inputs resident in HBM",
-        "value": value,
+        "value": head["examined_GBps"],
         "unit": "GB/s",
-        "ms_per_step": 1e3 * step_s,
+        "ms_per_step": head["ms_per_step"],
         "dtype": "u8",
         "config": {
             "workload": f"DFA memory-bound face, non-latching control: random complete DFA "
                         f"{n} states x 256 (uint16 table, accept_prob 0), {count} inputs x "
                         f"{length} uniform bytes (seed 0)",
             "num_states": n, "num_inputs": count, "input_bytes": length,
-            "technique": args.technique,
+            "technique": args.technique, "resolved_technique": tech,
         },
         "roofline": {
             "bound": "hbm",
@@ -202,8 +233,8 @@ def bench_dfa(args, torch) -> dict:
             "unit": "GB/s",
             "cores": threads,
             "kind": "port",
-            "sample": f"full batch ({count} x {length} B, non-latching), C oracle over "
-                      f"{threads} threads, mean of {reps} runs",
+            "sample": f"full batch ({count} x {length} B, non-latching, {n} states), C oracle "
+                      f"over {threads} threads, mean of {reps} runs",
         },
         "extras": extras,
     }
@@ -215,7 +246,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="dfa", choices=["dfa"])
-    ap.add_argument("--states", type=int, default=4096)
+    ap.add_argument("--states", type=int, default=256)
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--length", type=int, default=65536)
     ap.add_argument("--technique", default="auto")

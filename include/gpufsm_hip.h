@@ -55,6 +55,8 @@ extern "C" {
 #define GFSM_DFA_AUTO 0
 #define GFSM_DFA_GATHER 1     /* one lane per input, table in global memory (dfa_kernel) */
 #define GFSM_DFA_LDS 2        /* one lane per input, uint16 table staged in LDS */
+#define GFSM_DFA_SPLIT 3      /* exact in-input parallel walk: fixed segments of the batch,
+                                 speculative entries repaired by checkpoint merging */
 
 typedef struct gfsm_nfa gfsm_nfa;
 typedef struct gfsm_dfa gfsm_dfa;
@@ -115,10 +117,12 @@ int gfsm_nfa_run_host(gfsm_nfa *nfa, int32_t technique, const uint8_t *data,
                       int32_t *lens, float *kernel_ms);
 
 /* NFA batch, device buffers, asynchronous on `stream`. No upstream counterpart (upstream
- * always copies from the host); it is the resident form the batched and served paths use. */
+ * always copies from the host); it is the resident form the batched and served paths use.
+ * total_bytes must equal d_offsets[num_strings] (the host knows it; reading it back would
+ * cost a synchronisation): techniques that cut inputs into segments size scratch from it. */
 int gfsm_nfa_run_device(gfsm_nfa *nfa, int32_t technique, const uint8_t *d_data,
-                        const int64_t *d_offsets, int32_t num_strings, int32_t *d_flags,
-                        int32_t *d_lens, void *stream);
+                        const int64_t *d_offsets, int32_t num_strings, int64_t total_bytes,
+                        int32_t *d_flags, int32_t *d_lens, void *stream);
 
 /* DFA handles. Replaces the per-call trans/accept upload of run_dfa (api.hpp:101-104);
  * layout of `src/gpufsm/core/dfa.py:25-37` (trans[s * 256 + c]). */
@@ -132,10 +136,11 @@ int gfsm_dfa_run_host(gfsm_dfa *dfa, int32_t technique, const uint8_t *data,
                       const int64_t *offsets, int32_t num_strings, int32_t *flags,
                       int32_t *lens, float *kernel_ms);
 
-/* DFA batch, device buffers, asynchronous on `stream` (resident form). */
+/* DFA batch, device buffers, asynchronous on `stream` (resident form); total_bytes as for
+ * gfsm_nfa_run_device. */
 int gfsm_dfa_run_device(gfsm_dfa *dfa, int32_t technique, const uint8_t *d_data,
-                        const int64_t *d_offsets, int32_t num_strings, int32_t *d_flags,
-                        int32_t *d_lens, void *stream);
+                        const int64_t *d_offsets, int32_t num_strings, int64_t total_bytes,
+                        int32_t *d_flags, int32_t *d_lens, void *stream);
 
 #ifdef __cplusplus
 }

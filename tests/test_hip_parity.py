@@ -28,7 +28,7 @@ from oracle.reference import simulate, simulate_dfa
 pytestmark = pytest.mark.gpu
 
 NFA_TECHS = ["auto", "worklist"]
-DFA_TECHS = ["auto", "gather", "lds"]
+DFA_TECHS = ["auto", "gather", "lds", "split"]
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -65,7 +65,7 @@ def test_golden_nfa_corpus(golden, tech):
         assert (res.accepted, res.match_len) == (case["accepted"], case["match_len"]), case["id"]
 
 
-@pytest.mark.parametrize("tech", ["auto", "gather"])
+@pytest.mark.parametrize("tech", ["auto", "gather", "split"])
 def test_golden_dfa_corpus(golden, tech):
     for case in golden["dfa_cases"]:
         dfa = dfa_from_json(case["dfa"])
@@ -206,3 +206,68 @@ def test_plain_nfa_config():
     wf, wn, _ = native.nfa_batch(nfa, data, off)
     np.testing.assert_array_equal(f, wf)
     np.testing.assert_array_equal(n, wn)
+
+
+def _ragged(rng, count, max_len, empty_frac=0.1):
+    lens = rng.integers(0, max_len, size=count)
+    lens[rng.random(count) < empty_frac] = 0
+    data = rng.integers(0, 256, size=int(lens.sum()), dtype=np.uint8)
+    off = np.zeros(count + 1, dtype=np.int64)
+    np.cumsum(lens, out=off[1:])
+    return data, off
+
+
+def _check_dfa(dfa, data, off, tech):
+    with DeviceDFA(dfa) as dev:
+        f, n, _ = dev.run_host(data, off, _lib.DFA_TECHNIQUES[tech])
+    wf, wn, _ = native.dfa_batch(dfa, data, off)
+    np.testing.assert_array_equal(f, wf)
+    np.testing.assert_array_equal(n, wn)
+    return wf
+
+
+@pytest.mark.parametrize("n", [16, 256, 4096, 40000])
+def test_split_rare_accepts_in_continuation_segments(n):
+    """Accepting states entered rarely, so matches land deep inside long inputs."""
+    dfa = random_dfa(n, accept_prob=0.0, seed=n + 1)
+    dfa.accept[1] = True  # one accepting state: first visit after about n bytes
+    rng = np.random.default_rng(n)
+    data, off = _ragged(rng, 96, 12 * max(n, 1024))
+    wf = _check_dfa(dfa, data, off, "split")
+    assert wf.any()
+
+
+def test_split_permutation_dfa_never_merges():
+    """Every byte permutes a 16-state component, so speculative paths never merge and pass 3
+    re-walks every continuation segment. A second component counts consecutive 0xFF bytes
+    and accepts at two in a row: matches land deep inside the inputs."""
+    from gpufsm.core.dfa import DFA
+
+    m = 16
+    rng = np.random.default_rng(9)
+    perms = np.stack([rng.permutation(m) for _ in range(256)], axis=1)  # [p, byte]
+    n = 3 * m  # state = 3 * p + k, k = 0xFF run length capped at 2
+    trans = np.zeros((n, 256), dtype=np.int32)
+    for p in range(m):
+        for k in range(3):
+            for b in range(256):
+                k2 = min(k + 1, 2) if b == 0xFF else 0
+                trans[3 * p + k, b] = 3 * perms[p, b] + k2
+    for accepting in (False, True):
+        accept = np.zeros(n, dtype=bool)
+        if accepting:
+            accept[2::3] = True
+        dfa = DFA(num_states=n, start_state=0, accept=accept, trans=trans.reshape(-1))
+        data, off = _ragged(rng, 48, 60000)
+        wf = _check_dfa(dfa, data, off, "split")
+        if accepting:
+            assert wf.any() and not wf.all()
+
+
+def test_split_latching_and_many_short_inputs():
+    dfa = random_dfa(256, accept_prob=0.10, seed=3)
+    rng = np.random.default_rng(3)
+    data, off = _ragged(rng, 5000, 300)
+    _check_dfa(dfa, data, off, "split")
+    data, off = W.uniform_batch(64, 65536, seed=1)
+    _check_dfa(dfa, data, off, "split")
