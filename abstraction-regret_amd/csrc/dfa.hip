@@ -20,6 +20,7 @@
 #include "internal.hpp"
 
 #include <climits>
+#include <cstdlib>
 
 namespace gfsm {
 
@@ -197,95 +198,233 @@ __global__ void split_init_kernel(int32_t *flags, int32_t *lens, int32_t *done,
     }
 }
 
-template <typename T, bool kLds>
-__global__ __launch_bounds__(1024) void dfa_split_walk_kernel(const T *__restrict__ gtab,
+// ---- interleaved walking: K independent chains per lane -----------------------------------
+//
+// One dependent table lookup per byte makes a single chain latency-bound (a wave waits for
+// every LDS read). Each lane therefore advances K chains together, byte by byte, so K
+// independent loads are in flight per lane; a chain that finishes its part takes the next
+// segment of the lane's list. A round moves every chain by 32 bytes when all of them sit on
+// a 32-byte boundary with 32 bytes left in their part (the common case inside long inputs);
+// otherwise the chains that are not aligned advance byte by byte to the next boundary.
+
+constexpr int kRound = 64;               // bytes per chain per round
+constexpr int kVec = kRound / 16;        // 16-byte vectors per round
+
+template <typename T, int K>
+__device__ __forceinline__ void step_round(const char *tab, uint32_t col, uint32_t (&e)[K],
+                                           uint32_t (&mx)[K], const uint4 (&v)[K][kVec]) {
+#pragma unroll
+    for (int k = 0; k < kRound; ++k) {
+#pragma unroll
+        for (int c = 0; c < K; ++c) {
+            const uint4 &x = v[c][k >> 4];
+            const int kk = k & 15;
+            const uint32_t w = kk < 4 ? x.x : (kk < 8 ? x.y : (kk < 12 ? x.z : x.w));
+            e[c] = dstep<T>(tab, col, e[c], (w >> ((kk & 3) * 8)) & 0xffu);
+            mx[c] = e[c] > mx[c] ? e[c] : mx[c];
+        }
+    }
+}
+
+// Replay 16 bytes from e; index of the first step entering an accepting state, or -1.
+template <typename T>
+__device__ __forceinline__ int32_t find16(const char *tab, uint32_t col, uint32_t &e,
+                                          const uint4 &v, uint32_t acc) {
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+    for (int k = 0; k < 16; ++k) {
+        e = dstep<T>(tab, col, e, (w[k >> 2] >> ((k & 3) * 8)) & 0xffu);
+        if (e >= acc) return k;
+    }
+    return -1;
+}
+
+// Index (0..kRound-1) of the first accepting step of a round replayed from e.
+template <typename T>
+__device__ __forceinline__ int32_t find_round(const char *tab, uint32_t col, uint32_t e,
+                                              const uint4 (&v)[kVec], uint32_t acc) {
+    for (int j = 0; j < kVec; ++j) {
+        const int32_t k = find16<T>(tab, col, e, v[j], acc);
+        if (k >= 0) return 16 * j + k;
+    }
+    return kRound - 1;  // unreachable when the round's maximum was >= acc
+}
+
+// Checkpoints are stored interval-major ([interval][segment]) so that the lanes of a wave,
+// which hold consecutive segments, write and read whole contiguous lines.
+template <typename T>
+struct CkView {
+    const T *p;     // &ckpt[segment]
+    int64_t step;   // number of segments
+    __device__ __forceinline__ uint32_t operator[](int64_t k) const {
+        return static_cast<uint32_t>(p[k * step]);
+    }
+};
+
+template <typename T, bool kLds, int K>
+__global__ __launch_bounds__(512 / K) void dfa_split_walk_kernel(const T *__restrict__ gtab,
                                                               SplitArgs a) {
     extern __shared__ uint4 lds_raw[];
     const char *tab = stage_table<T, kLds>(gtab, a.table_bytes, lds_raw);
-    const uint32_t col = a.col, acc = a.acc;
+    const uint32_t col = a.col, acc = a.acc, start = a.start;
     const int64_t rmask = (1ll << a.rshift) - 1;
     T *ckpt = static_cast<T *>(a.ckpt);
-    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-    for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < a.segments;
-         g += stride) {
-        const int64_t s0 = g * a.seg;
+    const int64_t lanes = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    int64_t next_g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint4 zero = make_uint4(0, 0, 0, 0);
+
+    int64_t g[K], q[K], pend[K];
+    int32_t in[K], mode[K];  // mode: 0 idle, 1 head (speculative), 2 exact
+    uint32_t e[K], mx[K];
+    bool have[K];
+    uint4 nv[K][kVec];
+
+    // chain c takes the next segment of this lane (or goes idle)
+    auto pick = [&](int c) {
+        have[c] = false;
+        e[c] = start;
+        mx[c] = 0;
+        if (next_g >= a.segments) {
+            mode[c] = 0;
+            return;
+        }
+        g[c] = next_g;
+        next_g += lanes;
+        const int64_t s0 = g[c] * a.seg;
         const int64_t s1 = (s0 + a.seg) < a.total ? (s0 + a.seg) : a.total;
-        int32_t i = split_find_input(a.off, a.count, s0);
-        int64_t q = s0;
-        // ---- head: speculative walk with checkpoints (the segment starts inside input i)
-        if (a.off[i] < s0) {
-            const int64_t hend = a.off[i + 1] < s1 ? a.off[i + 1] : s1;
-            T *ck = ckpt + g * a.nck;
-            uint32_t e = a.start, mx = 0;
-            bool abandoned = false;
-            // 64-byte chunks (segment starts are 64-aligned), next chunk prefetched
-            uint4 v0 = make_uint4(0, 0, 0, 0), v1 = v0, v2 = v0, v3 = v0;
-            if (q + 64 <= hend) {
-                v0 = ld16(a.data + q);
-                v1 = ld16(a.data + q + 16);
-                v2 = ld16(a.data + q + 32);
-                v3 = ld16(a.data + q + 48);
+        const int32_t i = split_find_input(a.off, a.count, s0);
+        in[c] = i;
+        q[c] = s0;
+        pend[c] = a.off[i + 1] < s1 ? a.off[i + 1] : s1;
+        mode[c] = a.off[i] < s0 ? 1 : 2;
+    };
+    // chain c has reached the end of its part (q == pend); `normal`: walked to the end
+    auto part_end = [&](int c, bool normal) {
+        const int64_t s0 = g[c] * a.seg;
+        const int64_t s1 = (s0 + a.seg) < a.total ? (s0 + a.seg) : a.total;
+        if (normal) {
+            if (mode[c] == 1) {
+                const int64_t hl = pend[c] - s0;
+                if (hl & rmask)
+                    ckpt[(hl >> a.rshift) * a.segments + g[c]] =
+                        static_cast<T>(e[c] | (mx[c] >= acc ? 1u : 0u));
+                a.head_exit[g[c]] = e[c];
+            } else if (a.off[in[c] + 1] > s1) {
+                a.tail_exit[g[c]] = e[c];
             }
-            while (q + 64 <= hend) {
-                uint4 n0 = v0, n1 = v1, n2 = v2, n3 = v3;
-                if (q + 128 <= hend) {
-                    n0 = ld16(a.data + q + 64);
-                    n1 = ld16(a.data + q + 80);
-                    n2 = ld16(a.data + q + 96);
-                    n3 = ld16(a.data + q + 112);
-                }
-                e = step16<T>(tab, col, e, v0, mx);
-                e = step16<T>(tab, col, e, v1, mx);
-                e = step16<T>(tab, col, e, v2, mx);
-                e = step16<T>(tab, col, e, v3, mx);
-                q += 64;
-                const int64_t r = q - s0;
-                if ((r & rmask) == 0) {
-                    ck[(r >> a.rshift) - 1] = static_cast<T>(e | (mx >= acc ? 1u : 0u));
-                    mx = 0;
-                }
-                if ((r & 255) == 0 && __atomic_load_n(&a.done[i], __ATOMIC_RELAXED)) {
-                    abandoned = true;
-                    break;
-                }
-                v0 = n0;
-                v1 = n1;
-                v2 = n2;
-                v3 = n3;
+        }
+        if (pend[c] >= s1) {
+            pick(c);
+            return;
+        }
+        int32_t i = in[c] + 1;
+        while (a.off[i + 1] <= pend[c]) ++i;  // empty inputs starting here
+        in[c] = i;
+        q[c] = pend[c];
+        pend[c] = a.off[i + 1] < s1 ? a.off[i + 1] : s1;
+        mode[c] = 2;
+        e[c] = start;
+        mx[c] = 0;
+        have[c] = false;
+    };
+    auto accept_at = [&](int c, int64_t pos) {
+        const int32_t i = in[c];
+        a.flags[i] = 1;
+        a.lens[i] = static_cast<int32_t>(pos - a.off[i] + 1);
+        __atomic_store_n(&a.done[i], 1, __ATOMIC_RELAXED);
+    };
+
+#pragma unroll
+    for (int c = 0; c < K; ++c) pick(c);
+
+    for (;;) {
+        bool any = false, fast = true;
+#pragma unroll
+        for (int c = 0; c < K; ++c) {
+            if (mode[c]) {
+                any = true;
+                if ((q[c] & (kRound - 1)) || q[c] + kRound > pend[c]) fast = false;
             }
-            if (!abandoned) {
-                for (; q < hend; ++q) {
-                    e = dstep<T>(tab, col, e, a.data[q]);
-                    mx = e > mx ? e : mx;
-                    const int64_t r = q + 1 - s0;
+        }
+        if (!any) break;
+        if (fast) {
+            uint4 v[K][kVec];
+            uint32_t e0[K];
+#pragma unroll
+            for (int c = 0; c < K; ++c) {
+                e0[c] = e[c];
+                if (mode[c]) {
+#pragma unroll
+                    for (int j = 0; j < kVec; ++j)
+                        v[c][j] = have[c] ? nv[c][j] : ld16(a.data + q[c] + 16 * j);
+                    have[c] = q[c] + 2 * kRound <= pend[c];
+                    if (have[c]) {
+#pragma unroll
+                        for (int j = 0; j < kVec; ++j) nv[c][j] = ld16(a.data + q[c] + kRound + 16 * j);
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < kVec; ++j) v[c][j] = zero;
+                }
+            }
+            step_round<T, K>(tab, col, e, mx, v);
+#pragma unroll
+            for (int c = 0; c < K; ++c) {
+                if (!mode[c]) {
+                    e[c] = start;
+                    continue;
+                }
+                q[c] += kRound;
+                if (mode[c] == 2) {
+                    if (mx[c] >= acc) {
+                        accept_at(c, q[c] - kRound + find_round<T>(tab, col, e0[c], v[c], acc));
+                        q[c] = pend[c];
+                        part_end(c, false);
+                        continue;
+                    }
+                    mx[c] = 0;
+                } else {
+                    const int64_t r = q[c] - g[c] * a.seg;
                     if ((r & rmask) == 0) {
-                        ck[(r >> a.rshift) - 1] = static_cast<T>(e | (mx >= acc ? 1u : 0u));
-                        mx = 0;
+                        ckpt[((r >> a.rshift) - 1) * a.segments + g[c]] =
+                            static_cast<T>(e[c] | (mx[c] >= acc ? 1u : 0u));
+                        mx[c] = 0;
+                    }
+                    if ((r & 255) == 0 && __atomic_load_n(&a.done[in[c]], __ATOMIC_RELAXED)) {
+                        q[c] = pend[c];
+                        part_end(c, false);
+                        continue;
                     }
                 }
-                const int64_t hl = hend - s0;
-                if (hl & rmask) ck[hl >> a.rshift] = static_cast<T>(e | (mx >= acc ? 1u : 0u));
-                a.head_exit[g] = e;
+                if (q[c] == pend[c]) part_end(c, true);
             }
-            q = hend;
-            ++i;
-        }
-        // ---- inputs that start in this segment: exact
-        while (q < s1) {
-            while (a.off[i + 1] <= q) ++i;  // empty inputs that start at q
-            const int64_t ie = a.off[i + 1];
-            const int64_t wend = ie < s1 ? ie : s1;
-            uint32_t e = a.start;
-            const int64_t hit = walk_exact<T>(tab, col, acc, e, a.data, q, wend);
-            if (hit >= 0) {
-                a.flags[i] = 1;
-                a.lens[i] = static_cast<int32_t>(hit - a.off[i] + 1);
-                __atomic_store_n(&a.done[i], 1, __ATOMIC_RELAXED);
-            } else if (ie > s1) {
-                a.tail_exit[g] = e;
+        } else {
+#pragma unroll
+            for (int c = 0; c < K; ++c) {
+                if (!mode[c] || !((q[c] & (kRound - 1)) || q[c] + kRound > pend[c])) continue;
+                // byte by byte to the next 32-byte boundary (or the end of the part)
+                const int64_t up = (q[c] | (kRound - 1)) + 1;
+                const int64_t stop = up < pend[c] ? up : pend[c];
+                bool hit = false;
+                for (; q[c] < stop; ++q[c]) {
+                    e[c] = dstep<T>(tab, col, e[c], a.data[q[c]]);
+                    if (mode[c] == 2) {
+                        if (e[c] >= acc) {
+                            accept_at(c, q[c]);
+                            hit = true;
+                            break;
+                        }
+                    } else {
+                        mx[c] = e[c] > mx[c] ? e[c] : mx[c];
+                    }
+                }
+                have[c] = false;
+                if (hit) {
+                    q[c] = pend[c];
+                    part_end(c, false);
+                } else if (q[c] == pend[c]) {
+                    part_end(c, true);
+                }
             }
-            q = wend;
-            ++i;
         }
     }
 }
@@ -295,7 +434,7 @@ __global__ __launch_bounds__(1024) void dfa_split_walk_kernel(const T *__restric
 template <typename T>
 __device__ __forceinline__ void split_merge_walk(const char *tab, const SplitArgs &a,
                                                  int64_t s0, int64_t hend, uint32_t entry,
-                                                 const T *ck, uint32_t spec_exit,
+                                                 CkView<T> ck, uint32_t spec_exit,
                                                  int32_t &acc_out, uint32_t &exit_out) {
     const uint32_t col = a.col, acc = a.acc;
     const int64_t rmask = (1ll << a.rshift) - 1;
@@ -316,7 +455,7 @@ __device__ __forceinline__ void split_merge_walk(const char *tab, const SplitArg
         }
         q += 16;
         const int64_t r = q - s0;
-        if ((r & rmask) == 0 && (static_cast<uint32_t>(ck[(r >> a.rshift) - 1]) & ~1u) == e) {
+        if ((r & rmask) == 0 && (ck[(r >> a.rshift) - 1] & ~1u) == e) {
             merged = (r >> a.rshift) - 1;
             break;
         }
@@ -332,15 +471,15 @@ __device__ __forceinline__ void split_merge_walk(const char *tab, const SplitArg
         }
         exit_out = e;
         const int64_t hl = hend - s0;
-        if (!(hl & rmask) || (static_cast<uint32_t>(ck[hl >> a.rshift]) & ~1u) != e) return;
+        if (!(hl & rmask) || (ck[hl >> a.rshift] & ~1u) != e) return;
         merged = hl >> a.rshift;  // met at the final partial interval: nothing after it
     }
     // paths coincide after checkpoint `merged`: take the first flagged interval after it
     const int64_t nk = ((hend - s0) + rmask) >> a.rshift;
     int64_t k = merged + 1;
-    while (k < nk && !(static_cast<uint32_t>(ck[k]) & 1u)) ++k;
+    while (k < nk && !(ck[k] & 1u)) ++k;
     if (k < nk) {
-        uint32_t c = static_cast<uint32_t>(ck[k - 1]) & ~1u;
+        uint32_t c = ck[k - 1] & ~1u;
         const int64_t lo = s0 + (k << a.rshift);
         const int64_t hi = (lo + rmask + 1) < hend ? (lo + rmask + 1) : hend;
         const int64_t hit = walk_exact<T>(tab, col, acc, c, data, lo, hi);
@@ -349,40 +488,159 @@ __device__ __forceinline__ void split_merge_walk(const char *tab, const SplitArg
     exit_out = spec_exit;
 }
 
-// repair = 0: pass 2 for every head. repair = 1: again, only for heads whose assumed entry
-// differs from the previous segment's pass-2 exit (the previous head did not merge), now
-// starting from that exit. A head being repaired can race with the repair of its
-// predecessor; the event pass re-checks every head afterwards, so this only affects speed.
-template <typename T, bool kLds>
-__global__ __launch_bounds__(1024) void dfa_split_fix_kernel(const T *__restrict__ gtab,
+// After a merge at checkpoint `merged` the pass-1 path is the true one: the first accept
+// after it is in the first flagged interval, located by walking that interval.
+template <typename T>
+__device__ __forceinline__ int32_t split_accept_after(const char *tab, const SplitArgs &a,
+                                                      int64_t s0, int64_t hend, CkView<T> ck,
+                                                      int64_t merged) {
+    const int64_t rmask = (1ll << a.rshift) - 1;
+    const int64_t nk = ((hend - s0) + rmask) >> a.rshift;
+    int64_t k = merged + 1;
+    while (k < nk && !(ck[k] & 1u)) ++k;
+    if (k >= nk) return -1;
+    uint32_t c = ck[k - 1] & ~1u;
+    const int64_t lo = s0 + (k << a.rshift);
+    const int64_t hi = (lo + rmask + 1) < hend ? (lo + rmask + 1) : hend;
+    const int64_t hit = walk_exact<T>(tab, a.col, a.acc, c, a.data, lo, hi);
+    return static_cast<int32_t>(hit - s0);  // the flag guarantees a hit
+}
+
+// Pass 2 with K interleaved merge walks per lane (see the walk kernel). repair = 0: every
+// head, entry = the previous segment's pass-1 exit. repair = 1: only heads whose assumed
+// entry differs from the previous segment's pass-2 exit, now walked from that exit. A head
+// being repaired can race with the repair of its predecessor; the event pass re-checks every
+// head afterwards, so the race only affects speed.
+template <typename T, bool kLds, int K>
+__global__ __launch_bounds__(512 / K) void dfa_split_fix_kernel(const T *__restrict__ gtab,
                                                              SplitArgs a, int32_t repair) {
     extern __shared__ uint4 lds_raw[];
     const char *tab = stage_table<T, kLds>(gtab, a.table_bytes, lds_raw);
+    const uint32_t col = a.col, acc = a.acc;
+    const int64_t rmask = (1ll << a.rshift) - 1;
     const T *ckpt = static_cast<const T *>(a.ckpt);
-    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-    for (int64_t g = 1 + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-         g < a.segments; g += stride) {
-        const int64_t s0 = g * a.seg;
-        const int32_t i = split_find_input(a.off, a.count, s0);
-        if (a.off[i] >= s0 || a.done[i]) continue;
-        const int64_t s1 = (s0 + a.seg) < a.total ? (s0 + a.seg) : a.total;
-        const int64_t hend = a.off[i + 1] < s1 ? a.off[i + 1] : s1;
-        const bool started_before = a.off[i] >= (g - 1) * a.seg;
-        uint32_t entry;
-        if (repair) {
-            entry = started_before ? a.tail_exit[g - 1]
-                                   : __atomic_load_n(&a.fixed_exit[g - 1], __ATOMIC_RELAXED);
-            if (entry == a.fixed_entry[g]) continue;
-        } else {
-            entry = started_before ? a.tail_exit[g - 1] : a.head_exit[g - 1];
+    const int64_t lanes = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    int64_t next_g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint4 zero = make_uint4(0, 0, 0, 0);
+
+    int64_t g[K], q[K], hend[K];
+    uint32_t e[K], entry[K];
+    bool act[K], have[K];
+    uint4 nv[K][kVec];
+
+    auto pick = [&](int c) {
+        act[c] = false;
+        have[c] = false;
+        e[c] = a.start;
+        while (next_g < a.segments) {
+            const int64_t gg = next_g;
+            next_g += lanes;
+            if (gg == 0) continue;
+            const int64_t s0 = gg * a.seg;
+            const int32_t i = split_find_input(a.off, a.count, s0);
+            if (a.off[i] >= s0 || a.done[i]) continue;
+            const bool started_before = a.off[i] >= (gg - 1) * a.seg;
+            uint32_t en;
+            if (repair) {
+                en = started_before ? a.tail_exit[gg - 1]
+                                    : __atomic_load_n(&a.fixed_exit[gg - 1], __ATOMIC_RELAXED);
+                if (en == a.fixed_entry[gg]) continue;
+            } else {
+                en = started_before ? a.tail_exit[gg - 1] : a.head_exit[gg - 1];
+            }
+            const int64_t s1 = (s0 + a.seg) < a.total ? (s0 + a.seg) : a.total;
+            g[c] = gg;
+            q[c] = s0;
+            hend[c] = a.off[i + 1] < s1 ? a.off[i + 1] : s1;
+            e[c] = en;
+            entry[c] = en;
+            act[c] = true;
+            return;
         }
-        int32_t acc_at;
-        uint32_t ex;
-        split_merge_walk<T>(tab, a, s0, hend, entry, ckpt + g * a.nck, a.head_exit[g], acc_at,
-                            ex);
-        a.fixed_entry[g] = entry;
-        a.fixed_acc[g] = acc_at;
-        __atomic_store_n(&a.fixed_exit[g], ex, __ATOMIC_RELAXED);
+    };
+    auto finish = [&](int c, int32_t acc_at, uint32_t ex) {
+        a.fixed_entry[g[c]] = entry[c];
+        a.fixed_acc[g[c]] = acc_at;
+        __atomic_store_n(&a.fixed_exit[g[c]], ex, __ATOMIC_RELAXED);
+        pick(c);
+    };
+
+#pragma unroll
+    for (int c = 0; c < K; ++c) pick(c);
+
+    for (;;) {
+        bool any = false, fast = true;
+#pragma unroll
+        for (int c = 0; c < K; ++c) {
+            if (act[c]) {
+                any = true;
+                if ((q[c] & (kRound - 1)) || q[c] + kRound > hend[c]) fast = false;
+            }
+        }
+        if (!any) break;
+        if (fast) {
+            uint4 v[K][kVec];
+            uint32_t e0[K], mx[K];
+#pragma unroll
+            for (int c = 0; c < K; ++c) {
+                e0[c] = e[c];
+                mx[c] = 0;
+                if (act[c]) {
+#pragma unroll
+                    for (int j = 0; j < kVec; ++j)
+                        v[c][j] = have[c] ? nv[c][j] : ld16(a.data + q[c] + 16 * j);
+                    have[c] = q[c] + 2 * kRound <= hend[c];
+                    if (have[c]) {
+#pragma unroll
+                        for (int j = 0; j < kVec; ++j) nv[c][j] = ld16(a.data + q[c] + kRound + 16 * j);
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < kVec; ++j) v[c][j] = zero;
+                }
+            }
+            step_round<T, K>(tab, col, e, mx, v);
+#pragma unroll
+            for (int c = 0; c < K; ++c) {
+                if (!act[c]) {
+                    e[c] = a.start;
+                    continue;
+                }
+                const int64_t s0 = g[c] * a.seg;
+                q[c] += kRound;
+                if (mx[c] >= acc) {
+                    const int32_t k = find_round<T>(tab, col, e0[c], v[c], acc);
+                    finish(c, static_cast<int32_t>(q[c] - kRound + k - s0), e[c]);
+                    continue;
+                }
+                const int64_t r = q[c] - s0;
+                const CkView<T> ck{ckpt + g[c], a.segments};
+                if ((r & rmask) == 0 && (ck[(r >> a.rshift) - 1] & ~1u) == e[c]) {
+                    const int32_t after =
+                        split_accept_after<T>(tab, a, s0, hend[c], ck, (r >> a.rshift) - 1);
+                    finish(c, after, a.head_exit[g[c]]);
+                    continue;
+                }
+                if (q[c] == hend[c]) finish(c, -1, e[c]);
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < K; ++c) {
+                if (!act[c] || !((q[c] & (kRound - 1)) || q[c] + kRound > hend[c])) continue;
+                // the last partial round of a head: byte by byte to its end
+                const int64_t s0 = g[c] * a.seg;
+                int32_t hit = -1;
+                for (; q[c] < hend[c]; ++q[c]) {
+                    e[c] = dstep<T>(tab, col, e[c], a.data[q[c]]);
+                    if (e[c] >= acc) {
+                        hit = static_cast<int32_t>(q[c] - s0);
+                        break;
+                    }
+                }
+                have[c] = false;
+                finish(c, hit, e[c]);
+            }
+        }
     }
 }
 
@@ -438,7 +696,7 @@ __global__ __launch_bounds__(256) void dfa_split_finish_kernel(const T *__restri
             ex = a.fixed_exit[g];
         } else {
             const int64_t hend = (s0 + a.seg) < ie ? (s0 + a.seg) : ie;
-            split_merge_walk<T>(tab, a, s0, hend, cur, ckpt + g * a.nck, a.head_exit[g],
+            split_merge_walk<T>(tab, a, s0, hend, cur, CkView<T>{ckpt + g, a.segments}, a.head_exit[g],
                                 acc_at, ex);
         }
         if (acc_at >= 0) {
@@ -471,10 +729,24 @@ size_t dfa_lds_bytes(int32_t n) { return static_cast<size_t>(n) * kAlphabet * si
 
 static bool lds_fits(const DfaDev &d) { return !d.wide && dfa_lds_bytes(d.n) <= 160 * 1024; }
 
+// Tuning overrides for sweeps: GFSM_SPLIT_SEGMENT (bytes, power of two >= 256) and
+// GFSM_SPLIT_CHAINS (1, 2 or 4 interleaved chains per lane).
+static int64_t env_int(const char *name, int64_t fallback) {
+    const char *v = std::getenv(name);
+    if (!v || !*v) return fallback;
+    char *end = nullptr;
+    const long long x = std::strtoll(v, &end, 10);
+    return (end && *end == '\0' && x > 0) ? x : fallback;
+}
+
 DfaSplitPlan dfa_split_plan(const DfaDev &d, int64_t total_bytes, int32_t count) {
     DfaSplitPlan p;
+    p.chains = static_cast<int32_t>(env_int("GFSM_SPLIT_CHAINS", 1));
+    if (p.chains != 1 && p.chains != 2 && p.chains != 4) p.chains = 1;
     if (lds_fits(d)) {
-        p.seg_bytes = 1024;
+        int64_t c = env_int("GFSM_SPLIT_SEGMENT", 2048);
+        if (c < 256 || (c & (c - 1))) c = 2048;
+        p.seg_bytes = c;
         p.ckpt_shift = 6;
     } else {
         int64_t c = 1024;
@@ -503,38 +775,42 @@ static void allow_big_lds(K *kernel, size_t bytes) {
                                  static_cast<int>(bytes)));
 }
 
-template <typename T, bool kLds>
+template <typename T, bool kLds, int kChains>
 static void split_launch_passes(const DfaDev &d, SplitArgs &a, hipStream_t stream) {
     const T *tab = static_cast<const T *>(d.table);
     const size_t lds = kLds ? static_cast<size_t>(a.table_bytes) : 0;
     const int32_t cus = device_cu_count();
     static size_t attr = 0;  // largest table the attributes were raised for
     if (kLds && lds > attr) {
-        allow_big_lds(&dfa_split_walk_kernel<T, kLds>, lds);
-        allow_big_lds(&dfa_split_fix_kernel<T, kLds>, lds);
+        allow_big_lds(&dfa_split_walk_kernel<T, kLds, kChains>, lds);
+        allow_big_lds(&dfa_split_fix_kernel<T, kLds, kChains>, lds);
         allow_big_lds(&dfa_split_finish_kernel<T, kLds>, lds);
         attr = lds;
     }
-    // spread the segments over every CU
-    int64_t per_cu = (a.segments + cus - 1) / cus;
+    // spread the segments over every CU, kChains per lane
+    int64_t per_cu = (a.segments + static_cast<int64_t>(cus) * kChains - 1) /
+                     (static_cast<int64_t>(cus) * kChains);
     per_cu = ((per_cu + 63) / 64) * 64;
     int32_t threads, blocks;
-    if (kLds) {  // the table fills the LDS: one block per CU, up to 16 waves
-        threads = static_cast<int32_t>(per_cu < 64 ? 64 : (per_cu > 1024 ? 1024 : per_cu));
-        const int64_t need = (a.segments + threads - 1) / threads;
+    if (kLds) {  // the table fills the LDS: one block per CU, up to 8 waves
+        const int64_t cap = 512 / kChains;
+        threads = static_cast<int32_t>(per_cu < 64 ? 64 : (per_cu > cap ? cap : per_cu));
+        const int64_t need = (a.segments + static_cast<int64_t>(threads) * kChains - 1) /
+                             (static_cast<int64_t>(threads) * kChains);
         blocks = static_cast<int32_t>(need < cus ? (need > 0 ? need : 1) : cus);
     } else {
-        threads = per_cu >= 256 ? 256 : 64;
-        const int64_t need = (a.segments + threads - 1) / threads;
+        threads = per_cu >= 256 / kChains ? 256 / kChains : 64;
+        const int64_t need = (a.segments + static_cast<int64_t>(threads) * kChains - 1) /
+                             (static_cast<int64_t>(threads) * kChains);
         const int64_t cap = static_cast<int64_t>(cus) * 32;
         blocks = static_cast<int32_t>(need < cap ? (need > 0 ? need : 1) : cap);
     }
-    dfa_split_walk_kernel<T, kLds><<<blocks, threads, lds, stream>>>(tab, a);
+    dfa_split_walk_kernel<T, kLds, kChains><<<blocks, threads, lds, stream>>>(tab, a);
     GFSM_HIP(hipGetLastError());
     if (a.segments > 1) {
-        dfa_split_fix_kernel<T, kLds><<<blocks, threads, lds, stream>>>(tab, a, 0);
+        dfa_split_fix_kernel<T, kLds, kChains><<<blocks, threads, lds, stream>>>(tab, a, 0);
         GFSM_HIP(hipGetLastError());
-        dfa_split_fix_kernel<T, kLds><<<blocks, threads, lds, stream>>>(tab, a, 1);
+        dfa_split_fix_kernel<T, kLds, kChains><<<blocks, threads, lds, stream>>>(tab, a, 1);
         GFSM_HIP(hipGetLastError());
         const int64_t eblocks64 = (a.segments + 255) / 256;
         const int32_t eblocks = static_cast<int32_t>(eblocks64 < 65536 ? eblocks64 : 65536);
@@ -587,12 +863,19 @@ void launch_dfa_split(const DfaDev &d, const DfaSplitPlan &plan, const uint8_t *
                                                                 a.first_event, count);
     GFSM_HIP(hipGetLastError());
     if (plan.segments <= 0) return;
-    if (d.wide)
-        split_launch_passes<uint32_t, false>(d, a, stream);
-    else if (lds_fits(d))
-        split_launch_passes<uint16_t, true>(d, a, stream);
-    else
-        split_launch_passes<uint16_t, false>(d, a, stream);
+    const bool lds = lds_fits(d);
+    if (d.wide) {
+        split_launch_passes<uint32_t, false, 1>(d, a, stream);
+    } else if (lds) {
+        if (plan.chains == 4) split_launch_passes<uint16_t, true, 4>(d, a, stream);
+        else if (plan.chains == 1) split_launch_passes<uint16_t, true, 1>(d, a, stream);
+        else if (plan.chains == 2) split_launch_passes<uint16_t, true, 2>(d, a, stream);
+        else split_launch_passes<uint16_t, true, 1>(d, a, stream);
+    } else {
+        if (plan.chains == 4) split_launch_passes<uint16_t, false, 4>(d, a, stream);
+        else if (plan.chains == 1) split_launch_passes<uint16_t, false, 1>(d, a, stream);
+        else split_launch_passes<uint16_t, false, 2>(d, a, stream);
+    }
 }
 
 void launch_fill_result(int32_t *flags, int32_t *lens, int32_t count, int32_t flag,

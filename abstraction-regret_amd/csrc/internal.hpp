@@ -92,6 +92,7 @@ struct DfaSplitPlan {
     int64_t seg_bytes = 0;     // segment length C (multiple of the checkpoint interval)
     int32_t ckpt_shift = 0;    // log2 of the checkpoint interval R
     int64_t segments = 0;      // ceil(total_bytes / C)
+    int32_t chains = 1;        // interleaved chains per lane
     size_t scratch_bytes = 0;  // device scratch the launch needs
 };
 DfaSplitPlan dfa_split_plan(const DfaDev &d, int64_t total_bytes, int32_t count);

@@ -50,13 +50,13 @@ def _threads() -> int:
         return os.cpu_count() or 1
 
 
-def _traffic(kernel: str, workload: str):
-    """HBM bytes per launch from the committed rocprofv3 counter summary, if present."""
+def _traffic(technique: str, workload: str):
+    """HBM bytes per step from the committed rocprofv3 counter summary, if present."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             table = json.load(f)
-        return table.get(f"{kernel}:{workload}")
+        return table.get(f"{technique}:{workload}")
     except (OSError, ValueError):
         return None
 
@@ -99,10 +99,11 @@ def _time_steps(torch, fn, steps: int, warmup: int):
     return wall, [s.elapsed_time(e) for s, e in zip(starts, ends)]
 
 
+# The kernels one step of each technique launches (the roofline covers the whole step).
 KERNELS = {
-    "gather": "dfa_gather_kernel",
-    "lds": "dfa_lds_kernel",
-    "split": "dfa_split_walk_kernel",
+    "gather": "dfa_walk_kernel<gather>",
+    "lds": "dfa_walk_kernel<lds>",
+    "split": "dfa_split (init, walk, fix x2, event, finish)",
 }
 
 
@@ -159,8 +160,14 @@ def bench_dfa(args, torch) -> dict:
     dfa = W.dense_dfa(n, 0.0, seed=0)
     head = _measure_dfa(torch, batch, dfa, args.technique, data, offsets, args.steps,
                         args.warmup, f"dfa{n} non-latching")
+    if args.profile:
+        # a plain device copy of the same input bytes: the reference point that scales the
+        # FETCH_SIZE / WRITE_SIZE counters of a profiled run into HBM bytes
+        copy = batch.data.clone()
+        torch.cuda.synchronize()
+        del copy
     # the other variants of the same workload (not the headline)
-    for other_n in (256, 4096):
+    for other_n in (() if args.profile else (256, 4096)):
         for latching in (False, True):
             if other_n == n and not latching:
                 continue
@@ -184,15 +191,16 @@ def bench_dfa(args, torch) -> dict:
     cpu_s = (time.perf_counter() - t0) / reps
 
     # plain per-call drop-in (gpufsm.run_batch builds an executor, copies, launches, copies back)
-    inputs = [data[offsets[i]:offsets[i + 1]].tobytes() for i in range(count)]
-    run_batch(dfa, inputs[:4], backend="hip", technique=args.technique)
-    t0 = time.perf_counter()
-    res = run_batch(dfa, inputs, backend="hip", technique=args.technique)
-    plain_s = time.perf_counter() - t0
-    assert not any(r.accepted for r in res)
-    extras["plain_per_call"] = {"seconds": plain_s,
-                                "examined_GBps": head["examined_bytes"] / plain_s / 1e9,
-                                "kernel_ms": res[0].kernel_ms}
+    if not args.profile:
+        inputs = [data[offsets[i]:offsets[i + 1]].tobytes() for i in range(count)]
+        run_batch(dfa, inputs[:4], backend="hip", technique=args.technique)
+        t0 = time.perf_counter()
+        res = run_batch(dfa, inputs, backend="hip", technique=args.technique)
+        plain_s = time.perf_counter() - t0
+        assert not any(r.accepted for r in res)
+        extras["plain_per_call"] = {"seconds": plain_s,
+                                    "examined_GBps": head["examined_bytes"] / plain_s / 1e9,
+                                    "kernel_ms": res[0].kernel_ms}
 
     examined = head["examined_bytes"]
     compulsory = head["table_bytes"] + 8 * (count + 1) + 8 * count
@@ -223,7 +231,7 @@ def bench_dfa(args, torch) -> dict:
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": _traffic(kernel_name, workload_name),
+            "traffic": _traffic(tech, workload_name),
             "kernel": kernel_name,
             "kernel_ms": kernel_ms,
             "algorithmic_bytes_per_launch": examined + compulsory,
@@ -250,6 +258,9 @@ def main() -> int:
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--length", type=int, default=65536)
     ap.add_argument("--technique", default="auto")
+    ap.add_argument("--profile", action="store_true",
+                    help="headline variant only, plus one device copy of the input bytes "
+                         "(the calibration point for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); inputs are "
