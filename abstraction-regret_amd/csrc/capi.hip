@@ -23,6 +23,7 @@ struct gfsm_nfa {
     NfaWorklistDev wl{};
     bool chain_ok = false;
     NfaTileDev tile{};
+    NfaChainDev chain{};
     int32_t max_chain = 0;
     std::vector<void *> blocks;
     int64_t device_bytes = 0;
@@ -111,11 +112,13 @@ int32_t resolve_nfa(const gfsm_nfa *h, int32_t technique) {
         case GFSM_NFA_WORKLIST:
             return technique;
         case GFSM_NFA_BITPACKED:
+        case GFSM_NFA_CHAIN:
             if (!h->chain_ok)
                 throw Error(GFSM_EUNSUPPORTED,
-                            "bitpacked needs a chain-structured NFA (no epsilon edges, at most "
-                            "one predecessor and successor per non-start state, homogeneous "
-                            "labels, chains of at most 4096 states)");
+                            std::string(technique == GFSM_NFA_CHAIN ? "chain" : "bitpacked") +
+                                " needs a chain-structured NFA (no epsilon edges, at most "
+                                "one predecessor and successor per non-start state, homogeneous "
+                                "labels, chains of at most 4096 states)");
             return technique;
         default:
             throw Error(GFSM_EINVAL, "unknown NFA technique " + std::to_string(technique));
@@ -149,6 +152,12 @@ void nfa_enqueue(gfsm_nfa *h, int32_t tech, const uint8_t *d_data, const int64_t
     if (count == 0) return;
     if (h->start_accepts) {
         launch_fill_result(d_flags, d_lens, count, 1, 0, s);
+        return;
+    }
+    if (tech == GFSM_NFA_CHAIN) {
+        StreamBuf scratch(nfa_chain_scratch_bytes(count), s);
+        launch_nfa_chain(h->chain, h->tile, d_data, d_off, count, d_flags, d_lens, scratch.p,
+                         nfa_chain_chunk_bytes(), s);
         return;
     }
     if (tech == GFSM_NFA_BITPACKED) {
@@ -296,6 +305,15 @@ int gfsm_nfa_create(int32_t num_states, int32_t start_state, const uint8_t *acce
                 h->tile.tiles = t.tiles;
                 h->tile.start_always = t.start_always ? 1 : 0;
                 h->max_chain = t.max_chain;
+                h->chain.next = upload(B, by, t.next.data(), t.next.size());
+                h->chain.acc = h->tile.acc;
+                h->chain.first_off = upload(B, by, t.first_off.data(), t.first_off.size());
+                h->chain.first_slot = upload(B, by, t.first_slot.data(), t.first_slot.size());
+                h->chain.first_accepts =
+                    upload(B, by, t.first_accepts.data(), t.first_accepts.size());
+                for (int i = 0; i < 4; ++i) h->chain.selfmask[i] = t.selfmask[i];
+                h->chain.max_chain = t.max_chain;
+                h->chain.start_always = h->tile.start_always;
             }
         } catch (...) {
             free_blocks(h->blocks);
@@ -361,7 +379,8 @@ int gfsm_nfa_analyze(int32_t num_states, int32_t start_state, const uint8_t *acc
         add(w.accw.size(), 8); add(w.start_list.size(), 4);
         if (t.ok) {
             add(t.mask.size(), 8); add(t.first.size(), 8); add(t.hassucc.size(), 8);
-            add(t.acc.size(), 8);
+            add(t.acc.size(), 8); add(t.next.size(), 8); add(t.first_off.size(), 4);
+            add(t.first_slot.size(), 4); add(t.first_accepts.size(), 4);
         }
         out->device_bytes = bytes;
     });

@@ -731,7 +731,7 @@ static bool lds_fits(const DfaDev &d) { return !d.wide && dfa_lds_bytes(d.n) <= 
 
 // Tuning overrides for sweeps: GFSM_SPLIT_SEGMENT (bytes, power of two >= 256) and
 // GFSM_SPLIT_CHAINS (1, 2 or 4 interleaved chains per lane).
-static int64_t env_int(const char *name, int64_t fallback) {
+int64_t env_int(const char *name, int64_t fallback) {
     const char *v = std::getenv(name);
     if (!v || !*v) return fallback;
     char *end = nullptr;

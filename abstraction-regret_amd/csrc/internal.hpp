@@ -66,6 +66,20 @@ struct NfaTileDev {
     int32_t start_always;      // self-loop on every byte
 };
 
+// Sparse chain layout (same renumbering as the tiles): the active set is a list of slots;
+// slot s continues to s + 1 on byte c when hassucc(s) and c is in the class of s + 1
+// (bit of mask[tile][c]); the start state adds first_slot[first_off[c] .. first_off[c+1]).
+struct NfaChainDev {
+    const uint64_t *next;                  // [tiles][256][64]: mask & (hassucc << 1)
+    const uint64_t *acc;                   // tile accept bits (NfaTileDev)
+    const int32_t *first_off;              // [257]
+    const int32_t *first_slot;
+    const uint32_t *first_accepts;         // [256]
+    uint64_t selfmask[4];
+    int32_t max_chain;
+    int32_t start_always;
+};
+
 // ----- DFA ------------------------------------------------------------------------------
 
 // Table by byte, entries = successor's byte offset in a column (see layout.hpp): the step
@@ -100,6 +114,8 @@ void launch_dfa_split(const DfaDev &d, const DfaSplitPlan &plan, const uint8_t *
                       const int64_t *off, int32_t count, int64_t total_bytes, int32_t *flags,
                       int32_t *lens, void *scratch, hipStream_t stream);
 int32_t device_cu_count();
+// Positive integer from the environment (tuning overrides), else fallback.
+int64_t env_int(const char *name, int64_t fallback);
 
 void launch_nfa_worklist(const NfaWorklistDev &d, const uint8_t *data, const int64_t *off,
                          int32_t count, int32_t *flags, int32_t *lens, int32_t *scratch,
@@ -111,5 +127,12 @@ void nfa_worklist_geometry(const NfaWorklistDev &d, int32_t count, int32_t *grid
 void launch_nfa_bitpacked(const NfaTileDev &d, const uint8_t *data, const int64_t *off,
                           int32_t count, int32_t *flags, int32_t *lens, int32_t *best,
                           hipStream_t stream);
+
+// Sparse chain technique: persistent waves over (input, chunk) work items.
+size_t nfa_chain_scratch_bytes(int32_t count);
+int32_t nfa_chain_chunk_bytes();  // GFSM_CHAIN_CHUNK (read per call), default 4096
+void launch_nfa_chain(const NfaChainDev &d, const NfaTileDev &tiles, const uint8_t *data,
+                      const int64_t *off, int32_t count, int32_t *flags, int32_t *lens,
+                      void *scratch, int32_t chunk_bytes, hipStream_t stream);
 
 }  // namespace gfsm

@@ -178,7 +178,7 @@ TileHost build_tiles(const NfaInput &a) {
         h.why = "a cycle";
         return h;
     }
-    // first-fit decreasing into 4096-state tiles
+    // best-fit decreasing into 4096-state tiles
     std::vector<size_t> order(chains.size());
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(),
@@ -188,8 +188,10 @@ TileHost build_tiles(const NfaInput &a) {
     for (size_t ci : order) {
         const int32_t len = static_cast<int32_t>(chains[ci].size());
         h.max_chain = std::max(h.max_chain, len);
-        size_t b = 0;
-        while (b < fill.size() && fill[b] + len > kTileStates) ++b;
+        // best fit: the fullest tile that still has room
+        size_t b = fill.size();
+        for (size_t t = 0; t < fill.size(); ++t)
+            if (fill[t] + len <= kTileStates && (b == fill.size() || fill[t] > fill[b])) b = t;
         if (b == fill.size()) {
             fill.push_back(0);
             bins.emplace_back();
@@ -218,6 +220,37 @@ TileHost build_tiles(const NfaInput &a) {
                 if (succ[u] != -1) h.hassucc[w] |= bit;
                 if (a.accept[u]) h.acc[w] |= bit;
                 ++slot;
+            }
+        }
+    }
+    // per-byte lists of the start state's successors (in slot order)
+    std::vector<std::vector<int32_t>> by_byte(256);
+    h.first_accepts.assign(256, 0u);
+    for (int32_t u = 0; u < n; ++u) {
+        if (u == s0 || !from_start[u].any()) continue;
+        for (int c = 0; c < 256; ++c) {
+            if (!from_start[u].has(c)) continue;
+            by_byte[c].push_back(h.slot_of_state[u]);
+            if (a.accept[u]) h.first_accepts[c] = 1u;
+        }
+    }
+    h.first_off.assign(257, 0);
+    for (int c = 0; c < 256; ++c) {
+        std::sort(by_byte[c].begin(), by_byte[c].end());
+        h.first_off[c + 1] = h.first_off[c] + static_cast<int32_t>(by_byte[c].size());
+    }
+    h.first_slot.reserve(static_cast<size_t>(h.first_off[256]));
+    for (int c = 0; c < 256; ++c)
+        h.first_slot.insert(h.first_slot.end(), by_byte[c].begin(), by_byte[c].end());
+    h.next.assign((T + 1) * 256 * kTileWords, 0ull);
+    for (size_t b = 0; b < T; ++b) {
+        const uint64_t *hs = &h.hassucc[b * kTileWords];
+        for (int c = 0; c < 256; ++c) {
+            const uint64_t *m = &h.mask[(b * 256 + c) * kTileWords];
+            uint64_t *o = &h.next[(b * 256 + c) * kTileWords];
+            for (int w = 0; w < kTileWords; ++w) {
+                const uint64_t shifted = (hs[w] << 1) | (w > 0 ? hs[w - 1] >> 63 : 0ull);
+                o[w] = m[w] & shifted;
             }
         }
     }

@@ -34,6 +34,14 @@ struct TileHost {
     uint64_t selfmask[4] = {0, 0, 0, 0};
     std::vector<uint64_t> mask, first, hassucc, acc;  // see NfaTileDev
     std::vector<int32_t> slot_of_state;               // renumbering (-1 for the start)
+    // successors of the start state by byte: slots of first positions whose class holds c
+    std::vector<int32_t> first_off;                   // [257]
+    std::vector<int32_t> first_slot;                  // [first_off[256]]
+    std::vector<uint32_t> first_accepts;              // [256] some first position accepts
+    // chain continuation by byte, [tiles + 1][256][64]: bit t of next[tile][c] is set when
+    // slot t - 1 has a chain successor and the class of slot t holds c (mask & hassucc << 1).
+    // The extra all-zero tile lets a step read slot last + 1 without a bounds test.
+    std::vector<uint64_t> next;
 };
 
 // Validates the CSR (throws Error(GFSM_EINVAL) on a malformed automaton).

@@ -15,10 +15,15 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 
 # status codes and technique ids (gpufsm_hip.h)
 OK, EINVAL, ENOMEM, EUNSUPPORTED, EDEVICE = 0, 1, 2, 3, 4
-NFA_AUTO, NFA_WORKLIST, NFA_BITPACKED = 0, 1, 2
+NFA_AUTO, NFA_WORKLIST, NFA_BITPACKED, NFA_CHAIN = 0, 1, 2, 3
 DFA_AUTO, DFA_GATHER, DFA_LDS, DFA_SPLIT = 0, 1, 2, 3
 
-NFA_TECHNIQUES = {"auto": NFA_AUTO, "worklist": NFA_WORKLIST, "bitpacked": NFA_BITPACKED}
+NFA_TECHNIQUES = {
+    "auto": NFA_AUTO,
+    "worklist": NFA_WORKLIST,
+    "bitpacked": NFA_BITPACKED,
+    "chain": NFA_CHAIN,
+}
 DFA_TECHNIQUES = {"auto": DFA_AUTO, "gather": DFA_GATHER, "lds": DFA_LDS, "split": DFA_SPLIT}
 
 # Every symbol the header declares; the CPU test suite checks the library exports them all.

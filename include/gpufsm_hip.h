@@ -50,6 +50,10 @@ extern "C" {
                                  bitmap (worklist_warp / worklist_shared / worklist_compact) */
 #define GFSM_NFA_BITPACKED 2  /* bit-parallel chain step, 4096-state tiles, class masks in
                                  LDS (bitpacked / multistream); chain-structured NFAs only */
+#define GFSM_NFA_CHAIN 3      /* sparse chain step: persistent waves over (input, chunk)
+                                 items, active slots listed in LDS and compacted by ballot;
+                                 long inputs cut into overlapping chunks when the start state
+                                 loops on every byte; chain-structured NFAs only */
 
 /* DFA techniques */
 #define GFSM_DFA_AUTO 0

@@ -50,6 +50,7 @@ def test_header_constants_match_python():
     assert consts["GFSM_ABI_VERSION"] == _lib.load().gfsm_abi_version()
     assert consts["GFSM_NFA_WORKLIST"] == _lib.NFA_WORKLIST
     assert consts["GFSM_NFA_BITPACKED"] == _lib.NFA_BITPACKED
+    assert consts["GFSM_NFA_CHAIN"] == _lib.NFA_CHAIN
     assert consts["GFSM_DFA_GATHER"] == _lib.DFA_GATHER
     assert consts["GFSM_DFA_LDS"] == _lib.DFA_LDS
     assert consts["GFSM_DFA_SPLIT"] == _lib.DFA_SPLIT

@@ -13,7 +13,7 @@ import random
 import numpy as np
 import pytest
 
-from fixtures import EXAMPLES, dfa_from_json, nfa_from_json, random_nfa
+from fixtures import EXAMPLES, chain_nfa, dfa_from_json, nfa_from_json, random_nfa
 from gpufsm import Backend, is_available, run, run_batch
 from gpufsm import workloads as W
 from gpufsm.core.dfa import random_dfa
@@ -157,7 +157,7 @@ def test_dfa_non_latching_long_inputs(tech):
     np.testing.assert_array_equal(n, wn)
 
 
-@pytest.mark.parametrize("tech", ["auto", "bitpacked", "worklist"])
+@pytest.mark.parametrize("tech", ["auto", "bitpacked", "worklist", "chain"])
 def test_pattern_union_ragged(tech):
     pu, (data, off) = W.workload_ragged(count=300, num_patterns=300, seed=3)
     # shorten to keep the oracle quick
@@ -192,11 +192,99 @@ def test_pattern_union_latching_bitpacked_vs_worklist():
     with DeviceNFA(pu.nfa) as dev:
         fb, nb, _ = dev.run_host(data, off, _lib.NFA_BITPACKED)
         fw, nw, _ = dev.run_host(data, off, _lib.NFA_WORKLIST)
+        fc, nc, _ = dev.run_host(data, off, _lib.NFA_CHAIN)
     wf, wn, _ = native.nfa_batch(pu.nfa, data, off)
-    np.testing.assert_array_equal(fb, wf)
-    np.testing.assert_array_equal(nb, wn)
-    np.testing.assert_array_equal(fw, wf)
-    np.testing.assert_array_equal(nw, wn)
+    for f, n in ((fb, nb), (fw, nw), (fc, nc)):
+        np.testing.assert_array_equal(f, wf)
+        np.testing.assert_array_equal(n, wn)
+
+
+def _check_nfa(nfa, data, off, tech):
+    with DeviceNFA(nfa) as dev:
+        f, n, _ = dev.run_host(data, off, _lib.NFA_TECHNIQUES[tech])
+    wf, wn, _ = native.nfa_batch(nfa, data, off)
+    np.testing.assert_array_equal(f, wf)
+    np.testing.assert_array_equal(n, wn)
+    return wf, wn
+
+
+def _class_batch(rng, used, count, max_len, p_class=0.8, empty_frac=0.1):
+    lens = rng.integers(0, max_len, size=count)
+    lens[rng.random(count) < empty_frac] = 0
+    total = int(lens.sum())
+    pool = np.frombuffer(used, dtype=np.uint8)
+    data = np.where(rng.random(total) < p_class, rng.choice(pool, size=total),
+                    rng.integers(0, 256, size=total)).astype(np.uint8)
+    off = np.zeros(count + 1, dtype=np.int64)
+    np.cumsum(lens, out=off[1:])
+    return data, off
+
+
+@pytest.mark.parametrize("self_loop", ["any", "all", "partial", "none"])
+@pytest.mark.parametrize("chunk", ["64", "333", "4096"])
+def test_chain_random_automata(self_loop, chunk, monkeypatch):
+    """Shuffled ids, shared first positions, accepting inner states, every start self-loop
+    kind; chunked (wildcard start) and unchunked inputs, chunk sizes not multiples of 4."""
+    monkeypatch.setenv("GFSM_CHAIN_CHUNK", chunk)
+    rng = random.Random(f"{self_loop}-{chunk}")
+    nrng = np.random.default_rng(rng.randrange(1 << 30))
+    for trial in range(4):
+        nfa, used = chain_nfa(rng, rng.randint(1, 200), (3, 30), (1, 12), self_loop=self_loop,
+                              p_accept=0.0005, p_last=0.03,
+                              alphabet=bytes(rng.sample(range(256), 24)))
+        data, off = _class_batch(nrng, used, 97, 3000)
+        _check_nfa(nfa, data, off, "chain")
+
+
+@pytest.mark.parametrize("chunk", ["64", "1000"])
+def test_chain_long_inputs_match_positions(chunk, monkeypatch):
+    """Wildcard-start union with a few accepting patterns: first matches fall deep into
+    64 KiB inputs, at every phase relative to the chunk boundaries."""
+    monkeypatch.setenv("GFSM_CHAIN_CHUNK", chunk)
+    pu = W.pattern_union(400, (3, 8), (4, 32), seed=8, accepting=False)
+    last = np.cumsum(pu.lengths)
+    pu.nfa.accept[last[:3]] = True
+    lens = np.full(48, 1 << 16, dtype=np.int64)
+    data, off = W.pattern_inputs(pu, lens, 0.9, seed=9)
+    wf, wn = _check_nfa(pu.nfa, data, off, "chain")
+    assert wf.any() and not wf.all()
+    assert len(set((wn[wf.astype(bool)] % int(chunk)).tolist())) > 5
+
+
+def test_chain_overflow_falls_back_to_tiles():
+    """Active sets of more than 1024 slots: 40 chains of wildcard-like classes, each 40 long,
+    so every input overflows the LDS list and is rerun by the tiled pass."""
+    b = NFABuilder()
+    s0 = b.add_state()
+    b.set_start(s0)
+    b.add_transition(s0, 256, s0)
+    for chain in range(40):
+        prev = s0
+        for pos in range(40):
+            s = b.add_state(accept=(pos == 39 and chain < 4))
+            for c in range(255 if pos == 39 else 256):
+                b.add_transition(prev, c if pos < 39 else (c + chain) % 256, s)
+            prev = s
+    nfa = b.build()
+    nrng = np.random.default_rng(12)
+    data, off = _ragged(nrng, 40, 2000)
+    wf, wn = _check_nfa(nfa, data, off, "chain")
+    assert wf.any()
+
+
+def test_chain_run_batch_api():
+    pu = W.pattern_union(120, seed=13)
+    lens = np.random.default_rng(13).integers(0, 9000, size=60)
+    data, off = W.pattern_inputs(pu, lens, 0.7, seed=14)
+    batch = [data[off[i]:off[i + 1]].tobytes() for i in range(60)]
+    got = _verdicts(run_batch(pu.nfa, batch, backend="hip", technique="chain"))
+    assert got == _oracle_nfa(pu.nfa, batch)
+
+
+def test_chain_rejects_non_chain_nfa():
+    nfa, _ = EXAMPLES["ab_star_c_plus_d"]()
+    with pytest.raises(Exception, match="chain-structured"):
+        run_batch(nfa, [b"abcd"], backend="hip", technique="chain")
 
 
 def test_plain_nfa_config():
