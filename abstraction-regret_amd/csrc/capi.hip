@@ -309,9 +309,17 @@ int gfsm_nfa_create(int32_t num_states, int32_t start_state, const uint8_t *acce
                 h->chain.acc = h->tile.acc;
                 h->chain.first_off = upload(B, by, t.first_off.data(), t.first_off.size());
                 h->chain.first_slot = upload(B, by, t.first_slot.data(), t.first_slot.size());
-                h->chain.first_accepts =
-                    upload(B, by, t.first_accepts.data(), t.first_accepts.size());
-                for (int i = 0; i < 4; ++i) h->chain.selfmask[i] = t.selfmask[i];
+                if (!t.pair.empty()) {
+                    h->chain.pair = upload(B, by, t.pair.data(), t.pair.size());
+                    h->chain.pair_slot = upload(B, by, t.pair_slot.data(), t.pair_slot.size());
+                }
+                h->chain.bflags = upload(B, by, t.bflags.data(), t.bflags.size());
+                for (int i = 0; i < 4; ++i) {
+                    h->chain.selfmask[i] = t.selfmask[i];
+                    h->chain.first_any[i] = t.first_any[i];
+                    h->chain.first_acc[i] = t.first_acc[i];
+                }
+                h->chain.tiles = t.tiles;
                 h->chain.max_chain = t.max_chain;
                 h->chain.start_always = h->tile.start_always;
             }
@@ -380,7 +388,8 @@ int gfsm_nfa_analyze(int32_t num_states, int32_t start_state, const uint8_t *acc
         if (t.ok) {
             add(t.mask.size(), 8); add(t.first.size(), 8); add(t.hassucc.size(), 8);
             add(t.acc.size(), 8); add(t.next.size(), 8); add(t.first_off.size(), 4);
-            add(t.first_slot.size(), 4); add(t.first_accepts.size(), 4);
+            add(t.first_slot.size(), 4); add(t.bflags.size(), 4);
+            if (!t.pair.empty()) { add(t.pair.size(), 4); add(t.pair_slot.size(), 4); }
         }
         out->device_bytes = bytes;
     });

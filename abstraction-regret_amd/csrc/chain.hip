@@ -1,0 +1,460 @@
+// Sparse chain technique for chain-structured NFAs (pattern unions) on gfx950 (wave64).
+//
+// Same automata as the tiled bit-parallel technique in nfa.hip (every non-start state has
+// at most one non-start predecessor and successor, no epsilon edges, homogeneous labels),
+// same slot numbering (4096-state tiles, successor of slot s is s + 1), same semantics
+// (`src/gpufsm/reference.py:36-65`: latch-first-match, an empty set ends the input with
+// (0, 0)). The tiled pass pays for every state on every byte; here the work follows the
+// active set, which in a pattern union is a few dozen of tens of thousands of states.
+//
+// Work items. Persistent waves take (input, chunk) items from an atomic counter. Long
+// inputs are cut into chunks when the start state loops on every byte: every non-start
+// state is then at most max_chain - 1 bytes from the start, so a chunk started
+// max_chain - 1 bytes early from the start state alone has the exact active set from its
+// first byte on; accepts inside that overlap belong to the previous chunk and are ignored.
+// An input's verdict is the minimum first-accept position over its chunks (atomicMin on
+// best[]), which also lets later chunks of an input with a known earlier match stop.
+//
+// Depth-1 states are never listed. The first positions entered on byte p only matter for
+// their accept bit (bflags), for emptiness, and for their successors, which the next byte c
+// adds from pair_slot[pair[p][c]]: the second positions whose first class holds p and whose
+// own class holds c, with one accept bit for the whole list. Listed states continue to
+// s + 1 on c when bit s + 1 of next[tile][c] is set: one 64-bit load per listed state,
+// issued together with the load of its accept word. A chain state has one predecessor, so
+// no deduplication is needed (a state that is both a first position and a chain successor
+// may be listed twice, which changes no verdict).
+//
+// Three tiers, each run only on the inputs the previous one could not finish:
+//   reg  — the listed states live one per lane in a register, at most 64 per wave. Lanes
+//          whose state dies free up; the additions of the next byte go to the lanes free at
+//          its start, in lane order (mbcnt over the free mask), so there is no compaction
+//          and no LDS. The pair entry and byte flags of the next byte are fetched while the
+//          current byte is stepped. An input that needs more than 64 lanes is flagged.
+//   lds  — the list lives in LDS (up to 1024 states per wave), compacted by ballot + mbcnt
+//          each byte. Also the only tier when the pair lists would be too large; it then
+//          adds the start state's successors from per-byte lists of first positions.
+//   tile — inputs that overflow the LDS list are run by the tiled bit-parallel pass.
+// Upstream has no counterpart: its chain-shaped automata go through the bit-packed
+// multistream kernels (`native/bitpacked.cu:12-166`), i.e. the tile tier.
+#include "internal.hpp"
+
+#include <climits>
+
+namespace gfsm {
+
+namespace {
+
+constexpr int kChainWaves = 4;  // waves per block (all chain kernels)
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int32_t lane_rank(uint64_t m) {
+    return static_cast<int32_t>(__builtin_amdgcn_mbcnt_hi(
+        static_cast<uint32_t>(m >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u)));
+}
+
+// Scratch of one launch: best / ovf1 / ovf2 per input, chunk bases, two work counters.
+struct ChainScratch {
+    int32_t *best, *ovf1, *ovf2;
+    int64_t *cbase;            // [count + 1]: first item of each input, total at [count]
+    unsigned int *counter;     // [2]: reg tier, lds tier
+};
+
+__global__ void chain_prep_kernel(const int64_t *__restrict__ off, int32_t count, int64_t chunk,
+                                  ChainScratch sc) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) {
+        sc.best[i] = INT_MAX;
+        sc.ovf1[i] = 0;
+        sc.ovf2[i] = 0;
+        const int64_t len = off[i + 1] - off[i];
+        sc.cbase[i] = (len + chunk - 1) / chunk;  // chunk counts, scanned next
+    }
+}
+
+// Exclusive scan of cbase[0..count) in place, cbase[count] = total; resets the counters.
+__global__ __launch_bounds__(1024) void chain_scan_kernel(ChainScratch sc, int32_t count) {
+    __shared__ int64_t part[1024];
+    int64_t carry = 0;
+    for (int32_t base = 0; base < count; base += 1024) {
+        const int32_t i = base + static_cast<int32_t>(threadIdx.x);
+        const int64_t v = i < count ? sc.cbase[i] : 0;
+        part[threadIdx.x] = v;
+        __syncthreads();
+        for (int d = 1; d < 1024; d <<= 1) {
+            const int64_t add = threadIdx.x >= static_cast<unsigned>(d) ? part[threadIdx.x - d] : 0;
+            __syncthreads();
+            part[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (i < count) sc.cbase[i] = carry + part[threadIdx.x] - v;
+        const int64_t tile_total = part[1023];
+        __syncthreads();
+        carry += tile_total;
+    }
+    if (threadIdx.x == 0) {
+        sc.cbase[count] = carry;
+        sc.counter[0] = 0u;
+        sc.counter[1] = 0u;
+    }
+}
+
+// Bytes [a, a + 4) of the batch as one little-endian word (a is 4-aligned in absolute
+// address terms); bytes outside [0, data_end) read as zero, words at or past stop as zero.
+__device__ __forceinline__ uint32_t chain_load_word(const uint8_t *__restrict__ data, int64_t a,
+                                                    int64_t stop, int64_t data_end) {
+    if (a >= stop) return 0;
+    if (a >= 0 && a + 4 <= data_end) return *reinterpret_cast<const uint32_t *>(data + a);
+    uint32_t w = 0;
+    for (int q = 0; q < 4; ++q)
+        if (a + q >= 0 && a + q < data_end) w |= static_cast<uint32_t>(data[a + q]) << (8 * q);
+    return w;
+}
+
+// Byte r (0..255) of the 256-byte window whose 4-byte words are spread over the lanes.
+__device__ __forceinline__ uint32_t window_byte(uint32_t word, int32_t r) {
+    return (static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(word), r >> 2)) >>
+            ((r & 3) << 3)) & 0xffu;
+}
+
+// One work item, decoded.
+struct Item {
+    int32_t in;
+    int64_t ib, cs, ce, ps;
+};
+
+// Takes the next item for the whole wave; false when none are left. Every lane takes part
+// in the atomic (lane 0 adds 1, the others 0) and the value is read from lane 0 under a
+// full mask: no branch precedes the broadcast. (A lane-0-only branch there lets the
+// compiler peel lane 0 off into a loop of its own, after which the other lanes never see a
+// new item.)
+__device__ __forceinline__ bool next_item(unsigned int *counter, const int64_t *__restrict__ cbase,
+                                          const int64_t *__restrict__ off, int32_t count,
+                                          int64_t items, int64_t chunk, int64_t overlap, int lane,
+                                          Item &it) {
+    const unsigned int mine = atomicAdd(counter, lane == 0 ? 1u : 0u);
+    const unsigned int item = __builtin_amdgcn_readfirstlane(mine);
+    if (static_cast<int64_t>(item) >= items) return false;
+    int32_t lo = 0, hi = count;  // input: largest i with cbase[i] <= item
+    while (hi - lo > 1) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (cbase[mid] <= static_cast<int64_t>(item)) lo = mid; else hi = mid;
+    }
+    it.in = lo;
+    it.ib = off[lo];
+    const int64_t ie = off[lo + 1];
+    it.cs = it.ib + (static_cast<int64_t>(item) - cbase[lo]) * chunk;
+    it.ce = (it.cs + chunk) < ie ? (it.cs + chunk) : ie;
+    it.ps = (it.cs - overlap) > it.ib ? (it.cs - overlap) : it.ib;
+    return true;
+}
+
+// An earlier first-accept than anything at or after position `from` is already known.
+__device__ __forceinline__ bool settled(const int32_t *best, const Item &it, int64_t from) {
+    const int32_t bst = __atomic_load_n(&best[it.in], __ATOMIC_RELAXED);
+    return bst != INT_MAX && it.ib + bst - 1 < from;
+}
+
+// ------------------------------------------------------------------------- reg tier ----
+
+// Flags of the 4 bytes of `word` (4 bits each, bflags layout), for a lane's window word.
+__device__ __forceinline__ uint32_t word_flags(const uint32_t *__restrict__ bflags, uint32_t word) {
+    return bflags[word & 0xffu] | (bflags[(word >> 8) & 0xffu] << 4) |
+           (bflags[(word >> 16) & 0xffu] << 8) | (bflags[word >> 24] << 12);
+}
+
+__global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_reg_kernel(
+    NfaChainDev d, const uint8_t *__restrict__ data, const int64_t *__restrict__ off,
+    int32_t count, ChainScratch sc, int64_t chunk) {
+    const int lane = threadIdx.x & 63;
+    const int64_t items = sc.cbase[count];
+    const int64_t data_end = off[count];
+    const int64_t overlap = d.max_chain > 0 ? d.max_chain - 1 : 0;
+    Item it;
+    while (next_item(&sc.counter[0], sc.cbase, off, count, items, chunk, overlap, lane, it)) {
+        bool stop = settled(sc.best, it, it.cs);  // wave-uniform throughout
+        int32_t slot = -1;                        // this lane's listed state, -1 = free
+        uint64_t occ = 0;                         // lanes holding a state
+        bool s0 = true, ovf = false;
+        int32_t d1 = 256;                         // byte of the depth-1 set; 256 = none
+        uint32_t last_b = 0, last_fl = 0;         // the byte before the window and its flags
+        int32_t found = 0;
+        const int64_t abase =
+            it.ps - static_cast<int64_t>((reinterpret_cast<uintptr_t>(data) + it.ps) & 3u);
+        for (int64_t w0 = abase; w0 < it.ce && !stop; w0 += 256) {
+            const uint32_t word = chain_load_word(data, w0 + 4 * lane, it.ce, data_end);
+            const uint32_t fw = word_flags(d.bflags, word);
+            const int32_t lo = it.ps > w0 ? static_cast<int32_t>(it.ps - w0) : 0;
+            const int32_t hi = (it.ce - w0) < 256 ? static_cast<int32_t>(it.ce - w0) : 256;
+            const int32_t hit_from = it.cs > w0 ? static_cast<int32_t>(it.cs - w0) : 0;
+            // The depth-1 set before each byte depends only on the bytes and on whether the
+            // start state is still active, so the pair entries of the whole window are
+            // loaded up front, four per lane. kpos: first byte of the window (>= lo) on
+            // which the start state has no self-loop (it is active before byte r while
+            // kpos >= r).
+            int32_t kpos = 1 << 20;
+            if (!d.start_always) {
+                int32_t mine = 1 << 20;
+#pragma unroll
+                for (int q = 3; q >= 0; --q) {
+                    const int32_t r = 4 * lane + q;
+                    if (r >= lo && r < hi && !((fw >> (4 * q + 2)) & 1u)) mine = r;
+                }
+                const uint64_t m = __ballot(mine < (1 << 20));
+                if (m) kpos = __builtin_amdgcn_readlane(mine, __builtin_ctzll(m));
+            }
+            const uint32_t pword = static_cast<uint32_t>(__shfl_up(static_cast<int>(word), 1));
+            const uint32_t pfw = static_cast<uint32_t>(__shfl_up(static_cast<int>(fw), 1));
+            int32_t pe_off[4], pe_w[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int32_t r = 4 * lane + q;
+                const uint32_t bq = (word >> (8 * q)) & 0xffu;
+                uint32_t bp, fp;
+                if (q > 0) {
+                    bp = (word >> (8 * (q - 1))) & 0xffu;
+                    fp = (fw >> (4 * (q - 1))) & 0xfu;
+                } else {
+                    bp = lane > 0 ? pword >> 24 : last_b;
+                    fp = lane > 0 ? (pfw >> 12) & 0xfu : last_fl;
+                }
+                const bool s0p = s0 && kpos >= r - 1;
+                const int32_t d1q =
+                    r == lo ? d1 : ((s0p && (fp & 1u)) ? static_cast<int32_t>(bp) : 256);
+                const int32_t idx =
+                    (r >= lo && r < hi) ? 2 * (d1q * 256 + static_cast<int32_t>(bq)) : 0;
+                pe_off[q] = d.pair[idx];
+                pe_w[q] = d.pair[idx + 1];
+            }
+            for (int32_t j = lo >> 2; j < 64 && !stop; ++j) {
+                const uint32_t wj = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(word), j));
+                const uint32_t fj = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(fw), j));
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int32_t r = 4 * j + q;
+                    if (r < lo || r >= hi || stop) continue;
+                    const uint32_t b = (wj >> (8 * q)) & 0xffu;
+                    const uint32_t fl = (fj >> (4 * q)) & 0xfu;
+                    const int32_t add_off = __builtin_amdgcn_readlane(pe_off[q], j);
+                    const int32_t add_w = __builtin_amdgcn_readlane(pe_w[q], j);
+                    const int32_t add_cnt = add_w & INT_MAX;
+                    // additions go to the lanes free at the start of the byte, in lane
+                    // order; every lane loads (padding keeps the index in range)
+                    const int32_t frank = lane_rank(~occ);
+                    const int32_t cand = d.pair_slot[add_off + (frank & 63)];
+                    // continuation of held states; free lanes look up slot 0 and ignore it
+                    const bool have = slot >= 0;
+                    const uint32_t t = have ? static_cast<uint32_t>(slot + 1) : 0u;
+                    const uint32_t tile = t >> 12;  // chains never cross a tile
+                    const uint32_t sl = t & 4095u;
+                    const uint64_t nw =
+                        d.next[(static_cast<size_t>(tile) * 256 + b) * kTileWords + (sl >> 6)];
+                    const uint64_t aw = d.acc[tile * kTileWords + (sl >> 6)];
+                    const bool alive = have & static_cast<bool>((nw >> (sl & 63)) & 1ull);
+                    const bool hit = alive & static_cast<bool>((aw >> (sl & 63)) & 1ull);
+                    if (add_cnt > 64 - __popcll(occ)) {
+                        ovf = true;
+                        stop = true;
+                        continue;
+                    }
+                    slot = alive ? static_cast<int32_t>(t) : ((!have && frank < add_cnt) ? cand : -1);
+                    occ = __ballot(slot >= 0);
+                    if ((__ballot(hit) != 0 || add_w < 0 || (s0 && (fl & 2u))) && r >= hit_from) {
+                        found = static_cast<int32_t>(w0 + r - it.ib + 1);
+                        stop = true;
+                        continue;
+                    }
+                    d1 = (s0 && (fl & 1u)) ? static_cast<int32_t>(b) : 256;
+                    s0 = s0 && (d.start_always || (fl & 4u));
+                    last_b = b;
+                    last_fl = fl;
+                    if (occ == 0 && d1 == 256 && !s0) stop = true;
+                }
+            }
+            if (!stop && ((w0 - abase) & 4095) == 3840 && settled(sc.best, it, w0 + 256)) stop = true;
+        }
+        if (lane == 0) {
+            if (found > 0) atomicMin(&sc.best[it.in], found);
+            if (ovf) sc.ovf1[it.in] = 1;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------- lds tier ----
+
+template <bool kPairs, int kCap>
+__global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_lds_kernel(
+    NfaChainDev d, const uint8_t *__restrict__ data, const int64_t *__restrict__ off,
+    int32_t count, ChainScratch sc, int64_t chunk, const int32_t *__restrict__ only,
+    int32_t *ovf_out) {
+    __shared__ int32_t lists[kChainWaves][2][kCap];
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int64_t items = sc.cbase[count];
+    const int64_t data_end = off[count];
+    const int64_t overlap = d.max_chain > 0 ? d.max_chain - 1 : 0;
+    Item it;
+    while (next_item(&sc.counter[1], sc.cbase, off, count, items, chunk, overlap, lane, it)) {
+        bool stop = (only && !only[it.in]) || settled(sc.best, it, it.cs);  // wave-uniform
+        int32_t *A = lists[wv][0];
+        int32_t *B = lists[wv][1];
+        int32_t k = 0;
+        bool s0 = true, ovf = false;
+        int32_t d1 = 256;  // pair mode: byte of the depth-1 set; 256 = none
+        int32_t found = 0;
+        const int64_t abase =
+            it.ps - static_cast<int64_t>((reinterpret_cast<uintptr_t>(data) + it.ps) & 3u);
+        for (int64_t w0 = abase; w0 < it.ce && !stop; w0 += 256) {
+            const uint32_t word = chain_load_word(data, w0 + 4 * lane, it.ce, data_end);
+            const int32_t lo = it.ps > w0 ? static_cast<int32_t>(it.ps - w0) : 0;
+            const int32_t hi = (it.ce - w0) < 256 ? static_cast<int32_t>(it.ce - w0) : 256;
+            const int32_t hit_from = it.cs > w0 ? static_cast<int32_t>(it.cs - w0) : 0;
+            for (int32_t r = lo; r < hi; ++r) {
+                const uint32_t b = window_byte(word, r);
+                const uint32_t fl = d.bflags[b];
+                int32_t add_off = 0, add_cnt = 0;
+                bool add_acc = false;
+                if (kPairs) {
+                    const int32_t *pe = d.pair + 2 * (d1 * 256 + static_cast<int32_t>(b));
+                    add_off = pe[0];
+                    add_cnt = pe[1] & INT_MAX;
+                    add_acc = pe[1] < 0;
+                } else if (s0) {
+                    add_off = d.first_off[b];
+                    add_cnt = d.first_off[b + 1] - add_off;
+                }
+                const int32_t *add_src = kPairs ? d.pair_slot : d.first_slot;
+                int32_t nk = 0;
+                bool hit = add_acc || (s0 && (fl & 2u));
+                for (int32_t j0 = 0; j0 < k; j0 += 64) {
+                    const int32_t jj = j0 + lane;
+                    bool alive = false;
+                    int32_t t = 0;
+                    if (jj < k) {
+                        t = A[jj] + 1;
+                        const uint32_t tile = static_cast<uint32_t>(t) >> 12;
+                        const uint32_t sl = static_cast<uint32_t>(t) & 4095u;
+                        const uint64_t nw =
+                            d.next[(static_cast<size_t>(tile) * 256 + b) * kTileWords + (sl >> 6)];
+                        const uint64_t aw = d.acc[tile * kTileWords + (sl >> 6)];
+                        alive = (nw >> (sl & 63)) & 1ull;
+                        hit |= alive && ((aw >> (sl & 63)) & 1ull);
+                    }
+                    const uint64_t m = __ballot(alive);
+                    const int32_t cnt = __popcll(m);
+                    if (nk + cnt > kCap) {
+                        ovf = true;
+                        break;
+                    }
+                    if (alive) B[nk + lane_rank(m)] = t;
+                    nk += cnt;
+                }
+                if (!ovf && add_cnt > 0) {
+                    if (nk + add_cnt > kCap) {
+                        ovf = true;
+                    } else {
+                        for (int32_t jj = lane; jj < add_cnt; jj += 64)
+                            B[nk + jj] = add_src[add_off + jj];
+                        nk += add_cnt;
+                    }
+                }
+                if (ovf) {
+                    stop = true;
+                    break;
+                }
+                if (kPairs) d1 = (s0 && (fl & 1u)) ? static_cast<int32_t>(b) : 256;
+                if (!d.start_always) s0 = s0 && (fl & 4u);
+                wave_sync();
+                int32_t *tmp = A;
+                A = B;
+                B = tmp;
+                k = nk;
+                if (__ballot(hit) != 0 && r >= hit_from) {
+                    found = static_cast<int32_t>(w0 + r - it.ib + 1);
+                    stop = true;
+                    break;
+                }
+                if (k == 0 && !s0 && (!kPairs || d1 == 256)) {
+                    stop = true;
+                    break;
+                }
+            }
+            if (!stop && ((w0 - abase) & 4095) == 3840 && settled(sc.best, it, w0 + 256)) stop = true;
+        }
+        if (lane == 0) {
+            if (found > 0) atomicMin(&sc.best[it.in], found);
+            if (ovf) ovf_out[it.in] = 1;
+        }
+    }
+}
+
+// Blocks of a persistent kernel: as many as are resident at once on every CU.
+int persistent_grid(const void *kernel, int threads) {
+    int per_cu = 0;
+    GFSM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0));
+    return device_cu_count() * (per_cu > 0 ? per_cu : 1);
+}
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+}  // namespace
+
+int32_t nfa_chain_chunk_bytes() {
+    const int64_t c = env_int("GFSM_CHAIN_CHUNK", 4096);
+    return static_cast<int32_t>(c < 64 ? 64 : (c > (1 << 30) ? (1 << 30) : c));
+}
+
+size_t nfa_chain_scratch_bytes(int32_t count) {
+    const size_t c = static_cast<size_t>(count);
+    return 3 * align256(c * 4) + align256((c + 1) * 8) + 256;
+}
+
+void launch_nfa_chain(const NfaChainDev &d, const NfaTileDev &tiles, const uint8_t *data,
+                      const int64_t *off, int32_t count, int32_t *flags, int32_t *lens,
+                      void *scratch, int32_t chunk_bytes, hipStream_t stream) {
+    if (count <= 0) return;
+    const size_t c = static_cast<size_t>(count);
+    char *p = static_cast<char *>(scratch);
+    ChainScratch sc;
+    sc.best = reinterpret_cast<int32_t *>(p);
+    p += align256(c * 4);
+    sc.ovf1 = reinterpret_cast<int32_t *>(p);
+    p += align256(c * 4);
+    sc.ovf2 = reinterpret_cast<int32_t *>(p);
+    p += align256(c * 4);
+    sc.cbase = reinterpret_cast<int64_t *>(p);
+    p += align256((c + 1) * 8);
+    sc.counter = reinterpret_cast<unsigned int *>(p);
+    // without an always-active start state, inputs are not cut
+    const int64_t chunk = d.start_always ? static_cast<int64_t>(chunk_bytes) : (int64_t(1) << 62);
+    chain_prep_kernel<<<(count + 255) / 256, 256, 0, stream>>>(off, count, chunk, sc);
+    GFSM_HIP(hipGetLastError());
+    chain_scan_kernel<<<1, 1024, 0, stream>>>(sc, count);
+    GFSM_HIP(hipGetLastError());
+    const int threads = 64 * kChainWaves;
+    static const int grid_reg =
+        persistent_grid(reinterpret_cast<const void *>(&nfa_chain_reg_kernel), threads);
+    static const int grid_lds_pairs = persistent_grid(
+        reinterpret_cast<const void *>(&nfa_chain_lds_kernel<true, 1024>), threads);
+    static const int grid_lds_first = persistent_grid(
+        reinterpret_cast<const void *>(&nfa_chain_lds_kernel<false, 1024>), threads);
+    // GFSM_CHAIN_FIRST=1 skips the pair lists (first-position lists in the lds tier only)
+    if (d.pair && env_int("GFSM_CHAIN_FIRST", 0) != 1) {
+        nfa_chain_reg_kernel<<<grid_reg, threads, 0, stream>>>(d, data, off, count, sc, chunk);
+        GFSM_HIP(hipGetLastError());
+        nfa_chain_lds_kernel<true, 1024><<<grid_lds_pairs, threads, 0, stream>>>(
+            d, data, off, count, sc, chunk, sc.ovf1, sc.ovf2);
+    } else {
+        nfa_chain_lds_kernel<false, 1024><<<grid_lds_first, threads, 0, stream>>>(
+            d, data, off, count, sc, chunk, nullptr, sc.ovf2);
+    }
+    GFSM_HIP(hipGetLastError());
+    launch_nfa_bitpacked_core(tiles, data, off, count, sc.best, sc.ovf2, stream);
+    launch_nfa_best_finish(sc.best, count, flags, lens, stream);
+}
+
+}  // namespace gfsm

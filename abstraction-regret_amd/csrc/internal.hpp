@@ -22,6 +22,7 @@ constexpr int kBigRow = 16;              // rows longer than this get a per-labe
 constexpr int kBigStride = 258;          // per-label index: 257 label starts + row end
 constexpr int kTileStates = 4096;        // bitpacked tile: 64 lanes x 64-bit words
 constexpr int kTileWords = 64;
+constexpr int64_t kMaxPairSlots = int64_t(1) << 24;  // byte-pair list cap (entries)
 
 // Thrown inside the library, converted to a status code at the C boundary.
 struct Error : std::runtime_error {
@@ -67,17 +68,25 @@ struct NfaTileDev {
 };
 
 // Sparse chain layout (same renumbering as the tiles): the active set is a list of slots;
-// slot s continues to s + 1 on byte c when hassucc(s) and c is in the class of s + 1
-// (bit of mask[tile][c]); the start state adds first_slot[first_off[c] .. first_off[c+1]).
+// slot s continues to s + 1 on byte c when bit s + 1 of next[tile][c] is set. The start
+// state's successors come either from byte-pair lists of second positions (pair != null) or
+// from per-byte lists of first positions (see layout.hpp).
 struct NfaChainDev {
-    const uint64_t *next;                  // [tiles][256][64]: mask & (hassucc << 1)
-    const uint64_t *acc;                   // tile accept bits (NfaTileDev)
-    const int32_t *first_off;              // [257]
+    const uint64_t *next;          // [tiles + 1][256][64]: mask & (hassucc << 1)
+    const uint64_t *acc;           // [tiles + 1][64] accepting slots (last tile all zero)
+    const int32_t *first_off;      // [257]
     const int32_t *first_slot;
-    const uint32_t *first_accepts;         // [256]
-    uint64_t selfmask[4];
+    const int32_t *pair;           // [257 * 256][2] {offset, count | accepts << 31}, or
+                                   // null; row 256 is all zero (no depth-1 set)
+    const int32_t *pair_slot;      // lists + 64 padding entries
+    const uint32_t *bflags;        // [256] bit 0 first_any, 1 first_acc, 2 start self-loop
+    uint64_t first_any[4];         // bytes on which the start state has a successor
+    uint64_t first_acc[4];         // ... an accepting one
+    uint64_t selfmask[4];          // bytes on which the start state stays active
+    int32_t tiles;
     int32_t max_chain;
     int32_t start_always;
+    int32_t pad;
 };
 
 // ----- DFA ------------------------------------------------------------------------------
@@ -127,6 +136,14 @@ void nfa_worklist_geometry(const NfaWorklistDev &d, int32_t count, int32_t *grid
 void launch_nfa_bitpacked(const NfaTileDev &d, const uint8_t *data, const int64_t *off,
                           int32_t count, int32_t *flags, int32_t *lens, int32_t *best,
                           hipStream_t stream);
+// The tiled pass alone over inputs with only[i] != 0 (all when only is null), folding its
+// first-accept positions into best[] (INT_MAX = none yet) with atomicMin.
+void launch_nfa_bitpacked_core(const NfaTileDev &d, const uint8_t *data, const int64_t *off,
+                               int32_t count, int32_t *best, const int32_t *only,
+                               hipStream_t stream);
+// best[] (1-based first-accept position, INT_MAX = none) -> (flags, lens).
+void launch_nfa_best_finish(const int32_t *best, int32_t count, int32_t *flags, int32_t *lens,
+                            hipStream_t stream);
 
 // Sparse chain technique: persistent waves over (input, chunk) work items.
 size_t nfa_chain_scratch_bytes(int32_t count);

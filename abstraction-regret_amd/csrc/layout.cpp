@@ -2,6 +2,7 @@
 #include "layout.hpp"
 
 #include <algorithm>
+#include <climits>
 #include <numeric>
 #include <string>
 
@@ -27,6 +28,16 @@ struct Bits256 {
         return w[0] == o.w[0] && w[1] == o.w[1] && w[2] == o.w[2] && w[3] == o.w[3];
     }
     bool has(int b) const { return (w[b >> 6] >> (b & 63)) & 1ull; }
+    int count() const {
+        return __builtin_popcountll(w[0]) + __builtin_popcountll(w[1]) +
+               __builtin_popcountll(w[2]) + __builtin_popcountll(w[3]);
+    }
+    std::vector<int> bytes() const {
+        std::vector<int> out;
+        for (int c = 0; c < 256; ++c)
+            if (has(c)) out.push_back(c);
+        return out;
+    }
 };
 
 }  // namespace
@@ -204,7 +215,7 @@ TileHost build_tiles(const NfaInput &a) {
     h.mask.assign(T * 256 * kTileWords, 0ull);
     h.first.assign(T * kTileWords, 0ull);
     h.hassucc.assign(T * kTileWords, 0ull);
-    h.acc.assign(T * kTileWords, 0ull);
+    h.acc.assign((T + 1) * kTileWords, 0ull);  // one all-zero tile past the last
     h.slot_of_state.assign(static_cast<size_t>(n), -1);
     for (size_t b = 0; b < bins.size(); ++b) {
         int32_t slot = 0;
@@ -242,6 +253,58 @@ TileHost build_tiles(const NfaInput &a) {
     h.first_slot.reserve(static_cast<size_t>(h.first_off[256]));
     for (int c = 0; c < 256; ++c)
         h.first_slot.insert(h.first_slot.end(), by_byte[c].begin(), by_byte[c].end());
+    for (int c = 0; c < 256; ++c) {
+        if (h.first_off[c + 1] > h.first_off[c]) h.first_any[c >> 6] |= 1ull << (c & 63);
+        if (h.first_accepts[c]) h.first_acc[c >> 6] |= 1ull << (c & 63);
+    }
+    // byte-pair lists of second positions (counting pass, then fill)
+    {
+        std::vector<int32_t> firsts;  // first positions with a chain successor
+        int64_t total = 0;
+        for (int32_t u = 0; u < n; ++u) {
+            if (u == s0 || !from_start[u].any() || succ[u] == -1) continue;
+            firsts.push_back(u);
+            total += static_cast<int64_t>(from_start[u].count()) * from_pred[succ[u]].count();
+        }
+        if (total <= kMaxPairSlots) {
+            std::vector<int64_t> cnt(65536, 0);
+            for (int32_t u : firsts) {
+                const std::vector<int> c2s = from_pred[succ[u]].bytes();
+                for (int c : from_start[u].bytes())
+                    for (int c2 : c2s) ++cnt[c * 256 + c2];
+            }
+            h.pair.assign(257 * 256 * 2, 0);
+            std::vector<int64_t> fill(65536, 0);
+            int64_t o = 0;
+            for (int i = 0; i < 65536; ++i) {
+                fill[i] = o;
+                h.pair[2 * i] = static_cast<int32_t>(o);
+                h.pair[2 * i + 1] = static_cast<int32_t>(cnt[i]);
+                o += cnt[i];
+            }
+            // 64 padding entries: a wave may read up to 63 past the end of a list
+            h.pair_slot.assign(static_cast<size_t>(total) + 64, 0);
+            for (int32_t u : firsts) {
+                const int32_t v = succ[u];
+                const int32_t sv = h.slot_of_state[v];
+                const std::vector<int> c2s = from_pred[v].bytes();
+                for (int c : from_start[u].bytes())
+                    for (int c2 : c2s) {
+                        const int i = c * 256 + c2;
+                        h.pair_slot[static_cast<size_t>(fill[i]++)] = sv;
+                        if (a.accept[v]) h.pair[2 * i + 1] |= INT32_MIN;
+                    }
+            }
+            for (int i = 0; i < 65536; ++i)
+                std::sort(h.pair_slot.begin() + h.pair[2 * i],
+                          h.pair_slot.begin() + h.pair[2 * i] + (h.pair[2 * i + 1] & INT32_MAX));
+        }
+    }
+    h.bflags.assign(256, 0u);
+    for (int c = 0; c < 256; ++c)
+        h.bflags[c] = ((h.first_any[c >> 6] >> (c & 63)) & 1ull) |
+                      (((h.first_acc[c >> 6] >> (c & 63)) & 1ull) << 1) |
+                      (static_cast<uint32_t>(self.has(c)) << 2);
     h.next.assign((T + 1) * 256 * kTileWords, 0ull);
     for (size_t b = 0; b < T; ++b) {
         const uint64_t *hs = &h.hassucc[b * kTileWords];

@@ -32,7 +32,8 @@ struct TileHost {
     int32_t max_chain = 0;
     bool start_always = false;
     uint64_t selfmask[4] = {0, 0, 0, 0};
-    std::vector<uint64_t> mask, first, hassucc, acc;  // see NfaTileDev
+    std::vector<uint64_t> mask, first, hassucc, acc;  // see NfaTileDev; acc has one extra
+                                                      // all-zero tile
     std::vector<int32_t> slot_of_state;               // renumbering (-1 for the start)
     // successors of the start state by byte: slots of first positions whose class holds c
     std::vector<int32_t> first_off;                   // [257]
@@ -42,6 +43,17 @@ struct TileHost {
     // slot t - 1 has a chain successor and the class of slot t holds c (mask & hassucc << 1).
     // The extra all-zero tile lets a step read slot last + 1 without a bounds test.
     std::vector<uint64_t> next;
+    // the same by bits: the start state has a successor on c / one that accepts
+    uint64_t first_any[4] = {0, 0, 0, 0};
+    uint64_t first_acc[4] = {0, 0, 0, 0};
+    // second positions by byte pair: pair[c1 * 256 + c2] = {offset, count | accepts << 31}
+    // into pair_slot, the slots v = succ(u) of first positions u whose start class holds c1
+    // and whose own class holds c2. Empty when the lists would exceed kMaxPairSlots.
+    std::vector<int32_t> pair;                        // [257 * 256 * 2] or empty; row 256
+                                                      // (no depth-1 set) is all zero
+    std::vector<int32_t> pair_slot;                   // + 64 zero padding entries
+    std::vector<uint32_t> bflags;                     // [256] first_any | first_acc << 1 |
+                                                      // start self-loop << 2
 };
 
 // Validates the CSR (throws Error(GFSM_EINVAL) on a malformed automaton).

@@ -303,7 +303,7 @@ __global__ __launch_bounds__(kBpThreads) void nfa_bitpacked_kernel(
 
 // The tiled bit-parallel pass alone (best[] must hold INT_MAX or earlier results): every
 // input, or only those with only[i] != 0.
-static void bitpacked_core(const NfaTileDev &d, const uint8_t *data, const int64_t *off,
+void launch_nfa_bitpacked_core(const NfaTileDev &d, const uint8_t *data, const int64_t *off,
                            int32_t count, int32_t *best, const int32_t *only,
                            hipStream_t stream) {
     static bool attr_set = false;
@@ -330,243 +330,12 @@ void launch_nfa_bitpacked(const NfaTileDev &d, const uint8_t *data, const int64_
     if (count <= 0) return;
     bp_init_kernel<<<(count + 255) / 256, 256, 0, stream>>>(best, count);
     GFSM_HIP(hipGetLastError());
-    bitpacked_core(d, data, off, count, best, nullptr, stream);
-    bp_finish_kernel<<<(count + 255) / 256, 256, 0, stream>>>(best, count, flags, lens);
-    GFSM_HIP(hipGetLastError());
+    launch_nfa_bitpacked_core(d, data, off, count, best, nullptr, stream);
+    launch_nfa_best_finish(best, count, flags, lens, stream);
 }
 
-// ---------------------------------------------------------------- sparse chain -----------
-//
-// For chain-structured NFAs whose active sets are sparse (pattern unions: a few hundred of
-// tens of thousands of states). One wave per work item (an input, or a chunk of one); the
-// active set is a list of slots in LDS. Per byte c:
-//   * every active slot s continues to s + 1 when next[tile][c] has bit s + 1 (the class of
-//     s + 1 holds c and s has a chain successor): one 64-bit load per active slot;
-//   * the start state, while active, adds the precomputed slots first_slot[first_off[c]..];
-//   * survivors are compacted with a 64-bit ballot and mbcnt, no atomics: a chain state has
-//     one predecessor, so the list needs no deduplication;
-//   * the accept test (bit of acc, or first_accepts[c]) is fused into the step.
-// Long inputs are cut into chunks when the start state is always active (wildcard
-// self-loop): every non-start state is then at most max_chain - 1 bytes from the start, so a
-// chunk started max_chain - 1 bytes early from the start state alone reproduces the active
-// set exactly from its first byte on; accepts inside that overlap belong to the previous
-// chunk and are ignored. An input's verdict is the minimum first-accept position over its
-// chunks (atomicMin), which also lets later chunks skip work. A wave whose list outgrows its
-// LDS capacity flags the input; flagged inputs are then run by the tiled bit-parallel pass.
-
-constexpr int kChainWaves = 4;    // waves per block
-constexpr int kChainCap = 1024;   // active-list capacity per wave (slots)
-
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__global__ void chain_prep_kernel(const int64_t *__restrict__ off, int32_t count, int64_t chunk,
-                                  int32_t *best, int32_t *overflow, int64_t *cbase) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) {
-        best[i] = INT_MAX;
-        overflow[i] = 0;
-        const int64_t len = off[i + 1] - off[i];
-        cbase[i] = (len + chunk - 1) / chunk;  // chunk counts, scanned next
-    }
-}
-
-// Exclusive scan of cbase[0..count) in place, cbase[count] = total; resets the counter.
-__global__ __launch_bounds__(1024) void chain_scan_kernel(int64_t *cbase, int32_t count,
-                                                          unsigned int *counter) {
-    __shared__ int64_t part[1024];
-    int64_t carry = 0;
-    for (int32_t base = 0; base < count; base += 1024) {
-        const int32_t i = base + static_cast<int32_t>(threadIdx.x);
-        const int64_t v = i < count ? cbase[i] : 0;
-        part[threadIdx.x] = v;
-        __syncthreads();
-        for (int d = 1; d < 1024; d <<= 1) {
-            const int64_t add = threadIdx.x >= static_cast<unsigned>(d) ? part[threadIdx.x - d] : 0;
-            __syncthreads();
-            part[threadIdx.x] += add;
-            __syncthreads();
-        }
-        if (i < count) cbase[i] = carry + part[threadIdx.x] - v;
-        const int64_t tile_total = part[1023];
-        __syncthreads();
-        carry += tile_total;
-    }
-    if (threadIdx.x == 0) {
-        cbase[count] = carry;
-        *counter = 0u;
-    }
-}
-
-// Bytes [a, a + 4) of the batch as one little-endian word (a is 4-aligned in absolute
-// address terms); bytes outside [0, data_end) read as zero, words at or past stop as zero.
-__device__ __forceinline__ uint32_t chain_load_word(const uint8_t *__restrict__ data, int64_t a,
-                                                    int64_t stop, int64_t data_end) {
-    if (a >= stop) return 0;
-    if (a >= 0 && a + 4 <= data_end) return *reinterpret_cast<const uint32_t *>(data + a);
-    uint32_t w = 0;
-    for (int q = 0; q < 4; ++q)
-        if (a + q >= 0 && a + q < data_end) w |= static_cast<uint32_t>(data[a + q]) << (8 * q);
-    return w;
-}
-
-__global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_kernel(
-    NfaChainDev d, const uint8_t *__restrict__ data, const int64_t *__restrict__ off,
-    int32_t count, const int64_t *__restrict__ cbase, unsigned int *counter, int64_t chunk,
-    int32_t *best, int32_t *overflow) {
-    __shared__ int32_t lists[kChainWaves][2][kChainCap];
-    const int lane = threadIdx.x & 63;
-    const int wv = threadIdx.x >> 6;
-    const int64_t items = cbase[count];
-    const int64_t data_end = off[count];
-    const int64_t overlap = d.max_chain > 0 ? d.max_chain - 1 : 0;
-    for (;;) {
-        // Every lane takes part in the atomic (lane 0 adds 1, the others 0) and the value is
-        // read from lane 0 under a full mask. No branch precedes the broadcast: a lane-0-only
-        // branch there lets the compiler peel lane 0 off into its own loop, after which the
-        // other lanes never see a new item.
-        const unsigned int mine = atomicAdd(counter, lane == 0 ? 1u : 0u);
-        const unsigned int item = __builtin_amdgcn_readfirstlane(mine);
-        if (static_cast<int64_t>(item) >= items) break;
-        int32_t lo = 0, hi = count;  // input: largest i with cbase[i] <= item
-        while (hi - lo > 1) {
-            const int32_t mid = (lo + hi) >> 1;
-            if (cbase[mid] <= static_cast<int64_t>(item)) lo = mid; else hi = mid;
-        }
-        const int32_t in = lo;
-        const int64_t ib = off[in], ie = off[in + 1];
-        const int64_t cs = ib + (static_cast<int64_t>(item) - cbase[in]) * chunk;
-        const int64_t ce = (cs + chunk) < ie ? (cs + chunk) : ie;
-        bool stop = false;  // wave-uniform throughout
-        {
-            const int32_t bst = __atomic_load_n(&best[in], __ATOMIC_RELAXED);
-            if (bst != INT_MAX && ib + bst - 1 < cs) stop = true;  // an earlier match is known
-        }
-        const int64_t ps = (cs - overlap) > ib ? (cs - overlap) : ib;
-        int32_t *A = lists[wv][0];
-        int32_t *B = lists[wv][1];
-        int32_t k = 0;
-        bool s0 = true, ovf = false;
-        int32_t found = 0;
-        const int64_t abase =
-            ps - static_cast<int64_t>((reinterpret_cast<uintptr_t>(data) + ps) & 3u);
-        for (int64_t w0 = abase; w0 < ce && !stop; w0 += 256) {
-            const uint32_t word = chain_load_word(data, w0 + 4 * lane, ce, data_end);
-            for (int32_t j = 0; j < 64 && !stop; ++j) {
-                const uint32_t w4 = __builtin_amdgcn_readlane(word, j);
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int64_t at = w0 + 4 * j + q;
-                    if (at < ps || at >= ce || stop) continue;
-                    const uint32_t b = (w4 >> (8 * q)) & 0xffu;
-                    int32_t nk = 0;
-                    bool hit = false;
-                    for (int32_t j0 = 0; j0 < k; j0 += 64) {
-                        const int32_t jj = j0 + lane;
-                        bool alive = false;
-                        int32_t t = 0;
-                        if (jj < k) {
-                            t = A[jj] + 1;  // chains never cross a tile
-                            const uint32_t tile = static_cast<uint32_t>(t) >> 12;
-                            const uint32_t sl = static_cast<uint32_t>(t) & 4095u;
-                            const uint64_t nw =
-                                d.next[(static_cast<size_t>(tile) * 256 + b) * kTileWords + (sl >> 6)];
-                            alive = (nw >> (sl & 63)) & 1ull;
-                            if (alive) hit |= (d.acc[tile * kTileWords + (sl >> 6)] >> (sl & 63)) & 1ull;
-                        }
-                        const uint64_t m = __ballot(alive);
-                        const int32_t cnt = __popcll(m);
-                        if (nk + cnt > kChainCap) {
-                            ovf = true;
-                            break;
-                        }
-                        if (alive) {
-                            const int32_t rank = __builtin_amdgcn_mbcnt_hi(
-                                static_cast<uint32_t>(m >> 32),
-                                __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
-                            B[nk + rank] = t;
-                        }
-                        nk += cnt;
-                    }
-                    if (!ovf && s0) {
-                        const int32_t fo = d.first_off[b];
-                        const int32_t fc = d.first_off[b + 1] - fo;
-                        if (nk + fc > kChainCap) {
-                            ovf = true;
-                        } else {
-                            for (int32_t jj = lane; jj < fc; jj += 64) B[nk + jj] = d.first_slot[fo + jj];
-                            nk += fc;
-                            hit |= d.first_accepts[b] != 0u;
-                        }
-                    }
-                    if (ovf) {
-                        stop = true;
-                        continue;
-                    }
-                    if (!d.start_always) s0 = s0 && ((d.selfmask[b >> 6] >> (b & 63)) & 1ull);
-                    wave_sync();
-                    int32_t *tmp = A;
-                    A = B;
-                    B = tmp;
-                    k = nk;
-                    if (__any(hit) && at >= cs) {
-                        found = static_cast<int32_t>(at - ib + 1);
-                        stop = true;
-                    } else if (k == 0 && !s0) {
-                        stop = true;
-                    }
-                }
-            }
-            if (!stop && ((w0 - abase) & 4095) == 3840) {
-                const int32_t bst = __atomic_load_n(&best[in], __ATOMIC_RELAXED);
-                if (bst != INT_MAX && ib + bst - 1 < w0 + 256) stop = true;
-            }
-        }
-        if (lane == 0) {
-            if (found > 0) atomicMin(&best[in], found);
-            if (ovf) overflow[in] = 1;
-        }
-    }
-}
-
-int32_t nfa_chain_chunk_bytes() {
-    const int64_t c = env_int("GFSM_CHAIN_CHUNK", 4096);
-    return static_cast<int32_t>(c < 64 ? 64 : (c > (1 << 30) ? (1 << 30) : c));
-}
-
-size_t nfa_chain_scratch_bytes(int32_t count) {
-    const size_t c = static_cast<size_t>(count);
-    return ((c * 4 + 255) & ~size_t(255)) * 2 + (((c + 1) * 8 + 255) & ~size_t(255)) + 256;
-}
-
-void launch_nfa_chain(const NfaChainDev &d, const NfaTileDev &tiles, const uint8_t *data,
-                      const int64_t *off, int32_t count, int32_t *flags, int32_t *lens,
-                      void *scratch, int32_t chunk_bytes, hipStream_t stream) {
-    if (count <= 0) return;
-    const size_t c = static_cast<size_t>(count);
-    char *base = static_cast<char *>(scratch);
-    int32_t *best = reinterpret_cast<int32_t *>(base);
-    base += (c * 4 + 255) & ~size_t(255);
-    int32_t *overflow = reinterpret_cast<int32_t *>(base);
-    base += (c * 4 + 255) & ~size_t(255);
-    int64_t *cbase = reinterpret_cast<int64_t *>(base);
-    base += ((c + 1) * 8 + 255) & ~size_t(255);
-    unsigned int *counter = reinterpret_cast<unsigned int *>(base);
-    // without an always-active start state, inputs are not cut
-    const int64_t chunk = d.start_always ? static_cast<int64_t>(chunk_bytes) : (int64_t(1) << 62);
-    chain_prep_kernel<<<(count + 255) / 256, 256, 0, stream>>>(off, count, chunk, best, overflow,
-                                                               cbase);
-    GFSM_HIP(hipGetLastError());
-    chain_scan_kernel<<<1, 1024, 0, stream>>>(cbase, count, counter);
-    GFSM_HIP(hipGetLastError());
-    int grid = device_cu_count() * 5;
-    nfa_chain_kernel<<<grid, 64 * kChainWaves, 0, stream>>>(d, data, off, count, cbase, counter,
-                                                           chunk, best, overflow);
-    GFSM_HIP(hipGetLastError());
-    bitpacked_core(tiles, data, off, count, best, overflow, stream);
+void launch_nfa_best_finish(const int32_t *best, int32_t count, int32_t *flags, int32_t *lens,
+                            hipStream_t stream) {
     bp_finish_kernel<<<(count + 255) / 256, 256, 0, stream>>>(best, count, flags, lens);
     GFSM_HIP(hipGetLastError());
 }

@@ -220,12 +220,15 @@ def _class_batch(rng, used, count, max_len, p_class=0.8, empty_frac=0.1):
     return data, off
 
 
+@pytest.mark.parametrize("mode", ["pairs", "first"])
 @pytest.mark.parametrize("self_loop", ["any", "all", "partial", "none"])
 @pytest.mark.parametrize("chunk", ["64", "333", "4096"])
-def test_chain_random_automata(self_loop, chunk, monkeypatch):
+def test_chain_random_automata(self_loop, chunk, mode, monkeypatch):
     """Shuffled ids, shared first positions, accepting inner states, every start self-loop
-    kind; chunked (wildcard start) and unchunked inputs, chunk sizes not multiples of 4."""
+    kind; chunked (wildcard start) and unchunked inputs, chunk sizes not multiples of 4;
+    second positions from byte-pair lists or first positions from per-byte lists."""
     monkeypatch.setenv("GFSM_CHAIN_CHUNK", chunk)
+    monkeypatch.setenv("GFSM_CHAIN_FIRST", "1" if mode == "first" else "0")
     rng = random.Random(f"{self_loop}-{chunk}")
     nrng = np.random.default_rng(rng.randrange(1 << 30))
     for trial in range(4):
@@ -236,11 +239,13 @@ def test_chain_random_automata(self_loop, chunk, monkeypatch):
         _check_nfa(nfa, data, off, "chain")
 
 
+@pytest.mark.parametrize("mode", ["pairs", "first"])
 @pytest.mark.parametrize("chunk", ["64", "1000"])
-def test_chain_long_inputs_match_positions(chunk, monkeypatch):
+def test_chain_long_inputs_match_positions(chunk, mode, monkeypatch):
     """Wildcard-start union with a few accepting patterns: first matches fall deep into
     64 KiB inputs, at every phase relative to the chunk boundaries."""
     monkeypatch.setenv("GFSM_CHAIN_CHUNK", chunk)
+    monkeypatch.setenv("GFSM_CHAIN_FIRST", "1" if mode == "first" else "0")
     pu = W.pattern_union(400, (3, 8), (4, 32), seed=8, accepting=False)
     last = np.cumsum(pu.lengths)
     pu.nfa.accept[last[:3]] = True
@@ -251,9 +256,11 @@ def test_chain_long_inputs_match_positions(chunk, monkeypatch):
     assert len(set((wn[wf.astype(bool)] % int(chunk)).tolist())) > 5
 
 
-def test_chain_overflow_falls_back_to_tiles():
+@pytest.mark.parametrize("mode", ["pairs", "first"])
+def test_chain_overflow_falls_back_to_tiles(mode, monkeypatch):
     """Active sets of more than 1024 slots: 40 chains of wildcard-like classes, each 40 long,
     so every input overflows the LDS list and is rerun by the tiled pass."""
+    monkeypatch.setenv("GFSM_CHAIN_FIRST", "1" if mode == "first" else "0")
     b = NFABuilder()
     s0 = b.add_state()
     b.set_start(s0)
