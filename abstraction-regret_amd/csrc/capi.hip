@@ -108,7 +108,9 @@ void check_batch_args(const void *data, const int64_t *offsets, int32_t count,
 int32_t resolve_nfa(const gfsm_nfa *h, int32_t technique) {
     switch (technique) {
         case GFSM_NFA_AUTO:
-            return h->chain_ok ? GFSM_NFA_BITPACKED : GFSM_NFA_WORKLIST;
+            // chain-structured automata: the sparse chain walk (its own tiers fall back to
+            // the tiled pass on inputs whose active sets outgrow the lists)
+            return h->chain_ok ? GFSM_NFA_CHAIN : GFSM_NFA_WORKLIST;
         case GFSM_NFA_WORKLIST:
             return technique;
         case GFSM_NFA_BITPACKED:
@@ -125,12 +127,29 @@ int32_t resolve_nfa(const gfsm_nfa *h, int32_t technique) {
     }
 }
 
-int32_t resolve_dfa(const gfsm_dfa *h, int32_t technique, int32_t count, int64_t total) {
+struct Tuning {
+    int32_t chunk = 0, segment = 0, flags = 0;
+};
+
+Tuning read_tuning(const gfsm_tuning *t) {
+    Tuning r;
+    if (t) {
+        if (t->chunk_bytes < 0 || t->segment_bytes < 0)
+            throw Error(GFSM_EINVAL, "tuning sizes must be >= 0");
+        r.chunk = t->chunk_bytes;
+        r.segment = t->segment_bytes;
+        r.flags = t->flags;
+    }
+    return r;
+}
+
+int32_t resolve_dfa(const gfsm_dfa *h, int32_t technique, int32_t count, int64_t total,
+                    const Tuning &tu = Tuning()) {
     switch (technique) {
         case GFSM_DFA_AUTO: {
             // Segment the batch when inputs span several segments: one lane per input
             // would serialise each of them into one long dependent chain.
-            const DfaSplitPlan plan = dfa_split_plan(h->dev, total, count);
+            const DfaSplitPlan plan = dfa_split_plan(h->dev, total, count, tu.segment);
             const int64_t avg = count > 0 ? total / count : 0;
             if (avg >= 4 * plan.seg_bytes) return GFSM_DFA_SPLIT;
             return h->lds_ok ? GFSM_DFA_LDS : GFSM_DFA_GATHER;
@@ -147,55 +166,93 @@ int32_t resolve_dfa(const gfsm_dfa *h, int32_t technique, int32_t count, int64_t
     }
 }
 
-void nfa_enqueue(gfsm_nfa *h, int32_t tech, const uint8_t *d_data, const int64_t *d_off,
-                 int32_t count, int32_t *d_flags, int32_t *d_lens, hipStream_t s) {
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// Device scratch of one NFA run (resolved technique).
+size_t nfa_workspace(const gfsm_nfa *h, int32_t tech, int32_t count) {
+    if (count <= 0 || h->start_accepts) return 0;
+    if (tech == GFSM_NFA_CHAIN) return nfa_chain_scratch_bytes(count);
+    if (tech == GFSM_NFA_BITPACKED) return align256(static_cast<size_t>(count) * sizeof(int32_t));
+    int32_t grid = 1;
+    int64_t scratch_elems = 0;
+    nfa_worklist_geometry(h->wl, count, &grid, &scratch_elems);
+    return align256(static_cast<size_t>(scratch_elems) * sizeof(int32_t));
+}
+
+// Enqueue one NFA run; ws == nullptr allocates the scratch stream-ordered.
+void nfa_enqueue(gfsm_nfa *h, int32_t tech, const Tuning &tu, const uint8_t *d_data,
+                 const int64_t *d_off, int32_t count, int32_t *d_flags, int32_t *d_lens,
+                 void *ws, size_t ws_bytes, hipStream_t s) {
     if (count == 0) return;
     if (h->start_accepts) {
         launch_fill_result(d_flags, d_lens, count, 1, 0, s);
         return;
     }
+    const size_t need = nfa_workspace(h, tech, count);
+    if (!ws && need > 0) {
+        StreamBuf buf(need, s);
+        nfa_enqueue(h, tech, tu, d_data, d_off, count, d_flags, d_lens, buf.p, need, s);
+        return;
+    }
+    if (ws_bytes < need)
+        throw Error(GFSM_EINVAL, "workspace of " + std::to_string(ws_bytes) + " bytes, " +
+                                     std::to_string(need) + " needed");
     if (tech == GFSM_NFA_CHAIN) {
-        StreamBuf scratch(nfa_chain_scratch_bytes(count), s);
-        launch_nfa_chain(h->chain, h->tile, d_data, d_off, count, d_flags, d_lens, scratch.p,
-                         nfa_chain_chunk_bytes(), s);
+        const bool pairs = !(tu.flags & GFSM_TUNE_CHAIN_FIRST) && env_int("GFSM_CHAIN_FIRST", 0) != 1;
+        launch_nfa_chain(h->chain, h->tile, d_data, d_off, count, d_flags, d_lens, ws,
+                         nfa_chain_chunk_bytes(tu.chunk), pairs, s);
         return;
     }
     if (tech == GFSM_NFA_BITPACKED) {
-        StreamBuf best(static_cast<size_t>(count) * sizeof(int32_t), s);
         launch_nfa_bitpacked(h->tile, d_data, d_off, count, d_flags, d_lens,
-                             static_cast<int32_t *>(best.p), s);
+                             static_cast<int32_t *>(ws), s);
         return;
     }
     int32_t grid = 1;
     int64_t scratch_elems = 0;
     nfa_worklist_geometry(h->wl, count, &grid, &scratch_elems);
-    if (scratch_elems > 0) {
-        StreamBuf scratch(static_cast<size_t>(scratch_elems) * sizeof(int32_t), s);
-        launch_nfa_worklist(h->wl, d_data, d_off, count, d_flags, d_lens,
-                            static_cast<int32_t *>(scratch.p), grid, s);
-    } else {
-        launch_nfa_worklist(h->wl, d_data, d_off, count, d_flags, d_lens, nullptr, grid, s);
-    }
+    launch_nfa_worklist(h->wl, d_data, d_off, count, d_flags, d_lens,
+                        scratch_elems > 0 ? static_cast<int32_t *>(ws) : nullptr, grid, s);
 }
 
-void dfa_enqueue(gfsm_dfa *h, int32_t tech, const uint8_t *d_data, const int64_t *d_off,
-                 int32_t count, int64_t total, int32_t *d_flags, int32_t *d_lens,
-                 hipStream_t s) {
+size_t dfa_workspace(const gfsm_dfa *h, int32_t tech, int32_t count, int64_t total,
+                     const Tuning &tu) {
+    if (count <= 0 || h->start_accepts || tech != GFSM_DFA_SPLIT) return 0;
+    return dfa_split_plan(h->dev, total, count, tu.segment).scratch_bytes;
+}
+
+void dfa_enqueue(gfsm_dfa *h, int32_t tech, const Tuning &tu, const uint8_t *d_data,
+                 const int64_t *d_off, int32_t count, int64_t total, int32_t *d_flags,
+                 int32_t *d_lens, void *ws, size_t ws_bytes, hipStream_t s) {
     if (count == 0) return;
     if (h->start_accepts) {
         launch_fill_result(d_flags, d_lens, count, 1, 0, s);
         return;
     }
     if (tech == GFSM_DFA_SPLIT) {
-        const DfaSplitPlan plan = dfa_split_plan(h->dev, total, count);
-        StreamBuf scratch(plan.scratch_bytes, s);
-        launch_dfa_split(h->dev, plan, d_data, d_off, count, total, d_flags, d_lens, scratch.p, s);
+        const DfaSplitPlan plan = dfa_split_plan(h->dev, total, count, tu.segment);
+        if (!ws) {
+            StreamBuf buf(plan.scratch_bytes, s);
+            launch_dfa_split(h->dev, plan, d_data, d_off, count, total, d_flags, d_lens, buf.p, s);
+            return;
+        }
+        if (ws_bytes < plan.scratch_bytes)
+            throw Error(GFSM_EINVAL, "workspace of " + std::to_string(ws_bytes) + " bytes, " +
+                                         std::to_string(plan.scratch_bytes) + " needed");
+        launch_dfa_split(h->dev, plan, d_data, d_off, count, total, d_flags, d_lens, ws, s);
         return;
     }
     if (tech == GFSM_DFA_LDS)
         launch_dfa_lds(h->dev, d_data, d_off, count, d_flags, d_lens, s);
     else
         launch_dfa_gather(h->dev, d_data, d_off, count, d_flags, d_lens, s);
+}
+
+void check_ws(const void *ws, int64_t ws_bytes) {
+    if (ws_bytes < 0) throw Error(GFSM_EINVAL, "workspace_bytes must be >= 0");
+    if (ws_bytes > 0 && !ws) throw Error(GFSM_EINVAL, "null workspace");
+    if (reinterpret_cast<uintptr_t>(ws) & 255u)
+        throw Error(GFSM_EINVAL, "workspace must be 256-byte aligned");
 }
 
 // Host form shared by NFA and DFA: stage the batch, time the launch with events, copy back.
@@ -404,7 +461,9 @@ int gfsm_nfa_run_host(gfsm_nfa *nfa, int32_t technique, const uint8_t *data,
         const int32_t tech = resolve_nfa(nfa, technique);
         run_host(data, offsets, num_strings, flags, lens, kernel_ms,
                  [&](const uint8_t *dd, const int64_t *doff, int64_t, int32_t *df, int32_t *dl,
-                     hipStream_t s) { nfa_enqueue(nfa, tech, dd, doff, num_strings, df, dl, s); });
+                     hipStream_t s) {
+                     nfa_enqueue(nfa, tech, Tuning(), dd, doff, num_strings, df, dl, nullptr, 0, s);
+                 });
     });
 }
 
@@ -416,8 +475,51 @@ int gfsm_nfa_run_device(gfsm_nfa *nfa, int32_t technique, const uint8_t *d_data,
         check_batch_args(d_data, d_offsets, num_strings, d_flags, d_lens);
         if (total_bytes < 0) throw Error(GFSM_EINVAL, "total_bytes must be >= 0");
         const int32_t tech = resolve_nfa(nfa, technique);
-        nfa_enqueue(nfa, tech, d_data, d_offsets, num_strings, d_flags, d_lens,
+        nfa_enqueue(nfa, tech, Tuning(), d_data, d_offsets, num_strings, d_flags, d_lens,
+                    nullptr, 0, static_cast<hipStream_t>(stream));
+    });
+}
+
+int gfsm_nfa_workspace_size(gfsm_nfa *nfa, int32_t technique, int32_t num_strings,
+                            int64_t total_bytes, const gfsm_tuning *tuning,
+                            int64_t *out_bytes) {
+    return guarded([&] {
+        if (!nfa || !out_bytes) throw Error(GFSM_EINVAL, "null argument");
+        if (num_strings < 0 || total_bytes < 0) throw Error(GFSM_EINVAL, "negative size");
+        (void)read_tuning(tuning);
+        *out_bytes = static_cast<int64_t>(nfa_workspace(nfa, resolve_nfa(nfa, technique), num_strings));
+    });
+}
+
+int gfsm_nfa_run_device_ws(gfsm_nfa *nfa, int32_t technique, const gfsm_tuning *tuning,
+                           const uint8_t *d_data, const int64_t *d_offsets,
+                           int32_t num_strings, int64_t total_bytes, int32_t *d_flags,
+                           int32_t *d_lens, void *d_workspace, int64_t workspace_bytes,
+                           void *stream) {
+    return guarded([&] {
+        if (!nfa) throw Error(GFSM_EINVAL, "null handle");
+        check_batch_args(d_data, d_offsets, num_strings, d_flags, d_lens);
+        if (total_bytes < 0) throw Error(GFSM_EINVAL, "total_bytes must be >= 0");
+        check_ws(d_workspace, workspace_bytes);
+        const Tuning tu = read_tuning(tuning);
+        const int32_t tech = resolve_nfa(nfa, technique);
+        const size_t need = nfa_workspace(nfa, tech, num_strings);
+        if (static_cast<size_t>(workspace_bytes) < need)
+            throw Error(GFSM_EINVAL, "workspace of " + std::to_string(workspace_bytes) +
+                                         " bytes, " + std::to_string(need) + " needed");
+        nfa_enqueue(nfa, tech, tu, d_data, d_offsets, num_strings, d_flags, d_lens,
+                    need > 0 ? d_workspace : nullptr, static_cast<size_t>(workspace_bytes),
                     static_cast<hipStream_t>(stream));
+    });
+}
+
+int gfsm_nfa_resolve(gfsm_nfa *nfa, int32_t technique, int32_t num_strings,
+                     int64_t total_bytes, int32_t *out_technique) {
+    return guarded([&] {
+        if (!nfa || !out_technique) throw Error(GFSM_EINVAL, "null argument");
+        (void)num_strings;
+        (void)total_bytes;
+        *out_technique = resolve_nfa(nfa, technique);
     });
 }
 
@@ -483,7 +585,8 @@ int gfsm_dfa_run_host(gfsm_dfa *dfa, int32_t technique, const uint8_t *data,
         run_host(data, offsets, num_strings, flags, lens, kernel_ms,
                  [&](const uint8_t *dd, const int64_t *doff, int64_t tot, int32_t *df,
                      int32_t *dl, hipStream_t s) {
-                     dfa_enqueue(dfa, tech, dd, doff, num_strings, tot, df, dl, s);
+                     dfa_enqueue(dfa, tech, Tuning(), dd, doff, num_strings, tot, df, dl,
+                                 nullptr, 0, s);
                  });
     });
 }
@@ -496,8 +599,51 @@ int gfsm_dfa_run_device(gfsm_dfa *dfa, int32_t technique, const uint8_t *d_data,
         check_batch_args(d_data, d_offsets, num_strings, d_flags, d_lens);
         if (total_bytes < 0) throw Error(GFSM_EINVAL, "total_bytes must be >= 0");
         const int32_t tech = resolve_dfa(dfa, technique, num_strings, total_bytes);
-        dfa_enqueue(dfa, tech, d_data, d_offsets, num_strings, total_bytes, d_flags, d_lens,
+        dfa_enqueue(dfa, tech, Tuning(), d_data, d_offsets, num_strings, total_bytes, d_flags,
+                    d_lens, nullptr, 0, static_cast<hipStream_t>(stream));
+    });
+}
+
+int gfsm_dfa_workspace_size(gfsm_dfa *dfa, int32_t technique, int32_t num_strings,
+                            int64_t total_bytes, const gfsm_tuning *tuning,
+                            int64_t *out_bytes) {
+    return guarded([&] {
+        if (!dfa || !out_bytes) throw Error(GFSM_EINVAL, "null argument");
+        if (num_strings < 0 || total_bytes < 0) throw Error(GFSM_EINVAL, "negative size");
+        const Tuning tu = read_tuning(tuning);
+        const int32_t tech = resolve_dfa(dfa, technique, num_strings, total_bytes, tu);
+        *out_bytes = static_cast<int64_t>(dfa_workspace(dfa, tech, num_strings, total_bytes, tu));
+    });
+}
+
+int gfsm_dfa_run_device_ws(gfsm_dfa *dfa, int32_t technique, const gfsm_tuning *tuning,
+                           const uint8_t *d_data, const int64_t *d_offsets,
+                           int32_t num_strings, int64_t total_bytes, int32_t *d_flags,
+                           int32_t *d_lens, void *d_workspace, int64_t workspace_bytes,
+                           void *stream) {
+    return guarded([&] {
+        if (!dfa) throw Error(GFSM_EINVAL, "null handle");
+        check_batch_args(d_data, d_offsets, num_strings, d_flags, d_lens);
+        if (total_bytes < 0) throw Error(GFSM_EINVAL, "total_bytes must be >= 0");
+        check_ws(d_workspace, workspace_bytes);
+        const Tuning tu = read_tuning(tuning);
+        const int32_t tech = resolve_dfa(dfa, technique, num_strings, total_bytes, tu);
+        const size_t need = dfa_workspace(dfa, tech, num_strings, total_bytes, tu);
+        if (static_cast<size_t>(workspace_bytes) < need)
+            throw Error(GFSM_EINVAL, "workspace of " + std::to_string(workspace_bytes) +
+                                         " bytes, " + std::to_string(need) + " needed");
+        dfa_enqueue(dfa, tech, tu, d_data, d_offsets, num_strings, total_bytes, d_flags, d_lens,
+                    d_workspace, static_cast<size_t>(workspace_bytes),
                     static_cast<hipStream_t>(stream));
+    });
+}
+
+int gfsm_dfa_resolve(gfsm_dfa *dfa, int32_t technique, int32_t num_strings,
+                     int64_t total_bytes, int32_t *out_technique) {
+    return guarded([&] {
+        if (!dfa || !out_technique) throw Error(GFSM_EINVAL, "null argument");
+        if (num_strings < 0 || total_bytes < 0) throw Error(GFSM_EINVAL, "negative size");
+        *out_technique = resolve_dfa(dfa, technique, num_strings, total_bytes);
     });
 }
 

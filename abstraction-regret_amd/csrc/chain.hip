@@ -403,8 +403,8 @@ size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 }  // namespace
 
-int32_t nfa_chain_chunk_bytes() {
-    const int64_t c = env_int("GFSM_CHAIN_CHUNK", 4096);
+int32_t nfa_chain_chunk_bytes(int32_t requested) {
+    const int64_t c = requested > 0 ? requested : env_int("GFSM_CHAIN_CHUNK", 4096);
     return static_cast<int32_t>(c < 64 ? 64 : (c > (1 << 30) ? (1 << 30) : c));
 }
 
@@ -415,7 +415,7 @@ size_t nfa_chain_scratch_bytes(int32_t count) {
 
 void launch_nfa_chain(const NfaChainDev &d, const NfaTileDev &tiles, const uint8_t *data,
                       const int64_t *off, int32_t count, int32_t *flags, int32_t *lens,
-                      void *scratch, int32_t chunk_bytes, hipStream_t stream) {
+                      void *scratch, int32_t chunk_bytes, bool pairs, hipStream_t stream) {
     if (count <= 0) return;
     const size_t c = static_cast<size_t>(count);
     char *p = static_cast<char *>(scratch);
@@ -442,8 +442,7 @@ void launch_nfa_chain(const NfaChainDev &d, const NfaTileDev &tiles, const uint8
         reinterpret_cast<const void *>(&nfa_chain_lds_kernel<true, 1024>), threads);
     static const int grid_lds_first = persistent_grid(
         reinterpret_cast<const void *>(&nfa_chain_lds_kernel<false, 1024>), threads);
-    // GFSM_CHAIN_FIRST=1 skips the pair lists (first-position lists in the lds tier only)
-    if (d.pair && env_int("GFSM_CHAIN_FIRST", 0) != 1) {
+    if (d.pair && pairs) {
         nfa_chain_reg_kernel<<<grid_reg, threads, 0, stream>>>(d, data, off, count, sc, chunk);
         GFSM_HIP(hipGetLastError());
         nfa_chain_lds_kernel<true, 1024><<<grid_lds_pairs, threads, 0, stream>>>(

@@ -739,18 +739,21 @@ int64_t env_int(const char *name, int64_t fallback) {
     return (end && *end == '\0' && x > 0) ? x : fallback;
 }
 
-DfaSplitPlan dfa_split_plan(const DfaDev &d, int64_t total_bytes, int32_t count) {
+DfaSplitPlan dfa_split_plan(const DfaDev &d, int64_t total_bytes, int32_t count,
+                            int64_t seg_bytes) {
     DfaSplitPlan p;
     p.chains = static_cast<int32_t>(env_int("GFSM_SPLIT_CHAINS", 1));
     if (p.chains != 1 && p.chains != 2 && p.chains != 4) p.chains = 1;
+    const bool valid = seg_bytes >= 256 && seg_bytes <= (1 << 20) && !(seg_bytes & (seg_bytes - 1));
     if (lds_fits(d)) {
-        int64_t c = env_int("GFSM_SPLIT_SEGMENT", 2048);
-        if (c < 256 || (c & (c - 1))) c = 2048;
+        int64_t c = valid ? seg_bytes : env_int("GFSM_SPLIT_SEGMENT", 2048);
+        if (c < 256 || c > (1 << 20) || (c & (c - 1))) c = 2048;
         p.seg_bytes = c;
         p.ckpt_shift = 6;
     } else {
         int64_t c = 1024;
         while (c < 4ll * d.n && c < 65536) c <<= 1;
+        if (valid) c = seg_bytes;
         p.seg_bytes = c;
         p.ckpt_shift = 6;
         while ((1ll << p.ckpt_shift) * 16 < c) ++p.ckpt_shift;

@@ -118,7 +118,9 @@ struct DfaSplitPlan {
     int32_t chains = 1;        // interleaved chains per lane
     size_t scratch_bytes = 0;  // device scratch the launch needs
 };
-DfaSplitPlan dfa_split_plan(const DfaDev &d, int64_t total_bytes, int32_t count);
+// seg_bytes: requested segment length (a power of two in [256, 1 MiB]); 0 = default
+DfaSplitPlan dfa_split_plan(const DfaDev &d, int64_t total_bytes, int32_t count,
+                            int64_t seg_bytes = 0);
 void launch_dfa_split(const DfaDev &d, const DfaSplitPlan &plan, const uint8_t *data,
                       const int64_t *off, int32_t count, int64_t total_bytes, int32_t *flags,
                       int32_t *lens, void *scratch, hipStream_t stream);
@@ -147,9 +149,11 @@ void launch_nfa_best_finish(const int32_t *best, int32_t count, int32_t *flags, 
 
 // Sparse chain technique: persistent waves over (input, chunk) work items.
 size_t nfa_chain_scratch_bytes(int32_t count);
-int32_t nfa_chain_chunk_bytes();  // GFSM_CHAIN_CHUNK (read per call), default 4096
+// chunk_bytes when the caller has no preference: GFSM_CHAIN_CHUNK (read per call), or 4096
+int32_t nfa_chain_chunk_bytes(int32_t requested);
+// pairs: use the byte-pair lists (register tier first) when the handle has them.
 void launch_nfa_chain(const NfaChainDev &d, const NfaTileDev &tiles, const uint8_t *data,
                       const int64_t *off, int32_t count, int32_t *flags, int32_t *lens,
-                      void *scratch, int32_t chunk_bytes, hipStream_t stream);
+                      void *scratch, int32_t chunk_bytes, bool pairs, hipStream_t stream);
 
 }  // namespace gfsm

@@ -4,10 +4,10 @@ Registers ``Backend.HIP`` techniques with the unchanged registry
 (`src/gpufsm/core/registry.py:62-117`), the way the upstream CUDA backend registers its own
 (`src/gpufsm/backends/cuda/__init__.py:19-28`):
 
-* NFA: ``auto`` (default; ``bitpacked`` when the automaton is chain-structured, else
-  ``worklist``), ``worklist``, ``bitpacked``;
-* DFA: ``auto`` (default; ``lds`` for tables of at most 320 states, else ``gather``),
-  ``gather``, ``lds``.
+* NFA: ``auto`` (default; ``chain`` when the automaton is chain-structured, else
+  ``worklist``), ``worklist``, ``bitpacked``, ``chain``;
+* DFA: ``auto`` (default; ``split`` when inputs are long, else ``lds`` for tables of at
+  most 320 states, else ``gather``), ``gather``, ``lds``, ``split``.
 
 Techniques are registered whenever the library loads. The availability probe, registered
 unconditionally, additionally needs a visible gfx950 device; when it says no, the reason is

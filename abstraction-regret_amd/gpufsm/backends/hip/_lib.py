@@ -8,6 +8,7 @@ fallback to any other implementation.
 from __future__ import annotations
 
 import ctypes
+import importlib.util
 import os
 
 LIB_NAME = "libgpufsm_hip.so"
@@ -42,7 +43,15 @@ EXPORTS = (
     "gfsm_dfa_query",
     "gfsm_dfa_run_host",
     "gfsm_dfa_run_device",
+    "gfsm_nfa_workspace_size",
+    "gfsm_nfa_run_device_ws",
+    "gfsm_nfa_resolve",
+    "gfsm_dfa_workspace_size",
+    "gfsm_dfa_run_device_ws",
+    "gfsm_dfa_resolve",
 )
+
+TUNE_CHAIN_FIRST = 1
 
 
 class HIPUnavailable(RuntimeError):
@@ -85,6 +94,17 @@ class DfaProps(ctypes.Structure):
     ]
 
 
+class Tuning(ctypes.Structure):
+    """``gfsm_tuning``: zero fields mean the library's choice."""
+
+    _fields_ = [
+        ("chunk_bytes", ctypes.c_int32),
+        ("segment_bytes", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
 _c = ctypes
 _vp = _c.c_void_p
 _i32 = _c.c_int32
@@ -110,10 +130,45 @@ _SIGNATURES = {
     "gfsm_dfa_query": (_c.c_int, [_vp, _c.POINTER(DfaProps)]),
     "gfsm_dfa_run_host": (_c.c_int, [_vp, _i32, _u8p, _i64p, _i32, _i32p, _i32p, _fp]),
     "gfsm_dfa_run_device": (_c.c_int, [_vp, _i32, _vp, _vp, _i32, _c.c_int64, _vp, _vp, _vp]),
+    "gfsm_nfa_workspace_size": (_c.c_int, [_vp, _i32, _i32, _c.c_int64, _c.POINTER(Tuning),
+                                           _i64p]),
+    "gfsm_nfa_run_device_ws": (_c.c_int, [_vp, _i32, _c.POINTER(Tuning), _vp, _vp, _i32,
+                                          _c.c_int64, _vp, _vp, _vp, _c.c_int64, _vp]),
+    "gfsm_nfa_resolve": (_c.c_int, [_vp, _i32, _i32, _c.c_int64, _i32p]),
+    "gfsm_dfa_workspace_size": (_c.c_int, [_vp, _i32, _i32, _c.c_int64, _c.POINTER(Tuning),
+                                           _i64p]),
+    "gfsm_dfa_run_device_ws": (_c.c_int, [_vp, _i32, _c.POINTER(Tuning), _vp, _vp, _i32,
+                                          _c.c_int64, _vp, _vp, _vp, _c.c_int64, _vp]),
+    "gfsm_dfa_resolve": (_c.c_int, [_vp, _i32, _i32, _c.c_int64, _i32p]),
 }
 
 _lib: ctypes.CDLL | None = None
 _load_error: str | None = None
+
+
+def _share_torch_hip_runtime() -> None:
+    """Make this library use the same HIP runtime as PyTorch, if PyTorch-ROCm is installed.
+
+    PyTorch wheels carry their own ``libamdhip64.so`` (same soname as the system one) and
+    reach it by file name through their RPATH. Loaded in the other order, a process would
+    hold two HIP runtimes, and the second to initialise finds no device. Loading PyTorch's
+    copy first (without importing torch) lets the dynamic loader bind this library to it by
+    soname, and lets torch reuse the same file later. ``GFSM_HIP_RUNTIME=system`` skips this.
+    """
+    if os.environ.get("GFSM_HIP_RUNTIME") == "system":
+        return
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        return
+    if spec is None or not spec.origin:
+        return
+    candidate = os.path.join(os.path.dirname(spec.origin), "lib", "libamdhip64.so")
+    if os.path.exists(candidate):
+        try:
+            ctypes.CDLL(candidate, mode=ctypes.RTLD_GLOBAL)
+        except OSError:
+            pass
 
 
 def load() -> ctypes.CDLL:
@@ -123,6 +178,7 @@ def load() -> ctypes.CDLL:
         return _lib
     if _load_error is not None:
         raise HIPUnavailable(_load_error)
+    _share_torch_hip_runtime()
     if not os.path.exists(LIB_PATH):
         _load_error = (
             f"{LIB_NAME} is not built ({LIB_PATH} missing); run "

@@ -27,10 +27,10 @@ class HIPNFAExecutor:
         self.technique = technique
         self._tech = _lib.NFA_TECHNIQUES[technique]
         self._dev = DeviceNFA(nfa)
-        if technique == "bitpacked" and not self._dev.props.chain_ok:
+        if technique in ("bitpacked", "chain") and not self._dev.props.chain_ok:
             self._dev.close()
             raise ValueError(
-                "HIP technique 'bitpacked' needs a chain-structured NFA (pattern union: no "
+                f"HIP technique {technique!r} needs a chain-structured NFA (pattern union: no "
                 "epsilon edges, one predecessor and successor per non-start state, "
                 "homogeneous labels); use 'worklist' or 'auto'")
 

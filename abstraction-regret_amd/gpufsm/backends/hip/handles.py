@@ -46,11 +46,48 @@ def analyze_nfa(nfa: NFA) -> NfaProps:
     return props
 
 
+def make_tuning(chunk_bytes: int = 0, segment_bytes: int = 0, flags: int = 0):
+    """A ``gfsm_tuning`` (None when every field is the library's choice)."""
+    if not (chunk_bytes or segment_bytes or flags):
+        return None
+    return _lib.Tuning(int(chunk_bytes), int(segment_bytes), int(flags), 0)
+
+
 class _Handle:
     _destroy = ""
+    _prefix = ""
 
     def __init__(self) -> None:
         self._h = ctypes.c_void_p()
+
+    def _fn(self, name: str):
+        return getattr(_lib.load(), f"{self._prefix}_{name}")
+
+    def resolve(self, count: int, total_bytes: int, technique: int = 0) -> int:
+        """The concrete technique id ``technique`` (auto = 0) resolves to for this shape."""
+        out = ctypes.c_int32(0)
+        check(self._fn("resolve")(self.handle, technique, int(count), int(total_bytes),
+                                  ctypes.byref(out)))
+        return int(out.value)
+
+    def workspace_size(self, count: int, total_bytes: int, technique: int = 0,
+                       tuning=None) -> int:
+        """Device scratch bytes :meth:`run_device_ws` needs for this batch shape."""
+        out = ctypes.c_int64(0)
+        check(self._fn("workspace_size")(
+            self.handle, technique, int(count), int(total_bytes),
+            ctypes.byref(tuning) if tuning is not None else None, ctypes.byref(out)))
+        return int(out.value)
+
+    def run_device_ws(self, d_data: int, d_offsets: int, count: int, total_bytes: int,
+                      d_flags: int, d_lens: int, d_ws: int, ws_bytes: int, stream: int = 0,
+                      technique: int = 0, tuning=None) -> None:
+        """Enqueue on ``stream`` with caller scratch: no allocation, graph-capturable."""
+        check(self._fn("run_device_ws")(
+            self.handle, technique, ctypes.byref(tuning) if tuning is not None else None,
+            ctypes.c_void_p(d_data), ctypes.c_void_p(d_offsets), int(count),
+            int(total_bytes), ctypes.c_void_p(d_flags), ctypes.c_void_p(d_lens),
+            ctypes.c_void_p(d_ws), int(ws_bytes), ctypes.c_void_p(stream)))
 
     @property
     def handle(self) -> ctypes.c_void_p:
@@ -80,6 +117,7 @@ class DeviceNFA(_Handle):
     """An :class:`~gpufsm.core.nfa.NFA` resident in HBM in both device layouts."""
 
     _destroy = "gfsm_nfa_destroy"
+    _prefix = "gfsm_nfa"
 
     def __init__(self, nfa: NFA) -> None:
         super().__init__()
@@ -119,6 +157,7 @@ class DeviceDFA(_Handle):
     """A :class:`~gpufsm.core.dfa.DFA` resident in HBM (uint16 or uint32 entries)."""
 
     _destroy = "gfsm_dfa_destroy"
+    _prefix = "gfsm_dfa"
 
     def __init__(self, dfa: DFA) -> None:
         super().__init__()

@@ -1,0 +1,281 @@
+"""Batched, device-resident ``run_batch``: the resident form of the hot path.
+
+The plain ``gpufsm.run_batch`` builds a fresh executor per call (as upstream does,
+`src/gpufsm/api.py:48-49`): automaton preprocessing and upload, packing, host-to-device
+copy, one launch, copy back. :class:`BatchEngine` keeps what repeats resident instead:
+
+* automata stay in HBM in a :class:`ResidentCache` keyed by automaton identity (``NFA`` and
+  ``DFA`` hash by identity, `src/gpufsm/core/nfa.py:43-54`), least recently used first out
+  under a byte budget;
+* a :class:`DeviceBatch` holds packed inputs (uint8 bytes, int64 offsets), outputs and the
+  launch workspace in HBM; inputs are staged through pinned host memory with asynchronous
+  copies;
+* the technique and its tuning come from the :class:`~gpufsm.tune.Autotuner`;
+* a launch is the persistent ragged kernel sequence of the chosen technique enqueued with
+  caller workspace (no allocation, no memset), so it can be captured once into a HIP graph
+  and replayed (``graph=True``).
+
+PyTorch provides device memory, pinned host memory, streams, events and graph capture; the
+work itself is the C ABI of ``libgpufsm_hip.so``.
+"""
+
+from __future__ import annotations
+
+import collections
+import threading
+import time
+
+import numpy as np
+
+from .backends.hip import _lib
+from .backends.hip.handles import DeviceDFA, DeviceNFA
+from .core.packing import pack_bytes
+from .core.registry import Kind
+from .core.result import Result, batch_results
+from .tune import Autotuner, Choice
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise _lib.HIPUnavailable("BatchEngine needs a visible GPU (torch.cuda.is_available())")
+    return torch
+
+
+class ResidentCache:
+    """Device-resident automaton handles keyed by identity, LRU under ``budget_bytes``."""
+
+    def __init__(self, budget_bytes: int | None = None) -> None:
+        self.budget_bytes = budget_bytes
+        self._items: collections.OrderedDict[int, tuple[object, object]] = collections.OrderedDict()
+        self._lock = threading.Lock()
+        self.uploads = 0
+
+    @property
+    def bytes(self) -> int:
+        return sum(h.props.device_bytes for _, h in self._items.values())
+
+    def get(self, automaton):
+        key = id(automaton)
+        with self._lock:
+            hit = self._items.get(key)
+            if hit is not None and hit[0] is automaton:
+                self._items.move_to_end(key)
+                return hit[1]
+        kind = Kind.of(automaton)
+        handle = DeviceNFA(automaton) if kind is Kind.NFA else DeviceDFA(automaton)
+        with self._lock:
+            self.uploads += 1
+            self._items[key] = (automaton, handle)  # holds the automaton: id stays unique
+            if self.budget_bytes is not None:
+                while len(self._items) > 1 and self.bytes > self.budget_bytes:
+                    _, (_, old) = self._items.popitem(last=False)
+                    old.close()
+        return handle
+
+    def close(self) -> None:
+        with self._lock:
+            for _, h in self._items.values():
+                h.close()
+            self._items.clear()
+
+
+class DeviceBatch:
+    """Packed inputs, outputs and launch workspace resident in HBM.
+
+    ``count`` inputs at ``data[offsets[i]:offsets[i+1]]``. The buffers keep their addresses
+    for the life of the batch (a graph captured on them stays valid); :meth:`load` replaces
+    the contents with another batch that fits the capacity.
+    """
+
+    def __init__(self, data: np.ndarray, offsets: np.ndarray, *, capacity_bytes: int = 0,
+                 capacity_count: int = 0, stream=None) -> None:
+        torch = _torch()
+        self._torch = torch
+        count = int(offsets.size) - 1
+        total = int(offsets[-1]) if count >= 0 else 0
+        self.cap_bytes = max(int(capacity_bytes), total, 1)
+        self.cap_count = max(int(capacity_count), count, 1)
+        dev = torch.device("cuda")
+        self.data = torch.empty(self.cap_bytes, dtype=torch.uint8, device=dev)
+        self.offsets = torch.zeros(self.cap_count + 1, dtype=torch.int64, device=dev)
+        self.flags = torch.zeros(self.cap_count, dtype=torch.int32, device=dev)
+        self.lens = torch.zeros(self.cap_count, dtype=torch.int32, device=dev)
+        self._h_data = torch.empty(self.cap_bytes, dtype=torch.uint8, pin_memory=True)
+        self._h_off = torch.empty(self.cap_count + 1, dtype=torch.int64, pin_memory=True)
+        self._h_out = torch.empty(2 * self.cap_count, dtype=torch.int32, pin_memory=True)
+        self.workspace = None
+        self.count = 0
+        self.total = 0
+        self.load(data, offsets, stream=stream)
+
+    def load(self, data: np.ndarray, offsets: np.ndarray, stream=None) -> None:
+        """Stage a packed batch (asynchronously on ``stream``) into the resident buffers."""
+        torch = self._torch
+        count = int(offsets.size) - 1
+        total = int(offsets[-1])
+        if count > self.cap_count or total > self.cap_bytes:
+            raise ValueError(f"batch of {count} inputs / {total} bytes exceeds the capacity "
+                             f"{self.cap_count} / {self.cap_bytes}")
+        stream = stream or torch.cuda.current_stream()
+        self._h_data[:total].numpy()[:] = np.asarray(data[:total], dtype=np.uint8)
+        self._h_off[: count + 1].numpy()[:] = np.asarray(offsets, dtype=np.int64)
+        with torch.cuda.stream(stream):
+            self.data[:total].copy_(self._h_data[:total], non_blocking=True)
+            self.offsets[: count + 1].copy_(self._h_off[: count + 1], non_blocking=True)
+        self.count, self.total = count, total
+
+    def ensure_workspace(self, nbytes: int) -> int:
+        """Device pointer of a workspace of at least ``nbytes`` (256-byte aligned)."""
+        torch = self._torch
+        if self.workspace is None or self.workspace.numel() < nbytes:
+            self.workspace = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device="cuda")
+        return self.workspace.data_ptr()
+
+    def fetch(self, stream=None) -> tuple[np.ndarray, np.ndarray]:
+        """Copy the verdicts back (waits for ``stream``)."""
+        torch = self._torch
+        stream = stream or torch.cuda.current_stream()
+        c = self.count
+        with torch.cuda.stream(stream):
+            self._h_out[:c].copy_(self.flags[:c], non_blocking=True)
+            self._h_out[self.cap_count:self.cap_count + c].copy_(self.lens[:c], non_blocking=True)
+        stream.synchronize()
+        out = self._h_out.numpy()
+        return out[:c].copy(), out[self.cap_count:self.cap_count + c].copy()
+
+
+def _techniques(kind: Kind) -> dict[str, int]:
+    return _lib.NFA_TECHNIQUES if kind is Kind.NFA else _lib.DFA_TECHNIQUES
+
+
+class BatchEngine:
+    """Resident automata + resident batches + autotuned, optionally graph-captured launches."""
+
+    def __init__(self, *, budget_bytes: int | None = None, tuner: Autotuner | None | bool = True,
+                 arch: str | None = None) -> None:
+        self._torch = _torch()
+        self.cache = ResidentCache(budget_bytes)
+        if tuner is True:
+            tuner = Autotuner()
+        self.tuner = tuner or None
+        self.arch = arch or _lib.device_arch(0)
+        self._graphs: dict[tuple, object] = {}
+
+    # -- technique selection ------------------------------------------------------------
+    def choose(self, automaton, batch: DeviceBatch, technique: str = "auto",
+               stream=None) -> Choice:
+        """The technique + tuning a launch of ``batch`` uses (``technique`` unless "auto")."""
+        if technique != "auto" or self.tuner is None or batch.count == 0:
+            return Choice(technique)
+        kind = Kind.of(automaton)
+        dev = self.cache.get(automaton)
+        key = self.tuner.signature(kind.value, self.arch, dev.props, batch.count, batch.total)
+        offsets = None
+
+        def timer(choice: Choice):
+            nonlocal offsets
+            torch = self._torch
+            if offsets is None:
+                offsets = batch.offsets[: batch.count + 1].cpu().numpy()
+            k = int(np.searchsorted(offsets, self.tuner.sample_bytes, side="right")) - 1
+            k = max(1, min(batch.count, k))
+            s = stream or torch.cuda.current_stream()
+            self._launch(dev, kind, batch, choice, s, count=k)  # warm-up (and attributes)
+            times = []
+            for _ in range(self.tuner.repeats):
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(s)
+                self._launch(dev, kind, batch, choice, s, count=k)
+                e1.record(s)
+                e1.synchronize()
+                times.append(e0.elapsed_time(e1) / 1e3)
+            c = k
+            flags = batch.flags[:c].cpu().numpy()
+            lens = batch.lens[:c].cpu().numpy().astype(np.int64)
+            span = np.diff(offsets[: c + 1])
+            examined = int(np.where(flags != 0, lens, span).sum())
+            return min(times), max(examined, 1)
+
+        return self.tuner.decide(key, kind.value, dev.props, batch.count, batch.total,
+                                 timer).choice
+
+    # -- launches -----------------------------------------------------------------------
+    def _launch(self, dev, kind: Kind, batch: DeviceBatch, choice: Choice, stream,
+                count: int | None = None) -> None:
+        tech = _techniques(kind)[choice.technique]
+        tuning = choice.tuning()
+        c = batch.count if count is None else count
+        total = int(batch.total) if count is None else None
+        if total is None:
+            total = int(batch.offsets[c].item())
+        need = dev.workspace_size(c, total, tech, tuning)
+        ws = batch.ensure_workspace(need)
+        dev.run_device_ws(batch.data.data_ptr(), batch.offsets.data_ptr(), c, total,
+                          batch.flags.data_ptr(), batch.lens.data_ptr(), ws,
+                          batch.workspace.numel(), stream.cuda_stream, tech, tuning)
+
+    def enqueue(self, automaton, batch: DeviceBatch, technique: str = "auto", stream=None,
+                graph: bool = False) -> Choice:
+        """Enqueue one run of ``automaton`` over ``batch`` on ``stream``; returns the choice.
+
+        With ``graph=True`` the launch sequence is captured into a HIP graph on first use
+        (per automaton, batch, choice and shape) and replayed afterwards.
+        """
+        torch = self._torch
+        stream = stream or torch.cuda.current_stream()
+        kind = Kind.of(automaton)
+        dev = self.cache.get(automaton)
+        choice = self.choose(automaton, batch, technique, stream)
+        if batch.count == 0:
+            return choice
+        if not graph:
+            self._launch(dev, kind, batch, choice, stream)
+            return choice
+        key = (id(dev), id(batch), choice, batch.count, batch.total)
+        g = self._graphs.get(key)
+        if g is None:
+            # one eager run first: it sizes the workspace and sets kernel attributes, so the
+            # capture records kernel launches only
+            self._launch(dev, kind, batch, choice, stream)
+            stream.synchronize()
+            g = torch.cuda.CUDAGraph()
+            side = torch.cuda.Stream()
+            side.wait_stream(stream)
+            with torch.cuda.graph(g, stream=side):
+                self._launch(dev, kind, batch, choice, torch.cuda.current_stream())
+            self._graphs[key] = g
+            stream.wait_stream(side)
+        with torch.cuda.stream(stream):
+            g.replay()
+        return choice
+
+    def run_packed(self, automaton, data: np.ndarray, offsets: np.ndarray,
+                   technique: str = "auto") -> tuple[np.ndarray, np.ndarray, float]:
+        """(flags, lens, kernel_ms) of one batch given packed host arrays."""
+        torch = self._torch
+        stream = torch.cuda.current_stream()
+        batch = DeviceBatch(data, offsets, stream=stream)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        self.enqueue(automaton, batch, technique, stream)
+        e1.record(stream)
+        flags, lens = batch.fetch(stream)
+        return flags, lens, e0.elapsed_time(e1)
+
+    def run_batch(self, automaton, inputs: list[bytes], technique: str = "auto") -> list[Result]:
+        """Drop-in ``run_batch`` over the resident path (same results, same Result layout)."""
+        if not inputs:
+            return []
+        t0 = time.perf_counter()
+        data, offsets = pack_bytes(inputs)
+        transfer_ms = (time.perf_counter() - t0) * 1000.0
+        flags, lens, kernel_ms = self.run_packed(automaton, data, offsets, technique)
+        return batch_results(flags, lens, kernel_ms, transfer_ms)
+
+    def close(self) -> None:
+        self._graphs.clear()
+        self.cache.close()

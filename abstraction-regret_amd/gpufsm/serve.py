@@ -1,0 +1,330 @@
+"""Serving loop: continuous batching of match requests against device-resident automata.
+
+Requests ``(automaton name, bytes)`` arrive at any time from any thread. One scheduler
+thread runs the loop:
+
+1. **Completion.** Every in-flight launch slot whose completion event has fired hands its
+   verdicts to the requests it carried and gives their bytes back to the budget.
+2. **Admission.** Waiting requests are admitted in arrival order while the HBM byte budget
+   (resident automata + the input bytes of every admitted, unfinished request) has room. A
+   request that could never fit is rejected at once; one that waits longer than the queue
+   timeout is rejected as timed out.
+3. **Batching.** Each free slot takes the oldest admitted request and every other admitted
+   request for the same automaton, up to the slot's byte and count capacity, packs them
+   (uint8 bytes, int64 offsets) into pinned memory and enqueues, on the slot's own stream:
+   the host-to-device copy, the launch (the persistent ragged kernels of the technique for
+   that automaton) and the copy back, followed by an event.
+
+Slots have separate streams and buffers, so requests that arrive while others are on the
+GPU are admitted and launched as slots and budget free up, and their work overlaps on the
+device. Automata stay resident for the life of the loop (:class:`~gpufsm.batch.ResidentCache`).
+
+The device side sits behind a small slot interface (:class:`GpuSlots`), so the scheduling
+logic also runs against a host-side stand-in in the CPU test suite.
+
+No upstream counterpart: upstream has no serving path.
+"""
+
+from __future__ import annotations
+
+import collections
+import itertools
+import math
+import threading
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass(eq=False)
+class Request:
+    """One match request; :meth:`wait` blocks until it is done."""
+
+    name: str
+    data: bytes
+    arrival: float
+    rid: int = 0
+    status: str = "queued"        # queued | admitted | running | ok | rejected | timeout
+    accepted: bool = False
+    match_len: int = 0
+    done_at: float = 0.0
+    _event: threading.Event = field(default_factory=threading.Event, repr=False)
+
+    @property
+    def latency(self) -> float:
+        return self.done_at - self.arrival
+
+    def finish(self, status: str, now: float, accepted: bool = False, match_len: int = 0) -> None:
+        self.status = status
+        self.accepted = bool(accepted)
+        self.match_len = int(match_len)
+        self.done_at = now
+        self._event.set()
+
+    def wait(self, timeout: float | None = None) -> bool:
+        return self._event.wait(timeout)
+
+
+class Admission:
+    """HBM byte budget: resident automata plus the inputs of admitted, unfinished requests."""
+
+    def __init__(self, budget_bytes: int, resident_bytes: int = 0) -> None:
+        if resident_bytes > budget_bytes:
+            raise ValueError(f"resident automata need {resident_bytes} bytes, more than the "
+                             f"budget of {budget_bytes}")
+        self.budget = int(budget_bytes)
+        self.resident = int(resident_bytes)
+        self.in_flight = 0
+
+    def ever_fits(self, nbytes: int) -> bool:
+        return self.resident + nbytes <= self.budget
+
+    def try_admit(self, nbytes: int) -> bool:
+        if self.resident + self.in_flight + nbytes > self.budget:
+            return False
+        self.in_flight += nbytes
+        return True
+
+    def release(self, nbytes: int) -> None:
+        self.in_flight -= nbytes
+        assert self.in_flight >= 0
+
+
+def form_batch(admitted: collections.deque, max_bytes: int, max_count: int) -> list[Request]:
+    """The oldest admitted request plus later ones for the same automaton that fit."""
+    if not admitted:
+        return []
+    head = admitted[0]
+    batch, used, keep = [], 0, collections.deque()
+    while admitted:
+        r = admitted.popleft()
+        if (r.name == head.name and len(batch) < max_count
+                and (not batch or used + len(r.data) <= max_bytes)):
+            batch.append(r)
+            used += len(r.data)
+        else:
+            keep.append(r)
+    admitted.extend(keep)
+    return batch
+
+
+class GpuSlots:
+    """Launch slots on the GPU: each has a stream, a resident batch buffer and an event."""
+
+    def __init__(self, engine, automata: dict, slots: int, capacity_bytes: int,
+                 capacity_count: int, technique: str = "auto") -> None:
+        import torch
+
+        from .batch import DeviceBatch
+
+        self.torch = torch
+        self.engine = engine
+        self.automata = automata
+        self.technique = technique
+        empty_off = np.zeros(1, dtype=np.int64)
+        self.streams = [torch.cuda.Stream() for _ in range(slots)]
+        self.batches = [DeviceBatch(np.zeros(0, np.uint8), empty_off,
+                                    capacity_bytes=capacity_bytes,
+                                    capacity_count=capacity_count) for _ in range(slots)]
+        self.events: list = [None] * slots
+        for a in automata.values():  # resident before the first request
+            engine.cache.get(a)
+        torch.cuda.synchronize()
+
+    def __len__(self) -> int:
+        return len(self.streams)
+
+    def resident_bytes(self) -> int:
+        return self.engine.cache.bytes
+
+    def launch(self, i: int, name: str, data: np.ndarray, offsets: np.ndarray) -> None:
+        s, b = self.streams[i], self.batches[i]
+        b.load(data, offsets, stream=s)
+        self.engine.enqueue(self.automata[name], b, self.technique, stream=s)
+        c = b.count
+        with self.torch.cuda.stream(s):
+            b._h_out[:c].copy_(b.flags[:c], non_blocking=True)
+            b._h_out[b.cap_count:b.cap_count + c].copy_(b.lens[:c], non_blocking=True)
+            ev = self.torch.cuda.Event()
+            ev.record(s)
+        self.events[i] = ev
+
+    def ready(self, i: int) -> bool:
+        return self.events[i] is not None and self.events[i].query()
+
+    def results(self, i: int) -> tuple[np.ndarray, np.ndarray]:
+        b = self.batches[i]
+        out = b._h_out.numpy()
+        self.events[i] = None
+        return out[:b.count].copy(), out[b.cap_count:b.cap_count + b.count].copy()
+
+
+class ServingLoop:
+    """Continuous batching over device-resident automata; see the module docstring."""
+
+    def __init__(self, automata: dict, *, budget_bytes: int = 32 << 30, slots: int = 4,
+                 max_batch_bytes: int = 16 << 20, max_batch_count: int = 4096,
+                 queue_timeout_s: float | None = None, technique: str = "auto",
+                 engine=None, backend=None) -> None:
+        self.automata = dict(automata)
+        self.max_batch_bytes = int(max_batch_bytes)
+        self.max_batch_count = int(max_batch_count)
+        self.queue_timeout_s = queue_timeout_s
+        if backend is None:
+            from .batch import BatchEngine
+
+            self.engine = engine or BatchEngine(tuner=None)
+            backend = GpuSlots(self.engine, self.automata, slots, self.max_batch_bytes,
+                               self.max_batch_count, technique)
+        self.slots = backend
+        self.admission = Admission(budget_bytes, backend.resident_bytes())
+        self._waiting: collections.deque[Request] = collections.deque()
+        self._admitted: collections.deque[Request] = collections.deque()
+        self._running: dict[int, list[Request]] = {}
+        self._free = list(range(len(backend)))
+        self._cv = threading.Condition()
+        self._ids = itertools.count()
+        self._stop = False
+        self._thread: threading.Thread | None = None
+        self.completed: list[Request] = []
+        self.launches = 0
+
+    # -- client side --------------------------------------------------------------------
+    def submit(self, name: str, data: bytes) -> Request:
+        if name not in self.automata:
+            raise KeyError(f"unknown automaton {name!r}; have {sorted(self.automata)}")
+        r = Request(name, bytes(data), time.perf_counter(), next(self._ids))
+        with self._cv:
+            self._waiting.append(r)
+            self._cv.notify()
+        return r
+
+    def start(self) -> ServingLoop:
+        self._thread = threading.Thread(target=self._run, name="gpufsm-serve", daemon=True)
+        self._thread.start()
+        return self
+
+    def stop(self, drain: bool = True, timeout: float = 60.0) -> None:
+        if drain:
+            deadline = time.perf_counter() + timeout
+            while time.perf_counter() < deadline:
+                with self._cv:
+                    idle = not (self._waiting or self._admitted or self._running)
+                if idle:
+                    break
+                time.sleep(0.001)
+        with self._cv:
+            self._stop = True
+            self._cv.notify()
+        if self._thread is not None:
+            self._thread.join(timeout)
+            self._thread = None
+
+    def __enter__(self):
+        return self.start()
+
+    def __exit__(self, *exc) -> None:
+        self.stop()
+
+    # -- scheduler ----------------------------------------------------------------------
+    def step(self) -> bool:
+        """One pass of completion, admission and batching; True if anything happened."""
+        now = time.perf_counter()
+        busy = False
+        for i, reqs in list(self._running.items()):
+            if not self.slots.ready(i):
+                continue
+            flags, lens = self.slots.results(i)
+            t = time.perf_counter()
+            for r, f, n in zip(reqs, flags, lens):
+                self.admission.release(len(r.data))
+                r.finish("ok", t, bool(f), int(n))
+                self.completed.append(r)
+            del self._running[i]
+            self._free.append(i)
+            busy = True
+        with self._cv:
+            while self._waiting:
+                r = self._waiting[0]
+                if not self.admission.ever_fits(len(r.data)) or len(r.data) > self.max_batch_bytes:
+                    self._waiting.popleft()
+                    r.finish("rejected", now)
+                    self.completed.append(r)
+                    busy = True
+                    continue
+                if self.admission.try_admit(len(r.data)):
+                    self._waiting.popleft()
+                    r.status = "admitted"
+                    self._admitted.append(r)
+                    busy = True
+                    continue
+                if self.queue_timeout_s is not None and now - r.arrival > self.queue_timeout_s:
+                    self._waiting.popleft()
+                    r.finish("timeout", now)
+                    self.completed.append(r)
+                    busy = True
+                    continue
+                break
+        while self._free and self._admitted:
+            batch = form_batch(self._admitted, self.max_batch_bytes, self.max_batch_count)
+            i = self._free.pop()
+            lens = np.fromiter((len(r.data) for r in batch), dtype=np.int64, count=len(batch))
+            offsets = np.zeros(len(batch) + 1, dtype=np.int64)
+            np.cumsum(lens, out=offsets[1:])
+            data = np.frombuffer(b"".join(r.data for r in batch), dtype=np.uint8)
+            for r in batch:
+                r.status = "running"
+            self.slots.launch(i, batch[0].name, data, offsets)
+            self._running[i] = batch
+            self.launches += 1
+            busy = True
+        return busy
+
+    def _run(self) -> None:
+        while True:
+            with self._cv:
+                if self._stop:
+                    return
+            if not self.step():
+                with self._cv:
+                    if not self._waiting and not self._stop:
+                        self._cv.wait(0.0005 if self._running else 0.01)
+
+    # -- reporting ----------------------------------------------------------------------
+    def stats(self, since: float = 0.0) -> dict:
+        done = [r for r in self.completed if r.arrival >= since]
+        ok = [r for r in done if r.status == "ok"]
+        lat = np.array([r.latency for r in ok]) if ok else np.zeros(0)
+        span = (max(r.done_at for r in ok) - min(r.arrival for r in ok)) if ok else 0.0
+        credited = sum(len(r.data) for r in ok)
+        examined = sum((r.match_len if r.accepted else len(r.data)) for r in ok)
+        return {
+            "requests": len(done),
+            "ok": len(ok),
+            "rejected": sum(r.status == "rejected" for r in done),
+            "timeout": sum(r.status == "timeout" for r in done),
+            "p50_ms": float(np.percentile(lat, 50) * 1e3) if ok else math.nan,
+            "p99_ms": float(np.percentile(lat, 99) * 1e3) if ok else math.nan,
+            "credited_GBps": credited / span / 1e9 if span > 0 else math.nan,
+            "examined_GBps_upper": examined / span / 1e9 if span > 0 else math.nan,
+            "launches": self.launches,
+        }
+
+
+def poisson_drive(loop: ServingLoop, requests: list[tuple[str, bytes]], qps: float,
+                  seed: int = 0) -> list[Request]:
+    """Submit ``requests`` open-loop at Poisson arrivals of rate ``qps``; wait for all."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / qps, size=len(requests))
+    t0 = time.perf_counter()
+    at = t0 + np.cumsum(gaps)
+    out = []
+    for (name, data), when in zip(requests, at):
+        delay = when - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        out.append(loop.submit(name, data))
+    for r in out:
+        r.wait()
+    return out

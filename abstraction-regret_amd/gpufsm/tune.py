@@ -1,0 +1,178 @@
+"""Autotuner: picks the technique and launch tuning per automaton and batch shape.
+
+Every technique is bit-exact (the parity suite checks each against the oracle), so the
+choice only moves time. The tuner times each candidate on a bounded sample of the real batch
+(a prefix of its inputs, run in place: same device buffers, smaller count), keeps the
+fastest, and records every candidate's measured examined-byte rate next to the choice, in a
+JSON table keyed by a shape signature:
+
+    kind | device arch | states (power-of-two bucket) | structure flags | mean input length
+    (bucket) | batch size (bucket)
+
+so a later batch of the same class reuses the decision without timing anything. The table
+persists across processes (``GFSM_TUNE_TABLE`` or ``~/.cache/gpufsm/hip_autotune.json``).
+
+Candidates
+  NFA, chain-structured: ``chain`` with chunks of 1, 4 and 16 KiB, ``chain`` without the
+  byte-pair lists, ``bitpacked`` (dense active sets) and ``worklist``.
+  NFA, general: ``worklist`` only.
+  DFA: ``gather``; ``lds`` when the table fits LDS; ``split`` (segment 1, 2 and 4 KiB when
+  the table fits LDS, the library's choice otherwise) when inputs average at least 1 KiB.
+
+No upstream counterpart: upstream picks techniques by hand (`bench/` drivers).
+"""
+
+from __future__ import annotations
+
+import json
+import math
+import os
+import threading
+from dataclasses import dataclass, field
+
+from .backends.hip import _lib
+from .backends.hip.handles import make_tuning
+
+
+@dataclass(frozen=True)
+class Choice:
+    """A technique id plus tuning fields (zeros = library default)."""
+
+    technique: str
+    chunk_bytes: int = 0
+    segment_bytes: int = 0
+    flags: int = 0
+
+    def label(self) -> str:
+        parts = [self.technique]
+        if self.chunk_bytes:
+            parts.append(f"chunk={self.chunk_bytes}")
+        if self.segment_bytes:
+            parts.append(f"segment={self.segment_bytes}")
+        if self.flags & _lib.TUNE_CHAIN_FIRST:
+            parts.append("first")
+        return ":".join(parts)
+
+    def tuning(self):
+        return make_tuning(self.chunk_bytes, self.segment_bytes, self.flags)
+
+
+@dataclass
+class Decision:
+    choice: Choice
+    rates: dict[str, float] = field(default_factory=dict)  # label -> examined GB/s
+
+
+def _bucket(x: int) -> int:
+    return 0 if x <= 0 else 1 << int(math.ceil(math.log2(x)))
+
+
+def default_table_path() -> str:
+    env = os.environ.get("GFSM_TUNE_TABLE")
+    if env:
+        return env
+    return os.path.join(os.path.expanduser("~"), ".cache", "gpufsm", "hip_autotune.json")
+
+
+def candidates(kind: str, props, mean_len: float) -> list[Choice]:
+    if kind == "nfa":
+        if not props.chain_ok:
+            return [Choice("worklist")]
+        return [Choice("chain", chunk_bytes=1024), Choice("chain", chunk_bytes=4096),
+                Choice("chain", chunk_bytes=16384),
+                Choice("chain", chunk_bytes=4096, flags=_lib.TUNE_CHAIN_FIRST),
+                Choice("bitpacked"), Choice("worklist")]
+    out = [Choice("gather")]
+    if props.lds_ok:
+        out.append(Choice("lds"))
+    if mean_len >= 1024:
+        if props.lds_ok:
+            out += [Choice("split", segment_bytes=s) for s in (1024, 2048, 4096)]
+        else:
+            out.append(Choice("split"))
+    return out
+
+
+class Autotuner:
+    """Chooses a :class:`Choice` per shape class; see the module docstring."""
+
+    def __init__(self, path: str | None = None, sample_bytes: int = 64 << 20,
+                 repeats: int = 3, persist: bool = True) -> None:
+        self.path = path or default_table_path()
+        self.sample_bytes = int(sample_bytes)
+        self.repeats = int(repeats)
+        self.persist = persist
+        self._lock = threading.Lock()
+        self.table: dict[str, dict] = {}
+        if persist and os.path.exists(self.path):
+            try:
+                with open(self.path) as f:
+                    self.table = json.load(f)
+            except (OSError, ValueError):
+                self.table = {}
+
+    @staticmethod
+    def signature(kind: str, arch: str, props, count: int, total: int) -> str:
+        mean = total // count if count else 0
+        if kind == "nfa":
+            struct = f"chain{props.chain_ok}always{props.start_always}"
+        else:
+            struct = f"w{props.entry_bytes}lds{props.lds_ok}"
+        return (f"{kind}|{arch}|n{_bucket(props.num_states)}|{struct}|len{_bucket(mean)}"
+                f"|cnt{_bucket(count)}")
+
+    def lookup(self, key: str) -> Decision | None:
+        entry = self.table.get(key)
+        if not entry:
+            return None
+        c = entry["choice"]
+        return Decision(Choice(c["technique"], c.get("chunk_bytes", 0),
+                               c.get("segment_bytes", 0), c.get("flags", 0)),
+                        dict(entry.get("rates", {})))
+
+    def decide(self, key: str, kind: str, props, count: int, total: int, timer) -> Decision:
+        """The recorded decision for ``key``, or time the candidates with ``timer``.
+
+        ``timer(choice, sample_count) -> (seconds, examined_bytes)`` runs one candidate
+        over the first ``sample_count`` inputs; it may raise for a candidate that does not
+        apply, which then drops out.
+        """
+        with self._lock:
+            hit = self.lookup(key)
+            if hit is not None:
+                return hit
+        mean = total / count if count else 0.0
+        best: tuple[float, Choice] | None = None
+        rates: dict[str, float] = {}
+        for cand in candidates(kind, props, mean):
+            try:
+                secs, examined = timer(cand)
+            except _lib.HIPError:
+                continue
+            rate = examined / secs / 1e9 if secs > 0 else float("inf")
+            rates[cand.label()] = round(rate, 4)
+            if best is None or secs < best[0]:
+                best = (secs, cand)
+        if best is None:
+            raise RuntimeError(f"no technique could run shape {key}")
+        dec = Decision(best[1], rates)
+        with self._lock:
+            self.table[key] = {"choice": {"technique": dec.choice.technique,
+                                          "chunk_bytes": dec.choice.chunk_bytes,
+                                          "segment_bytes": dec.choice.segment_bytes,
+                                          "flags": dec.choice.flags},
+                               "rates": rates}
+            self.save()
+        return dec
+
+    def save(self) -> None:
+        if not self.persist:
+            return
+        try:
+            os.makedirs(os.path.dirname(self.path) or ".", exist_ok=True)
+            tmp = self.path + ".tmp"
+            with open(tmp, "w") as f:
+                json.dump(self.table, f, indent=1, sort_keys=True)
+            os.replace(tmp, self.path)
+        except OSError:
+            pass  # an unwritable cache only costs re-tuning in the next process

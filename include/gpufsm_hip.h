@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define GFSM_ABI_VERSION 1
+#define GFSM_ABI_VERSION 2
 
 /* status codes */
 #define GFSM_OK 0
@@ -64,6 +64,16 @@ extern "C" {
 
 typedef struct gfsm_nfa gfsm_nfa;
 typedef struct gfsm_dfa gfsm_dfa;
+
+/* Tuning of one run (nullable everywhere; a zero field means "the library's choice"). No
+ * upstream counterpart: upstream kernels have fixed launch shapes. */
+#define GFSM_TUNE_CHAIN_FIRST 1  /* chain: no byte-pair lists (first-position lists only) */
+typedef struct {
+    int32_t chunk_bytes;      /* chain: bytes per work item when inputs are cut (>= 64) */
+    int32_t segment_bytes;    /* dfa split: segment length, a power of two in [256, 1 MiB] */
+    int32_t flags;            /* GFSM_TUNE_* bits */
+    int32_t reserved;
+} gfsm_tuning;
 
 typedef struct {
     int32_t num_states;
@@ -128,6 +138,27 @@ int gfsm_nfa_run_device(gfsm_nfa *nfa, int32_t technique, const uint8_t *d_data,
                         const int64_t *d_offsets, int32_t num_strings, int64_t total_bytes,
                         int32_t *d_flags, int32_t *d_lens, void *stream);
 
+/* Device scratch (bytes) that gfsm_nfa_run_device_ws needs for this technique and batch
+ * shape. No upstream counterpart. */
+int gfsm_nfa_workspace_size(gfsm_nfa *nfa, int32_t technique, int32_t num_strings,
+                            int64_t total_bytes, const gfsm_tuning *tuning,
+                            int64_t *out_bytes);
+
+/* gfsm_nfa_run_device with tuning and caller-provided device scratch (at least
+ * gfsm_nfa_workspace_size bytes, 256-byte aligned): enqueues kernels only, no allocation
+ * and no memset, so the call can be captured into a HIP graph. No upstream counterpart
+ * (it is the persistent/captured run_batch of the batched path). */
+int gfsm_nfa_run_device_ws(gfsm_nfa *nfa, int32_t technique, const gfsm_tuning *tuning,
+                           const uint8_t *d_data, const int64_t *d_offsets,
+                           int32_t num_strings, int64_t total_bytes, int32_t *d_flags,
+                           int32_t *d_lens, void *d_workspace, int64_t workspace_bytes,
+                           void *stream);
+
+/* The concrete technique GFSM_NFA_AUTO (or `technique`) resolves to for this batch shape;
+ * GFSM_EUNSUPPORTED if the technique cannot run the automaton. No upstream counterpart. */
+int gfsm_nfa_resolve(gfsm_nfa *nfa, int32_t technique, int32_t num_strings,
+                     int64_t total_bytes, int32_t *out_technique);
+
 /* DFA handles. Replaces the per-call trans/accept upload of run_dfa (api.hpp:101-104);
  * layout of `src/gpufsm/core/dfa.py:25-37` (trans[s * 256 + c]). */
 int gfsm_dfa_create(int32_t num_states, int32_t start_state, const uint8_t *accept,
@@ -145,6 +176,19 @@ int gfsm_dfa_run_host(gfsm_dfa *dfa, int32_t technique, const uint8_t *data,
 int gfsm_dfa_run_device(gfsm_dfa *dfa, int32_t technique, const uint8_t *d_data,
                         const int64_t *d_offsets, int32_t num_strings, int64_t total_bytes,
                         int32_t *d_flags, int32_t *d_lens, void *stream);
+
+/* DFA counterparts of gfsm_nfa_workspace_size / gfsm_nfa_run_device_ws /
+ * gfsm_nfa_resolve. No upstream counterpart. */
+int gfsm_dfa_workspace_size(gfsm_dfa *dfa, int32_t technique, int32_t num_strings,
+                            int64_t total_bytes, const gfsm_tuning *tuning,
+                            int64_t *out_bytes);
+int gfsm_dfa_run_device_ws(gfsm_dfa *dfa, int32_t technique, const gfsm_tuning *tuning,
+                           const uint8_t *d_data, const int64_t *d_offsets,
+                           int32_t num_strings, int64_t total_bytes, int32_t *d_flags,
+                           int32_t *d_lens, void *d_workspace, int64_t workspace_bytes,
+                           void *stream);
+int gfsm_dfa_resolve(gfsm_dfa *dfa, int32_t technique, int32_t num_strings,
+                     int64_t total_bytes, int32_t *out_technique);
 
 #ifdef __cplusplus
 }

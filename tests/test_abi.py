@@ -56,11 +56,13 @@ def test_header_constants_match_python():
     assert consts["GFSM_DFA_SPLIT"] == _lib.DFA_SPLIT
     assert consts["GFSM_EINVAL"] == _lib.EINVAL
     assert consts["GFSM_EUNSUPPORTED"] == _lib.EUNSUPPORTED
+    assert consts["GFSM_TUNE_CHAIN_FIRST"] == _lib.TUNE_CHAIN_FIRST
 
 
 def test_props_struct_layout(tmp_path):
     """The ctypes mirrors have the C structs' size and field offsets (compiled with gcc)."""
-    fields = {"gfsm_nfa_props": _lib.NfaProps, "gfsm_dfa_props": _lib.DfaProps}
+    fields = {"gfsm_nfa_props": _lib.NfaProps, "gfsm_dfa_props": _lib.DfaProps,
+              "gfsm_tuning": _lib.Tuning}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "gpufsm_hip.h"',
              "int main(void) {"]
     for cname, py in fields.items():
@@ -155,6 +157,13 @@ def test_null_handles_are_errors_not_crashes():
     assert rc == _lib.EINVAL and b"null handle" in lib.gfsm_last_error()
     assert lib.gfsm_nfa_destroy(None) == _lib.OK
     assert lib.gfsm_dfa_query(None, None) == _lib.EINVAL
+    out = ctypes.c_int64(0)
+    assert lib.gfsm_nfa_workspace_size(None, 0, 1, 1, None, ctypes.byref(out)) == _lib.EINVAL
+    assert lib.gfsm_dfa_workspace_size(None, 0, 1, 1, None, ctypes.byref(out)) == _lib.EINVAL
+    tech = ctypes.c_int32(0)
+    assert lib.gfsm_nfa_resolve(None, 0, 1, 1, ctypes.byref(tech)) == _lib.EINVAL
+    assert lib.gfsm_dfa_run_device_ws(None, 0, None, None, None, 0, 0, None, None, None, 0,
+                                      None) == _lib.EINVAL
 
 
 def test_dfa_create_validates_before_touching_the_device():

@@ -1,0 +1,172 @@
+"""Resident batch engine, autotuner, graph replay and serving loop on the GPU.
+
+Every verdict is checked against the oracle; graph replays against the eager run of the
+same launch (several replays, with other eager work between them).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from gpufsm import workloads as W
+from gpufsm.backends.hip import _lib
+from gpufsm.batch import BatchEngine, DeviceBatch
+from gpufsm.core.packing import pack_bytes
+from gpufsm.serve import ServingLoop, poisson_drive
+from gpufsm.tune import Autotuner
+from fixtures import random_nfa
+from oracle import native
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle(a, data, off):
+    fn = native.dfa_batch if hasattr(a, "trans") else native.nfa_batch
+    f, n, _ = fn(a, data, off)
+    return f, n
+
+
+@pytest.fixture(scope="module")
+def engine(tmp_path_factory):
+    path = tmp_path_factory.mktemp("tune") / "table.json"
+    eng = BatchEngine(tuner=Autotuner(str(path), sample_bytes=4 << 20, repeats=2))
+    yield eng
+    eng.close()
+
+
+def test_engine_matches_oracle_on_every_kind(engine):
+    import random
+
+    pu = W.pattern_union(300, seed=21)
+    lens = np.random.default_rng(21).integers(0, 20000, size=200)
+    ud, uo = W.pattern_inputs(pu, lens, 0.7, seed=22)
+    gen = random.Random(23)
+    gnfa = random_nfa(gen, 300, b"abcdefgh", max_sym=3, max_eps=1, p_accept=0.01, p_any=0.02)
+    gd = np.frombuffer(bytes(gen.choice(b"abcdefghi") for _ in range(40000)), dtype=np.uint8)
+    go = np.linspace(0, gd.size, 101).astype(np.int64)
+    dfa = W.dense_dfa(4096, 0.0, seed=24)
+    dd, do = W.uniform_batch(64, 1 << 16, seed=25)
+    small = W.dense_dfa(256, 0.001, seed=26)
+    for a, d, o in ((pu.nfa, ud, uo), (gnfa, gd, go), (dfa, dd, do), (small, dd, do)):
+        f, n, _ = engine.run_packed(a, d, o)
+        wf, wn = _oracle(a, d, o)
+        np.testing.assert_array_equal(f, wf)
+        np.testing.assert_array_equal(n, wn)
+    table = engine.tuner.table
+    assert len(table) >= 3
+    for entry in table.values():
+        assert entry["rates"] and entry["choice"]["technique"]
+
+
+def test_run_batch_is_a_drop_in(engine):
+    pu = W.pattern_union(100, seed=31)
+    lens = np.random.default_rng(31).integers(0, 3000, size=50)
+    d, o = W.pattern_inputs(pu, lens, 0.7, seed=32)
+    inputs = [d[o[i]:o[i + 1]].tobytes() for i in range(50)]
+    res = engine.run_batch(pu.nfa, inputs)
+    wf, wn = _oracle(pu.nfa, *pack_bytes(inputs))
+    assert [(r.accepted, r.match_len) for r in res] == [(bool(a), int(b)) for a, b in zip(wf, wn)]
+    assert res[0].kernel_ms > 0 and all(r.kernel_ms == 0 for r in res[1:])
+    assert engine.run_batch(pu.nfa, []) == []
+
+
+@pytest.mark.parametrize("technique", ["chain", "split"])
+def test_graph_replays_match_eager(engine, technique):
+    import torch
+
+    if technique == "chain":
+        pu = W.pattern_union(200, seed=41)
+        a = pu.nfa
+        d, o = W.pattern_inputs(pu, np.full(64, 9000, dtype=np.int64), 0.7, seed=42)
+    else:
+        a = W.dense_dfa(256, 0.0005, seed=43)
+        d, o = W.uniform_batch(32, 1 << 15, seed=44)
+    other = W.dense_dfa(64, 0.01, seed=45)
+    od, oo = W.uniform_batch(16, 4096, seed=46)
+    wf, wn = _oracle(a, d, o)
+    s = torch.cuda.current_stream()
+    batch = DeviceBatch(d, o, stream=s)
+    obatch = DeviceBatch(od, oo, stream=s)
+    engine.enqueue(a, batch, technique, stream=s)
+    f, n = batch.fetch(s)
+    np.testing.assert_array_equal(f, wf)
+    np.testing.assert_array_equal(n, wn)
+    for rep in range(4):
+        batch.flags.fill_(-1)
+        batch.lens.fill_(-1)
+        engine.enqueue(a, batch, technique, stream=s, graph=True)
+        f, n = batch.fetch(s)
+        np.testing.assert_array_equal(f, wf, err_msg=f"replay {rep}")
+        np.testing.assert_array_equal(n, wn, err_msg=f"replay {rep}")
+        engine.enqueue(other, obatch, "gather", stream=s)  # eager work between replays
+        obatch.fetch(s)
+
+
+def test_resident_cache_reuses_and_evicts():
+    eng = BatchEngine(tuner=None, budget_bytes=3 << 20)
+    try:
+        a = W.dense_dfa(4096, 0.1, seed=51)   # 2 MiB table
+        b = W.dense_dfa(4096, 0.1, seed=52)
+        d, o = W.uniform_batch(8, 512, seed=53)
+        eng.run_packed(a, d, o)
+        eng.run_packed(a, d, o)
+        assert eng.cache.uploads == 1
+        eng.run_packed(b, d, o)               # over budget: a is evicted
+        assert eng.cache.uploads == 2 and eng.cache.bytes <= 3 << 20
+        f, n, _ = eng.run_packed(a, d, o)
+        assert eng.cache.uploads == 3
+        wf, wn = _oracle(a, d, o)
+        np.testing.assert_array_equal(f, wf)
+        np.testing.assert_array_equal(n, wn)
+    finally:
+        eng.close()
+
+
+def test_workspace_too_small_is_an_error(engine):
+    import torch
+
+    pu = W.pattern_union(50, seed=61)
+    d, o = W.pattern_inputs(pu, np.full(8, 2000, dtype=np.int64), 0.7, seed=62)
+    batch = DeviceBatch(d, o)
+    dev = engine.cache.get(pu.nfa)
+    need = dev.workspace_size(batch.count, batch.total, _lib.NFA_CHAIN)
+    assert need > 0
+    ws = torch.empty(need, dtype=torch.uint8, device="cuda")
+    with pytest.raises(_lib.HIPError, match="workspace"):
+        dev.run_device_ws(batch.data.data_ptr(), batch.offsets.data_ptr(), batch.count,
+                          batch.total, batch.flags.data_ptr(), batch.lens.data_ptr(),
+                          ws.data_ptr(), need - 256, 0, _lib.NFA_CHAIN)
+
+
+def test_serving_mix_equals_alone():
+    """Mixed automata, Poisson arrivals, 3 slots: each result equals its lone run."""
+    automata = {
+        "dfa256": W.dense_dfa(256, 0.0005, seed=71),
+        "dfa4096": W.dense_dfa(4096, 0.0002, seed=72),
+        "pu1k": W.pattern_union(40, seed=73).nfa,
+        "pu16k": W.pattern_union(700, seed=74).nfa,
+    }
+    rng = np.random.default_rng(75)
+    names = sorted(automata)
+    reqs = []
+    for k in range(150):
+        name = names[rng.integers(0, 4)]
+        length = int(min(np.exp(rng.normal(np.log(16384), 1.0)), 1 << 20))
+        reqs.append((name, rng.integers(0, 256, size=length, dtype=np.uint8).tobytes()))
+    loop = ServingLoop(automata, budget_bytes=1 << 30, slots=3, max_batch_bytes=4 << 20,
+                       max_batch_count=64)
+    with loop:
+        done = poisson_drive(loop, reqs, qps=3000, seed=76)
+    assert all(r.status == "ok" for r in done)
+    alone = BatchEngine(tuner=None)
+    try:
+        for (name, data), r in zip(reqs, done):
+            d = np.frombuffer(data, dtype=np.uint8)
+            o = np.array([0, len(data)], dtype=np.int64)
+            f, n, _ = alone.run_packed(automata[name], d, o)
+            wf, wn = _oracle(automata[name], d, o)
+            assert (r.accepted, r.match_len) == (bool(f[0]), int(n[0])) == (bool(wf[0]), int(wn[0]))
+    finally:
+        alone.close()
+    assert loop.launches < len(reqs)
