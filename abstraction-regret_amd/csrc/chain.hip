@@ -161,19 +161,28 @@ __device__ __forceinline__ bool settled(const int32_t *best, const Item &it, int
 
 // ------------------------------------------------------------------------- reg tier ----
 
-// Flags of the 4 bytes of `word` (4 bits each, bflags layout), for a lane's window word.
-__device__ __forceinline__ uint32_t word_flags(const uint32_t *__restrict__ bflags, uint32_t word) {
-    return bflags[word & 0xffu] | (bflags[(word >> 8) & 0xffu] << 4) |
-           (bflags[(word >> 16) & 0xffu] << 8) | (bflags[word >> 24] << 12);
-}
+constexpr int kAccLdsTiles = 32;  // accept bits staged in LDS up to this many tiles
 
+// Register tier. Per 256-byte window, each lane holds 4 bytes of input (for the loads) and,
+// for the bytes r = 64 q + lane (q = 0..3), the byte, its flags and the pair entry of the
+// depth-1 set before it, so a byte step reads everything it needs with three readlanes and
+// issues its (at most three) memory loads together. A byte step has one branch, taken only
+// on an event (an accept, an empty set, a list overflow).
+template <bool kAlways, bool kAccLds>
 __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_reg_kernel(
     NfaChainDev d, const uint8_t *__restrict__ data, const int64_t *__restrict__ off,
     int32_t count, ChainScratch sc, int64_t chunk) {
+    __shared__ uint32_t s_flags[256];
+    __shared__ uint64_t s_acc[kAccLds ? (kAccLdsTiles + 1) * kTileWords : 1];
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) s_flags[i] = d.bflags[i];
+    if (kAccLds)
+        for (int i = threadIdx.x; i < (d.tiles + 1) * kTileWords; i += blockDim.x) s_acc[i] = d.acc[i];
+    __syncthreads();
     const int lane = threadIdx.x & 63;
     const int64_t items = sc.cbase[count];
     const int64_t data_end = off[count];
     const int64_t overlap = d.max_chain > 0 ? d.max_chain - 1 : 0;
+    const char *__restrict__ next_base = reinterpret_cast<const char *>(d.next);
     Item it;
     while (next_item(&sc.counter[0], sc.cbase, off, count, items, chunk, overlap, lane, it)) {
         bool stop = settled(sc.best, it, it.cs);  // wave-uniform throughout
@@ -181,98 +190,102 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_reg_kernel(
         uint64_t occ = 0;                         // lanes holding a state
         bool s0 = true, ovf = false;
         int32_t d1 = 256;                         // byte of the depth-1 set; 256 = none
-        uint32_t last_b = 0, last_fl = 0;         // the byte before the window and its flags
         int32_t found = 0;
         const int64_t abase =
             it.ps - static_cast<int64_t>((reinterpret_cast<uintptr_t>(data) + it.ps) & 3u);
         for (int64_t w0 = abase; w0 < it.ce && !stop; w0 += 256) {
             const uint32_t word = chain_load_word(data, w0 + 4 * lane, it.ce, data_end);
-            const uint32_t fw = word_flags(d.bflags, word);
             const int32_t lo = it.ps > w0 ? static_cast<int32_t>(it.ps - w0) : 0;
             const int32_t hi = (it.ce - w0) < 256 ? static_cast<int32_t>(it.ce - w0) : 256;
             const int32_t hit_from = it.cs > w0 ? static_cast<int32_t>(it.cs - w0) : 0;
-            // The depth-1 set before each byte depends only on the bytes and on whether the
-            // start state is still active, so the pair entries of the whole window are
-            // loaded up front, four per lane. kpos: first byte of the window (>= lo) on
-            // which the start state has no self-loop (it is active before byte r while
-            // kpos >= r).
+            uint32_t bq[4], fq[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t wq = static_cast<uint32_t>(
+                    __shfl(static_cast<int>(word), 16 * q + (lane >> 2)));
+                bq[q] = (wq >> (8 * (lane & 3))) & 0xffu;
+                fq[q] = s_flags[bq[q]];
+            }
+            // kpos: first byte of the window (>= lo) on which the start state has no
+            // self-loop; the start state is active before byte r while kpos >= r
             int32_t kpos = 1 << 20;
-            if (!d.start_always) {
-                int32_t mine = 1 << 20;
+            if (!kAlways) {
 #pragma unroll
                 for (int q = 3; q >= 0; --q) {
-                    const int32_t r = 4 * lane + q;
-                    if (r >= lo && r < hi && !((fw >> (4 * q + 2)) & 1u)) mine = r;
+                    const int32_t r = 64 * q + lane;
+                    const uint64_t m = __ballot(r >= lo && r < hi && !(fq[q] & 4u));
+                    if (m) kpos = 64 * q + static_cast<int32_t>(__builtin_ctzll(m));
                 }
-                const uint64_t m = __ballot(mine < (1 << 20));
-                if (m) kpos = __builtin_amdgcn_readlane(mine, __builtin_ctzll(m));
             }
-            const uint32_t pword = static_cast<uint32_t>(__shfl_up(static_cast<int>(word), 1));
-            const uint32_t pfw = static_cast<uint32_t>(__shfl_up(static_cast<int>(fw), 1));
+            uint32_t info[4];
             int32_t pe_off[4], pe_w[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int32_t r = 4 * lane + q;
-                const uint32_t bq = (word >> (8 * q)) & 0xffu;
-                uint32_t bp, fp;
-                if (q > 0) {
-                    bp = (word >> (8 * (q - 1))) & 0xffu;
-                    fp = (fw >> (4 * (q - 1))) & 0xfu;
-                } else {
-                    bp = lane > 0 ? pword >> 24 : last_b;
-                    fp = lane > 0 ? (pfw >> 12) & 0xfu : last_fl;
+                const int32_t r = 64 * q + lane;
+                uint32_t bp = static_cast<uint32_t>(__shfl_up(static_cast<int>(bq[q]), 1));
+                uint32_t fp = static_cast<uint32_t>(__shfl_up(static_cast<int>(fq[q]), 1));
+                if (q > 0 && lane == 0) {
+                    bp = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(bq[q - 1]), 63));
+                    fp = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(fq[q - 1]), 63));
                 }
-                const bool s0p = s0 && kpos >= r - 1;
+                // r == lo takes the running d1; every later r derives it from byte r - 1
+                const bool s0p = kAlways || (s0 && kpos >= r - 1);
                 const int32_t d1q =
                     r == lo ? d1 : ((s0p && (fp & 1u)) ? static_cast<int32_t>(bp) : 256);
                 const int32_t idx =
-                    (r >= lo && r < hi) ? 2 * (d1q * 256 + static_cast<int32_t>(bq)) : 0;
+                    (r >= lo && r < hi) ? 2 * (d1q * 256 + static_cast<int32_t>(bq[q])) : 0;
                 pe_off[q] = d.pair[idx];
                 pe_w[q] = d.pair[idx + 1];
+                info[q] = bq[q] | (fq[q] << 8);
             }
-            for (int32_t j = lo >> 2; j < 64 && !stop; ++j) {
-                const uint32_t wj = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(word), j));
-                const uint32_t fj = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(fw), j));
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int32_t r = 4 * j + q;
-                    if (r < lo || r >= hi || stop) continue;
-                    const uint32_t b = (wj >> (8 * q)) & 0xffu;
-                    const uint32_t fl = (fj >> (4 * q)) & 0xfu;
-                    const int32_t add_off = __builtin_amdgcn_readlane(pe_off[q], j);
-                    const int32_t add_w = __builtin_amdgcn_readlane(pe_w[q], j);
+            for (int q = 0; q < 4; ++q) {
+                const int32_t ilo = lo - 64 * q > 0 ? lo - 64 * q : 0;
+                const int32_t ihi = hi - 64 * q < 64 ? hi - 64 * q : 64;
+                for (int32_t i = ilo; i < ihi && !stop; ++i) {
+                    const uint32_t inf = static_cast<uint32_t>(
+                        __builtin_amdgcn_readlane(static_cast<int>(info[q]), i));
+                    const uint32_t b = inf & 0xffu;
+                    const uint32_t fl = inf >> 8;
+                    const int32_t add_off = __builtin_amdgcn_readlane(pe_off[q], i);
+                    const int32_t add_w = __builtin_amdgcn_readlane(pe_w[q], i);
                     const int32_t add_cnt = add_w & INT_MAX;
-                    // additions go to the lanes free at the start of the byte, in lane
-                    // order; every lane loads (padding keeps the index in range)
-                    const int32_t frank = lane_rank(~occ);
-                    const int32_t cand = d.pair_slot[add_off + (frank & 63)];
-                    // continuation of held states; free lanes look up slot 0 and ignore it
+                    // additions go to the lanes free at the start of the byte, in lane order
                     const bool have = slot >= 0;
+                    const int32_t frank = lane_rank(~occ);
+                    int32_t cand = -1;
+                    if (!have && frank < add_cnt) cand = d.pair_slot[add_off + frank];
+                    // continuation of held states; free lanes look up slot 0 and ignore it
                     const uint32_t t = have ? static_cast<uint32_t>(slot + 1) : 0u;
-                    const uint32_t tile = t >> 12;  // chains never cross a tile
-                    const uint32_t sl = t & 4095u;
-                    const uint64_t nw =
-                        d.next[(static_cast<size_t>(tile) * 256 + b) * kTileWords + (sl >> 6)];
-                    const uint64_t aw = d.acc[tile * kTileWords + (sl >> 6)];
-                    const bool alive = have & static_cast<bool>((nw >> (sl & 63)) & 1ull);
-                    const bool hit = alive & static_cast<bool>((aw >> (sl & 63)) & 1ull);
-                    if (add_cnt > 64 - __popcll(occ)) {
-                        ovf = true;
-                        stop = true;
-                        continue;
-                    }
-                    slot = alive ? static_cast<int32_t>(t) : ((!have && frank < add_cnt) ? cand : -1);
+                    const uint32_t noff = ((t >> 12) << 17) | (b << 9) | ((t >> 3) & 0x1f8u);
+                    const uint64_t nw = *reinterpret_cast<const uint64_t *>(next_base + noff);
+                    const uint64_t aw = kAccLds ? s_acc[t >> 6] : d.acc[t >> 6];
+                    const uint32_t bit = t & 63u;
+                    const bool alive = have && ((nw >> bit) & 1ull);
+                    const bool hit = alive && ((aw >> bit) & 1ull);
+                    const uint64_t occ_prev = occ;
+                    slot = alive ? static_cast<int32_t>(t) : cand;
                     occ = __ballot(slot >= 0);
-                    if ((__ballot(hit) != 0 || add_w < 0 || (s0 && (fl & 2u))) && r >= hit_from) {
-                        found = static_cast<int32_t>(w0 + r - it.ib + 1);
-                        stop = true;
-                        continue;
+                    const uint64_t hitm = __ballot(hit);
+                    const int32_t d1n = (s0 && (fl & 1u)) ? static_cast<int32_t>(b) : 256;
+                    const bool s0n = kAlways || (s0 && (fl & 4u));
+                    const bool ovf_now = add_cnt > 64 - __popcll(occ_prev);
+                    const bool acc_now = hitm != 0 || add_w < 0 || (s0 && (fl & 2u));
+                    const bool dead = occ == 0 && d1n == 256 && !s0n;
+                    s0 = s0n;
+                    d1 = d1n;
+                    if (ovf_now || acc_now || dead) {
+                        const int32_t r = 64 * q + i;
+                        if (ovf_now) {
+                            ovf = true;
+                            stop = true;
+                        } else if (acc_now && r >= hit_from) {
+                            found = static_cast<int32_t>(w0 + r - it.ib + 1);
+                            stop = true;
+                        } else if (dead) {
+                            stop = true;
+                        }
                     }
-                    d1 = (s0 && (fl & 1u)) ? static_cast<int32_t>(b) : 256;
-                    s0 = s0 && (d.start_always || (fl & 4u));
-                    last_b = b;
-                    last_fl = fl;
-                    if (occ == 0 && d1 == 256 && !s0) stop = true;
                 }
             }
             if (!stop && ((w0 - abase) & 4095) == 3840 && settled(sc.best, it, w0 + 256)) stop = true;
@@ -436,14 +449,22 @@ void launch_nfa_chain(const NfaChainDev &d, const NfaTileDev &tiles, const uint8
     chain_scan_kernel<<<1, 1024, 0, stream>>>(sc, count);
     GFSM_HIP(hipGetLastError());
     const int threads = 64 * kChainWaves;
-    static const int grid_reg =
-        persistent_grid(reinterpret_cast<const void *>(&nfa_chain_reg_kernel), threads);
+    static const int grid_reg = persistent_grid(
+        reinterpret_cast<const void *>(&nfa_chain_reg_kernel<true, true>), threads);
     static const int grid_lds_pairs = persistent_grid(
         reinterpret_cast<const void *>(&nfa_chain_lds_kernel<true, 1024>), threads);
     static const int grid_lds_first = persistent_grid(
         reinterpret_cast<const void *>(&nfa_chain_lds_kernel<false, 1024>), threads);
     if (d.pair && pairs) {
-        nfa_chain_reg_kernel<<<grid_reg, threads, 0, stream>>>(d, data, off, count, sc, chunk);
+        const bool acc_lds = d.tiles <= kAccLdsTiles;
+        if (d.start_always && acc_lds)
+            nfa_chain_reg_kernel<true, true><<<grid_reg, threads, 0, stream>>>(d, data, off, count, sc, chunk);
+        else if (d.start_always)
+            nfa_chain_reg_kernel<true, false><<<grid_reg, threads, 0, stream>>>(d, data, off, count, sc, chunk);
+        else if (acc_lds)
+            nfa_chain_reg_kernel<false, true><<<grid_reg, threads, 0, stream>>>(d, data, off, count, sc, chunk);
+        else
+            nfa_chain_reg_kernel<false, false><<<grid_reg, threads, 0, stream>>>(d, data, off, count, sc, chunk);
         GFSM_HIP(hipGetLastError());
         nfa_chain_lds_kernel<true, 1024><<<grid_lds_pairs, threads, 0, stream>>>(
             d, data, off, count, sc, chunk, sc.ovf1, sc.ovf2);

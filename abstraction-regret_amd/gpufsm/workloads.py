@@ -205,3 +205,41 @@ def workload_divergent(count: int = 1024, length: int = 1 << 20, seed: int = 0,
                        accepting=accepting)
     lengths = np.full(count, length, dtype=np.int64)
     return pu, pattern_inputs(pu, lengths, 0.95, seed=seed + 2)
+
+
+def serving_automata(seed: int = 0) -> tuple[dict, dict]:
+    """The serving mix: 4 random DFAs (256, 1024, 4096, 16384 states; accept probability
+    1e-4, so most requests are scanned to the end) and 4 pattern-union NFAs built as in
+    the ragged workload (about 1K, 4K, 16K and 64K states). Returns (automata by name,
+    pattern unions by name)."""
+    automata: dict = {}
+    unions: dict = {}
+    for k, n in enumerate((256, 1024, 4096, 16384)):
+        automata[f"dfa{n}"] = dense_dfa(n, 1e-4, seed=seed + k)
+    for k, n in enumerate((1024, 4096, 16384, 65536)):
+        patterns = max(1, round(n / 22.5))
+        pu = pattern_union(patterns, (5, 40), (1, 32), seed=seed + 10 + k, total_states=n)
+        automata[f"nfa{n}"] = pu.nfa
+        unions[f"nfa{n}"] = pu
+    return automata, unions
+
+
+def serving_requests(automata: dict, unions: dict, count: int, seed: int = 0,
+                     median: int = 16384, sigma: float = 1.0,
+                     max_len: int = 4 << 20) -> list[tuple[str, bytes]]:
+    """``count`` requests: uniform automaton choice, log-normal lengths (median 16 KiB,
+    clipped to 4 MiB); NFA requests draw 70% of bytes from the union's classes."""
+    rng = np.random.default_rng(seed)
+    names = sorted(automata)
+    lengths = lognormal_lengths(count, median, sigma, 1, max_len, seed=seed + 1)
+    out = []
+    for k in range(count):
+        name = names[int(rng.integers(0, len(names)))]
+        n = int(lengths[k])
+        if name in unions:
+            d, _ = pattern_inputs(unions[name], np.array([n], dtype=np.int64), 0.70,
+                                  seed=seed * 100003 + k)
+        else:
+            d = rng.integers(0, 256, size=n, dtype=np.uint8)
+        out.append((name, d.tobytes()))
+    return out

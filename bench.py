@@ -1,14 +1,19 @@
 #!/usr/bin/env python3
 """Benchmark of the gpufsm HIP hot path on one MI355X; prints ONE JSON line.
 
-Workload (default ``dfa``): the DFA memory-bound face — a seeded random complete DFA with
-4096 states (uint16 table with folded accept bits, 2 MiB), a batch of 4096 inputs of 64 KiB
-uniform random bytes already resident in HBM. The headline ``value`` is measured on the
-non-latching control (``accept_prob = 0``, as the upstream latch control does), where every
-input byte is examined, so input GB/s counts bytes actually read. The latching variant as
-configured (10% accepting states) is reported beside it with both examined and credited
-bytes, and so is the plain per-call drop-in (``gpufsm.run_batch``: upload, copy, launch,
-copy back).
+Headline: the DFA memory-bound face — a seeded random complete DFA (``--states``, default
+256: the uint16-table variant that fits LDS; 4096 is reported beside it), a batch of 4096
+inputs of 64 KiB uniform random bytes already resident in HBM. The ``value`` is measured on
+the non-latching control (``accept_prob = 0``, as the upstream latch control does), where
+every input byte is examined, so input GB/s counts bytes actually read. The latching
+variant as configured (10% accepting states) is reported beside it with both examined and
+credited bytes, and so is the plain per-call drop-in (``gpufsm.run_batch``: upload, copy,
+launch, copy back).
+
+Extras (each gated against the C oracle before it is timed): the ragged NFA mix, plain
+per-call ``run_batch`` versus the resident, autotuned, graph-captured engine; the plain NFA
+drop-in in microseconds per call; the serving loop's p50/p99 latency and throughput over a
+QPS sweep.
 
 Order of work:
   1. build the workload, upload it, run once, and gate every verdict against the C oracle
@@ -213,15 +218,18 @@ def bench_dfa(args, torch) -> dict:
     kernel_name = KERNELS.get(tech, tech)
     workload_name = f"dfa{n}_{count}x{length}_nonlatching"
     return {
-        "metric": "input GB/s (examined bytes), DFA walk, inputs resident in HBM",
+        "metric": "input GB/s batched vs plain run_batch on MI355X; p50/p99 latency at fixed QPS",
         "value": head["examined_GBps"],
         "unit": "GB/s",
         "ms_per_step": head["ms_per_step"],
         "dtype": "u8",
         "config": {
-            "workload": f"DFA memory-bound face, non-latching control: random complete DFA "
+            "workload": f"DFA memory-bound face (batched: inputs resident in HBM; value = "
+                        f"examined input GB/s), non-latching control: random complete DFA "
                         f"{n} states x 256 (uint16 table, accept_prob 0), {count} inputs x "
-                        f"{length} uniform bytes (seed 0)",
+                        f"{length} uniform bytes (seed 0); plain per-call, the latching "
+                        f"variant, the 4096-state table, the ragged NFA mix, the plain NFA "
+                        f"drop-in and the serving sweep are under extras",
             "num_states": n, "num_inputs": count, "input_bytes": length,
             "technique": args.technique, "resolved_technique": tech,
         },
@@ -248,6 +256,195 @@ def bench_dfa(args, torch) -> dict:
     }
 
 
+def _examined_upper(flags: np.ndarray, lens: np.ndarray, offsets: np.ndarray) -> int:
+    """Bytes up to each first match (whole input when nothing matched)."""
+    return int(np.where(flags != 0, lens, np.diff(offsets)).sum())
+
+
+def bench_ragged(args, torch) -> dict:
+    """Ragged NFA mix: plain per-call ``gpufsm.run_batch`` vs the resident engine.
+
+    Plain: a fresh executor per call (automaton preprocessing and upload, packing of the
+    16384 ``bytes`` objects, host-to-device copy, launch, copy back, Result objects), as
+    upstream's per-call path. Batched: the automaton resident, the packed batch resident in
+    HBM, technique and tuning from the autotuner, the launch sequence captured into a HIP
+    graph and replayed. Both are gated against the C oracle (which also gives the examined
+    byte count); throughput is examined bytes per second of wall time per batch.
+    """
+    from gpufsm import run_batch, workloads as W
+    from gpufsm.batch import BatchEngine, DeviceBatch
+    from gpufsm.tune import Autotuner
+
+    from oracle import gate, native
+
+    count = args.ragged_count
+    t0 = time.perf_counter()
+    pu, (data, off) = W.workload_ragged(count=count)
+    _log(f"ragged nfa: {pu.nfa.num_states} states, {count} inputs, {off[-1] / 1e6:.1f} MB "
+         f"({time.perf_counter() - t0:.1f} s to generate)")
+    threads = _threads()
+    t0 = time.perf_counter()
+    wf, wn, wex = native.nfa_batch(pu.nfa, data, off, threads=threads)
+    cpu_s = time.perf_counter() - t0
+    examined, credited = int(wex.sum()), int(off[-1])
+    _log(f"ragged nfa: oracle {cpu_s:.1f} s on {threads} threads, {int(wf.sum())} accepted, "
+         f"{examined / 1e6:.1f} MB examined")
+
+    # batched: resident automaton + batch, autotuned, graph-captured
+    s = torch.cuda.current_stream()
+    engine = BatchEngine(tuner=Autotuner(persist=False))
+    t0 = time.perf_counter()
+    batch = DeviceBatch(data, off, stream=s)
+    choice = engine.enqueue(pu.nfa, batch, "auto", stream=s)
+    f, n = batch.fetch(s)
+    tune_s = time.perf_counter() - t0
+    gate.require_arrays(f, n, wf, wn, f"hip nfa engine {choice.label()} (ragged)")
+    steps = max(3, args.steps // 2)
+    wall, ks = _time_steps(torch, lambda: engine.enqueue(pu.nfa, batch, "auto", stream=s,
+                                                         graph=True), steps, 1)
+    f, n = batch.fetch(s)
+    gate.require_arrays(f, n, wf, wn, "hip nfa engine graph replay (ragged)")
+    batched_s = wall / steps
+    # the same with the host batch staged every time (pinned copy + H2D + launch + D2H)
+    t0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        batch.load(data, off, stream=s)
+        engine.enqueue(pu.nfa, batch, "auto", stream=s, graph=True)
+        batch.fetch(s)
+    staged_s = (time.perf_counter() - t0) / reps
+    key = next(iter(engine.tuner.table))
+    decision = engine.tuner.table[key]
+
+    # plain per-call drop-in
+    inputs = [data[off[i]:off[i + 1]].tobytes() for i in range(count)]
+    res = run_batch(pu.nfa, inputs, backend="hip")
+    got_f = np.fromiter((r.accepted for r in res), dtype=np.int32, count=count)
+    got_n = np.fromiter((r.match_len for r in res), dtype=np.int32, count=count)
+    gate.require_arrays(got_f, got_n, wf, wn, "hip nfa plain run_batch (ragged)")
+    plain = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        res = run_batch(pu.nfa, inputs, backend="hip")
+        plain.append(time.perf_counter() - t0)
+    plain_s = float(np.median(plain))
+    engine.close()
+    out = {
+        "workload": f"ragged NFA mix: {pu.nfa.num_states}-state union of 2000 patterns "
+                    f"(lengths 5-40, class widths 1-32, wildcard start), {count} inputs, "
+                    f"log-normal lengths (median 4 KiB, sigma 1, 64 B-1 MiB), 70% pattern bytes",
+        "examined_bytes": examined,
+        "credited_bytes": credited,
+        "accepted": int(wf.sum()),
+        "batched": {
+            "technique": choice.label(),
+            "ms_per_batch": 1e3 * batched_s,
+            "kernel_ms": float(np.mean(ks)),
+            "examined_GBps": examined / batched_s / 1e9,
+            "credited_GBps": credited / batched_s / 1e9,
+            "tuning_seconds": tune_s,
+            "autotuner": {"key": key, "rates_GBps_sample": decision["rates"]},
+        },
+        "batched_host_staged": {"ms_per_batch": 1e3 * staged_s,
+                                "examined_GBps": examined / staged_s / 1e9},
+        "plain_per_call": {"ms_per_batch": 1e3 * plain_s,
+                           "examined_GBps": examined / plain_s / 1e9,
+                           "credited_GBps": credited / plain_s / 1e9,
+                           "kernel_ms": res[0].kernel_ms},
+        "batched_over_plain": plain_s / batched_s,
+        "cpu_oracle": {"examined_GBps": examined / cpu_s / 1e9, "threads": threads,
+                       "seconds": cpu_s},
+    }
+    _log(f"ragged nfa: batched {out['batched']['examined_GBps']:.2f} GB/s "
+         f"({choice.label()}), plain {out['plain_per_call']['examined_GBps']:.3f} GB/s, "
+         f"ratio {out['batched_over_plain']:.1f}x")
+    return out
+
+
+def bench_plain_nfa(args, torch) -> dict:
+    """Plain NFA drop-in (1024 states, 64 x 4 KiB): microseconds per call."""
+    from gpufsm import run_batch, workloads as W
+    from gpufsm.batch import BatchEngine, DeviceBatch
+
+    from oracle import gate, native
+
+    nfa, (data, off) = W.workload_plain()
+    wf, wn, wex = native.nfa_batch(nfa, data, off)
+    inputs = [data[off[i]:off[i + 1]].tobytes() for i in range(off.size - 1)]
+    res = run_batch(nfa, inputs, backend="hip")
+    gate.require_arrays(np.array([r.accepted for r in res], np.int32),
+                        np.array([r.match_len for r in res], np.int32), wf, wn,
+                        "hip plain nfa run_batch")
+    calls = []
+    for _ in range(20):
+        t0 = time.perf_counter()
+        run_batch(nfa, inputs, backend="hip")
+        calls.append(time.perf_counter() - t0)
+    s = torch.cuda.current_stream()
+    engine = BatchEngine(tuner=None)
+    batch = DeviceBatch(data, off, stream=s)
+    engine.enqueue(nfa, batch, "auto", stream=s)
+    f, n = batch.fetch(s)
+    gate.require_arrays(f, n, wf, wn, "hip plain nfa engine")
+    wall, ks = _time_steps(torch, lambda: engine.enqueue(nfa, batch, "auto", stream=s,
+                                                         graph=True), 50, 5)
+    engine.close()
+    per_call = float(np.median(calls))
+    return {
+        "workload": "plain NFA drop-in: random NFA 1024 states x 4 edges, 5% accepting, "
+                    "64 inputs x 4096 uniform bytes",
+        "examined_bytes": int(wex.sum()),
+        "credited_bytes": int(off[-1]),
+        "plain_per_call_us": 1e6 * per_call,
+        "plain_per_call_p90_us": 1e6 * float(np.percentile(calls, 90)),
+        "resident_graph_us": 1e6 * wall / 50,
+        "resident_kernel_us": 1e3 * float(np.mean(ks)),
+        "plain_credited_GBps": int(off[-1]) / per_call / 1e9,
+    }
+
+
+def bench_serving(args, torch) -> dict:
+    """Serving loop: 8 resident automata, Poisson arrivals at a QPS sweep."""
+    from gpufsm import workloads as W
+    from gpufsm.serve import ServingLoop, poisson_drive
+
+    from oracle import gate, native
+
+    automata, unions = W.serving_automata()
+    loop = ServingLoop(automata, budget_bytes=32 << 30, slots=4, max_batch_bytes=16 << 20,
+                       max_batch_count=1024)
+    sweep = []
+    with loop:
+        warm = W.serving_requests(automata, unions, 64, seed=99)
+        poisson_drive(loop, warm, qps=2000, seed=99)
+        for k, qps in enumerate(args.serve_qps):
+            n = max(64, int(qps * args.serve_seconds))
+            reqs = W.serving_requests(automata, unions, n, seed=1000 + k)
+            t_start = time.perf_counter()
+            done = poisson_drive(loop, reqs, qps=qps, seed=2000 + k)
+            st = loop.stats(since=t_start)
+            st["qps_offered"] = qps
+            st["seconds"] = time.perf_counter() - t_start
+            st["mean_request_bytes"] = float(np.mean([len(d) for _, d in reqs]))
+            sweep.append(st)
+            _log(f"serving qps={qps}: p50 {st['p50_ms']:.2f} ms p99 {st['p99_ms']:.2f} ms, "
+                 f"{st['credited_GBps']:.2f} GB/s credited, {st['rejected']} rejected")
+            # gate a sample of the served verdicts against the oracle
+            for r, (name, data) in list(zip(done, reqs))[:: max(1, n // 48)]:
+                a = automata[name]
+                d = np.frombuffer(data, dtype=np.uint8)
+                o = np.array([0, len(data)], dtype=np.int64)
+                fn = native.dfa_batch if name.startswith("dfa") else native.nfa_batch
+                wf, wn, _ = fn(a, d, o)
+                gate.require_arrays(np.array([int(r.accepted)], np.int32),
+                                    np.array([r.match_len], np.int32), wf, wn,
+                                    f"served {name} request")
+    return {"workload": "8 resident automata (DFAs 256/1K/4K/16K states, pattern unions "
+                        "1K/4K/16K/64K states), Poisson arrivals, log-normal request "
+                        "lengths (median 16 KiB, clipped 4 MiB), HBM budget 32 GiB, 4 slots",
+            "sweep": sweep}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -258,6 +455,12 @@ def main() -> int:
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--length", type=int, default=65536)
     ap.add_argument("--technique", default="auto")
+    ap.add_argument("--ragged-count", type=int, default=16384)
+    ap.add_argument("--serve-qps", type=float, nargs="+", default=[1000.0, 4000.0, 16000.0])
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--only", default="",
+                    help="comma list of extra sections to run (ragged,plain,serving); "
+                         "default all")
     ap.add_argument("--profile", action="store_true",
                     help="headline variant only, plus one device copy of the input bytes "
                          "(the calibration point for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -276,6 +479,14 @@ def main() -> int:
         print("bench.py needs a GPU (torch.cuda.is_available() is False)", file=sys.stderr)
         return 2
     out = bench_dfa(args, torch)
+    if not args.profile:
+        want = set(filter(None, args.only.split(","))) or {"ragged", "plain", "serving"}
+        if "ragged" in want:
+            out["extras"]["ragged_nfa"] = bench_ragged(args, torch)
+        if "plain" in want:
+            out["extras"]["plain_nfa"] = bench_plain_nfa(args, torch)
+        if "serving" in want:
+            out["extras"]["serving"] = bench_serving(args, torch)
     line = {
         "metric": out["metric"],
         "value": out["value"],
