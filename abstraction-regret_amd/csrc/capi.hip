@@ -169,9 +169,11 @@ int32_t resolve_dfa(const gfsm_dfa *h, int32_t technique, int32_t count, int64_t
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 // Device scratch of one NFA run (resolved technique).
-size_t nfa_workspace(const gfsm_nfa *h, int32_t tech, int32_t count) {
+size_t nfa_workspace(const gfsm_nfa *h, int32_t tech, int32_t count, int64_t total,
+                     const Tuning &tu) {
     if (count <= 0 || h->start_accepts) return 0;
-    if (tech == GFSM_NFA_CHAIN) return nfa_chain_scratch_bytes(count);
+    if (tech == GFSM_NFA_CHAIN)
+        return nfa_chain_scratch_bytes(h->chain, count, total, nfa_chain_chunk_bytes(tu.chunk));
     if (tech == GFSM_NFA_BITPACKED) return align256(static_cast<size_t>(count) * sizeof(int32_t));
     int32_t grid = 1;
     int64_t scratch_elems = 0;
@@ -181,17 +183,17 @@ size_t nfa_workspace(const gfsm_nfa *h, int32_t tech, int32_t count) {
 
 // Enqueue one NFA run; ws == nullptr allocates the scratch stream-ordered.
 void nfa_enqueue(gfsm_nfa *h, int32_t tech, const Tuning &tu, const uint8_t *d_data,
-                 const int64_t *d_off, int32_t count, int32_t *d_flags, int32_t *d_lens,
-                 void *ws, size_t ws_bytes, hipStream_t s) {
+                 const int64_t *d_off, int32_t count, int64_t total, int32_t *d_flags,
+                 int32_t *d_lens, void *ws, size_t ws_bytes, hipStream_t s) {
     if (count == 0) return;
     if (h->start_accepts) {
         launch_fill_result(d_flags, d_lens, count, 1, 0, s);
         return;
     }
-    const size_t need = nfa_workspace(h, tech, count);
+    const size_t need = nfa_workspace(h, tech, count, total, tu);
     if (!ws && need > 0) {
         StreamBuf buf(need, s);
-        nfa_enqueue(h, tech, tu, d_data, d_off, count, d_flags, d_lens, buf.p, need, s);
+        nfa_enqueue(h, tech, tu, d_data, d_off, count, total, d_flags, d_lens, buf.p, need, s);
         return;
     }
     if (ws_bytes < need)
@@ -199,7 +201,7 @@ void nfa_enqueue(gfsm_nfa *h, int32_t tech, const Tuning &tu, const uint8_t *d_d
                                      std::to_string(need) + " needed");
     if (tech == GFSM_NFA_CHAIN) {
         const bool pairs = !(tu.flags & GFSM_TUNE_CHAIN_FIRST) && env_int("GFSM_CHAIN_FIRST", 0) != 1;
-        launch_nfa_chain(h->chain, h->tile, d_data, d_off, count, d_flags, d_lens, ws,
+        launch_nfa_chain(h->chain, h->tile, d_data, d_off, count, total, d_flags, d_lens, ws,
                          nfa_chain_chunk_bytes(tu.chunk), pairs, s);
         return;
     }
@@ -460,9 +462,10 @@ int gfsm_nfa_run_host(gfsm_nfa *nfa, int32_t technique, const uint8_t *data,
         check_batch_args(data, offsets, num_strings, flags, lens);
         const int32_t tech = resolve_nfa(nfa, technique);
         run_host(data, offsets, num_strings, flags, lens, kernel_ms,
-                 [&](const uint8_t *dd, const int64_t *doff, int64_t, int32_t *df, int32_t *dl,
-                     hipStream_t s) {
-                     nfa_enqueue(nfa, tech, Tuning(), dd, doff, num_strings, df, dl, nullptr, 0, s);
+                 [&](const uint8_t *dd, const int64_t *doff, int64_t tot, int32_t *df,
+                     int32_t *dl, hipStream_t s) {
+                     nfa_enqueue(nfa, tech, Tuning(), dd, doff, num_strings, tot, df, dl,
+                                 nullptr, 0, s);
                  });
     });
 }
@@ -475,8 +478,8 @@ int gfsm_nfa_run_device(gfsm_nfa *nfa, int32_t technique, const uint8_t *d_data,
         check_batch_args(d_data, d_offsets, num_strings, d_flags, d_lens);
         if (total_bytes < 0) throw Error(GFSM_EINVAL, "total_bytes must be >= 0");
         const int32_t tech = resolve_nfa(nfa, technique);
-        nfa_enqueue(nfa, tech, Tuning(), d_data, d_offsets, num_strings, d_flags, d_lens,
-                    nullptr, 0, static_cast<hipStream_t>(stream));
+        nfa_enqueue(nfa, tech, Tuning(), d_data, d_offsets, num_strings, total_bytes, d_flags,
+                    d_lens, nullptr, 0, static_cast<hipStream_t>(stream));
     });
 }
 
@@ -486,8 +489,9 @@ int gfsm_nfa_workspace_size(gfsm_nfa *nfa, int32_t technique, int32_t num_string
     return guarded([&] {
         if (!nfa || !out_bytes) throw Error(GFSM_EINVAL, "null argument");
         if (num_strings < 0 || total_bytes < 0) throw Error(GFSM_EINVAL, "negative size");
-        (void)read_tuning(tuning);
-        *out_bytes = static_cast<int64_t>(nfa_workspace(nfa, resolve_nfa(nfa, technique), num_strings));
+        const Tuning tu = read_tuning(tuning);
+        *out_bytes = static_cast<int64_t>(
+            nfa_workspace(nfa, resolve_nfa(nfa, technique), num_strings, total_bytes, tu));
     });
 }
 
@@ -503,11 +507,11 @@ int gfsm_nfa_run_device_ws(gfsm_nfa *nfa, int32_t technique, const gfsm_tuning *
         check_ws(d_workspace, workspace_bytes);
         const Tuning tu = read_tuning(tuning);
         const int32_t tech = resolve_nfa(nfa, technique);
-        const size_t need = nfa_workspace(nfa, tech, num_strings);
+        const size_t need = nfa_workspace(nfa, tech, num_strings, total_bytes, tu);
         if (static_cast<size_t>(workspace_bytes) < need)
             throw Error(GFSM_EINVAL, "workspace of " + std::to_string(workspace_bytes) +
                                          " bytes, " + std::to_string(need) + " needed");
-        nfa_enqueue(nfa, tech, tu, d_data, d_offsets, num_strings, d_flags, d_lens,
+        nfa_enqueue(nfa, tech, tu, d_data, d_offsets, num_strings, total_bytes, d_flags, d_lens,
                     need > 0 ? d_workspace : nullptr, static_cast<size_t>(workspace_bytes),
                     static_cast<hipStream_t>(stream));
     });

@@ -57,12 +57,36 @@ __device__ __forceinline__ int32_t lane_rank(uint64_t m) {
         static_cast<uint32_t>(m >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u)));
 }
 
-// Scratch of one launch: best / ovf1 / ovf2 per input, chunk bases, two work counters.
+// Scratch of one launch. Work items are handed out chunk-major: round k holds chunk k of
+// every input that has more than k chunks, so the chunks of one input run one round apart
+// and a later chunk usually starts after an earlier one has found the input's first match
+// (it then skips at once). Inputs are ordered by chunk count, descending (a counting sort),
+// which makes round k a prefix of that order: item x of round k is perm[x - rbase[k]].
 struct ChainScratch {
     int32_t *best, *ovf1, *ovf2;
-    int64_t *cbase;            // [count + 1]: first item of each input, total at [count]
+    int32_t *nch;              // [count] chunks per input
+    int32_t *perm;             // [count] inputs by chunk count, descending
+    int32_t *hist;             // [K] inputs with exactly c chunks
+    int32_t *cursor;           // [K] scatter cursors
+    int32_t *rcount;           // [K] R_c = inputs with more than c chunks
+    int64_t *rbase;            // [K] first item of round k; total items at [maxk]
+    int32_t *maxk;             // [1] most chunks of any input
     unsigned int *counter;     // [2]: reg tier, lds tier
+    int32_t K;                 // capacity of the per-round arrays (> max chunks + 1)
 };
+
+__global__ void chain_zero_kernel(ChainScratch sc) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < sc.K) {
+        sc.hist[i] = 0;
+        sc.cursor[i] = 0;
+    }
+    if (i == 0) {
+        *sc.maxk = 0;
+        sc.counter[0] = 0u;
+        sc.counter[1] = 0u;
+    }
+}
 
 __global__ void chain_prep_kernel(const int64_t *__restrict__ off, int32_t count, int64_t chunk,
                                   ChainScratch sc) {
@@ -72,34 +96,65 @@ __global__ void chain_prep_kernel(const int64_t *__restrict__ off, int32_t count
         sc.ovf1[i] = 0;
         sc.ovf2[i] = 0;
         const int64_t len = off[i + 1] - off[i];
-        sc.cbase[i] = (len + chunk - 1) / chunk;  // chunk counts, scanned next
+        const int32_t c = static_cast<int32_t>((len + chunk - 1) / chunk);
+        sc.nch[i] = c;
+        if (c > 0) {
+            atomicAdd(&sc.hist[c], 1);
+            atomicMax(sc.maxk, c);
+        }
     }
 }
 
-// Exclusive scan of cbase[0..count) in place, cbase[count] = total; resets the counters.
-__global__ __launch_bounds__(1024) void chain_scan_kernel(ChainScratch sc, int32_t count) {
+// One block: R_c = sum of hist over (c, maxk] (a suffix sum), then rbase = exclusive prefix
+// sums of R over rounds 0..maxk-1 (rbase[maxk] = number of items).
+__global__ __launch_bounds__(1024) void chain_rounds_kernel(ChainScratch sc) {
     __shared__ int64_t part[1024];
+    const int32_t maxk = *sc.maxk;
+    const int32_t t = static_cast<int32_t>(threadIdx.x);
     int64_t carry = 0;
-    for (int32_t base = 0; base < count; base += 1024) {
-        const int32_t i = base + static_cast<int32_t>(threadIdx.x);
-        const int64_t v = i < count ? sc.cbase[i] : 0;
-        part[threadIdx.x] = v;
+    // suffix sums: position p of the walk is c = maxk - p
+    for (int32_t base = 0; base <= maxk; base += 1024) {
+        const int32_t c = maxk - (base + t);
+        const int64_t v = (c >= 1) ? sc.hist[c] : 0;
+        part[t] = v;
         __syncthreads();
         for (int d = 1; d < 1024; d <<= 1) {
-            const int64_t add = threadIdx.x >= static_cast<unsigned>(d) ? part[threadIdx.x - d] : 0;
+            const int64_t add = t >= d ? part[t - d] : 0;
             __syncthreads();
-            part[threadIdx.x] += add;
+            part[t] += add;
             __syncthreads();
         }
-        if (i < count) sc.cbase[i] = carry + part[threadIdx.x] - v;
+        // inclusive sum over [c, maxk] minus hist[c] = R_c
+        if (c >= 0) sc.rcount[c] = static_cast<int32_t>(carry + part[t] - v);
         const int64_t tile_total = part[1023];
         __syncthreads();
         carry += tile_total;
     }
-    if (threadIdx.x == 0) {
-        sc.cbase[count] = carry;
-        sc.counter[0] = 0u;
-        sc.counter[1] = 0u;
+    __syncthreads();
+    carry = 0;
+    for (int32_t base = 0; base <= maxk; base += 1024) {
+        const int32_t k = base + t;
+        const int64_t v = k < maxk ? sc.rcount[k] : 0;
+        part[t] = v;
+        __syncthreads();
+        for (int d = 1; d < 1024; d <<= 1) {
+            const int64_t add = t >= d ? part[t - d] : 0;
+            __syncthreads();
+            part[t] += add;
+            __syncthreads();
+        }
+        if (k <= maxk) sc.rbase[k] = carry + part[t] - v;
+        const int64_t tile_total = part[1023];
+        __syncthreads();
+        carry += tile_total;
+    }
+}
+
+__global__ void chain_scatter_kernel(ChainScratch sc, int32_t count) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) {
+        const int32_t c = sc.nch[i];
+        if (c > 0) sc.perm[sc.rcount[c] + atomicAdd(&sc.cursor[c], 1)] = i;
     }
 }
 
@@ -132,22 +187,23 @@ struct Item {
 // full mask: no branch precedes the broadcast. (A lane-0-only branch there lets the
 // compiler peel lane 0 off into a loop of its own, after which the other lanes never see a
 // new item.)
-__device__ __forceinline__ bool next_item(unsigned int *counter, const int64_t *__restrict__ cbase,
-                                          const int64_t *__restrict__ off, int32_t count,
+__device__ __forceinline__ bool next_item(unsigned int *counter, const ChainScratch &sc,
+                                          const int64_t *__restrict__ off, int32_t maxk,
                                           int64_t items, int64_t chunk, int64_t overlap, int lane,
                                           Item &it) {
     const unsigned int mine = atomicAdd(counter, lane == 0 ? 1u : 0u);
     const unsigned int item = __builtin_amdgcn_readfirstlane(mine);
     if (static_cast<int64_t>(item) >= items) return false;
-    int32_t lo = 0, hi = count;  // input: largest i with cbase[i] <= item
+    int32_t lo = 0, hi = maxk;  // round: largest k with rbase[k] <= item
     while (hi - lo > 1) {
         const int32_t mid = (lo + hi) >> 1;
-        if (cbase[mid] <= static_cast<int64_t>(item)) lo = mid; else hi = mid;
+        if (sc.rbase[mid] <= static_cast<int64_t>(item)) lo = mid; else hi = mid;
     }
-    it.in = lo;
-    it.ib = off[lo];
-    const int64_t ie = off[lo + 1];
-    it.cs = it.ib + (static_cast<int64_t>(item) - cbase[lo]) * chunk;
+    const int32_t in = sc.perm[static_cast<int64_t>(item) - sc.rbase[lo]];
+    it.in = in;
+    it.ib = off[in];
+    const int64_t ie = off[in + 1];
+    it.cs = it.ib + static_cast<int64_t>(lo) * chunk;
     it.ce = (it.cs + chunk) < ie ? (it.cs + chunk) : ie;
     it.ps = (it.cs - overlap) > it.ib ? (it.cs - overlap) : it.ib;
     return true;
@@ -179,12 +235,13 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_reg_kernel(
         for (int i = threadIdx.x; i < (d.tiles + 1) * kTileWords; i += blockDim.x) s_acc[i] = d.acc[i];
     __syncthreads();
     const int lane = threadIdx.x & 63;
-    const int64_t items = sc.cbase[count];
+    const int32_t maxk = *sc.maxk;
+    const int64_t items = sc.rbase[maxk];
     const int64_t data_end = off[count];
     const int64_t overlap = d.max_chain > 0 ? d.max_chain - 1 : 0;
     const char *__restrict__ next_base = reinterpret_cast<const char *>(d.next);
     Item it;
-    while (next_item(&sc.counter[0], sc.cbase, off, count, items, chunk, overlap, lane, it)) {
+    while (next_item(&sc.counter[0], sc, off, maxk, items, chunk, overlap, lane, it)) {
         bool stop = settled(sc.best, it, it.cs);  // wave-uniform throughout
         int32_t slot = -1;                        // this lane's listed state, -1 = free
         uint64_t occ = 0;                         // lanes holding a state
@@ -288,7 +345,8 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_reg_kernel(
                     }
                 }
             }
-            if (!stop && ((w0 - abase) & 4095) == 3840 && settled(sc.best, it, w0 + 256)) stop = true;
+            // another chunk of this input may have found an earlier match meanwhile
+            if (!stop && settled(sc.best, it, w0 + 256)) stop = true;
         }
         if (lane == 0) {
             if (found > 0) atomicMin(&sc.best[it.in], found);
@@ -307,11 +365,12 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_lds_kernel(
     __shared__ int32_t lists[kChainWaves][2][kCap];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
-    const int64_t items = sc.cbase[count];
+    const int32_t maxk = *sc.maxk;
+    const int64_t items = sc.rbase[maxk];
     const int64_t data_end = off[count];
     const int64_t overlap = d.max_chain > 0 ? d.max_chain - 1 : 0;
     Item it;
-    while (next_item(&sc.counter[1], sc.cbase, off, count, items, chunk, overlap, lane, it)) {
+    while (next_item(&sc.counter[1], sc, off, maxk, items, chunk, overlap, lane, it)) {
         bool stop = (only && !only[it.in]) || settled(sc.best, it, it.cs);  // wave-uniform
         int32_t *A = lists[wv][0];
         int32_t *B = lists[wv][1];
@@ -396,7 +455,8 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_lds_kernel(
                     break;
                 }
             }
-            if (!stop && ((w0 - abase) & 4095) == 3840 && settled(sc.best, it, w0 + 256)) stop = true;
+            // another chunk of this input may have found an earlier match meanwhile
+            if (!stop && settled(sc.best, it, w0 + 256)) stop = true;
         }
         if (lane == 0) {
             if (found > 0) atomicMin(&sc.best[it.in], found);
@@ -421,32 +481,58 @@ int32_t nfa_chain_chunk_bytes(int32_t requested) {
     return static_cast<int32_t>(c < 64 ? 64 : (c > (1 << 30) ? (1 << 30) : c));
 }
 
-size_t nfa_chain_scratch_bytes(int32_t count) {
+// Chunk length actually used: inputs are cut only when the start state is always active.
+static int64_t effective_chunk(const NfaChainDev &d, int32_t chunk_bytes) {
+    return d.start_always ? static_cast<int64_t>(chunk_bytes) : (int64_t(1) << 62);
+}
+
+// Capacity of the per-round arrays: more than the most chunks any input can have.
+static int32_t round_capacity(int64_t total_bytes, int64_t chunk) {
+    const int64_t k = total_bytes / chunk + 3;
+    return static_cast<int32_t>(k < (int64_t(1) << 30) ? k : (int64_t(1) << 30));
+}
+
+size_t nfa_chain_scratch_bytes(const NfaChainDev &d, int32_t count, int64_t total_bytes,
+                               int32_t chunk_bytes) {
     const size_t c = static_cast<size_t>(count);
-    return 3 * align256(c * 4) + align256((c + 1) * 8) + 256;
+    const size_t K = static_cast<size_t>(round_capacity(total_bytes, effective_chunk(d, chunk_bytes)));
+    return 5 * align256(c * 4) + 3 * align256(K * 4) + align256(K * 8) + 256 + 256;
 }
 
 void launch_nfa_chain(const NfaChainDev &d, const NfaTileDev &tiles, const uint8_t *data,
-                      const int64_t *off, int32_t count, int32_t *flags, int32_t *lens,
-                      void *scratch, int32_t chunk_bytes, bool pairs, hipStream_t stream) {
+                      const int64_t *off, int32_t count, int64_t total_bytes, int32_t *flags,
+                      int32_t *lens, void *scratch, int32_t chunk_bytes, bool pairs,
+                      hipStream_t stream) {
     if (count <= 0) return;
+    const int64_t chunk = effective_chunk(d, chunk_bytes);
     const size_t c = static_cast<size_t>(count);
-    char *p = static_cast<char *>(scratch);
     ChainScratch sc;
-    sc.best = reinterpret_cast<int32_t *>(p);
-    p += align256(c * 4);
-    sc.ovf1 = reinterpret_cast<int32_t *>(p);
-    p += align256(c * 4);
-    sc.ovf2 = reinterpret_cast<int32_t *>(p);
-    p += align256(c * 4);
-    sc.cbase = reinterpret_cast<int64_t *>(p);
-    p += align256((c + 1) * 8);
-    sc.counter = reinterpret_cast<unsigned int *>(p);
-    // without an always-active start state, inputs are not cut
-    const int64_t chunk = d.start_always ? static_cast<int64_t>(chunk_bytes) : (int64_t(1) << 62);
+    sc.K = round_capacity(total_bytes, chunk);
+    const size_t K = static_cast<size_t>(sc.K);
+    char *p = static_cast<char *>(scratch);
+    auto take = [&](size_t bytes) {
+        char *q = p;
+        p += align256(bytes);
+        return q;
+    };
+    sc.best = reinterpret_cast<int32_t *>(take(c * 4));
+    sc.ovf1 = reinterpret_cast<int32_t *>(take(c * 4));
+    sc.ovf2 = reinterpret_cast<int32_t *>(take(c * 4));
+    sc.nch = reinterpret_cast<int32_t *>(take(c * 4));
+    sc.perm = reinterpret_cast<int32_t *>(take(c * 4));
+    sc.hist = reinterpret_cast<int32_t *>(take(K * 4));
+    sc.cursor = reinterpret_cast<int32_t *>(take(K * 4));
+    sc.rcount = reinterpret_cast<int32_t *>(take(K * 4));
+    sc.rbase = reinterpret_cast<int64_t *>(take(K * 8));
+    sc.maxk = reinterpret_cast<int32_t *>(take(4));
+    sc.counter = reinterpret_cast<unsigned int *>(take(8));
+    chain_zero_kernel<<<(sc.K + 255) / 256, 256, 0, stream>>>(sc);
+    GFSM_HIP(hipGetLastError());
     chain_prep_kernel<<<(count + 255) / 256, 256, 0, stream>>>(off, count, chunk, sc);
     GFSM_HIP(hipGetLastError());
-    chain_scan_kernel<<<1, 1024, 0, stream>>>(sc, count);
+    chain_rounds_kernel<<<1, 1024, 0, stream>>>(sc);
+    GFSM_HIP(hipGetLastError());
+    chain_scatter_kernel<<<(count + 255) / 256, 256, 0, stream>>>(sc, count);
     GFSM_HIP(hipGetLastError());
     const int threads = 64 * kChainWaves;
     static const int grid_reg = persistent_grid(

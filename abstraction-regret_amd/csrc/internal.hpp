@@ -148,12 +148,14 @@ void launch_nfa_best_finish(const int32_t *best, int32_t count, int32_t *flags, 
                             hipStream_t stream);
 
 // Sparse chain technique: persistent waves over (input, chunk) work items.
-size_t nfa_chain_scratch_bytes(int32_t count);
+size_t nfa_chain_scratch_bytes(const NfaChainDev &d, int32_t count, int64_t total_bytes,
+                               int32_t chunk_bytes);
 // chunk_bytes when the caller has no preference: GFSM_CHAIN_CHUNK (read per call), or 4096
 int32_t nfa_chain_chunk_bytes(int32_t requested);
 // pairs: use the byte-pair lists (register tier first) when the handle has them.
 void launch_nfa_chain(const NfaChainDev &d, const NfaTileDev &tiles, const uint8_t *data,
-                      const int64_t *off, int32_t count, int32_t *flags, int32_t *lens,
-                      void *scratch, int32_t chunk_bytes, bool pairs, hipStream_t stream);
+                      const int64_t *off, int32_t count, int64_t total_bytes, int32_t *flags,
+                      int32_t *lens, void *scratch, int32_t chunk_bytes, bool pairs,
+                      hipStream_t stream);
 
 }  // namespace gfsm
