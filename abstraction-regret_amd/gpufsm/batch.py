@@ -174,7 +174,7 @@ class BatchEngine:
         key = self.tuner.signature(kind.value, self.arch, dev.props, batch.count, batch.total)
         offsets = None
 
-        def timer(choice: Choice):
+        def timer(choice: Choice, give_up: float | None = None):
             nonlocal offsets
             torch = self._torch
             if offsets is None:
@@ -182,9 +182,15 @@ class BatchEngine:
             k = int(np.searchsorted(offsets, self.tuner.sample_bytes, side="right")) - 1
             k = max(1, min(batch.count, k))
             s = stream or torch.cuda.current_stream()
+            w0 = torch.cuda.Event(enable_timing=True)
+            w1 = torch.cuda.Event(enable_timing=True)
+            w0.record(s)
             self._launch(dev, kind, batch, choice, s, count=k)  # warm-up (and attributes)
-            times = []
-            for _ in range(self.tuner.repeats):
+            w1.record(s)
+            w1.synchronize()
+            times = [w0.elapsed_time(w1) / 1e3]
+            repeats = self.tuner.repeats if give_up is None or times[0] < give_up else 0
+            for _ in range(repeats):
                 e0 = torch.cuda.Event(enable_timing=True)
                 e1 = torch.cuda.Event(enable_timing=True)
                 e0.record(s)

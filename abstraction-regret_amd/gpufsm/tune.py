@@ -13,8 +13,8 @@ so a later batch of the same class reuses the decision without timing anything. 
 persists across processes (``GFSM_TUNE_TABLE`` or ``~/.cache/gpufsm/hip_autotune.json``).
 
 Candidates
-  NFA, chain-structured: ``chain`` with chunks of 1, 4 and 16 KiB, ``chain`` without the
-  byte-pair lists, ``bitpacked`` (dense active sets) and ``worklist``.
+  NFA, chain-structured: ``chain`` with chunks of 2, 4 and 8 KiB, ``chain`` without the
+  byte-pair lists, ``bitpacked`` (dense active sets) and, up to 4096 states, ``worklist``.
   NFA, general: ``worklist`` only.
   DFA: ``gather``; ``lds`` when the table fits LDS; ``split`` (segment 1, 2 and 4 KiB when
   the table fits LDS, the library's choice otherwise) when inputs average at least 1 KiB.
@@ -78,10 +78,12 @@ def candidates(kind: str, props, mean_len: float) -> list[Choice]:
     if kind == "nfa":
         if not props.chain_ok:
             return [Choice("worklist")]
-        return [Choice("chain", chunk_bytes=1024), Choice("chain", chunk_bytes=4096),
-                Choice("chain", chunk_bytes=16384),
-                Choice("chain", chunk_bytes=4096, flags=_lib.TUNE_CHAIN_FIRST),
-                Choice("bitpacked"), Choice("worklist")]
+        out = [Choice("chain", chunk_bytes=c) for c in (2048, 4096, 8192)]
+        out += [Choice("chain", chunk_bytes=4096, flags=_lib.TUNE_CHAIN_FIRST),
+                Choice("bitpacked")]
+        if props.num_states <= 4096:  # the general list walk only competes on small unions
+            out.append(Choice("worklist"))
+        return out
     out = [Choice("gather")]
     if props.lds_ok:
         out.append(Choice("lds"))
@@ -96,7 +98,7 @@ def candidates(kind: str, props, mean_len: float) -> list[Choice]:
 class Autotuner:
     """Chooses a :class:`Choice` per shape class; see the module docstring."""
 
-    def __init__(self, path: str | None = None, sample_bytes: int = 64 << 20,
+    def __init__(self, path: str | None = None, sample_bytes: int = 16 << 20,
                  repeats: int = 3, persist: bool = True) -> None:
         self.path = path or default_table_path()
         self.sample_bytes = int(sample_bytes)
@@ -133,9 +135,9 @@ class Autotuner:
     def decide(self, key: str, kind: str, props, count: int, total: int, timer) -> Decision:
         """The recorded decision for ``key``, or time the candidates with ``timer``.
 
-        ``timer(choice, sample_count) -> (seconds, examined_bytes)`` runs one candidate
-        over the first ``sample_count`` inputs; it may raise for a candidate that does not
-        apply, which then drops out.
+        ``timer(choice, give_up) -> (seconds, examined_bytes)`` runs one candidate over the
+        sample (repeating it only while it stays under ``give_up`` seconds, if given); it may
+        raise for a candidate that does not apply, which then drops out.
         """
         with self._lock:
             hit = self.lookup(key)
@@ -146,7 +148,7 @@ class Autotuner:
         rates: dict[str, float] = {}
         for cand in candidates(kind, props, mean):
             try:
-                secs, examined = timer(cand)
+                secs, examined = timer(cand, None if best is None else 8.0 * best[0])
             except _lib.HIPError:
                 continue
             rate = examined / secs / 1e9 if secs > 0 else float("inf")

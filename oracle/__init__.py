@@ -20,7 +20,6 @@ Contents and the upstream code each restates:
                     ``bitmap`` (`src/gpufsm/backends/cpu.py:28-67`) for tests.
 * ``gate.py``       the oracle gate ``compare`` / ``matches`` / ``require``
                     (`src/gpufsm/bench/oracle.py:14-80`).
-* ``configs.py``    seeded generators of the five benchmark workloads.
 
 Parity status of every function is recorded in ``tests/test_oracle.py``; none is unpinned.
 """
