@@ -166,9 +166,10 @@ class BatchEngine:
     # -- technique selection ------------------------------------------------------------
     def choose(self, automaton, batch: DeviceBatch, technique: str = "auto",
                stream=None) -> Choice:
-        """The technique + tuning a launch of ``batch`` uses (``technique`` unless "auto")."""
+        """The technique + tuning a launch of ``batch`` uses (``technique`` unless "auto";
+        a label such as ``"chain:chunk=2048"`` fixes the tuning too)."""
         if technique != "auto" or self.tuner is None or batch.count == 0:
-            return Choice(technique)
+            return Choice.parse(technique)
         kind = Kind.of(automaton)
         dev = self.cache.get(automaton)
         key = self.tuner.signature(kind.value, self.arch, dev.props, batch.count, batch.total)

@@ -56,6 +56,22 @@ class Choice:
     def tuning(self):
         return make_tuning(self.chunk_bytes, self.segment_bytes, self.flags)
 
+    @classmethod
+    def parse(cls, label: str) -> Choice:
+        """Inverse of :meth:`label`: ``"chain:chunk=2048"``, ``"split:segment=1024"``, ..."""
+        name, *parts = label.split(":")
+        kw = {"chunk_bytes": 0, "segment_bytes": 0, "flags": 0}
+        for part in parts:
+            if part == "first":
+                kw["flags"] |= _lib.TUNE_CHAIN_FIRST
+            elif part.startswith("chunk="):
+                kw["chunk_bytes"] = int(part[6:])
+            elif part.startswith("segment="):
+                kw["segment_bytes"] = int(part[8:])
+            else:
+                raise ValueError(f"bad technique label {label!r}")
+        return cls(name, **kw)
+
 
 @dataclass
 class Decision:
