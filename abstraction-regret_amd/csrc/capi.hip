@@ -371,6 +371,12 @@ int gfsm_nfa_create(int32_t num_states, int32_t start_state, const uint8_t *acce
                 if (!t.pair.empty()) {
                     h->chain.pair = upload(B, by, t.pair.data(), t.pair.size());
                     h->chain.pair_slot = upload(B, by, t.pair_slot.data(), t.pair_slot.size());
+                    if (static_cast<int64_t>(t.tiles) * kTileStates <= 65536) {
+                        // half the footprint: the register tier's L2 working set is the
+                        // continuation table plus these lists
+                        std::vector<uint16_t> s16(t.pair_slot.begin(), t.pair_slot.end());
+                        h->chain.pair_slot16 = upload(B, by, s16.data(), s16.size());
+                    }
                 }
                 h->chain.bflags = upload(B, by, t.bflags.data(), t.bflags.size());
                 for (int i = 0; i < 4; ++i) {
@@ -448,7 +454,11 @@ int gfsm_nfa_analyze(int32_t num_states, int32_t start_state, const uint8_t *acc
             add(t.mask.size(), 8); add(t.first.size(), 8); add(t.hassucc.size(), 8);
             add(t.acc.size(), 8); add(t.next.size(), 8); add(t.first_off.size(), 4);
             add(t.first_slot.size(), 4); add(t.bflags.size(), 4);
-            if (!t.pair.empty()) { add(t.pair.size(), 4); add(t.pair_slot.size(), 4); }
+            if (!t.pair.empty()) {
+                add(t.pair.size(), 4);
+                add(t.pair_slot.size(), 4);
+                if (static_cast<int64_t>(t.tiles) * kTileStates <= 65536) add(t.pair_slot.size(), 2);
+            }
         }
         out->device_bytes = bytes;
     });

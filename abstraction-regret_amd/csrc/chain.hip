@@ -163,7 +163,9 @@ __global__ void chain_scatter_kernel(ChainScratch sc, int32_t count) {
 __device__ __forceinline__ uint32_t chain_load_word(const uint8_t *__restrict__ data, int64_t a,
                                                     int64_t stop, int64_t data_end) {
     if (a >= stop) return 0;
-    if (a >= 0 && a + 4 <= data_end) return *reinterpret_cast<const uint32_t *>(data + a);
+    // streamed once: non-temporal, so it does not push the automaton tables out of L2
+    if (a >= 0 && a + 4 <= data_end)
+        return __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(data + a));
     uint32_t w = 0;
     for (int q = 0; q < 4; ++q)
         if (a + q >= 0 && a + q < data_end) w |= static_cast<uint32_t>(data[a + q]) << (8 * q);
@@ -217,22 +219,30 @@ __device__ __forceinline__ bool settled(const int32_t *best, const Item &it, int
 
 // ------------------------------------------------------------------------- reg tier ----
 
-constexpr int kAccLdsTiles = 32;  // accept bits staged in LDS up to this many tiles
+// "Small" automata (at most 16 tiles, 65536 slots): accept bits staged in LDS and 16-bit
+// pair lists, which keeps the continuation table and the lists in each XCD's L2 together.
+constexpr int kSmallTiles = 16;
 
 // Register tier. Per 256-byte window, each lane holds 4 bytes of input (for the loads) and,
-// for the bytes r = 64 q + lane (q = 0..3), the byte, its flags and the pair entry of the
-// depth-1 set before it, so a byte step reads everything it needs with three readlanes and
-// issues its (at most three) memory loads together. A byte step has one branch, taken only
-// on an event (an accept, an empty set, a list overflow).
-template <bool kAlways, bool kAccLds>
+// for the bytes r = 64 q + lane (q = 0..3), everything about byte r that does not depend on
+// the listed states: the byte, the pair entry of the depth-1 set before it, whether the
+// start state's side accepts on it (a depth-1 accept, or the pair list's accept bit), and
+// whether the start state's side is empty after it. The depth-1 set and the start state
+// follow from the bytes alone, so this is computed for the whole window up front. A byte
+// step then reads three values with readlanes, issues its (at most three) loads together,
+// and has one branch, taken only on an event (an accept, an empty set, a list overflow).
+template <bool kAlways, bool kSmall>
 __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_reg_kernel(
     NfaChainDev d, const uint8_t *__restrict__ data, const int64_t *__restrict__ off,
     int32_t count, ChainScratch sc, int64_t chunk) {
     __shared__ uint32_t s_flags[256];
-    __shared__ uint64_t s_acc[kAccLds ? (kAccLdsTiles + 1) * kTileWords : 1];
+    __shared__ uint32_t s_acc[kSmall ? (kSmallTiles + 1) * kTileWords * 2 : 1];
     for (int i = threadIdx.x; i < 256; i += blockDim.x) s_flags[i] = d.bflags[i];
-    if (kAccLds)
-        for (int i = threadIdx.x; i < (d.tiles + 1) * kTileWords; i += blockDim.x) s_acc[i] = d.acc[i];
+    if (kSmall) {
+        const uint32_t *acc32 = reinterpret_cast<const uint32_t *>(d.acc);
+        for (int i = threadIdx.x; i < (d.tiles + 1) * kTileWords * 2; i += blockDim.x)
+            s_acc[i] = acc32[i];
+    }
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const int32_t maxk = *sc.maxk;
@@ -240,12 +250,15 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_reg_kernel(
     const int64_t data_end = off[count];
     const int64_t overlap = d.max_chain > 0 ? d.max_chain - 1 : 0;
     const char *__restrict__ next_base = reinterpret_cast<const char *>(d.next);
+    const char *__restrict__ acc_base = reinterpret_cast<const char *>(d.acc);
+    const char *__restrict__ slot_base = kSmall ? reinterpret_cast<const char *>(d.pair_slot16)
+                                                : reinterpret_cast<const char *>(d.pair_slot);
     Item it;
     while (next_item(&sc.counter[0], sc, off, maxk, items, chunk, overlap, lane, it)) {
         bool stop = settled(sc.best, it, it.cs);  // wave-uniform throughout
         int32_t slot = -1;                        // this lane's listed state, -1 = free
         uint64_t occ = 0;                         // lanes holding a state
-        bool s0 = true, ovf = false;
+        bool s0 = true, ovf = false;              // start state active before the window
         int32_t d1 = 256;                         // byte of the depth-1 set; 256 = none
         int32_t found = 0;
         const int64_t abase =
@@ -264,7 +277,7 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_reg_kernel(
                 fq[q] = s_flags[bq[q]];
             }
             // kpos: first byte of the window (>= lo) on which the start state has no
-            // self-loop; the start state is active before byte r while kpos >= r
+            // self-loop; the start state is active before byte r while s0 && kpos >= r
             int32_t kpos = 1 << 20;
             if (!kAlways) {
 #pragma unroll
@@ -275,7 +288,9 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_reg_kernel(
                 }
             }
             uint32_t info[4];
-            int32_t pe_off[4], pe_w[4];
+            int32_t pe_off[4], pe_cnt[4];
+            int32_t d1_after = 256;  // lane 63, q = 3: state of the start side after byte 255
+            bool s0_after = true;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int32_t r = 64 * q + lane;
@@ -285,15 +300,25 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_reg_kernel(
                     bp = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(bq[q - 1]), 63));
                     fp = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(fq[q - 1]), 63));
                 }
+                const bool s0p = kAlways || (s0 && kpos >= r - 1);  // before byte r - 1
+                const bool s0r = kAlways || (s0 && kpos >= r);      // before byte r
+                const bool s0n = kAlways || (s0 && kpos > r);       // after byte r
                 // r == lo takes the running d1; every later r derives it from byte r - 1
-                const bool s0p = kAlways || (s0 && kpos >= r - 1);
                 const int32_t d1q =
                     r == lo ? d1 : ((s0p && (fp & 1u)) ? static_cast<int32_t>(bp) : 256);
+                const int32_t d1n = (s0r && (fq[q] & 1u)) ? static_cast<int32_t>(bq[q]) : 256;
                 const int32_t idx =
                     (r >= lo && r < hi) ? 2 * (d1q * 256 + static_cast<int32_t>(bq[q])) : 0;
+                const int32_t pw = d.pair[idx + 1];
                 pe_off[q] = d.pair[idx];
-                pe_w[q] = d.pair[idx + 1];
-                info[q] = bq[q] | (fq[q] << 8);
+                pe_cnt[q] = pw & INT_MAX;
+                const uint32_t start_acc = (pw < 0 || (s0r && (fq[q] & 2u))) ? 1u : 0u;
+                const uint32_t start_dead = (d1n == 256 && !s0n) ? 1u : 0u;
+                info[q] = bq[q] | (start_acc << 8) | (start_dead << 9);
+                if (q == 3) {
+                    d1_after = d1n;
+                    s0_after = s0n;
+                }
             }
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -303,34 +328,35 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_reg_kernel(
                     const uint32_t inf = static_cast<uint32_t>(
                         __builtin_amdgcn_readlane(static_cast<int>(info[q]), i));
                     const uint32_t b = inf & 0xffu;
-                    const uint32_t fl = inf >> 8;
-                    const int32_t add_off = __builtin_amdgcn_readlane(pe_off[q], i);
-                    const int32_t add_w = __builtin_amdgcn_readlane(pe_w[q], i);
-                    const int32_t add_cnt = add_w & INT_MAX;
+                    const uint32_t add_off = static_cast<uint32_t>(__builtin_amdgcn_readlane(pe_off[q], i));
+                    const int32_t add_cnt = __builtin_amdgcn_readlane(pe_cnt[q], i);
                     // additions go to the lanes free at the start of the byte, in lane order
                     const bool have = slot >= 0;
                     const int32_t frank = lane_rank(~occ);
                     int32_t cand = -1;
-                    if (!have && frank < add_cnt) cand = d.pair_slot[add_off + frank];
+                    if (!have && frank < add_cnt) {
+                        const uint32_t k = add_off + static_cast<uint32_t>(frank);
+                        cand = kSmall ? static_cast<int32_t>(
+                                            *reinterpret_cast<const uint16_t *>(slot_base + (k << 1)))
+                                      : *reinterpret_cast<const int32_t *>(slot_base + (k << 2));
+                    }
                     // continuation of held states; free lanes look up slot 0 and ignore it
                     const uint32_t t = have ? static_cast<uint32_t>(slot + 1) : 0u;
-                    const uint32_t noff = ((t >> 12) << 17) | (b << 9) | ((t >> 3) & 0x1f8u);
-                    const uint64_t nw = *reinterpret_cast<const uint64_t *>(next_base + noff);
-                    const uint64_t aw = kAccLds ? s_acc[t >> 6] : d.acc[t >> 6];
-                    const uint32_t bit = t & 63u;
-                    const bool alive = have && ((nw >> bit) & 1ull);
-                    const bool hit = alive && ((aw >> bit) & 1ull);
+                    // 32-bit words: tile t >> 12, byte b, word (t & 4095) >> 5
+                    const uint32_t noff = ((t >> 12) << 17) | (b << 9) | ((t >> 3) & 0x1fcu);
+                    const uint32_t nw = *reinterpret_cast<const uint32_t *>(next_base + noff);
+                    const uint32_t aw = kSmall ? s_acc[t >> 5]
+                                               : *reinterpret_cast<const uint32_t *>(acc_base + ((t >> 5) << 2));
+                    const uint32_t bit = t & 31u;
+                    const bool alive = have & static_cast<bool>((nw >> bit) & 1u);
+                    const bool hit = alive & static_cast<bool>((aw >> bit) & 1u);
                     const uint64_t occ_prev = occ;
                     slot = alive ? static_cast<int32_t>(t) : cand;
                     occ = __ballot(slot >= 0);
                     const uint64_t hitm = __ballot(hit);
-                    const int32_t d1n = (s0 && (fl & 1u)) ? static_cast<int32_t>(b) : 256;
-                    const bool s0n = kAlways || (s0 && (fl & 4u));
                     const bool ovf_now = add_cnt > 64 - __popcll(occ_prev);
-                    const bool acc_now = hitm != 0 || add_w < 0 || (s0 && (fl & 2u));
-                    const bool dead = occ == 0 && d1n == 256 && !s0n;
-                    s0 = s0n;
-                    d1 = d1n;
+                    const bool acc_now = hitm != 0 || (inf & 0x100u);
+                    const bool dead = occ == 0 && (inf & 0x200u);
                     if (ovf_now || acc_now || dead) {
                         const int32_t r = 64 * q + i;
                         if (ovf_now) {
@@ -345,6 +371,9 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_reg_kernel(
                     }
                 }
             }
+            // the start side after the window's last byte (only full windows continue)
+            d1 = __builtin_amdgcn_readlane(d1_after, 63);
+            s0 = kAlways || (__builtin_amdgcn_readlane(s0_after ? 1 : 0, 63) != 0);
             // another chunk of this input may have found an earlier match meanwhile
             if (!stop && settled(sc.best, it, w0 + 256)) stop = true;
         }
@@ -542,12 +571,12 @@ void launch_nfa_chain(const NfaChainDev &d, const NfaTileDev &tiles, const uint8
     static const int grid_lds_first = persistent_grid(
         reinterpret_cast<const void *>(&nfa_chain_lds_kernel<false, 1024>), threads);
     if (d.pair && pairs) {
-        const bool acc_lds = d.tiles <= kAccLdsTiles;
-        if (d.start_always && acc_lds)
+        const bool small = d.tiles <= kSmallTiles && d.pair_slot16 != nullptr;
+        if (d.start_always && small)
             nfa_chain_reg_kernel<true, true><<<grid_reg, threads, 0, stream>>>(d, data, off, count, sc, chunk);
         else if (d.start_always)
             nfa_chain_reg_kernel<true, false><<<grid_reg, threads, 0, stream>>>(d, data, off, count, sc, chunk);
-        else if (acc_lds)
+        else if (small)
             nfa_chain_reg_kernel<false, true><<<grid_reg, threads, 0, stream>>>(d, data, off, count, sc, chunk);
         else
             nfa_chain_reg_kernel<false, false><<<grid_reg, threads, 0, stream>>>(d, data, off, count, sc, chunk);

@@ -79,6 +79,7 @@ struct NfaChainDev {
     const int32_t *pair;           // [257 * 256][2] {offset, count | accepts << 31}, or
                                    // null; row 256 is all zero (no depth-1 set)
     const int32_t *pair_slot;      // lists + 64 padding entries
+    const uint16_t *pair_slot16;   // the same as uint16 when every slot is < 65536, else null
     const uint32_t *bflags;        // [256] bit 0 first_any, 1 first_acc, 2 start self-loop
     uint64_t first_any[4];         // bytes on which the start state has a successor
     uint64_t first_acc[4];         // ... an accepting one
