@@ -16,7 +16,8 @@ fixed here and named in the docstrings:
                       literal), class bytes drawn without replacement; the last position of
                       every pattern accepts (none with ``accepting=False``).
 * ``pattern_inputs``  each byte is, with probability ``p_pattern``, a uniform byte of the
-                      class of a uniformly chosen pattern position, otherwise a uniform byte.
+                      class of a uniformly chosen pattern position, otherwise a uniform byte
+                      (i.i.d. bytes, drawn from that mixture by an inverse-CDF table).
 * ``lognormal_lengths`` lengths ``exp(N(ln median, sigma))`` rounded and clipped.
 """
 
@@ -153,23 +154,33 @@ def lognormal_lengths(count: int, median: int, sigma: float, lo: int, hi: int,
 
 def pattern_inputs(pu: PatternUnion, lengths: np.ndarray, p_pattern: float,
                    seed: int = 0) -> tuple[np.ndarray, np.ndarray]:
-    """Inputs of the given lengths, biased toward the union's class bytes."""
+    """Inputs of the given lengths, biased toward the union's class bytes.
+
+    Each byte is, independently, with probability ``p_pattern`` a uniform member of the class
+    of a uniformly chosen pattern position, else a uniform byte. Bytes are i.i.d., so they
+    are drawn directly from that 256-way mixture through a 65536-entry inverse-CDF table
+    (probabilities rounded to 1/65536 by largest remainders): one table lookup per byte.
+    """
     rng = np.random.default_rng(seed)
     offsets = np.zeros(lengths.size + 1, dtype=np.int64)
     np.cumsum(lengths, out=offsets[1:])
     total = int(offsets[-1])
-    data = rng.integers(0, 256, size=total, dtype=np.uint8)
     positions = pu.class_start.size - 1
-    chunk = 1 << 24
+    width = np.diff(pu.class_start).astype(np.float64)
+    owner = np.repeat(np.arange(positions), np.diff(pu.class_start))
+    prob = np.full(256, (1.0 - p_pattern) / 256.0)
+    np.add.at(prob, pu.class_bytes.astype(np.int64), p_pattern / positions / width[owner])
+    slots = prob * 65536.0
+    counts = np.floor(slots).astype(np.int64)
+    rest = 65536 - int(counts.sum())
+    if rest > 0:
+        counts[np.argsort(-(slots - counts), kind="stable")[:rest]] += 1
+    table = np.repeat(np.arange(256, dtype=np.uint8), counts)
+    data = np.empty(total, dtype=np.uint8)
+    chunk = 1 << 26
     for a in range(0, total, chunk):
         b = min(total, a + chunk)
-        m = b - a
-        use = rng.random(m) < p_pattern
-        pos = rng.integers(0, positions, size=m)
-        width = pu.class_start[pos + 1] - pu.class_start[pos]
-        pick = pu.class_start[pos] + (rng.random(m) * width).astype(np.int64)
-        seg = data[a:b]
-        seg[use] = pu.class_bytes[pick[use]]
+        np.take(table, rng.integers(0, 65536, size=b - a, dtype=np.uint16), out=data[a:b])
     return data, offsets
 
 

@@ -465,6 +465,94 @@ def bench_serving(args, torch) -> dict:
             "sweep": sweep}
 
 
+def bench_divergent(args, torch) -> dict:
+    """High-divergence NFA: 65,536-state union (8000 patterns, class widths 16-64), 95%
+    pattern bytes; bitpacked vs chain vs worklist, and where the autotuner's choice flips.
+
+    The workload as configured latches within a few bytes per input (wide classes complete
+    short patterns quickly): it is run on the full 1024 x 1 MiB batch. The non-latching
+    control (no accepting states) keeps thousands of states active on every byte; the oracle
+    cannot walk a full GiB of it, so the control is timed on a bounded sample and each
+    technique is gated on a smaller one. A density sweep (fraction of pattern bytes) on the
+    control shows the crossover between the techniques.
+    """
+    from gpufsm import workloads as W
+    from gpufsm.backends.hip import _lib
+    from gpufsm.backends.hip.handles import DeviceNFA
+
+    from oracle import gate, native
+
+    stream = torch.cuda.current_stream().cuda_stream
+    out: dict = {"workload": "65536-state pattern union, 8000 patterns (lengths 5-12 adjusted to "
+                             "the state count), class widths 16-64, 95% pattern bytes"}
+
+    def run(dev, batch, tech):
+        batch.run(dev, _lib.NFA_TECHNIQUES[tech], stream)
+        torch.cuda.synchronize()
+        return batch.results()
+
+    def timed(dev, batch, tech, reps):
+        wall, ks = _time_steps(torch, lambda: batch.run(dev, _lib.NFA_TECHNIQUES[tech], stream),
+                               reps, 1)
+        return wall / reps
+
+    # as configured (latching), full batch
+    t0 = time.perf_counter()
+    pu, (data, off) = W.workload_divergent(count=args.div_count, length=args.div_length)
+    _log(f"divergent: {pu.nfa.num_states} states, {off[-1] / 2**20:.0f} MiB "
+         f"({time.perf_counter() - t0:.1f} s to generate)")
+    wf, wn, wex = native.nfa_batch(pu.nfa, data, off)
+    batch = DeviceBatch(torch, data, off)
+    latching = {"examined_bytes": int(wex.sum()), "credited_bytes": int(off[-1]),
+                "accepted": int(wf.sum()), "techniques": {}}
+    with DeviceNFA(pu.nfa) as dev:
+        latching["num_tiles"] = int(dev.props.num_tiles)
+        for tech in ("bitpacked", "chain", "worklist"):
+            f, n = run(dev, batch, tech)
+            gate.require_arrays(f, n, wf, wn, f"hip divergent latching {tech}")
+            sec = timed(dev, batch, tech, 3)
+            latching["techniques"][tech] = {"ms": 1e3 * sec,
+                                            "examined_GBps": int(wex.sum()) / sec / 1e9,
+                                            "credited_GBps": int(off[-1]) / sec / 1e9}
+    out["latching"] = latching
+    del batch
+
+    # non-latching control and the density sweep, bounded samples
+    ctl = {}
+    for frac in args.div_sweep:
+        pu0 = W.pattern_union(8000, (5, 12), (16, 64), seed=0, total_states=65536, accepting=False)
+        lens_t = np.full(args.div_sample_inputs, args.div_sample_len, dtype=np.int64)
+        d_t, o_t = W.pattern_inputs(pu0, lens_t, frac, seed=7)
+        lens_g = np.full(4, args.div_gate_len, dtype=np.int64)
+        d_g, o_g = W.pattern_inputs(pu0, lens_g, frac, seed=8)
+        gf, gn, gex = native.nfa_batch(pu0.nfa, d_g, o_g)
+        bt = DeviceBatch(torch, d_t, o_t)
+        bg = DeviceBatch(torch, d_g, o_g)
+        row = {}
+        with DeviceNFA(pu0.nfa) as dev:
+            for tech in ("bitpacked", "chain", "worklist"):
+                f, n = run(dev, bg, tech)
+                gate.require_arrays(f, n, gf, gn, f"hip divergent control {tech} p={frac}")
+                # the list walk is far slower on dense sets: time it on a prefix only
+                if tech == "worklist":
+                    k = max(1, args.div_sample_inputs // 16)
+                    sub = DeviceBatch(torch, d_t[: int(o_t[k])], o_t[: k + 1])
+                    sec = timed(dev, sub, tech, 1)
+                    nbytes = int(o_t[k])
+                else:
+                    sec = timed(dev, bt, tech, 2)
+                    nbytes = int(o_t[-1])
+                row[tech] = {"examined_GBps": nbytes / sec / 1e9, "sample_bytes": nbytes}
+        best = max(row, key=lambda t: row[t]["examined_GBps"])
+        ctl[f"p_pattern={frac}"] = {"rates": row, "fastest": best,
+                                    "gate_bytes": int(o_g[-1]),
+                                    "mean_active_examined": float(gex.sum())}
+        _log(f"divergent control p={frac}: " + ", ".join(
+            f"{t} {row[t]['examined_GBps']:.3f} GB/s" for t in row) + f" -> {best}")
+    out["non_latching_control"] = ctl
+    return out
+
+
 def ragged_line(args, torch) -> dict:
     """The ragged NFA mix as the headline: resident, autotuned, graph-captured batches."""
     r = bench_ragged(args, torch)
@@ -532,10 +620,16 @@ def main() -> int:
     ap.add_argument("--ragged-technique", default="auto",
                     help='"auto" (autotuned) or a fixed label such as "chain:chunk=2048"')
     ap.add_argument("--serve-qps", type=float, nargs="+", default=[1000.0, 4000.0, 16000.0])
+    ap.add_argument("--div-count", type=int, default=1024)
+    ap.add_argument("--div-length", type=int, default=1 << 20)
+    ap.add_argument("--div-sweep", type=float, nargs="+", default=[0.3, 0.6, 0.95])
+    ap.add_argument("--div-sample-inputs", type=int, default=32)
+    ap.add_argument("--div-sample-len", type=int, default=1 << 18)
+    ap.add_argument("--div-gate-len", type=int, default=1 << 14)
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--only", default="",
-                    help="comma list of extra sections to run (ragged,plain,serving); "
-                         "default all")
+                    help="comma list of extra sections to run (ragged,plain,serving,"
+                         "divergent); default all")
     ap.add_argument("--profile", action="store_true",
                     help="headline variant only, plus one device copy of the input bytes "
                          "(the calibration point for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -558,13 +652,16 @@ def main() -> int:
     else:
         out = bench_dfa(args, torch)
     if not args.profile and args.workload == "dfa":
-        want = set(filter(None, args.only.split(","))) or {"ragged", "plain", "serving"}
+        want = (set(filter(None, args.only.split(",")))
+                or {"ragged", "plain", "serving", "divergent"})
         if "ragged" in want:
             out["extras"]["ragged_nfa"] = bench_ragged(args, torch)
         if "plain" in want:
             out["extras"]["plain_nfa"] = bench_plain_nfa(args, torch)
         if "serving" in want:
             out["extras"]["serving"] = bench_serving(args, torch)
+        if "divergent" in want:
+            out["extras"]["divergent_nfa"] = bench_divergent(args, torch)
     line = {
         "metric": out["metric"],
         "value": out["value"],

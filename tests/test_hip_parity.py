@@ -159,12 +159,11 @@ def test_dfa_non_latching_long_inputs(tech):
 
 @pytest.mark.parametrize("tech", ["auto", "bitpacked", "worklist", "chain"])
 def test_pattern_union_ragged(tech):
-    pu, (data, off) = W.workload_ragged(count=300, num_patterns=300, seed=3)
-    # shorten to keep the oracle quick
-    lens = np.minimum(np.diff(off), 3000)
-    data2 = np.concatenate([data[off[i]:off[i] + lens[i]] for i in range(lens.size)])
-    off2 = np.zeros(lens.size + 1, dtype=np.int64)
-    np.cumsum(lens, out=off2[1:])
+    """Ragged log-normal lengths (capped at 3000 to keep the oracle quick) over a union of
+    short patterns, so that some inputs match and some do not."""
+    pu = W.pattern_union(300, (3, 12), (1, 32), seed=3)
+    lens = np.minimum(W.lognormal_lengths(300, 1024, 1.0, 0, 1 << 20, seed=4), 3000)
+    data2, off2 = W.pattern_inputs(pu, lens, 0.7, seed=5)
     with DeviceNFA(pu.nfa) as dev:
         assert dev.props.chain_ok == 1
         f, n, _ = dev.run_host(data2, off2, _lib.NFA_TECHNIQUES[tech])
@@ -366,3 +365,42 @@ def test_split_latching_and_many_short_inputs():
     _check_dfa(dfa, data, off, "split")
     data, off = W.uniform_batch(64, 65536, seed=1)
     _check_dfa(dfa, data, off, "split")
+
+
+def _union_as_anml(pu) -> str:
+    """A pattern union written as ANML: one all-input STE chain per pattern."""
+    parts = ["<anml><automata-network>"]
+    pos = 0
+    for p, length in enumerate(pu.lengths):
+        for i in range(int(length)):
+            cls = pu.class_bytes[pu.class_start[pos]:pu.class_start[pos + 1]]
+            sym = "[" + "".join(f"0x{int(c):02x}" for c in cls) + "]"
+            start = ' start="all-input"' if i == 0 else ""
+            inner = (f'<activate-on-match element="p{p}_{i + 1}"/>' if i + 1 < length
+                     else "<report-on-match/>")
+            parts.append(f'<state-transition-element id="p{p}_{i}" symbol-set="{sym}"{start}>'
+                         f"{inner}</state-transition-element>")
+            pos += 1
+    parts.append("</automata-network></anml>")
+    return "".join(parts)
+
+
+@pytest.mark.parametrize("fold", [False, True])
+def test_anml_pattern_set(tmp_path, fold):
+    """ANML pattern sets: with the synthetic start states (epsilon edges, general list walk)
+    and folded (chain-structured, chain technique), both equal to the oracle."""
+    from gpufsm.io import load_anml
+
+    pu = W.pattern_union(150, (3, 10), (1, 24), seed=17)
+    path = tmp_path / "set.anml"
+    path.write_text(_union_as_anml(pu))
+    nfa = load_anml(path, fold_starts=fold)
+    lens = np.random.default_rng(17).integers(0, 5000, size=80)
+    data, off = W.pattern_inputs(pu, lens, 0.7, seed=18)
+    with DeviceNFA(nfa) as dev:
+        assert dev.props.chain_ok == (1 if fold else 0)
+        f, n, _ = dev.run_host(data, off, _lib.NFA_AUTO)
+    wf, wn, _ = native.nfa_batch(nfa, data, off)
+    assert wf.any() and not wf.all()
+    np.testing.assert_array_equal(f, wf)
+    np.testing.assert_array_equal(n, wn)
