@@ -501,54 +501,62 @@ def bench_divergent(args, torch) -> dict:
     pu, (data, off) = W.workload_divergent(count=args.div_count, length=args.div_length)
     _log(f"divergent: {pu.nfa.num_states} states, {off[-1] / 2**20:.0f} MiB "
          f"({time.perf_counter() - t0:.1f} s to generate)")
+    t0 = time.perf_counter()
     wf, wn, wex = native.nfa_batch(pu.nfa, data, off)
+    _log(f"divergent: oracle {time.perf_counter() - t0:.1f} s, {int(wf.sum())} accepted, "
+         f"{int(wex.sum())} bytes examined")
     batch = DeviceBatch(torch, data, off)
     latching = {"examined_bytes": int(wex.sum()), "credited_bytes": int(off[-1]),
                 "accepted": int(wf.sum()), "techniques": {}}
+    t0 = time.perf_counter()
     with DeviceNFA(pu.nfa) as dev:
+        _log(f"divergent: automaton resident ({time.perf_counter() - t0:.1f} s, "
+             f"{dev.props.device_bytes / 1e6:.0f} MB)")
         latching["num_tiles"] = int(dev.props.num_tiles)
         for tech in ("bitpacked", "chain", "worklist"):
+            t0 = time.perf_counter()
             f, n = run(dev, batch, tech)
+            _log(f"divergent latching {tech}: first run {time.perf_counter() - t0:.2f} s")
             gate.require_arrays(f, n, wf, wn, f"hip divergent latching {tech}")
             sec = timed(dev, batch, tech, 3)
+            _log(f"divergent latching {tech}: {1e3 * sec:.2f} ms per batch")
             latching["techniques"][tech] = {"ms": 1e3 * sec,
                                             "examined_GBps": int(wex.sum()) / sec / 1e9,
                                             "credited_GBps": int(off[-1]) / sec / 1e9}
     out["latching"] = latching
     del batch
 
-    # non-latching control and the density sweep, bounded samples
+    # non-latching control and the density sweep: a batch wide enough to fill the GPU,
+    # gated on its first inputs (the oracle walks thousands of states per byte here)
     ctl = {}
-    for frac in args.div_sweep:
-        pu0 = W.pattern_union(8000, (5, 12), (16, 64), seed=0, total_states=65536, accepting=False)
-        lens_t = np.full(args.div_sample_inputs, args.div_sample_len, dtype=np.int64)
-        d_t, o_t = W.pattern_inputs(pu0, lens_t, frac, seed=7)
-        lens_g = np.full(4, args.div_gate_len, dtype=np.int64)
-        d_g, o_g = W.pattern_inputs(pu0, lens_g, frac, seed=8)
-        gf, gn, gex = native.nfa_batch(pu0.nfa, d_g, o_g)
-        bt = DeviceBatch(torch, d_t, o_t)
-        bg = DeviceBatch(torch, d_g, o_g)
-        row = {}
-        with DeviceNFA(pu0.nfa) as dev:
+    pu0 = W.pattern_union(8000, (5, 12), (16, 64), seed=0, total_states=65536, accepting=False)
+    with DeviceNFA(pu0.nfa) as dev:
+        for frac in args.div_sweep:
+            lens_t = np.full(args.div_sample_inputs, args.div_sample_len, dtype=np.int64)
+            d_t, o_t = W.pattern_inputs(pu0, lens_t, frac, seed=7)
+            k_gate = min(4, args.div_sample_inputs)
+            t0 = time.perf_counter()
+            gf, gn, gex = native.nfa_batch(pu0.nfa, d_t[: int(o_t[k_gate])], o_t[: k_gate + 1])
+            _log(f"divergent control p={frac}: oracle on {k_gate} inputs "
+                 f"{time.perf_counter() - t0:.1f} s")
+            bt = DeviceBatch(torch, d_t, o_t)
+            k_wl = min(args.div_sample_inputs, 16)
+            bw = DeviceBatch(torch, d_t[: int(o_t[k_wl])], o_t[: k_wl + 1])
+            row = {}
             for tech in ("bitpacked", "chain", "worklist"):
-                f, n = run(dev, bg, tech)
-                gate.require_arrays(f, n, gf, gn, f"hip divergent control {tech} p={frac}")
-                # the list walk is far slower on dense sets: time it on a prefix only
-                if tech == "worklist":
-                    k = max(1, args.div_sample_inputs // 16)
-                    sub = DeviceBatch(torch, d_t[: int(o_t[k])], o_t[: k + 1])
-                    sec = timed(dev, sub, tech, 1)
-                    nbytes = int(o_t[k])
-                else:
-                    sec = timed(dev, bt, tech, 2)
-                    nbytes = int(o_t[-1])
-                row[tech] = {"examined_GBps": nbytes / sec / 1e9, "sample_bytes": nbytes}
-        best = max(row, key=lambda t: row[t]["examined_GBps"])
-        ctl[f"p_pattern={frac}"] = {"rates": row, "fastest": best,
-                                    "gate_bytes": int(o_g[-1]),
-                                    "mean_active_examined": float(gex.sum())}
-        _log(f"divergent control p={frac}: " + ", ".join(
-            f"{t} {row[t]['examined_GBps']:.3f} GB/s" for t in row) + f" -> {best}")
+                bb = bw if tech == "worklist" else bt  # the list walk: a small prefix only
+                f, n = run(dev, bb, tech)
+                gate.require_arrays(f[:k_gate], n[:k_gate], gf, gn,
+                                    f"hip divergent control {tech} p={frac}")
+                sec = timed(dev, bb, tech, 1 if tech == "worklist" else 2)
+                nbytes = int(bb.total)
+                row[tech] = {"examined_GBps": nbytes / sec / 1e9, "sample_bytes": nbytes,
+                             "inputs": int(bb.count)}
+                _log(f"divergent control p={frac} {tech}: {row[tech]['examined_GBps']:.5f} "
+                     f"GB/s over {nbytes} bytes")
+            best = max(row, key=lambda t: row[t]["examined_GBps"])
+            ctl[f"p_pattern={frac}"] = {"rates": row, "fastest": best,
+                                        "gated_inputs": k_gate}
     out["non_latching_control"] = ctl
     return out
 
@@ -623,9 +631,8 @@ def main() -> int:
     ap.add_argument("--div-count", type=int, default=1024)
     ap.add_argument("--div-length", type=int, default=1 << 20)
     ap.add_argument("--div-sweep", type=float, nargs="+", default=[0.3, 0.6, 0.95])
-    ap.add_argument("--div-sample-inputs", type=int, default=32)
-    ap.add_argument("--div-sample-len", type=int, default=1 << 18)
-    ap.add_argument("--div-gate-len", type=int, default=1 << 14)
+    ap.add_argument("--div-sample-inputs", type=int, default=1024)
+    ap.add_argument("--div-sample-len", type=int, default=1 << 14)
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--only", default="",
                     help="comma list of extra sections to run (ragged,plain,serving,"
