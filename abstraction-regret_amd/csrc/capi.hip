@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -61,16 +62,54 @@ int guarded(F &&f) {
     }
 }
 
-template <typename T>
-T *upload(std::vector<void *> &blocks, int64_t &bytes, const T *src, size_t count) {
-    const size_t sz = (count ? count : 1) * sizeof(T);
-    void *p = nullptr;
-    if (hipMalloc(&p, sz) != hipSuccess) throw Error(GFSM_ENOMEM, "hipMalloc of automaton failed");
-    blocks.push_back(p);
-    bytes += static_cast<int64_t>(sz);
-    if (count) GFSM_HIP(hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice));
-    return static_cast<T *>(p);
-}
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// Collects the arrays of one automaton and uploads them as one allocation with one copy
+// (an automaton has up to 20 arrays; per-array allocations and copies dominated the plain
+// per-call path). Each array starts on a 256-byte boundary.
+class Packer {
+  public:
+    template <typename T>
+    void add(const T **field, const std::vector<T> &v) {
+        add(field, v.data(), v.size());
+    }
+    template <typename T>
+    void add(const T **field, const T *src, size_t count) {
+        const size_t bytes = (count ? count : 1) * sizeof(T);
+        const size_t off = total_;
+        total_ = align256(total_ + bytes);
+        parts_.push_back({src, count * sizeof(T), off,
+                          [field](char *base, size_t o) {
+                              *field = reinterpret_cast<const T *>(base + o);
+                          }});
+    }
+    // One device allocation (recorded in blocks), one host-to-device copy.
+    void commit(std::vector<void *> &blocks, int64_t &device_bytes) {
+        if (total_ == 0) return;
+        std::vector<char> host(total_, 0);
+        for (const Part &p : parts_)
+            if (p.bytes) std::memcpy(host.data() + p.off, p.src, p.bytes);
+        void *dev = nullptr;
+        if (hipMalloc(&dev, total_) != hipSuccess)
+            throw Error(GFSM_ENOMEM, "hipMalloc of " + std::to_string(total_) +
+                                         " bytes for the automaton failed");
+        blocks.push_back(dev);
+        device_bytes += static_cast<int64_t>(total_);
+        GFSM_HIP(hipMemcpy(dev, host.data(), total_, hipMemcpyHostToDevice));
+        for (const Part &p : parts_) p.set(static_cast<char *>(dev), p.off);
+    }
+
+    size_t total() const { return total_; }
+
+  private:
+    struct Part {
+        const void *src;
+        size_t bytes, off;
+        std::function<void(char *, size_t)> set;
+    };
+    std::vector<Part> parts_;
+    size_t total_ = 0;
+};
 
 void free_blocks(std::vector<void *> &blocks) {
     for (void *p : blocks) (void)hipFree(p);
@@ -166,8 +205,6 @@ int32_t resolve_dfa(const gfsm_dfa *h, int32_t technique, int32_t count, int64_t
     }
 }
 
-size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
-
 // Device scratch of one NFA run (resolved technique).
 size_t nfa_workspace(const gfsm_nfa *h, int32_t tech, int32_t count, int64_t total,
                      const Tuning &tu) {
@@ -257,10 +294,33 @@ void check_ws(const void *ws, int64_t ws_bytes) {
         throw Error(GFSM_EINVAL, "workspace must be 256-byte aligned");
 }
 
-// Host form shared by NFA and DFA: stage the batch, time the launch with events, copy back.
-template <typename Enqueue>
+// Pinned host staging of the calling thread (grows, kept for the life of the thread).
+struct HostStage {
+    char *p = nullptr;
+    size_t cap = 0;
+    char *get(size_t bytes) {
+        if (bytes > cap) {
+            if (p) (void)hipHostFree(p);
+            p = nullptr;
+            cap = 0;
+            const size_t want = bytes < (size_t(1) << 20) ? (size_t(1) << 20) : bytes;
+            GFSM_HIP(hipHostMalloc(reinterpret_cast<void **>(&p), want, hipHostMallocDefault));
+            cap = want;
+        }
+        return p;
+    }
+};
+
+// Batches up to this size go through the pinned staging buffer (one copy each way);
+// larger ones are copied straight from the caller's pageable memory.
+constexpr size_t kStageLimit = size_t(64) << 20;
+
+// Host form shared by NFA and DFA. One device allocation holds offsets, bytes, outputs and
+// the technique's scratch; small batches cross PCIe as one staged copy each way. The launch
+// is timed with HIP events, as the upstream entry points time theirs.
+template <typename WsBytes, typename Enqueue>
 void run_host(const uint8_t *data, const int64_t *offsets, int32_t count, int32_t *flags,
-              int32_t *lens, float *kernel_ms, Enqueue &&enqueue) {
+              int32_t *lens, float *kernel_ms, WsBytes &&ws_bytes, Enqueue &&enqueue) {
     if (kernel_ms) *kernel_ms = 0.0f;
     if (count == 0) return;
     const int64_t total = offsets[count] - offsets[0];
@@ -269,38 +329,106 @@ void run_host(const uint8_t *data, const int64_t *offsets, int32_t count, int32_
         if (offsets[i + 1] < offsets[i]) throw Error(GFSM_EINVAL, "offsets are not non-decreasing");
     if (total > 0 && !data) throw Error(GFSM_EINVAL, "null data");
     hipStream_t s = host_stream();
-    StreamBuf d_data(static_cast<size_t>(total), s);
-    StreamBuf d_off(static_cast<size_t>(count + 1) * sizeof(int64_t), s);
-    StreamBuf d_out(static_cast<size_t>(count) * 2 * sizeof(int32_t), s);
-    if (total > 0)
-        GFSM_HIP(hipMemcpyAsync(d_data.p, data, static_cast<size_t>(total), hipMemcpyHostToDevice, s));
-    GFSM_HIP(hipMemcpyAsync(d_off.p, offsets, static_cast<size_t>(count + 1) * sizeof(int64_t),
-                            hipMemcpyHostToDevice, s));
-    int32_t *d_flags = static_cast<int32_t *>(d_out.p);
+    const size_t off_bytes = static_cast<size_t>(count + 1) * sizeof(int64_t);
+    const size_t out_bytes = static_cast<size_t>(count) * 2 * sizeof(int32_t);
+    const size_t ws = ws_bytes(total);
+    // device layout: offsets | bytes | outputs | scratch
+    const size_t o_off = 0, o_data = align256(off_bytes);
+    const size_t o_out = o_data + align256(static_cast<size_t>(total));
+    const size_t o_ws = o_out + align256(out_bytes);
+    StreamBuf dev(o_ws + ws, s);
+    char *base = static_cast<char *>(dev.p);
+    thread_local HostStage stage;
+    const bool staged = static_cast<size_t>(total) <= kStageLimit;
+    if (staged) {
+        char *h = stage.get(o_out > out_bytes ? o_out : out_bytes);
+        std::memcpy(h + o_off, offsets, off_bytes);
+        if (total > 0) std::memcpy(h + o_data, data, static_cast<size_t>(total));
+        GFSM_HIP(hipMemcpyAsync(base, h, o_data + static_cast<size_t>(total),
+                                hipMemcpyHostToDevice, s));
+    } else {
+        GFSM_HIP(hipMemcpyAsync(base + o_off, offsets, off_bytes, hipMemcpyHostToDevice, s));
+        GFSM_HIP(hipMemcpyAsync(base + o_data, data, static_cast<size_t>(total),
+                                hipMemcpyHostToDevice, s));
+    }
+    int32_t *d_flags = reinterpret_cast<int32_t *>(base + o_out);
     int32_t *d_lens = d_flags + count;
-    hipEvent_t e0, e1;
-    GFSM_HIP(hipEventCreate(&e0));
-    GFSM_HIP(hipEventCreate(&e1));
-    try {
-        GFSM_HIP(hipEventRecord(e0, s));
-        enqueue(static_cast<const uint8_t *>(d_data.p), static_cast<const int64_t *>(d_off.p),
-                total, d_flags, d_lens, s);
-        GFSM_HIP(hipEventRecord(e1, s));
+    thread_local hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!e0) GFSM_HIP(hipEventCreate(&e0));
+    if (!e1) GFSM_HIP(hipEventCreate(&e1));
+    GFSM_HIP(hipEventRecord(e0, s));
+    enqueue(reinterpret_cast<const uint8_t *>(base + o_data),
+            reinterpret_cast<const int64_t *>(base + o_off), total, d_flags, d_lens,
+            ws ? static_cast<void *>(base + o_ws) : nullptr, ws, s);
+    GFSM_HIP(hipEventRecord(e1, s));
+    if (staged) {
+        char *h = stage.p;  // large enough: o_out >= out_bytes or sized for it above
+        GFSM_HIP(hipMemcpyAsync(h, d_flags, out_bytes, hipMemcpyDeviceToHost, s));
+        GFSM_HIP(hipStreamSynchronize(s));
+        std::memcpy(flags, h, static_cast<size_t>(count) * sizeof(int32_t));
+        std::memcpy(lens, h + static_cast<size_t>(count) * sizeof(int32_t),
+                    static_cast<size_t>(count) * sizeof(int32_t));
+    } else {
         GFSM_HIP(hipMemcpyAsync(flags, d_flags, static_cast<size_t>(count) * sizeof(int32_t),
                                 hipMemcpyDeviceToHost, s));
         GFSM_HIP(hipMemcpyAsync(lens, d_lens, static_cast<size_t>(count) * sizeof(int32_t),
                                 hipMemcpyDeviceToHost, s));
         GFSM_HIP(hipStreamSynchronize(s));
-        float ms = 0.0f;
-        GFSM_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (kernel_ms) *kernel_ms = ms;
-    } catch (...) {
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        throw;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    float ms = 0.0f;
+    GFSM_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (kernel_ms) *kernel_ms = ms;
+}
+
+// Lays out every device array of an NFA handle in `pk` (and fills the scalar fields).
+void pack_nfa(gfsm_nfa *h, int32_t num_states, const WorklistHost &w, const TileHost &t,
+              std::vector<uint16_t> &s16, Packer &pk) {
+    pk.add(&h->wl.row, w.row);
+    pk.add(&h->wl.edge, w.edge);
+    pk.add(&h->wl.bigidx, w.bigidx);
+    pk.add(&h->wl.bigoff, w.bigoff);
+    pk.add(&h->wl.erow, w.erow);
+    pk.add(&h->wl.etgt, w.etgt);
+    pk.add(&h->wl.accw, w.accw);
+    pk.add(&h->wl.start_list, w.start_list);
+    h->wl.n = num_states;
+    h->wl.nwords = w.nwords;
+    h->wl.k0 = static_cast<int32_t>(w.start_list.size());
+    h->wl.has_eps = w.has_eps ? 1 : 0;
+    h->chain_ok = t.ok;
+    if (t.ok) {
+        pk.add(&h->tile.mask, t.mask);
+        pk.add(&h->tile.first, t.first);
+        pk.add(&h->tile.hassucc, t.hassucc);
+        pk.add(&h->tile.acc, t.acc);
+        for (int i = 0; i < 4; ++i) h->tile.selfmask[i] = t.selfmask[i];
+        h->tile.tiles = t.tiles;
+        h->tile.start_always = t.start_always ? 1 : 0;
+        h->max_chain = t.max_chain;
+        pk.add(&h->chain.next, t.next);
+        pk.add(&h->chain.acc, t.acc);
+        pk.add(&h->chain.first_off, t.first_off);
+        pk.add(&h->chain.first_slot, t.first_slot);
+        if (!t.pair.empty()) {
+            pk.add(&h->chain.pair, t.pair);
+            pk.add(&h->chain.pair_slot, t.pair_slot);
+            if (static_cast<int64_t>(t.tiles) * kTileStates <= 65536) {
+                // half the footprint: the register tier's L2 working set is the
+                // continuation table plus these lists
+                s16.assign(t.pair_slot.begin(), t.pair_slot.end());
+                pk.add(&h->chain.pair_slot16, s16);
+            }
+        }
+        pk.add(&h->chain.bflags, t.bflags);
+        for (int i = 0; i < 4; ++i) {
+            h->chain.selfmask[i] = t.selfmask[i];
+            h->chain.first_any[i] = t.first_any[i];
+            h->chain.first_acc[i] = t.first_acc[i];
+        }
+        h->chain.tiles = t.tiles;
+        h->chain.max_chain = t.max_chain;
+        h->chain.start_always = h->tile.start_always;
+    }
 }
 
 }  // namespace
@@ -340,54 +468,10 @@ int gfsm_nfa_create(int32_t num_states, int32_t start_state, const uint8_t *acce
             h->nnz = sym_row_ptr[num_states];
             h->nne = eps_row_ptr[num_states];
             h->start_accepts = w.start_accepts;
-            auto &B = h->blocks;
-            auto &by = h->device_bytes;
-            h->wl.row = upload(B, by, w.row.data(), w.row.size());
-            h->wl.edge = upload(B, by, w.edge.data(), w.edge.size());
-            h->wl.bigidx = upload(B, by, w.bigidx.data(), w.bigidx.size());
-            h->wl.bigoff = upload(B, by, w.bigoff.data(), w.bigoff.size());
-            h->wl.erow = upload(B, by, w.erow.data(), w.erow.size());
-            h->wl.etgt = upload(B, by, w.etgt.data(), w.etgt.size());
-            h->wl.accw = upload(B, by, w.accw.data(), w.accw.size());
-            h->wl.start_list = upload(B, by, w.start_list.data(), w.start_list.size());
-            h->wl.n = num_states;
-            h->wl.nwords = w.nwords;
-            h->wl.k0 = static_cast<int32_t>(w.start_list.size());
-            h->wl.has_eps = w.has_eps ? 1 : 0;
-            h->chain_ok = t.ok;
-            if (t.ok) {
-                h->tile.mask = upload(B, by, t.mask.data(), t.mask.size());
-                h->tile.first = upload(B, by, t.first.data(), t.first.size());
-                h->tile.hassucc = upload(B, by, t.hassucc.data(), t.hassucc.size());
-                h->tile.acc = upload(B, by, t.acc.data(), t.acc.size());
-                for (int i = 0; i < 4; ++i) h->tile.selfmask[i] = t.selfmask[i];
-                h->tile.tiles = t.tiles;
-                h->tile.start_always = t.start_always ? 1 : 0;
-                h->max_chain = t.max_chain;
-                h->chain.next = upload(B, by, t.next.data(), t.next.size());
-                h->chain.acc = h->tile.acc;
-                h->chain.first_off = upload(B, by, t.first_off.data(), t.first_off.size());
-                h->chain.first_slot = upload(B, by, t.first_slot.data(), t.first_slot.size());
-                if (!t.pair.empty()) {
-                    h->chain.pair = upload(B, by, t.pair.data(), t.pair.size());
-                    h->chain.pair_slot = upload(B, by, t.pair_slot.data(), t.pair_slot.size());
-                    if (static_cast<int64_t>(t.tiles) * kTileStates <= 65536) {
-                        // half the footprint: the register tier's L2 working set is the
-                        // continuation table plus these lists
-                        std::vector<uint16_t> s16(t.pair_slot.begin(), t.pair_slot.end());
-                        h->chain.pair_slot16 = upload(B, by, s16.data(), s16.size());
-                    }
-                }
-                h->chain.bflags = upload(B, by, t.bflags.data(), t.bflags.size());
-                for (int i = 0; i < 4; ++i) {
-                    h->chain.selfmask[i] = t.selfmask[i];
-                    h->chain.first_any[i] = t.first_any[i];
-                    h->chain.first_acc[i] = t.first_acc[i];
-                }
-                h->chain.tiles = t.tiles;
-                h->chain.max_chain = t.max_chain;
-                h->chain.start_always = h->tile.start_always;
-            }
+            Packer pk;
+            std::vector<uint16_t> s16;
+            pack_nfa(h, num_states, w, t, s16, pk);
+            pk.commit(h->blocks, h->device_bytes);
         } catch (...) {
             free_blocks(h->blocks);
             delete h;
@@ -443,23 +527,11 @@ int gfsm_nfa_analyze(int32_t num_states, int32_t start_state, const uint8_t *acc
         out->num_tiles = t.ok ? t.tiles : 0;
         out->start_always = t.ok ? t.start_always : 0;
         out->max_chain = t.max_chain;
-        int64_t bytes = 0;
-        auto add = [&](size_t count, size_t elem) {
-            bytes += static_cast<int64_t>((count ? count : 1) * elem);
-        };
-        add(w.row.size(), 4); add(w.edge.size(), 4); add(w.bigidx.size(), 4);
-        add(w.bigoff.size(), 4); add(w.erow.size(), 4); add(w.etgt.size(), 4);
-        add(w.accw.size(), 8); add(w.start_list.size(), 4);
-        if (t.ok) {
-            add(t.mask.size(), 8); add(t.first.size(), 8); add(t.hassucc.size(), 8);
-            add(t.acc.size(), 8); add(t.next.size(), 8); add(t.first_off.size(), 4);
-            add(t.first_slot.size(), 4); add(t.bflags.size(), 4);
-            if (!t.pair.empty()) {
-                add(t.pair.size(), 4);
-                add(t.pair_slot.size(), 4);
-                if (static_cast<int64_t>(t.tiles) * kTileStates <= 65536) add(t.pair_slot.size(), 2);
-            }
-        }
+        gfsm_nfa tmp;
+        Packer pk;
+        std::vector<uint16_t> s16;
+        pack_nfa(&tmp, num_states, w, t, s16, pk);
+        const int64_t bytes = static_cast<int64_t>(pk.total());
         out->device_bytes = bytes;
     });
 }
@@ -472,10 +544,11 @@ int gfsm_nfa_run_host(gfsm_nfa *nfa, int32_t technique, const uint8_t *data,
         check_batch_args(data, offsets, num_strings, flags, lens);
         const int32_t tech = resolve_nfa(nfa, technique);
         run_host(data, offsets, num_strings, flags, lens, kernel_ms,
+                 [&](int64_t tot) { return nfa_workspace(nfa, tech, num_strings, tot, Tuning()); },
                  [&](const uint8_t *dd, const int64_t *doff, int64_t tot, int32_t *df,
-                     int32_t *dl, hipStream_t s) {
-                     nfa_enqueue(nfa, tech, Tuning(), dd, doff, num_strings, tot, df, dl,
-                                 nullptr, 0, s);
+                     int32_t *dl, void *ws, size_t wsb, hipStream_t s) {
+                     nfa_enqueue(nfa, tech, Tuning(), dd, doff, num_strings, tot, df, dl, ws,
+                                 wsb, s);
                  });
     });
 }
@@ -548,10 +621,15 @@ int gfsm_dfa_create(int32_t num_states, int32_t start_state, const uint8_t *acce
             h->n = num_states;
             h->start = start_state;
             h->start_accepts = t.start_accepts;
+            Packer pk;
+            const uint32_t *t32 = nullptr;
+            const uint16_t *t16 = nullptr;
             if (t.wide)
-                h->dev.table = upload(h->blocks, h->device_bytes, t.t32.data(), t.t32.size());
+                pk.add(&t32, t.t32);
             else
-                h->dev.table = upload(h->blocks, h->device_bytes, t.t16.data(), t.t16.size());
+                pk.add(&t16, t.t16);
+            pk.commit(h->blocks, h->device_bytes);
+            h->dev.table = t.wide ? static_cast<const void *>(t32) : static_cast<const void *>(t16);
             h->dev.n = num_states;
             h->dev.wide = t.wide ? 1 : 0;
             h->dev.start_entry = t.start_entry;
@@ -597,10 +675,11 @@ int gfsm_dfa_run_host(gfsm_dfa *dfa, int32_t technique, const uint8_t *data,
         const int64_t total = num_strings > 0 ? offsets[num_strings] : 0;
         const int32_t tech = resolve_dfa(dfa, technique, num_strings, total);
         run_host(data, offsets, num_strings, flags, lens, kernel_ms,
+                 [&](int64_t tot) { return dfa_workspace(dfa, tech, num_strings, tot, Tuning()); },
                  [&](const uint8_t *dd, const int64_t *doff, int64_t tot, int32_t *df,
-                     int32_t *dl, hipStream_t s) {
-                     dfa_enqueue(dfa, tech, Tuning(), dd, doff, num_strings, tot, df, dl,
-                                 nullptr, 0, s);
+                     int32_t *dl, void *ws, size_t wsb, hipStream_t s) {
+                     dfa_enqueue(dfa, tech, Tuning(), dd, doff, num_strings, tot, df, dl, ws,
+                                 wsb, s);
                  });
     });
 }
