@@ -612,6 +612,217 @@ def ragged_line(args, torch) -> dict:
     }
 
 
+GATE_COLUMNS = ("kind", "backend", "technique", "automaton", "inputs", "bytes", "mismatches",
+                "status", "seconds")
+
+
+def _environment() -> dict:
+    """What the numbers depend on: host, runtime and device."""
+    import platform
+
+    env = {"python": platform.python_version(), "numpy": np.__version__,
+           "host_cores": _threads(), "machine": platform.machine()}
+    try:
+        import torch
+
+        env["torch"] = torch.__version__
+        if torch.cuda.is_available():
+            p = torch.cuda.get_device_properties(0)
+            env.update(device=p.name, compute_units=p.multi_processor_count,
+                       hbm_bytes=int(p.total_memory),
+                       arch=getattr(p, "gcnArchName", ""))
+    except Exception as exc:  # pragma: no cover - depends on the machine
+        env["torch"] = f"unavailable: {exc}"
+    try:
+        with open("/opt/rocm/.info/version") as f:
+            env["rocm"] = f.read().strip()
+    except OSError:
+        env["rocm"] = "unknown"
+    try:
+        from gpufsm.backends.hip import _lib
+
+        env["gpufsm_hip_abi"] = _lib.load().gfsm_abi_version()
+        env["gpufsm_hip_lib"] = _lib.LIB_PATH
+    except Exception as exc:
+        env["gpufsm_hip_lib"] = f"unavailable: {exc}"
+    return env
+
+
+def gate_main(args) -> int:
+    """Strict oracle gate over every registered HIP technique; schema-checked CSV.
+
+    Walks the registry (so it checks whatever is registered, not a list in a comment) over
+    the vendored golden corpus, seeded random NFAs with epsilon edges and wildcards, pattern
+    unions (the chain-structured automata of the chain and bitpacked techniques) and random
+    DFAs from 16 to 40000 states, comparing (accepted, match_len) with the C oracle. A
+    required backend that is missing fails the run instead of skipping it
+    (`scripts/oracle_gate.py:1-20` upstream). Exit 0 only if every check agreed.
+    """
+    import csv
+    import random
+
+    from gpufsm import workloads as W
+    from gpufsm.core.dfa import DFA, random_dfa
+    from gpufsm.core.nfa import NFA, NFABuilder
+    from gpufsm.core.packing import pack_bytes
+    from gpufsm.core.registry import Backend, Kind, is_available, list_techniques, \
+        unavailable_reason
+    from gpufsm import run_batch
+
+    from oracle import native
+
+    required = {b.strip().lower() for b in args.require.split(",") if b.strip()}
+    env = _environment()
+    print("== environment ==")
+    for k, v in env.items():
+        print(f"  {k:16s} {v}")
+    print("== backends ==")
+    missing = []
+    for b in Backend:
+        avail = is_available(b)
+        print(f"  {b.value:8s} {'available' if avail else 'MISSING':10s} "
+              f"{unavailable_reason(b) or ''}")
+        if b.value in required and not avail:
+            missing.append(b.value)
+    if missing:
+        print(f"FAIL: required backend(s) not available: {', '.join(missing)}")
+        return 2
+
+    rng = random.Random(args.gate_seed)
+    nrng = np.random.default_rng(args.gate_seed)
+    cases: list[tuple[str, object, list[bytes]]] = []
+    golden = os.path.join(ROOT, "tests", "golden", "golden.json")
+    if os.path.exists(golden):
+        with open(golden) as f:
+            g = json.load(f)
+        for kind_key, make in (("nfa_cases", "nfa"), ("dfa_cases", "dfa")):
+            for c in g.get(kind_key, [])[: args.gate_golden]:
+                a = c[make]
+                if make == "nfa":
+                    auto = NFA(num_states=int(a["num_states"]), start_state=int(a["start_state"]),
+                               accept=np.array(a["accept"], bool),
+                               sym_row_ptr=np.array(a["sym_row_ptr"], np.int32),
+                               sym_targets=np.array(a["sym_targets"], np.int32),
+                               sym_symbols=np.array(a["sym_symbols"], np.int32),
+                               eps_row_ptr=np.array(a["eps_row_ptr"], np.int32),
+                               eps_targets=np.array(a["eps_targets"], np.int32))
+                else:
+                    auto = DFA(num_states=int(a["num_states"]), start_state=int(a["start_state"]),
+                               accept=np.array(a["accept"], bool),
+                               trans=np.array(a["trans"], np.int32))
+                cases.append((f"golden:{c.get('id', '?')}", auto,
+                              [bytes.fromhex(c["input_hex"])]))
+    for n in (8, 64, 700):
+        nb = NFABuilder()
+        for _ in range(n):
+            nb.add_state(accept=rng.random() < 0.02)
+        nb.set_start(rng.randrange(n))
+        for s_ in range(n):
+            for _ in range(rng.randint(0, 3)):
+                nb.add_transition(s_, 256 if rng.random() < 0.03 else rng.randrange(97, 105),
+                                  rng.randrange(n))
+            if rng.random() < 0.3:
+                nb.add_epsilon(s_, rng.randrange(n))
+        nfa = nb.build()
+        inputs = [bytes(rng.randrange(97, 106) for _ in range(rng.randint(0, args.gate_length)))
+                  for _ in range(args.gate_inputs)]
+        cases.append((f"random_nfa:{n}", nfa, inputs))
+    for patterns in (40, 600):
+        pu = W.pattern_union(patterns, (3, 20), (1, 32), seed=patterns)
+        lens = nrng.integers(0, args.gate_length * 8, size=args.gate_inputs)
+        d, o = W.pattern_inputs(pu, lens, 0.7, seed=patterns)
+        cases.append((f"pattern_union:{pu.nfa.num_states}", pu.nfa,
+                      [d[o[i]:o[i + 1]].tobytes() for i in range(lens.size)]))
+    for n in (16, 256, 4096, 40000):
+        dfa = random_dfa(n, accept_prob=0.3 / n, seed=n)
+        d, o = W.uniform_batch(args.gate_inputs, args.gate_length * 16, seed=n)
+        cases.append((f"random_dfa:{n}", dfa, [d[o[i]:o[i + 1]].tobytes()
+                                               for i in range(args.gate_inputs)]))
+
+    rows, failures, checked = [], 0, 0
+    print(f"== oracle gate: {len(cases)} automata ==")
+    for name, auto, inputs in cases:
+        kind = Kind.of(auto)
+        data, off = pack_bytes(inputs)
+        fn = native.nfa_batch if kind is Kind.NFA else native.dfa_batch
+        wf, wn, _ = fn(auto, data, off)
+        for backend in Backend:
+            if backend is Backend.CPU or not is_available(backend):
+                continue
+            for tech in list_techniques(backend, kind):
+                t0 = time.perf_counter()
+                try:
+                    res = run_batch(auto, inputs, backend=backend, technique=tech)
+                except ValueError:
+                    status, bad = "not-applicable", 0
+                except Exception as exc:
+                    status, bad = f"error:{type(exc).__name__}", len(inputs)
+                else:
+                    got_f = np.array([r.accepted for r in res], np.int32)
+                    got_n = np.array([r.match_len for r in res], np.int32)
+                    bad = int(((got_f != wf) | (got_n != wn)).sum())
+                    status = "ok" if bad == 0 else "MISMATCH"
+                secs = time.perf_counter() - t0
+                checked += status != "not-applicable"
+                failures += status not in ("ok", "not-applicable")
+                rows.append({"kind": kind.value, "backend": backend.value, "technique": tech,
+                             "automaton": name, "inputs": len(inputs),
+                             "bytes": int(off[-1]), "mismatches": bad, "status": status,
+                             "seconds": round(secs, 6)})
+                if status not in ("ok", "not-applicable"):
+                    print(f"  {status:14s} {kind.value}/{backend.value}/{tech} on {name}")
+    summary = collections_counter(rows)
+    for (kind, backend, tech), (n_ok, n_bad, n_na) in sorted(summary.items()):
+        print(f"  {kind}/{backend}/{tech:10s} ok {n_ok:4d}  failed {n_bad:3d}  "
+              f"not applicable {n_na:3d}")
+    if args.gate_csv:
+        with open(args.gate_csv, "w", newline="") as f:
+            f.write("# " + json.dumps(env, sort_keys=True) + "\n")
+            w = csv.DictWriter(f, fieldnames=GATE_COLUMNS)
+            w.writeheader()
+            w.writerows(rows)
+        _check_gate_csv(args.gate_csv)
+        print(f"  wrote {args.gate_csv} ({len(rows)} rows, schema checked)")
+    if checked == 0:
+        print("FAIL: nothing was checked")
+        return 2
+    print(f"{'PASS' if failures == 0 else 'FAIL'}: {checked} checks, {failures} failed")
+    return 0 if failures == 0 else 1
+
+
+def collections_counter(rows):
+    out: dict = {}
+    for r in rows:
+        key = (r["kind"], r["backend"], r["technique"])
+        ok, bad, na = out.get(key, (0, 0, 0))
+        if r["status"] == "ok":
+            ok += 1
+        elif r["status"] == "not-applicable":
+            na += 1
+        else:
+            bad += 1
+        out[key] = (ok, bad, na)
+    return out
+
+
+def _check_gate_csv(path: str) -> None:
+    """The written CSV has the environment line, the exact header and typed fields."""
+    import csv
+
+    with open(path) as f:
+        first = f.readline()
+        if not first.startswith("# "):
+            raise ValueError("gate CSV: missing environment line")
+        json.loads(first[2:])
+        r = csv.DictReader(f)
+        if tuple(r.fieldnames or ()) != GATE_COLUMNS:
+            raise ValueError(f"gate CSV: header {r.fieldnames} != {GATE_COLUMNS}")
+        for row in r:
+            int(row["inputs"]), int(row["bytes"]), int(row["mismatches"]), float(row["seconds"])
+            if row["kind"] not in ("nfa", "dfa") or not row["technique"] or not row["status"]:
+                raise ValueError(f"gate CSV: bad row {row}")
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -640,7 +851,18 @@ def main() -> int:
     ap.add_argument("--profile", action="store_true",
                     help="headline variant only, plus one device copy of the input bytes "
                          "(the calibration point for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--gate", action="store_true",
+                    help="run only the strict oracle gate over every registered technique")
+    ap.add_argument("--require", default="",
+                    help="with --gate: comma list of backends that must be available")
+    ap.add_argument("--gate-csv", default="", help="with --gate: write the results here")
+    ap.add_argument("--gate-inputs", type=int, default=64)
+    ap.add_argument("--gate-length", type=int, default=256)
+    ap.add_argument("--gate-golden", type=int, default=10**6)
+    ap.add_argument("--gate-seed", type=int, default=1048)
     args = ap.parse_args()
+    if args.gate:
+        return gate_main(args)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); inputs are "
               f"independent, so N GPUs would each run this batch: see the sharding notes.",
