@@ -172,6 +172,9 @@ struct SplitArgs {
     unsigned long long *first_event;
     uint32_t *head_exit, *tail_exit, *fixed_entry, *fixed_exit;
     int32_t *fixed_acc;
+    // [segments] when nck <= 64: bit k set when interval k of the head entered an
+    // accepting state on its speculative path (the flag bits of the checkpoints, gathered)
+    unsigned long long *flagmask;
     void *ckpt;  // T [segments * nck]
 };
 
@@ -260,9 +263,15 @@ struct CkView {
     }
 };
 
+// Threads per block of the walk (one block per CU: the table fills the LDS). The walk keeps
+// kWalkThreads * K chains in flight per CU; each step is one dependent LDS read, so the
+// number of chains in flight sets the walk's rate.
+constexpr int kWalkThreads = 1024;
+constexpr int kFixThreads = 512;
+
 template <typename T, bool kLds, int K>
-__global__ __launch_bounds__(512 / K) void dfa_split_walk_kernel(const T *__restrict__ gtab,
-                                                              SplitArgs a) {
+__global__ __launch_bounds__(kWalkThreads / K) void dfa_split_walk_kernel(
+    const T *__restrict__ gtab, SplitArgs a) {
     extern __shared__ uint4 lds_raw[];
     const char *tab = stage_table<T, kLds>(gtab, a.table_bytes, lds_raw);
     const uint32_t col = a.col, acc = a.acc, start = a.start;
@@ -272,9 +281,11 @@ __global__ __launch_bounds__(512 / K) void dfa_split_walk_kernel(const T *__rest
     int64_t next_g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const uint4 zero = make_uint4(0, 0, 0, 0);
 
+    const bool masks = a.nck <= 64;
     int64_t g[K], q[K], pend[K];
     int32_t in[K], mode[K];  // mode: 0 idle, 1 head (speculative), 2 exact
     uint32_t e[K], mx[K];
+    unsigned long long fm[K];  // head: flagged intervals so far
     bool have[K];
     uint4 nv[K][kVec];
 
@@ -283,6 +294,7 @@ __global__ __launch_bounds__(512 / K) void dfa_split_walk_kernel(const T *__rest
         have[c] = false;
         e[c] = start;
         mx[c] = 0;
+        fm[c] = 0;
         if (next_g >= a.segments) {
             mode[c] = 0;
             return;
@@ -304,9 +316,12 @@ __global__ __launch_bounds__(512 / K) void dfa_split_walk_kernel(const T *__rest
         if (normal) {
             if (mode[c] == 1) {
                 const int64_t hl = pend[c] - s0;
-                if (hl & rmask)
+                if (hl & rmask) {
                     ckpt[(hl >> a.rshift) * a.segments + g[c]] =
                         static_cast<T>(e[c] | (mx[c] >= acc ? 1u : 0u));
+                    if (mx[c] >= acc) fm[c] |= 1ull << ((hl >> a.rshift) & 63);
+                }
+                if (masks) a.flagmask[g[c]] = fm[c];
                 a.head_exit[g[c]] = e[c];
             } else if (a.off[in[c] + 1] > s1) {
                 a.tail_exit[g[c]] = e[c];
@@ -387,6 +402,7 @@ __global__ __launch_bounds__(512 / K) void dfa_split_walk_kernel(const T *__rest
                     if ((r & rmask) == 0) {
                         ckpt[((r >> a.rshift) - 1) * a.segments + g[c]] =
                             static_cast<T>(e[c] | (mx[c] >= acc ? 1u : 0u));
+                        if (mx[c] >= acc) fm[c] |= 1ull << (((r >> a.rshift) - 1) & 63);
                         mx[c] = 0;
                     }
                     if ((r & 255) == 0 && __atomic_load_n(&a.done[in[c]], __ATOMIC_RELAXED)) {
@@ -429,10 +445,26 @@ __global__ __launch_bounds__(512 / K) void dfa_split_walk_kernel(const T *__rest
     }
 }
 
-// Re-walk the head [s0, hend) from `entry` until it merges with the checkpoints `ck`.
-// acc_out: first accept offset relative to s0 (or -1); exit_out: state at hend.
+// First interval after `merged` whose checkpoint is flagged, or nk when none is: one load
+// of the head's flag mask when the segment has at most 64 intervals, else a scan.
 template <typename T>
-__device__ __forceinline__ void split_merge_walk(const char *tab, const SplitArgs &a,
+__device__ __forceinline__ int64_t split_next_flagged(const SplitArgs &a, int64_t g, CkView<T> ck,
+                                                      int64_t merged, int64_t nk) {
+    if (a.nck <= 64) {
+        const int64_t from = merged + 1;
+        const unsigned long long m = from < 64 ? (a.flagmask[g] >> from) : 0ull;
+        return m ? from + __builtin_ctzll(m) : nk;
+    }
+    int64_t k = merged + 1;
+    while (k < nk && !(ck[k] & 1u)) ++k;
+    return k;
+}
+
+// Re-walk the head [s0, hend) of segment g from `entry` until it merges with the
+// checkpoints `ck`. acc_out: first accept offset relative to s0 (or -1); exit_out: state at
+// hend.
+template <typename T>
+__device__ __forceinline__ void split_merge_walk(const char *tab, const SplitArgs &a, int64_t g,
                                                  int64_t s0, int64_t hend, uint32_t entry,
                                                  CkView<T> ck, uint32_t spec_exit,
                                                  int32_t &acc_out, uint32_t &exit_out) {
@@ -476,8 +508,7 @@ __device__ __forceinline__ void split_merge_walk(const char *tab, const SplitArg
     }
     // paths coincide after checkpoint `merged`: take the first flagged interval after it
     const int64_t nk = ((hend - s0) + rmask) >> a.rshift;
-    int64_t k = merged + 1;
-    while (k < nk && !(ck[k] & 1u)) ++k;
+    const int64_t k = split_next_flagged<T>(a, g, ck, merged, nk);
     if (k < nk) {
         uint32_t c = ck[k - 1] & ~1u;
         const int64_t lo = s0 + (k << a.rshift);
@@ -492,12 +523,11 @@ __device__ __forceinline__ void split_merge_walk(const char *tab, const SplitArg
 // after it is in the first flagged interval, located by walking that interval.
 template <typename T>
 __device__ __forceinline__ int32_t split_accept_after(const char *tab, const SplitArgs &a,
-                                                      int64_t s0, int64_t hend, CkView<T> ck,
-                                                      int64_t merged) {
+                                                      int64_t g, int64_t s0, int64_t hend,
+                                                      CkView<T> ck, int64_t merged) {
     const int64_t rmask = (1ll << a.rshift) - 1;
     const int64_t nk = ((hend - s0) + rmask) >> a.rshift;
-    int64_t k = merged + 1;
-    while (k < nk && !(ck[k] & 1u)) ++k;
+    const int64_t k = split_next_flagged<T>(a, g, ck, merged, nk);
     if (k >= nk) return -1;
     uint32_t c = ck[k - 1] & ~1u;
     const int64_t lo = s0 + (k << a.rshift);
@@ -512,8 +542,8 @@ __device__ __forceinline__ int32_t split_accept_after(const char *tab, const Spl
 // being repaired can race with the repair of its predecessor; the event pass re-checks every
 // head afterwards, so the race only affects speed.
 template <typename T, bool kLds, int K>
-__global__ __launch_bounds__(512 / K) void dfa_split_fix_kernel(const T *__restrict__ gtab,
-                                                             SplitArgs a, int32_t repair) {
+__global__ __launch_bounds__(kFixThreads / K) void dfa_split_fix_kernel(
+    const T *__restrict__ gtab, SplitArgs a, int32_t repair) {
     extern __shared__ uint4 lds_raw[];
     const char *tab = stage_table<T, kLds>(gtab, a.table_bytes, lds_raw);
     const uint32_t col = a.col, acc = a.acc;
@@ -580,12 +610,18 @@ __global__ __launch_bounds__(512 / K) void dfa_split_fix_kernel(const T *__restr
         if (!any) break;
         if (fast) {
             uint4 v[K][kVec];
-            uint32_t e0[K], mx[K];
+            uint32_t e0[K], mx[K], ckn[K];
 #pragma unroll
             for (int c = 0; c < K; ++c) {
                 e0[c] = e[c];
                 mx[c] = 0;
+                ckn[c] = ~0u;  // no checkpoint at the end of this round
                 if (act[c]) {
+                    // the checkpoint the round ends on, loaded while the round is walked
+                    const int64_t r_end = q[c] + kRound - g[c] * a.seg;
+                    if ((r_end & rmask) == 0)
+                        ckn[c] = static_cast<uint32_t>(
+                                     ckpt[((r_end >> a.rshift) - 1) * a.segments + g[c]]) & ~1u;
 #pragma unroll
                     for (int j = 0; j < kVec; ++j)
                         v[c][j] = have[c] ? nv[c][j] : ld16(a.data + q[c] + 16 * j);
@@ -615,9 +651,9 @@ __global__ __launch_bounds__(512 / K) void dfa_split_fix_kernel(const T *__restr
                 }
                 const int64_t r = q[c] - s0;
                 const CkView<T> ck{ckpt + g[c], a.segments};
-                if ((r & rmask) == 0 && (ck[(r >> a.rshift) - 1] & ~1u) == e[c]) {
-                    const int32_t after =
-                        split_accept_after<T>(tab, a, s0, hend[c], ck, (r >> a.rshift) - 1);
+                if (ckn[c] == e[c]) {
+                    const int32_t after = split_accept_after<T>(tab, a, g[c], s0, hend[c], ck,
+                                                                (r >> a.rshift) - 1);
                     finish(c, after, a.head_exit[g[c]]);
                     continue;
                 }
@@ -696,8 +732,8 @@ __global__ __launch_bounds__(256) void dfa_split_finish_kernel(const T *__restri
             ex = a.fixed_exit[g];
         } else {
             const int64_t hend = (s0 + a.seg) < ie ? (s0 + a.seg) : ie;
-            split_merge_walk<T>(tab, a, s0, hend, cur, CkView<T>{ckpt + g, a.segments}, a.head_exit[g],
-                                acc_at, ex);
+            split_merge_walk<T>(tab, a, g, s0, hend, cur, CkView<T>{ckpt + g, a.segments},
+                                a.head_exit[g], acc_at, ex);
         }
         if (acc_at >= 0) {
             a.flags[i] = 1;
@@ -764,7 +800,7 @@ DfaSplitPlan dfa_split_plan(const DfaDev &d, int64_t total_bytes, int32_t count,
     const size_t ent = d.wide ? 4 : 2;
     p.scratch_bytes = align256(static_cast<size_t>(count) * 4) +
                       align256(static_cast<size_t>(count) * 8) + 5 * align256(S * 4) +
-                      align256(S * nck * ent);
+                      align256(S * 8) + align256(S * nck * ent);
     return p;
 }
 
@@ -791,29 +827,36 @@ static void split_launch_passes(const DfaDev &d, SplitArgs &a, hipStream_t strea
         attr = lds;
     }
     // spread the segments over every CU, kChains per lane
-    int64_t per_cu = (a.segments + static_cast<int64_t>(cus) * kChains - 1) /
-                     (static_cast<int64_t>(cus) * kChains);
-    per_cu = ((per_cu + 63) / 64) * 64;
-    int32_t threads, blocks;
-    if (kLds) {  // the table fills the LDS: one block per CU, up to 8 waves
-        const int64_t cap = 512 / kChains;
-        threads = static_cast<int32_t>(per_cu < 64 ? 64 : (per_cu > cap ? cap : per_cu));
-        const int64_t need = (a.segments + static_cast<int64_t>(threads) * kChains - 1) /
-                             (static_cast<int64_t>(threads) * kChains);
-        blocks = static_cast<int32_t>(need < cus ? (need > 0 ? need : 1) : cus);
-    } else {
-        threads = per_cu >= 256 / kChains ? 256 / kChains : 64;
-        const int64_t need = (a.segments + static_cast<int64_t>(threads) * kChains - 1) /
-                             (static_cast<int64_t>(threads) * kChains);
-        const int64_t cap = static_cast<int64_t>(cus) * 32;
-        blocks = static_cast<int32_t>(need < cap ? (need > 0 ? need : 1) : cap);
-    }
-    dfa_split_walk_kernel<T, kLds, kChains><<<blocks, threads, lds, stream>>>(tab, a);
+    struct Shape {
+        int32_t blocks, threads;
+    };
+    auto shape = [&](int32_t max_threads) {
+        int64_t per_cu = (a.segments + static_cast<int64_t>(cus) * kChains - 1) /
+                         (static_cast<int64_t>(cus) * kChains);
+        per_cu = ((per_cu + 63) / 64) * 64;
+        Shape sh;
+        if (kLds) {  // the table fills the LDS: one block per CU
+            const int64_t cap = max_threads / kChains;
+            sh.threads = static_cast<int32_t>(per_cu < 64 ? 64 : (per_cu > cap ? cap : per_cu));
+            const int64_t need = (a.segments + static_cast<int64_t>(sh.threads) * kChains - 1) /
+                                 (static_cast<int64_t>(sh.threads) * kChains);
+            sh.blocks = static_cast<int32_t>(need < cus ? (need > 0 ? need : 1) : cus);
+        } else {
+            sh.threads = per_cu >= 256 / kChains ? 256 / kChains : 64;
+            const int64_t need = (a.segments + static_cast<int64_t>(sh.threads) * kChains - 1) /
+                                 (static_cast<int64_t>(sh.threads) * kChains);
+            const int64_t cap = static_cast<int64_t>(cus) * 32;
+            sh.blocks = static_cast<int32_t>(need < cap ? (need > 0 ? need : 1) : cap);
+        }
+        return sh;
+    };
+    const Shape walk = shape(kWalkThreads), fix = shape(kFixThreads);
+    dfa_split_walk_kernel<T, kLds, kChains><<<walk.blocks, walk.threads, lds, stream>>>(tab, a);
     GFSM_HIP(hipGetLastError());
     if (a.segments > 1) {
-        dfa_split_fix_kernel<T, kLds, kChains><<<blocks, threads, lds, stream>>>(tab, a, 0);
+        dfa_split_fix_kernel<T, kLds, kChains><<<fix.blocks, fix.threads, lds, stream>>>(tab, a, 0);
         GFSM_HIP(hipGetLastError());
-        dfa_split_fix_kernel<T, kLds, kChains><<<blocks, threads, lds, stream>>>(tab, a, 1);
+        dfa_split_fix_kernel<T, kLds, kChains><<<fix.blocks, fix.threads, lds, stream>>>(tab, a, 1);
         GFSM_HIP(hipGetLastError());
         const int64_t eblocks64 = (a.segments + 255) / 256;
         const int32_t eblocks = static_cast<int32_t>(eblocks64 < 65536 ? eblocks64 : 65536);
@@ -861,6 +904,8 @@ void launch_dfa_split(const DfaDev &d, const DfaSplitPlan &plan, const uint8_t *
     base += align256(S * 4);
     a.fixed_acc = reinterpret_cast<int32_t *>(base);
     base += align256(S * 4);
+    a.flagmask = reinterpret_cast<unsigned long long *>(base);
+    base += align256(S * 8);
     a.ckpt = base;
     split_init_kernel<<<(count + 255) / 256, 256, 0, stream>>>(flags, lens, a.done,
                                                                 a.first_event, count);

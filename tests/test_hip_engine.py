@@ -103,6 +103,39 @@ def test_graph_replays_match_eager(engine, technique):
         obatch.fetch(s)
 
 
+@pytest.mark.parametrize("segment", [256, 512, 1024, 8192])
+def test_split_segment_sizes(engine, segment):
+    """Split at fixed segment lengths: up to 64 checkpoint intervals per segment the fix
+    pass finds flagged intervals from one mask word per head, above that by a scan. Rare
+    accepting states put the first matches deep inside heads, after their merge points."""
+    import torch
+
+    from gpufsm.core.dfa import DFA
+
+    s = torch.cuda.current_stream()
+    for n, seed in ((256, 61), (48, 62)):
+        rng = np.random.default_rng(seed)
+        base = W.dense_dfa(n, 0.0, seed=seed)
+        acc_state = (base.start_state + 1) % n
+        trans = np.asarray(base.trans, dtype=np.int32).reshape(n, 256).copy()
+        trans[trans == acc_state] = (acc_state + 1) % n
+        # about one step in 2 segments enters the accepting state
+        k = max(2, (n * 256) // (2 * segment))
+        trans.reshape(-1)[rng.choice(n * 256, size=k, replace=False)] = acc_state
+        accept = np.zeros(n, dtype=bool)
+        accept[acc_state] = True
+        a = DFA(num_states=n, start_state=base.start_state, accept=accept,
+                trans=trans.reshape(-1))
+        d, o = W.uniform_batch(40, 4 * segment + 5000, seed=seed + 1)
+        wf, wn = _oracle(a, d, o)
+        assert wf.any() and wn[wf.astype(bool)].max() > segment
+        batch = DeviceBatch(d, o, stream=s)
+        engine.enqueue(a, batch, f"split:segment={segment}", stream=s)
+        f, m = batch.fetch(s)
+        np.testing.assert_array_equal(f, wf, err_msg=f"n={n}")
+        np.testing.assert_array_equal(m, wn, err_msg=f"n={n}")
+
+
 def test_resident_cache_reuses_and_evicts():
     eng = BatchEngine(tuner=None, budget_bytes=3 << 20)
     try:
