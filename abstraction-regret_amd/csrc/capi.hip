@@ -238,8 +238,10 @@ void nfa_enqueue(gfsm_nfa *h, int32_t tech, const Tuning &tu, const uint8_t *d_d
                                      std::to_string(need) + " needed");
     if (tech == GFSM_NFA_CHAIN) {
         const bool pairs = !(tu.flags & GFSM_TUNE_CHAIN_FIRST) && env_int("GFSM_CHAIN_FIRST", 0) != 1;
+        const bool paired =
+            !(tu.flags & GFSM_TUNE_CHAIN_WAVE64) && env_int("GFSM_CHAIN_WAVE64", 0) != 1;
         launch_nfa_chain(h->chain, h->tile, d_data, d_off, count, total, d_flags, d_lens, ws,
-                         nfa_chain_chunk_bytes(tu.chunk), pairs, s);
+                         nfa_chain_chunk_bytes(tu.chunk), pairs, paired, s);
         return;
     }
     if (tech == GFSM_NFA_BITPACKED) {

@@ -19,7 +19,7 @@
 // their accept bit (bflags), for emptiness, and for their successors, which the next byte c
 // adds from pair_slot[pair[p][c]]: the second positions whose first class holds p and whose
 // own class holds c, with one accept bit for the whole list. Listed states continue to
-// s + 1 on c when bit s + 1 of next[tile][c] is set: one 64-bit load per listed state,
+// s + 1 on c when bit s + 1 of next[c][tile] is set: one 32-bit load per listed state,
 // issued together with the load of its accept word. A chain state has one predecessor, so
 // no deduplication is needed (a state that is both a first position and a chain successor
 // may be listed twice, which changes no verdict).
@@ -63,7 +63,9 @@ __device__ __forceinline__ int32_t lane_rank(uint64_t m) {
 // (it then skips at once). Inputs are ordered by chunk count, descending (a counting sort),
 // which makes round k a prefix of that order: item x of round k is perm[x - rbase[k]].
 struct ChainScratch {
-    int32_t *best, *ovf1, *ovf2;
+    int32_t *best;             // [count] first accept (1-based), INT_MAX = none yet
+    int32_t *ovf2;             // [count] inputs the LDS tier hands to the tiles
+    int32_t *ovf_items;        // items the register tier hands to the LDS tier
     int32_t *nch;              // [count] chunks per input
     int32_t *perm;             // [count] inputs by chunk count, descending
     int32_t *hist;             // [K] inputs with exactly c chunks
@@ -71,7 +73,7 @@ struct ChainScratch {
     int32_t *rcount;           // [K] R_c = inputs with more than c chunks
     int64_t *rbase;            // [K] first item of round k; total items at [maxk]
     int32_t *maxk;             // [1] most chunks of any input
-    unsigned int *counter;     // [2]: reg tier, lds tier
+    unsigned int *counter;     // [4]: reg tier, lds tier, ovf_items fill, unused
     int32_t K;                 // capacity of the per-round arrays (> max chunks + 1)
 };
 
@@ -85,6 +87,8 @@ __global__ void chain_zero_kernel(ChainScratch sc) {
         *sc.maxk = 0;
         sc.counter[0] = 0u;
         sc.counter[1] = 0u;
+        sc.counter[2] = 0u;
+        sc.counter[3] = 0u;
     }
 }
 
@@ -93,7 +97,6 @@ __global__ void chain_prep_kernel(const int64_t *__restrict__ off, int32_t count
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) {
         sc.best[i] = INT_MAX;
-        sc.ovf1[i] = 0;
         sc.ovf2[i] = 0;
         const int64_t len = off[i + 1] - off[i];
         const int32_t c = static_cast<int32_t>((len + chunk - 1) / chunk);
@@ -181,33 +184,43 @@ __device__ __forceinline__ uint32_t window_byte(uint32_t word, int32_t r) {
 // One work item, decoded.
 struct Item {
     int32_t in;
+    uint32_t id;  // the item's number (round-major order)
     int64_t ib, cs, ce, ps;
 };
 
-// Takes the next item for the whole wave; false when none are left. Every lane takes part
-// in the atomic (lane 0 adds 1, the others 0) and the value is read from lane 0 under a
-// full mask: no branch precedes the broadcast. (A lane-0-only branch there lets the
-// compiler peel lane 0 off into a loop of its own, after which the other lanes never see a
-// new item.)
-__device__ __forceinline__ bool next_item(unsigned int *counter, const ChainScratch &sc,
-                                          const int64_t *__restrict__ off, int32_t maxk,
-                                          int64_t items, int64_t chunk, int64_t overlap, int lane,
-                                          Item &it) {
-    const unsigned int mine = atomicAdd(counter, lane == 0 ? 1u : 0u);
-    const unsigned int item = __builtin_amdgcn_readfirstlane(mine);
-    if (static_cast<int64_t>(item) >= items) return false;
-    int32_t lo = 0, hi = maxk;  // round: largest k with rbase[k] <= item
+// Item number `item` (< items) decoded: its round k is the largest with rbase[k] <= item.
+__device__ __forceinline__ void decode_item(uint32_t item, const ChainScratch &sc,
+                                            const int64_t *__restrict__ off, int32_t maxk,
+                                            int64_t chunk, int64_t overlap, Item &it) {
+    int32_t lo = 0, hi = maxk;
     while (hi - lo > 1) {
         const int32_t mid = (lo + hi) >> 1;
         if (sc.rbase[mid] <= static_cast<int64_t>(item)) lo = mid; else hi = mid;
     }
     const int32_t in = sc.perm[static_cast<int64_t>(item) - sc.rbase[lo]];
     it.in = in;
+    it.id = item;
     it.ib = off[in];
     const int64_t ie = off[in + 1];
     it.cs = it.ib + static_cast<int64_t>(lo) * chunk;
     it.ce = (it.cs + chunk) < ie ? (it.cs + chunk) : ie;
     it.ps = (it.cs - overlap) > it.ib ? (it.cs - overlap) : it.ib;
+}
+
+// Takes the next item for the whole wave; false when none are left. Items are numbered
+// 0..n-1, or are list[0..n-1] when a list is given. Every lane takes part in the atomic
+// (lane 0 adds 1, the others 0) and the value is read from lane 0 under a full mask: no
+// branch precedes the broadcast. (A lane-0-only branch there lets the compiler peel lane 0
+// off into a loop of its own, after which the other lanes never see a new item.)
+__device__ __forceinline__ bool next_item(unsigned int *counter, const int32_t *list,
+                                          const ChainScratch &sc, const int64_t *__restrict__ off,
+                                          int32_t maxk, int64_t n, int64_t chunk, int64_t overlap,
+                                          int lane, Item &it) {
+    const unsigned int mine = atomicAdd(counter, lane == 0 ? 1u : 0u);
+    const unsigned int k = __builtin_amdgcn_readfirstlane(mine);
+    if (static_cast<int64_t>(k) >= n) return false;
+    const uint32_t item = list ? static_cast<uint32_t>(list[k]) : k;
+    decode_item(item, sc, off, maxk, chunk, overlap, it);
     return true;
 }
 
@@ -250,11 +263,12 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_reg_kernel(
     const int64_t data_end = off[count];
     const int64_t overlap = d.max_chain > 0 ? d.max_chain - 1 : 0;
     const char *__restrict__ next_base = reinterpret_cast<const char *>(d.next);
+    const uint32_t plane = static_cast<uint32_t>(d.tiles + 1) * (kTileWords * 8);  // bytes per byte plane
     const char *__restrict__ acc_base = reinterpret_cast<const char *>(d.acc);
     const char *__restrict__ slot_base = kSmall ? reinterpret_cast<const char *>(d.pair_slot16)
                                                 : reinterpret_cast<const char *>(d.pair_slot);
     Item it;
-    while (next_item(&sc.counter[0], sc, off, maxk, items, chunk, overlap, lane, it)) {
+    while (next_item(&sc.counter[0], nullptr, sc, off, maxk, items, chunk, overlap, lane, it)) {
         bool stop = settled(sc.best, it, it.cs);  // wave-uniform throughout
         int32_t slot = -1;                        // this lane's listed state, -1 = free
         uint64_t occ = 0;                         // lanes holding a state
@@ -342,8 +356,8 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_reg_kernel(
                     }
                     // continuation of held states; free lanes look up slot 0 and ignore it
                     const uint32_t t = have ? static_cast<uint32_t>(slot + 1) : 0u;
-                    // 32-bit words: tile t >> 12, byte b, word (t & 4095) >> 5
-                    const uint32_t noff = ((t >> 12) << 17) | (b << 9) | ((t >> 3) & 0x1fcu);
+                    // plane b of the continuation table, 32-bit word t >> 5
+                    const uint32_t noff = b * plane + ((t >> 5) << 2);
                     const uint32_t nw = *reinterpret_cast<const uint32_t *>(next_base + noff);
                     const uint32_t aw = kSmall ? s_acc[t >> 5]
                                                : *reinterpret_cast<const uint32_t *>(acc_base + ((t >> 5) << 2));
@@ -379,7 +393,286 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_reg_kernel(
         }
         if (lane == 0) {
             if (found > 0) atomicMin(&sc.best[it.in], found);
-            if (ovf) sc.ovf1[it.in] = 1;
+            if (ovf) sc.ovf_items[atomicAdd(&sc.counter[2], 1u)] = static_cast<int32_t>(it.id);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- paired reg tier ----
+
+// Register tier with two work items per wave: lanes 0-31 run one (input, chunk) item and
+// lanes 32-63 another, each half holding up to 32 listed states, one per lane. The step is
+// the one-item tier's, but what was uniform over the wave is now uniform over a half: the
+// per-byte values (pair list offset, byte, clipped pair count, start-side accept, in range)
+// are written to LDS once per window and read back with one broadcast load per byte, ranks
+// among a half's free lanes come from mbcnt over that half's bits, and the events of both
+// halves (accept, overflow) are caught by one ballot. The per-item scalar work of the
+// one-item tier becomes vector work that serves two items per instruction.
+//
+// A half whose item stops inside a window (an accept, an overflow) clears the in-range bit
+// of its remaining bytes and idles to the window's end. An empty active set (possible only
+// without the start state's self-loop on every byte) is detected at window ends: the set
+// stays empty once the start state and its successors are gone.
+constexpr int kHalf = 32;                   // lanes per item
+constexpr int kHalfWin = 4 * kHalf;         // bytes per window and item
+constexpr uint32_t kInRange = 0x80000000u;  // packed byte info: bit 31
+constexpr uint32_t kStartAcc = 0x80u;       // bit 7; bits 0-6: pair count, clipped to 127
+// bits 9-28: the byte's plane of the continuation table in 32-bit words when the automaton
+// is small (byte * (tiles + 1) * 128 < 2^20), else the byte itself
+constexpr int kPlaneShift = 9;
+
+template <bool kAlways, bool kSmall>
+__global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_pair_kernel(
+    NfaChainDev d, const uint8_t *__restrict__ data, const int64_t *__restrict__ off,
+    int32_t count, ChainScratch sc, int64_t chunk) {
+    __shared__ uint32_t s_flags[256];
+    __shared__ uint32_t s_acc[kSmall ? (kSmallTiles + 1) * kTileWords * 2 : 1];
+    // {pair list offset, packed byte info} per byte; one spare entry per half, read by the
+    // prefetch after the window's last byte
+    __shared__ uint2 s_info[kChainWaves][2][kHalfWin + 1];
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) s_flags[i] = d.bflags[i];
+    if (kSmall) {
+        const uint32_t *acc32 = reinterpret_cast<const uint32_t *>(d.acc);
+        for (int i = threadIdx.x; i < (d.tiles + 1) * kTileWords * 2; i += blockDim.x)
+            s_acc[i] = acc32[i];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int half = lane >> 5;
+    const int hl = lane & 31;      // lane within the half
+    const int hbase = lane & 32;   // first lane of the half
+    const uint32_t lo_keep = half ? 0u : ~0u;
+    uint2 *info = s_info[threadIdx.x >> 6][half];
+    const int32_t maxk = *sc.maxk;
+    const int64_t items = sc.rbase[maxk];
+    const int64_t data_end = off[count];
+    const int64_t overlap = d.max_chain > 0 ? d.max_chain - 1 : 0;
+    const uint32_t plane_words = static_cast<uint32_t>(d.tiles + 1) * (kTileWords * 2);
+    const char *__restrict__ next_base = reinterpret_cast<const char *>(d.next);
+    const char *__restrict__ acc_base = reinterpret_cast<const char *>(d.acc);
+    const char *__restrict__ slot_base = kSmall ? reinterpret_cast<const char *>(d.pair_slot16)
+                                                : reinterpret_cast<const char *>(d.pair_slot);
+    // item state, uniform within a half
+    Item it{0, 0u, 0, 0, 0, 0};
+    bool running = false;    // the half holds an item
+    bool exhausted = false;  // wave-uniform: every item has been handed out
+    int64_t w0 = 0;          // first byte of the item's current window
+    int32_t d1 = 256;        // byte of the depth-1 set before the window; 256 = none
+    bool s0 = true;
+    // Stop state as integers, not flags: flags changed in divergent code inside the byte
+    // loop become lane masks that the compiler merges with exec on every iteration.
+    int32_t stopped = 0;     // nonzero once the item is finished
+    int32_t found = 0;       // > 0: first accept (1-based); -1: overflow to the LDS tier
+    int32_t slot = -1;       // this lane's listed state, -1 = free
+    // the first item of every half is its own index; later ones come from the counter,
+    // offset by the number of halves (a single counter handed out to every half at once
+    // serializes the start of the launch)
+    const uint32_t halves = gridDim.x * (kChainWaves * 2);
+    const uint32_t first_item = (blockIdx.x * kChainWaves + (threadIdx.x >> 6)) * 2 + half;
+    bool first_round = true;  // wave-uniform
+    for (;;) {
+        // halves without an item take the next one (all lanes take part in the atomic and
+        // the broadcast; items already settled are skipped here)
+        while (!exhausted && __ballot(!running) != 0) {
+            uint32_t item = first_item;
+            if (!first_round) {
+                const unsigned int mine =
+                    atomicAdd(&sc.counter[0], (hl == 0 && !running) ? 1u : 0u);
+                item = static_cast<uint32_t>(__shfl(static_cast<int>(mine), hbase)) + halves;
+            }
+            first_round = false;
+            bool none = false;
+            if (!running) {
+                if (static_cast<int64_t>(item) < items) {
+                    decode_item(item, sc, off, maxk, chunk, overlap, it);
+                    if (!settled(sc.best, it, it.cs)) {
+                        running = true;
+                        w0 = it.ps - static_cast<int64_t>(
+                                         (reinterpret_cast<uintptr_t>(data) + it.ps) & 3u);
+                        d1 = 256;
+                        s0 = true;
+                        stopped = 0;
+                        found = 0;
+                        slot = -1;
+                    }
+                } else {
+                    none = true;
+                }
+            }
+            exhausted = __ballot(none) != 0;
+        }
+        if (__ballot(running) == 0) break;  // nothing left to hand out and nothing running
+
+        // ---- the window: per-byte values of both halves into LDS
+        const uint32_t word = running ? chain_load_word(data, w0 + 4 * hl, it.ce, data_end) : 0u;
+        const int32_t lo = running ? (it.ps > w0 ? static_cast<int32_t>(it.ps - w0) : 0) : kHalfWin;
+        const int32_t hi =
+            running ? ((it.ce - w0) < kHalfWin ? static_cast<int32_t>(it.ce - w0) : kHalfWin) : 0;
+        const int32_t hit_from = it.cs > w0 ? static_cast<int32_t>(it.cs - w0) : 0;
+        uint32_t bq[4], fq[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t wq = static_cast<uint32_t>(
+                __shfl(static_cast<int>(word), hbase + 8 * q + (hl >> 2)));
+            bq[q] = (wq >> (8 * (hl & 3))) & 0xffu;
+            fq[q] = s_flags[bq[q]];
+        }
+        // kpos: first byte of the window (>= lo) on which the start state has no self-loop
+        int32_t kpos = 1 << 20;
+        if (!kAlways) {
+#pragma unroll
+            for (int q = 3; q >= 0; --q) {
+                const int32_t r = kHalf * q + hl;
+                const uint64_t m = __ballot(r >= lo && r < hi && !(fq[q] & 4u));
+                const uint32_t mh = static_cast<uint32_t>(m >> hbase);
+                if (mh) kpos = kHalf * q + static_cast<int32_t>(__builtin_ctz(mh));
+            }
+        }
+        int32_t d1_after = 256;  // hl 31, q 3: the start side after the window's last byte
+        bool s0_after = true;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int32_t r = kHalf * q + hl;
+            uint32_t bp = static_cast<uint32_t>(__shfl_up(static_cast<int>(bq[q]), 1, kHalf));
+            uint32_t fp = static_cast<uint32_t>(__shfl_up(static_cast<int>(fq[q]), 1, kHalf));
+            if (q > 0) {
+                const uint32_t cb = static_cast<uint32_t>(__shfl(static_cast<int>(bq[q - 1]), hbase + 31));
+                const uint32_t cf = static_cast<uint32_t>(__shfl(static_cast<int>(fq[q - 1]), hbase + 31));
+                if (hl == 0) {
+                    bp = cb;
+                    fp = cf;
+                }
+            }
+            const bool s0p = kAlways || (s0 && kpos >= r - 1);  // before byte r - 1
+            const bool s0r = kAlways || (s0 && kpos >= r);      // before byte r
+            const bool s0n = kAlways || (s0 && kpos > r);       // after byte r
+            const int32_t d1q =
+                r == lo ? d1 : ((s0p && (fp & 1u)) ? static_cast<int32_t>(bp) : 256);
+            const int32_t d1n = (s0r && (fq[q] & 1u)) ? static_cast<int32_t>(bq[q]) : 256;
+            const bool inr = r >= lo && r < hi;
+            const int32_t idx = inr ? 2 * (d1q * 256 + static_cast<int32_t>(bq[q])) : 0;
+            const int32_t pw = d.pair[idx + 1];
+            const int32_t cnt = pw & INT_MAX;
+            const uint32_t sacc = (inr && (pw < 0 || (s0r && (fq[q] & 2u)))) ? kStartAcc : 0u;
+            const uint32_t where = kSmall ? bq[q] * plane_words : bq[q];
+            const uint32_t pk = (where << kPlaneShift) | sacc |
+                                (inr ? (kInRange | static_cast<uint32_t>(cnt < 127 ? cnt : 127)) : 0u);
+            // an empty list reads entries 0-31 (one line that stays cached), not the scattered
+            // offset of its (depth-1 byte, byte) pair
+            const uint32_t po = (inr && cnt > 0) ? static_cast<uint32_t>(d.pair[idx]) : 0u;
+            info[r] = make_uint2(po, pk);
+            if (q == 3) {
+                d1_after = d1n;
+                s0_after = s0n;
+            }
+        }
+        wave_sync();
+
+        // ---- the bytes: both halves step together
+        const int32_t ilo = min(__builtin_amdgcn_readlane(lo, 0), __builtin_amdgcn_readlane(lo, 32));
+        const int32_t ihi = max(__builtin_amdgcn_readlane(hi, 0), __builtin_amdgcn_readlane(hi, 32));
+        uint2 e = info[ilo];
+        for (int32_t i = ilo; i < ihi; ++i) {
+            const uint2 en = info[i + 1];  // the next byte's values, read ahead
+            const uint32_t po = e.x, pk = e.y;
+            const bool inr = static_cast<int32_t>(pk) < 0;
+            const bool was_free = slot < 0;
+            const uint64_t freem = __ballot(was_free);
+            const uint32_t flo = static_cast<uint32_t>(freem) & lo_keep;
+            const uint32_t fhi = static_cast<uint32_t>(freem >> 32);
+            const int32_t frank = static_cast<int32_t>(
+                __builtin_amdgcn_mbcnt_hi(fhi, __builtin_amdgcn_mbcnt_lo(flo, 0u)));
+            // additions go to the half's free lanes in lane order; every lane reads an entry
+            // (at most 31 past the list, inside the padding) and only taking lanes use it
+            const uint32_t k = po + static_cast<uint32_t>(frank);
+            const int32_t cand =
+                kSmall ? static_cast<int32_t>(*reinterpret_cast<const uint16_t *>(slot_base + (k << 1)))
+                       : *reinterpret_cast<const int32_t *>(slot_base + (k << 2));
+            // continuation of held states; free lanes look up slot 0, whose bit is always 0
+            const uint32_t t = static_cast<uint32_t>(slot + 1);
+            const uint32_t wd = t >> 5;
+            const uint32_t noff =
+                kSmall ? (((pk >> kPlaneShift) & 0xfffffu) + wd) << 2
+                       : ((pk >> kPlaneShift) & 0xffu) * (plane_words << 2) + (wd << 2);
+            const uint32_t nw = *reinterpret_cast<const uint32_t *>(next_base + noff);
+            const uint32_t aw = kSmall ? s_acc[wd]
+                                       : *reinterpret_cast<const uint32_t *>(acc_base + (wd << 2));
+            // all loads in flight together: without this the compiler moves the pair list
+            // load under the "state died" branch, after the continuation load has returned
+            asm volatile("" ::"v"(nw), "v"(cand), "v"(aw));
+            const uint32_t bit = 1u << (t & 31u);
+            const uint32_t alive = nw & bit;
+            const bool hit = inr && (alive & aw) != 0;
+            const int32_t cnt = static_cast<int32_t>(pk & 127u);
+            const bool take = was_free && frank < cnt;
+            // lane 31 of a half: its rank plus its own free bit = the half's free lanes
+            const int32_t nfree = frank + (was_free ? 1 : 0);
+            const bool short_now = hl == 31 && cnt > nfree;
+            const int32_t nslot = alive ? static_cast<int32_t>(t) : (take ? cand : -1);
+            slot = inr ? nslot : slot;
+            e = en;
+            if (__ballot(hit || (pk & kStartAcc) || short_now) != 0) {
+                // more additions than lanes free before the byte (rare): the rest go to the
+                // lanes whose state died on this byte, in lane order; only when those are
+                // too few as well does the item overflow to the LDS tier
+                const uint64_t sm = __ballot(short_now);
+                const int32_t F = __shfl(nfree, hbase + 31);
+                const bool died = inr && !was_free && !alive;
+                const uint64_t dm = __ballot(died);
+                const uint32_t dlo = static_cast<uint32_t>(dm) & lo_keep;
+                const uint32_t dhi = static_cast<uint32_t>(dm >> 32);
+                const int32_t drank = static_cast<int32_t>(
+                    __builtin_amdgcn_mbcnt_hi(dhi, __builtin_amdgcn_mbcnt_lo(dlo, 0u)));
+                const int32_t ndied = __shfl(drank + (died ? 1 : 0), hbase + 31);
+                const bool short_half = static_cast<uint32_t>(sm >> hbase) != 0;
+                if (short_half && died && F + drank < cnt) {
+                    const uint32_t k2 = po + static_cast<uint32_t>(F + drank);
+                    slot = kSmall ? static_cast<int32_t>(
+                                        *reinterpret_cast<const uint16_t *>(slot_base + (k2 << 1)))
+                                  : *reinterpret_cast<const int32_t *>(slot_base + (k2 << 2));
+                }
+                const bool ovf_half = short_half && cnt > F + ndied;
+                const uint64_t em = __ballot(hit || (pk & kStartAcc) || ovf_half);
+                if (static_cast<uint32_t>(em >> hbase) != 0) {
+                    if (ovf_half) {
+                        found = -1;
+                        stopped = i + 1;
+                    } else if (i >= hit_from) {
+                        found = static_cast<int32_t>(w0 + i - it.ib + 1);
+                        stopped = i + 1;
+                    }
+                    if (stopped) {
+                        slot = -1;
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const int32_t r = kHalf * q + hl;
+                            if (r > i) info[r].y = 0u;
+                        }
+                    }
+                }
+                wave_sync();
+                e = info[i + 1];
+                if (__ballot(running && stopped == 0) == 0) break;
+            }
+        }
+
+        // ---- window end: carry the start side, finish items
+        d1 = __shfl(d1_after, hbase + 31);
+        s0 = kAlways || __shfl(s0_after ? 1 : 0, hbase + 31) != 0;
+        w0 += kHalfWin;
+        const bool held = static_cast<uint32_t>(__ballot(slot >= 0) >> hbase) != 0;
+        if (running && stopped == 0) {
+            if (w0 >= it.ce) stopped = 1;
+            else if (!kAlways && !held && d1 == 256 && !s0) stopped = 1;  // empty for good
+            else if (settled(sc.best, it, w0)) stopped = 1;
+        }
+        if (running && stopped != 0) {
+            if (hl == 0) {
+                if (found > 0) atomicMin(&sc.best[it.in], found);
+                if (found < 0) sc.ovf_items[atomicAdd(&sc.counter[2], 1u)] = static_cast<int32_t>(it.id);
+            }
+            running = false;
+            slot = -1;
         }
     }
 }
@@ -389,18 +682,20 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_reg_kernel(
 template <bool kPairs, int kCap>
 __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_lds_kernel(
     NfaChainDev d, const uint8_t *__restrict__ data, const int64_t *__restrict__ off,
-    int32_t count, ChainScratch sc, int64_t chunk, const int32_t *__restrict__ only,
-    int32_t *ovf_out) {
+    int32_t count, ChainScratch sc, int64_t chunk, bool from_list, int32_t *ovf_out) {
     __shared__ int32_t lists[kChainWaves][2][kCap];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
+    const size_t plane64 = static_cast<size_t>(d.tiles + 1) * kTileWords;  // words per byte plane
     const int32_t maxk = *sc.maxk;
-    const int64_t items = sc.rbase[maxk];
+    // every item, or only those the register tier could not finish
+    const int64_t items = from_list ? static_cast<int64_t>(sc.counter[2]) : sc.rbase[maxk];
+    const int32_t *list = from_list ? sc.ovf_items : nullptr;
     const int64_t data_end = off[count];
     const int64_t overlap = d.max_chain > 0 ? d.max_chain - 1 : 0;
     Item it;
-    while (next_item(&sc.counter[1], sc, off, maxk, items, chunk, overlap, lane, it)) {
-        bool stop = (only && !only[it.in]) || settled(sc.best, it, it.cs);  // wave-uniform
+    while (next_item(&sc.counter[1], list, sc, off, maxk, items, chunk, overlap, lane, it)) {
+        bool stop = settled(sc.best, it, it.cs);  // wave-uniform
         int32_t *A = lists[wv][0];
         int32_t *B = lists[wv][1];
         int32_t k = 0;
@@ -440,7 +735,7 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_lds_kernel(
                         const uint32_t tile = static_cast<uint32_t>(t) >> 12;
                         const uint32_t sl = static_cast<uint32_t>(t) & 4095u;
                         const uint64_t nw =
-                            d.next[(static_cast<size_t>(tile) * 256 + b) * kTileWords + (sl >> 6)];
+                            d.next[static_cast<size_t>(b) * plane64 + (static_cast<uint32_t>(t) >> 6)];
                         const uint64_t aw = d.acc[tile * kTileWords + (sl >> 6)];
                         alive = (nw >> (sl & 63)) & 1ull;
                         hit |= alive && ((aw >> (sl & 63)) & 1ull);
@@ -515,6 +810,13 @@ static int64_t effective_chunk(const NfaChainDev &d, int32_t chunk_bytes) {
     return d.start_always ? static_cast<int64_t>(chunk_bytes) : (int64_t(1) << 62);
 }
 
+// Most items a launch can have: every input contributes ceil(length / chunk) of them.
+static size_t item_capacity(const NfaChainDev &d, int32_t count, int64_t total_bytes,
+                            int32_t chunk_bytes) {
+    return static_cast<size_t>(total_bytes / effective_chunk(d, chunk_bytes)) +
+           static_cast<size_t>(count) + 1;
+}
+
 // Capacity of the per-round arrays: more than the most chunks any input can have.
 static int32_t round_capacity(int64_t total_bytes, int64_t chunk) {
     const int64_t k = total_bytes / chunk + 3;
@@ -525,13 +827,14 @@ size_t nfa_chain_scratch_bytes(const NfaChainDev &d, int32_t count, int64_t tota
                                int32_t chunk_bytes) {
     const size_t c = static_cast<size_t>(count);
     const size_t K = static_cast<size_t>(round_capacity(total_bytes, effective_chunk(d, chunk_bytes)));
-    return 5 * align256(c * 4) + 3 * align256(K * 4) + align256(K * 8) + 256 + 256;
+    return 4 * align256(c * 4) + align256(item_capacity(d, count, total_bytes, chunk_bytes) * 4) +
+           3 * align256(K * 4) + align256(K * 8) + 256 + 256;
 }
 
 void launch_nfa_chain(const NfaChainDev &d, const NfaTileDev &tiles, const uint8_t *data,
                       const int64_t *off, int32_t count, int64_t total_bytes, int32_t *flags,
                       int32_t *lens, void *scratch, int32_t chunk_bytes, bool pairs,
-                      hipStream_t stream) {
+                      bool paired, hipStream_t stream) {
     if (count <= 0) return;
     const int64_t chunk = effective_chunk(d, chunk_bytes);
     const size_t c = static_cast<size_t>(count);
@@ -545,7 +848,8 @@ void launch_nfa_chain(const NfaChainDev &d, const NfaTileDev &tiles, const uint8
         return q;
     };
     sc.best = reinterpret_cast<int32_t *>(take(c * 4));
-    sc.ovf1 = reinterpret_cast<int32_t *>(take(c * 4));
+    sc.ovf_items = reinterpret_cast<int32_t *>(
+        take(item_capacity(d, count, total_bytes, chunk_bytes) * 4));
     sc.ovf2 = reinterpret_cast<int32_t *>(take(c * 4));
     sc.nch = reinterpret_cast<int32_t *>(take(c * 4));
     sc.perm = reinterpret_cast<int32_t *>(take(c * 4));
@@ -554,7 +858,7 @@ void launch_nfa_chain(const NfaChainDev &d, const NfaTileDev &tiles, const uint8
     sc.rcount = reinterpret_cast<int32_t *>(take(K * 4));
     sc.rbase = reinterpret_cast<int64_t *>(take(K * 8));
     sc.maxk = reinterpret_cast<int32_t *>(take(4));
-    sc.counter = reinterpret_cast<unsigned int *>(take(8));
+    sc.counter = reinterpret_cast<unsigned int *>(take(16));
     chain_zero_kernel<<<(sc.K + 255) / 256, 256, 0, stream>>>(sc);
     GFSM_HIP(hipGetLastError());
     chain_prep_kernel<<<(count + 255) / 256, 256, 0, stream>>>(off, count, chunk, sc);
@@ -566,13 +870,23 @@ void launch_nfa_chain(const NfaChainDev &d, const NfaTileDev &tiles, const uint8
     const int threads = 64 * kChainWaves;
     static const int grid_reg = persistent_grid(
         reinterpret_cast<const void *>(&nfa_chain_reg_kernel<true, true>), threads);
+    static const int grid_pair = persistent_grid(
+        reinterpret_cast<const void *>(&nfa_chain_pair_kernel<true, true>), threads);
     static const int grid_lds_pairs = persistent_grid(
         reinterpret_cast<const void *>(&nfa_chain_lds_kernel<true, 1024>), threads);
     static const int grid_lds_first = persistent_grid(
         reinterpret_cast<const void *>(&nfa_chain_lds_kernel<false, 1024>), threads);
     if (d.pair && pairs) {
         const bool small = d.tiles <= kSmallTiles && d.pair_slot16 != nullptr;
-        if (d.start_always && small)
+        if (paired && d.start_always && small)
+            nfa_chain_pair_kernel<true, true><<<grid_pair, threads, 0, stream>>>(d, data, off, count, sc, chunk);
+        else if (paired && d.start_always)
+            nfa_chain_pair_kernel<true, false><<<grid_pair, threads, 0, stream>>>(d, data, off, count, sc, chunk);
+        else if (paired && small)
+            nfa_chain_pair_kernel<false, true><<<grid_pair, threads, 0, stream>>>(d, data, off, count, sc, chunk);
+        else if (paired)
+            nfa_chain_pair_kernel<false, false><<<grid_pair, threads, 0, stream>>>(d, data, off, count, sc, chunk);
+        else if (d.start_always && small)
             nfa_chain_reg_kernel<true, true><<<grid_reg, threads, 0, stream>>>(d, data, off, count, sc, chunk);
         else if (d.start_always)
             nfa_chain_reg_kernel<true, false><<<grid_reg, threads, 0, stream>>>(d, data, off, count, sc, chunk);
@@ -582,10 +896,10 @@ void launch_nfa_chain(const NfaChainDev &d, const NfaTileDev &tiles, const uint8
             nfa_chain_reg_kernel<false, false><<<grid_reg, threads, 0, stream>>>(d, data, off, count, sc, chunk);
         GFSM_HIP(hipGetLastError());
         nfa_chain_lds_kernel<true, 1024><<<grid_lds_pairs, threads, 0, stream>>>(
-            d, data, off, count, sc, chunk, sc.ovf1, sc.ovf2);
+            d, data, off, count, sc, chunk, true, sc.ovf2);
     } else {
         nfa_chain_lds_kernel<false, 1024><<<grid_lds_first, threads, 0, stream>>>(
-            d, data, off, count, sc, chunk, nullptr, sc.ovf2);
+            d, data, off, count, sc, chunk, false, sc.ovf2);
     }
     GFSM_HIP(hipGetLastError());
     launch_nfa_bitpacked_core(tiles, data, off, count, sc.best, sc.ovf2, stream);

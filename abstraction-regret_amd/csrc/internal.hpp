@@ -72,7 +72,7 @@ struct NfaTileDev {
 // state's successors come either from byte-pair lists of second positions (pair != null) or
 // from per-byte lists of first positions (see layout.hpp).
 struct NfaChainDev {
-    const uint64_t *next;          // [tiles + 1][256][64]: mask & (hassucc << 1)
+    const uint64_t *next;          // [256][tiles + 1][64]: mask & (hassucc << 1), by byte
     const uint64_t *acc;           // [tiles + 1][64] accepting slots (last tile all zero)
     const int32_t *first_off;      // [257]
     const int32_t *first_slot;
@@ -154,9 +154,11 @@ size_t nfa_chain_scratch_bytes(const NfaChainDev &d, int32_t count, int64_t tota
 // chunk_bytes when the caller has no preference: GFSM_CHAIN_CHUNK (read per call), or 4096
 int32_t nfa_chain_chunk_bytes(int32_t requested);
 // pairs: use the byte-pair lists (register tier first) when the handle has them.
+// paired: the register tier runs two items per wave (32 listed states each) instead of one
+// (64 listed states).
 void launch_nfa_chain(const NfaChainDev &d, const NfaTileDev &tiles, const uint8_t *data,
                       const int64_t *off, int32_t count, int64_t total_bytes, int32_t *flags,
                       int32_t *lens, void *scratch, int32_t chunk_bytes, bool pairs,
-                      hipStream_t stream);
+                      bool paired, hipStream_t stream);
 
 }  // namespace gfsm

@@ -305,12 +305,12 @@ TileHost build_tiles(const NfaInput &a) {
         h.bflags[c] = ((h.first_any[c >> 6] >> (c & 63)) & 1ull) |
                       (((h.first_acc[c >> 6] >> (c & 63)) & 1ull) << 1) |
                       (static_cast<uint32_t>(self.has(c)) << 2);
-    h.next.assign((T + 1) * 256 * kTileWords, 0ull);
+    h.next.assign(256 * (T + 1) * kTileWords, 0ull);
     for (size_t b = 0; b < T; ++b) {
         const uint64_t *hs = &h.hassucc[b * kTileWords];
         for (int c = 0; c < 256; ++c) {
             const uint64_t *m = &h.mask[(b * 256 + c) * kTileWords];
-            uint64_t *o = &h.next[(b * 256 + c) * kTileWords];
+            uint64_t *o = &h.next[(static_cast<size_t>(c) * (T + 1) + b) * kTileWords];
             for (int w = 0; w < kTileWords; ++w) {
                 const uint64_t shifted = (hs[w] << 1) | (w > 0 ? hs[w - 1] >> 63 : 0ull);
                 o[w] = m[w] & shifted;

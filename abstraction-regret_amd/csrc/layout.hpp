@@ -39,9 +39,12 @@ struct TileHost {
     std::vector<int32_t> first_off;                   // [257]
     std::vector<int32_t> first_slot;                  // [first_off[256]]
     std::vector<uint32_t> first_accepts;              // [256] some first position accepts
-    // chain continuation by byte, [tiles + 1][256][64]: bit t of next[tile][c] is set when
-    // slot t - 1 has a chain successor and the class of slot t holds c (mask & hassucc << 1).
-    // The extra all-zero tile lets a step read slot last + 1 without a bounds test.
+    // chain continuation by byte, [256][tiles + 1][64] (byte-major: one plane of
+    // (tiles + 1) * 64 words per byte, so slot t's word on byte c is plane c, word t >> 6):
+    // bit t of next[c][tile] is set when slot t - 1 has a chain successor and the class of
+    // slot t holds c (mask & hassucc << 1). The extra all-zero tile lets a step read slot
+    // last + 1 without a bounds test; bit 0 of every tile is zero (chains never cross
+    // tiles), so a lookup of slot 0 never continues anything.
     std::vector<uint64_t> next;
     // the same by bits: the start state has a successor on c / one that accepts
     uint64_t first_any[4] = {0, 0, 0, 0};

@@ -52,6 +52,7 @@ EXPORTS = (
 )
 
 TUNE_CHAIN_FIRST = 1
+TUNE_CHAIN_WAVE64 = 2
 
 
 class HIPUnavailable(RuntimeError):

@@ -13,8 +13,10 @@ so a later batch of the same class reuses the decision without timing anything. 
 persists across processes (``GFSM_TUNE_TABLE`` or ``~/.cache/gpufsm/hip_autotune.json``).
 
 Candidates
-  NFA, chain-structured: ``chain`` with chunks of 2, 4 and 8 KiB, ``chain`` without the
-  byte-pair lists, ``bitpacked`` (dense active sets) and, up to 4096 states, ``worklist``.
+  NFA, chain-structured: ``chain`` with chunks of 2, 4 and 8 KiB, ``chain`` with one item
+  per wave in the register tier (for active sets beyond 32 listed states), ``chain``
+  without the byte-pair lists, ``bitpacked`` (dense active sets) and, up to 4096 states,
+  ``worklist``.
   NFA, general: ``worklist`` only.
   DFA: ``gather``; ``lds`` when the table fits LDS; ``split`` (segment 1, 2 and 4 KiB when
   the table fits LDS, the library's choice otherwise) when inputs average at least 1 KiB.
@@ -51,6 +53,8 @@ class Choice:
             parts.append(f"segment={self.segment_bytes}")
         if self.flags & _lib.TUNE_CHAIN_FIRST:
             parts.append("first")
+        if self.flags & _lib.TUNE_CHAIN_WAVE64:
+            parts.append("wave64")
         return ":".join(parts)
 
     def tuning(self):
@@ -64,6 +68,8 @@ class Choice:
         for part in parts:
             if part == "first":
                 kw["flags"] |= _lib.TUNE_CHAIN_FIRST
+            elif part == "wave64":
+                kw["flags"] |= _lib.TUNE_CHAIN_WAVE64
             elif part.startswith("chunk="):
                 kw["chunk_bytes"] = int(part[6:])
             elif part.startswith("segment="):
@@ -95,7 +101,8 @@ def candidates(kind: str, props, mean_len: float) -> list[Choice]:
         if not props.chain_ok:
             return [Choice("worklist")]
         out = [Choice("chain", chunk_bytes=c) for c in (2048, 4096, 8192)]
-        out += [Choice("chain", chunk_bytes=4096, flags=_lib.TUNE_CHAIN_FIRST),
+        out += [Choice("chain", chunk_bytes=4096, flags=_lib.TUNE_CHAIN_WAVE64),
+                Choice("chain", chunk_bytes=4096, flags=_lib.TUNE_CHAIN_FIRST),
                 Choice("bitpacked")]
         if props.num_states <= 4096:  # the general list walk only competes on small unions
             out.append(Choice("worklist"))

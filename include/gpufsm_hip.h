@@ -68,6 +68,8 @@ typedef struct gfsm_dfa gfsm_dfa;
 /* Tuning of one run (nullable everywhere; a zero field means "the library's choice"). No
  * upstream counterpart: upstream kernels have fixed launch shapes. */
 #define GFSM_TUNE_CHAIN_FIRST 1  /* chain: no byte-pair lists (first-position lists only) */
+#define GFSM_TUNE_CHAIN_WAVE64 2 /* chain: one work item per wave in the register tier (64
+                                    listed states) instead of two (32 each) */
 typedef struct {
     int32_t chunk_bytes;      /* chain: bytes per work item when inputs are cut (>= 64) */
     int32_t segment_bytes;    /* dfa split: segment length, a power of two in [256, 1 MiB] */

@@ -57,6 +57,7 @@ def test_header_constants_match_python():
     assert consts["GFSM_EINVAL"] == _lib.EINVAL
     assert consts["GFSM_EUNSUPPORTED"] == _lib.EUNSUPPORTED
     assert consts["GFSM_TUNE_CHAIN_FIRST"] == _lib.TUNE_CHAIN_FIRST
+    assert consts["GFSM_TUNE_CHAIN_WAVE64"] == _lib.TUNE_CHAIN_WAVE64
 
 
 def test_props_struct_layout(tmp_path):

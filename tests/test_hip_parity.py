@@ -219,7 +219,14 @@ def _class_batch(rng, used, count, max_len, p_class=0.8, empty_frac=0.1):
     return data, off
 
 
-@pytest.mark.parametrize("mode", ["pairs", "first"])
+def _chain_mode(monkeypatch, mode):
+    """pairs: byte-pair lists, two items per wave in the register tier (the default);
+    wave64: byte-pair lists, one item per wave; first: per-byte lists of first positions."""
+    monkeypatch.setenv("GFSM_CHAIN_FIRST", "1" if mode == "first" else "0")
+    monkeypatch.setenv("GFSM_CHAIN_WAVE64", "1" if mode == "wave64" else "0")
+
+
+@pytest.mark.parametrize("mode", ["pairs", "wave64", "first"])
 @pytest.mark.parametrize("self_loop", ["any", "all", "partial", "none"])
 @pytest.mark.parametrize("chunk", ["64", "333", "4096"])
 def test_chain_random_automata(self_loop, chunk, mode, monkeypatch):
@@ -227,7 +234,7 @@ def test_chain_random_automata(self_loop, chunk, mode, monkeypatch):
     kind; chunked (wildcard start) and unchunked inputs, chunk sizes not multiples of 4;
     second positions from byte-pair lists or first positions from per-byte lists."""
     monkeypatch.setenv("GFSM_CHAIN_CHUNK", chunk)
-    monkeypatch.setenv("GFSM_CHAIN_FIRST", "1" if mode == "first" else "0")
+    _chain_mode(monkeypatch, mode)
     rng = random.Random(f"{self_loop}-{chunk}")
     nrng = np.random.default_rng(rng.randrange(1 << 30))
     for trial in range(4):
@@ -238,13 +245,13 @@ def test_chain_random_automata(self_loop, chunk, mode, monkeypatch):
         _check_nfa(nfa, data, off, "chain")
 
 
-@pytest.mark.parametrize("mode", ["pairs", "first"])
+@pytest.mark.parametrize("mode", ["pairs", "wave64", "first"])
 @pytest.mark.parametrize("chunk", ["64", "1000"])
 def test_chain_long_inputs_match_positions(chunk, mode, monkeypatch):
     """Wildcard-start union with a few accepting patterns: first matches fall deep into
     64 KiB inputs, at every phase relative to the chunk boundaries."""
     monkeypatch.setenv("GFSM_CHAIN_CHUNK", chunk)
-    monkeypatch.setenv("GFSM_CHAIN_FIRST", "1" if mode == "first" else "0")
+    _chain_mode(monkeypatch, mode)
     pu = W.pattern_union(400, (3, 8), (4, 32), seed=8, accepting=False)
     last = np.cumsum(pu.lengths)
     pu.nfa.accept[last[:3]] = True
@@ -255,11 +262,11 @@ def test_chain_long_inputs_match_positions(chunk, mode, monkeypatch):
     assert len(set((wn[wf.astype(bool)] % int(chunk)).tolist())) > 5
 
 
-@pytest.mark.parametrize("mode", ["pairs", "first"])
+@pytest.mark.parametrize("mode", ["pairs", "wave64", "first"])
 def test_chain_overflow_falls_back_to_tiles(mode, monkeypatch):
     """Active sets of more than 1024 slots: 40 chains of wildcard-like classes, each 40 long,
     so every input overflows the LDS list and is rerun by the tiled pass."""
-    monkeypatch.setenv("GFSM_CHAIN_FIRST", "1" if mode == "first" else "0")
+    _chain_mode(monkeypatch, mode)
     b = NFABuilder()
     s0 = b.add_state()
     b.set_start(s0)
@@ -276,6 +283,75 @@ def test_chain_overflow_falls_back_to_tiles(mode, monkeypatch):
     data, off = _ragged(nrng, 40, 2000)
     wf, wn = _check_nfa(nfa, data, off, "chain")
     assert wf.any()
+
+
+@pytest.mark.parametrize("mode", ["pairs", "wave64"])
+def test_chain_medium_active_sets(mode, monkeypatch):
+    """About 40 listed states at once: 3 chains of 15 positions whose classes hold every
+    byte but the last position's. Two items per wave overflow their 32 lanes (the LDS tier
+    reruns them), one item per wave holds them."""
+    _chain_mode(monkeypatch, mode)
+    monkeypatch.setenv("GFSM_CHAIN_CHUNK", "512")
+    b = NFABuilder()
+    s0 = b.add_state()
+    b.set_start(s0)
+    b.add_transition(s0, 256, s0)
+    for chain in range(3):
+        prev = s0
+        for pos in range(15):
+            s = b.add_state(accept=(pos == 14))
+            if pos < 14:
+                b.add_transition(prev, 256, s)
+            else:
+                b.add_transition(prev, 7 + chain, s)
+            prev = s
+    nfa = b.build()
+    nrng = np.random.default_rng(14)
+    lens = nrng.integers(0, 3000, size=64)
+    data = np.where(nrng.random(int(lens.sum())) < 0.998, 0x41, 7).astype(np.uint8)
+    off = np.zeros(65, dtype=np.int64)
+    np.cumsum(lens, out=off[1:])
+    wf, wn = _check_nfa(nfa, data, off, "chain")
+    assert wf.any() and not wf.all()
+
+
+@pytest.mark.parametrize("mode", ["pairs", "wave64"])
+def test_chain_additions_into_lanes_freed_on_the_same_byte(mode, monkeypatch):
+    """After A B...B, 32 states are listed (32 chains A B B ... B); on byte X all of them die
+    while 20 second positions are added (chains B X Y). Two items per wave have no lane
+    free before X, so every addition must go to a lane freed by the deaths on X, or the
+    match on the following Y is lost."""
+    _chain_mode(monkeypatch, mode)
+    A, B, X, Y = 0x41, 0x42, 0x58, 0x59
+    b = NFABuilder()
+    s0 = b.add_state()
+    b.set_start(s0)
+    b.add_transition(s0, 256, s0)
+    for _ in range(32):
+        prev = s0
+        for pos in range(30):
+            s = b.add_state()
+            b.add_transition(prev, A if pos == 0 else B, s)
+            prev = s
+    for k in range(20):
+        s1, s2, s3 = b.add_state(), b.add_state(), b.add_state(accept=k in (0, 19))
+        b.add_transition(s0, B, s1)
+        b.add_transition(s1, X, s2)
+        b.add_transition(s2, Y, s3)
+    nfa = b.build()
+    rng = np.random.default_rng(15)
+    inputs = []
+    for _ in range(96):
+        parts = []
+        for _ in range(rng.integers(1, 6)):
+            # noise from bytes no class holds: every segment starts from an empty list
+            parts.append(bytes(rng.integers(0, 64, size=rng.integers(1, 40)).astype(np.uint8)))
+            parts.append(bytes([A]) + bytes([B]) * int(rng.integers(1, 26)) +
+                         (bytes([X, Y]) if rng.random() < 0.3 else bytes([X])))
+        inputs.append(b"".join(parts))
+    data, off = pack_bytes(inputs)
+    wf, wn = _check_nfa(nfa, data, off, "chain")
+    assert wf.any() and not wf.all()
 
 
 def test_chain_run_batch_api():
