@@ -92,8 +92,18 @@ __global__ void chain_zero_kernel(ChainScratch sc) {
     }
 }
 
-__global__ void chain_prep_kernel(const int64_t *__restrict__ off, int32_t count, int64_t chunk,
-                                  ChainScratch sc) {
+// Chunk counts below kBins are counted per block in LDS first: most inputs have one or two
+// chunks, and one global atomic per input on those few counters serializes the kernel.
+constexpr int kBins = 256;
+
+__global__ __launch_bounds__(256) void chain_prep_kernel(const int64_t *__restrict__ off,
+                                                         int32_t count, int64_t chunk,
+                                                         ChainScratch sc) {
+    __shared__ int32_t h[kBins];
+    __shared__ int32_t bmax;
+    for (int k = threadIdx.x; k < kBins; k += blockDim.x) h[k] = 0;
+    if (threadIdx.x == 0) bmax = 0;
+    __syncthreads();
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) {
         sc.best[i] = INT_MAX;
@@ -102,10 +112,15 @@ __global__ void chain_prep_kernel(const int64_t *__restrict__ off, int32_t count
         const int32_t c = static_cast<int32_t>((len + chunk - 1) / chunk);
         sc.nch[i] = c;
         if (c > 0) {
-            atomicAdd(&sc.hist[c], 1);
-            atomicMax(sc.maxk, c);
+            if (c < kBins) atomicAdd(&h[c], 1);
+            else atomicAdd(&sc.hist[c], 1);
+            atomicMax(&bmax, c);
         }
     }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kBins; k += blockDim.x)
+        if (h[k]) atomicAdd(&sc.hist[k], h[k]);
+    if (threadIdx.x == 0 && bmax > 0) atomicMax(sc.maxk, bmax);
 }
 
 // One block: R_c = sum of hist over (c, maxk] (a suffix sum), then rbase = exclusive prefix
@@ -153,11 +168,23 @@ __global__ __launch_bounds__(1024) void chain_rounds_kernel(ChainScratch sc) {
     }
 }
 
-__global__ void chain_scatter_kernel(ChainScratch sc, int32_t count) {
+// perm[R_c + j] = the j-th input with exactly c chunks (any order within c). Ranks within
+// a block come from LDS counters; one global atomic per (block, chunk count) reserves them.
+__global__ __launch_bounds__(256) void chain_scatter_kernel(ChainScratch sc, int32_t count) {
+    __shared__ int32_t cnt[kBins], base[kBins];
+    for (int k = threadIdx.x; k < kBins; k += blockDim.x) cnt[k] = 0;
+    __syncthreads();
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) {
-        const int32_t c = sc.nch[i];
-        if (c > 0) sc.perm[sc.rcount[c] + atomicAdd(&sc.cursor[c], 1)] = i;
+    const int32_t c = i < count ? sc.nch[i] : 0;
+    int32_t local = 0;
+    if (c > 0 && c < kBins) local = atomicAdd(&cnt[c], 1);
+    __syncthreads();
+    for (int k = threadIdx.x; k < kBins; k += blockDim.x)
+        if (cnt[k]) base[k] = atomicAdd(&sc.cursor[k], cnt[k]);
+    __syncthreads();
+    if (c > 0) {
+        const int32_t pos = c < kBins ? base[c] + local : atomicAdd(&sc.cursor[c], 1);
+        sc.perm[sc.rcount[c] + pos] = i;
     }
 }
 
