@@ -816,13 +816,6 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_lds_kernel(
     }
 }
 
-// Blocks of a persistent kernel: as many as are resident at once on every CU.
-int persistent_grid(const void *kernel, int threads) {
-    int per_cu = 0;
-    GFSM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0));
-    return device_cu_count() * (per_cu > 0 ? per_cu : 1);
-}
-
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 }  // namespace
@@ -895,13 +888,14 @@ void launch_nfa_chain(const NfaChainDev &d, const NfaTileDev &tiles, const uint8
     chain_scatter_kernel<<<(count + 255) / 256, 256, 0, stream>>>(sc, count);
     GFSM_HIP(hipGetLastError());
     const int threads = 64 * kChainWaves;
-    static const int grid_reg = persistent_grid(
+    // persistent grids: as many blocks as stay resident at once on every CU
+    const int grid_reg = persistent_blocks(
         reinterpret_cast<const void *>(&nfa_chain_reg_kernel<true, true>), threads);
-    static const int grid_pair = persistent_grid(
+    const int grid_pair = persistent_blocks(
         reinterpret_cast<const void *>(&nfa_chain_pair_kernel<true, true>), threads);
-    static const int grid_lds_pairs = persistent_grid(
+    const int grid_lds_pairs = persistent_blocks(
         reinterpret_cast<const void *>(&nfa_chain_lds_kernel<true, 1024>), threads);
-    static const int grid_lds_first = persistent_grid(
+    const int grid_lds_first = persistent_blocks(
         reinterpret_cast<const void *>(&nfa_chain_lds_kernel<false, 1024>), threads);
     if (d.pair && pairs) {
         const bool small = d.tiles <= kSmallTiles && d.pair_slot16 != nullptr;

@@ -21,6 +21,9 @@
 
 #include <climits>
 #include <cstdlib>
+#include <map>
+#include <mutex>
+#include <utility>
 
 namespace gfsm {
 
@@ -748,17 +751,58 @@ __global__ __launch_bounds__(256) void dfa_split_finish_kernel(const T *__restri
 
 static size_t align256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
 
-int32_t device_cu_count() {
-    static int32_t cached[64] = {0};
+// Launch-shape facts cached per (device, kernel). Several host threads may launch at once
+// (the serving loop does), so the caches are guarded by one mutex; the values they hold
+// are the device's, so each device gets its own.
+namespace {
+std::mutex g_shape_mu;
+std::map<std::pair<int, const void *>, size_t> g_lds_limit;    // raised dynamic-LDS limit
+std::map<std::pair<int, const void *>, int> g_blocks_per_cu;  // occupancy at 256 threads
+std::map<int, int32_t> g_cus;
+
+int current_device() {
     int dev = 0;
     GFSM_HIP(hipGetDevice(&dev));
-    if (dev < 0 || dev >= 64) dev = 0;
-    if (!cached[dev]) {
-        int cus = 0;
-        GFSM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        cached[dev] = cus > 0 ? cus : 1;
+    return dev;
+}
+}  // namespace
+
+int32_t device_cu_count() {
+    const int dev = current_device();
+    std::lock_guard<std::mutex> lk(g_shape_mu);
+    auto it = g_cus.find(dev);
+    if (it != g_cus.end()) return it->second;
+    int cus = 0;
+    GFSM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    return g_cus[dev] = cus > 0 ? cus : 1;
+}
+
+void ensure_dynamic_lds(const void *kernel, size_t bytes) {
+    if (bytes <= 64 * 1024) return;  // the default limit
+    const int dev = current_device();
+    std::lock_guard<std::mutex> lk(g_shape_mu);
+    size_t &have = g_lds_limit[{dev, kernel}];
+    if (have >= bytes) return;
+    GFSM_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 static_cast<int>(bytes)));
+    have = bytes;
+}
+
+int persistent_blocks(const void *kernel, int threads) {
+    const int32_t cus = device_cu_count();
+    const int dev = current_device();
+    std::lock_guard<std::mutex> lk(g_shape_mu);
+    auto it = g_blocks_per_cu.find({dev, kernel});
+    int per_cu;
+    if (it != g_blocks_per_cu.end()) {
+        per_cu = it->second;
+    } else {
+        per_cu = 0;
+        GFSM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0));
+        if (per_cu < 1) per_cu = 1;
+        g_blocks_per_cu[{dev, kernel}] = per_cu;
     }
-    return cached[dev];
+    return cus * per_cu;
 }
 
 size_t dfa_lds_bytes(int32_t n) { return static_cast<size_t>(n) * kAlphabet * sizeof(uint16_t); }
@@ -808,10 +852,7 @@ DfaSplitPlan dfa_split_plan(const DfaDev &d, int64_t total_bytes, int32_t count,
 // the launch uses, not the CU maximum: the kernel's own static LDS counts against the CU.
 template <typename K>
 static void allow_big_lds(K *kernel, size_t bytes) {
-    if (bytes <= 64 * 1024) return;
-    GFSM_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 static_cast<int>(bytes)));
+    ensure_dynamic_lds(reinterpret_cast<const void *>(kernel), bytes);
 }
 
 template <typename T, bool kLds, int kChains>
@@ -819,12 +860,10 @@ static void split_launch_passes(const DfaDev &d, SplitArgs &a, hipStream_t strea
     const T *tab = static_cast<const T *>(d.table);
     const size_t lds = kLds ? static_cast<size_t>(a.table_bytes) : 0;
     const int32_t cus = device_cu_count();
-    static size_t attr = 0;  // largest table the attributes were raised for
-    if (kLds && lds > attr) {
+    if (kLds) {
         allow_big_lds(&dfa_split_walk_kernel<T, kLds, kChains>, lds);
         allow_big_lds(&dfa_split_fix_kernel<T, kLds, kChains>, lds);
         allow_big_lds(&dfa_split_finish_kernel<T, kLds>, lds);
-        attr = lds;
     }
     // spread the segments over every CU, kChains per lane
     struct Shape {
@@ -954,11 +993,7 @@ void launch_dfa_lds(const DfaDev &d, const uint8_t *data, const int64_t *off, in
     if (count <= 0) return;
     if (!lds_fits(d)) throw Error(3, "lds: the table does not fit in LDS");
     const uint32_t tb = d.col_bytes * kAlphabet;
-    static size_t attr = 0;
-    if (tb > attr) {
-        allow_big_lds(&dfa_walk_kernel<uint16_t, true>, tb);
-        attr = tb;
-    }
+    allow_big_lds(&dfa_walk_kernel<uint16_t, true>, tb);
     int grid = (count + 255) / 256;
     const int cap = 2 * device_cu_count();
     if (grid > cap) grid = cap;

@@ -125,7 +125,14 @@ DfaSplitPlan dfa_split_plan(const DfaDev &d, int64_t total_bytes, int32_t count,
 void launch_dfa_split(const DfaDev &d, const DfaSplitPlan &plan, const uint8_t *data,
                       const int64_t *off, int32_t count, int64_t total_bytes, int32_t *flags,
                       int32_t *lens, void *scratch, hipStream_t stream);
+// Compute units of the current device.
 int32_t device_cu_count();
+// Raises a kernel's dynamic-LDS limit on the current device to at least `bytes` (a no-op at
+// or below the default 64 KiB). Thread-safe; the attribute is set once per device and size.
+void ensure_dynamic_lds(const void *kernel, size_t bytes);
+// Blocks a persistent launch of `kernel` with `threads` per block keeps resident on every
+// CU of the current device (occupancy x CUs), cached per device and kernel.
+int persistent_blocks(const void *kernel, int threads);
 // Positive integer from the environment (tuning overrides), else fallback.
 int64_t env_int(const char *name, int64_t fallback);
 

@@ -172,11 +172,7 @@ void launch_nfa_worklist(const NfaWorklistDev &d, const uint8_t *data, const int
     if (count <= 0) return;
     const bool lds_lists = scratch == nullptr;
     const size_t bytes = wl_lds_bytes(d, lds_lists);
-    if (bytes > 64 * 1024) {
-        GFSM_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&nfa_worklist_kernel),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     static_cast<int>(bytes)));
-    }
+    ensure_dynamic_lds(reinterpret_cast<const void *>(&nfa_worklist_kernel), bytes);
     nfa_worklist_kernel<<<grid, 64, bytes, stream>>>(d, data, off, count, flags, lens, scratch,
                                                      lds_lists ? 1 : 0);
     GFSM_HIP(hipGetLastError());
@@ -306,14 +302,8 @@ __global__ __launch_bounds__(kBpThreads) void nfa_bitpacked_kernel(
 void launch_nfa_bitpacked_core(const NfaTileDev &d, const uint8_t *data, const int64_t *off,
                            int32_t count, int32_t *best, const int32_t *only,
                            hipStream_t stream) {
-    static bool attr_set = false;
     const size_t bytes = kBpLdsWords * sizeof(uint64_t);
-    if (!attr_set) {
-        GFSM_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&nfa_bitpacked_kernel),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     static_cast<int>(bytes)));
-        attr_set = true;
-    }
+    ensure_dynamic_lds(reinterpret_cast<const void *>(&nfa_bitpacked_kernel), bytes);
     const int32_t waves_per_block = kBpThreads / 64;
     int32_t G = (count + waves_per_block - 1) / waves_per_block;
     const int32_t target = (512 + d.tiles - 1) / d.tiles;  // about two blocks per CU

@@ -354,6 +354,24 @@ def test_chain_additions_into_lanes_freed_on_the_same_byte(mode, monkeypatch):
     assert wf.any() and not wf.all()
 
 
+@pytest.mark.parametrize("mode", ["pairs", "wave64", "first"])
+def test_chain_many_tiles(mode, monkeypatch):
+    """90,206 states in 23 tiles: above 65,536 slots the lists hold 32-bit slots and the
+    accept words are read from global memory instead of LDS. Accepting third positions of
+    50 patterns make about half the inputs match."""
+    _chain_mode(monkeypatch, mode)
+    pu = W.pattern_union(4000, (5, 40), (1, 8), seed=17)
+    first = 1 + np.concatenate(([0], np.cumsum(pu.lengths)[:-1]))
+    pu.nfa.accept[:] = False
+    pu.nfa.accept[first[:50] + 2] = True
+    lens = np.random.default_rng(17).integers(0, 6000, size=96)
+    data, off = W.pattern_inputs(pu, lens, 0.8, seed=18)
+    with DeviceNFA(pu.nfa) as dev:
+        assert dev.props.num_tiles > 16
+    wf, wn = _check_nfa(pu.nfa, data, off, "chain")
+    assert wf.any() and not wf.all()
+
+
 def test_chain_run_batch_api():
     pu = W.pattern_union(120, seed=13)
     lens = np.random.default_rng(13).integers(0, 9000, size=60)
