@@ -178,6 +178,14 @@ struct SplitArgs {
     // [segments] when nck <= 64: bit k set when interval k of the head entered an
     // accepting state on its speculative path (the flag bits of the checkpoints, gathered)
     unsigned long long *flagmask;
+    // Heads resolved by their full transition map (map_idx null when not in use):
+    int32_t *map_idx;          // [segments] the head's map, -1 = none
+    int32_t *map_list;         // [map_cap] heads to map, in flag order
+    unsigned int *map_count;   // heads flagged (may exceed map_cap)
+    uint16_t *map_exit;        // [map_cap][parts][n] exit entry of each part, every entry
+    int32_t *map_acc;          // [map_cap][parts][n] first accept in the part (offset in the
+                               // head), -1 = none
+    int32_t map_cap, n;        // n = states
     void *ckpt;  // T [segments * nck]
 };
 
@@ -194,8 +202,10 @@ __device__ __forceinline__ int32_t split_find_input(const int64_t *__restrict__ 
 }
 
 __global__ void split_init_kernel(int32_t *flags, int32_t *lens, int32_t *done,
-                                  unsigned long long *first_event, int32_t count) {
+                                  unsigned long long *first_event, int32_t count,
+                                  unsigned int *map_count) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0 && map_count) *map_count = 0u;
     if (i < count) {
         flags[i] = 0;
         lens[i] = 0;
@@ -216,11 +226,11 @@ __global__ void split_init_kernel(int32_t *flags, int32_t *lens, int32_t *done,
 constexpr int kRound = 64;               // bytes per chain per round
 constexpr int kVec = kRound / 16;        // 16-byte vectors per round
 
-template <typename T, int K>
+template <typename T, int K, int R = kRound>
 __device__ __forceinline__ void step_round(const char *tab, uint32_t col, uint32_t (&e)[K],
-                                           uint32_t (&mx)[K], const uint4 (&v)[K][kVec]) {
+                                           uint32_t (&mx)[K], const uint4 (&v)[K][R / 16]) {
 #pragma unroll
-    for (int k = 0; k < kRound; ++k) {
+    for (int k = 0; k < R; ++k) {
 #pragma unroll
         for (int c = 0; c < K; ++c) {
             const uint4 &x = v[c][k >> 4];
@@ -244,15 +254,15 @@ __device__ __forceinline__ int32_t find16(const char *tab, uint32_t col, uint32_
     return -1;
 }
 
-// Index (0..kRound-1) of the first accepting step of a round replayed from e.
-template <typename T>
+// Index (0..R-1) of the first accepting step of a round of R bytes replayed from e.
+template <typename T, int R = kRound>
 __device__ __forceinline__ int32_t find_round(const char *tab, uint32_t col, uint32_t e,
-                                              const uint4 (&v)[kVec], uint32_t acc) {
-    for (int j = 0; j < kVec; ++j) {
+                                              const uint4 (&v)[R / 16], uint32_t acc) {
+    for (int j = 0; j < R / 16; ++j) {
         const int32_t k = find16<T>(tab, col, e, v[j], acc);
         if (k >= 0) return 16 * j + k;
     }
-    return kRound - 1;  // unreachable when the round's maximum was >= acc
+    return R - 1;  // unreachable when the round's maximum was >= acc
 }
 
 // Checkpoints are stored interval-major ([interval][segment]) so that the lanes of a wave,
@@ -270,7 +280,10 @@ struct CkView {
 // kWalkThreads * K chains in flight per CU; each step is one dependent LDS read, so the
 // number of chains in flight sets the walk's rate.
 constexpr int kWalkThreads = 1024;
-constexpr int kFixThreads = 512;
+constexpr int kFixThreads = 1024;
+// The fix pass walks in rounds of R bytes: 64 up to 512 threads per CU, 32 above (hoisting
+// the byte offsets of a 64-byte round takes more VGPRs than 1024 threads leave).
+constexpr int fix_threads_for(int round) { return round >= 64 ? 512 : 1024; }
 
 template <typename T, bool kLds, int K>
 __global__ __launch_bounds__(kWalkThreads / K) void dfa_split_walk_kernel(
@@ -544,8 +557,8 @@ __device__ __forceinline__ int32_t split_accept_after(const char *tab, const Spl
 // entry differs from the previous segment's pass-2 exit, now walked from that exit. A head
 // being repaired can race with the repair of its predecessor; the event pass re-checks every
 // head afterwards, so the race only affects speed.
-template <typename T, bool kLds, int K>
-__global__ __launch_bounds__(kFixThreads / K) void dfa_split_fix_kernel(
+template <typename T, bool kLds, int K, int kFixRound>
+__global__ __launch_bounds__(fix_threads_for(kFixRound) / K) void dfa_split_fix_kernel(
     const T *__restrict__ gtab, SplitArgs a, int32_t repair) {
     extern __shared__ uint4 lds_raw[];
     const char *tab = stage_table<T, kLds>(gtab, a.table_bytes, lds_raw);
@@ -559,7 +572,8 @@ __global__ __launch_bounds__(kFixThreads / K) void dfa_split_fix_kernel(
     int64_t g[K], q[K], hend[K];
     uint32_t e[K], entry[K];
     bool act[K], have[K];
-    uint4 nv[K][kVec];
+    constexpr int kFixVec = kFixRound / 16;
+    uint4 nv[K][kFixVec];
 
     auto pick = [&](int c) {
         act[c] = false;
@@ -607,12 +621,12 @@ __global__ __launch_bounds__(kFixThreads / K) void dfa_split_fix_kernel(
         for (int c = 0; c < K; ++c) {
             if (act[c]) {
                 any = true;
-                if ((q[c] & (kRound - 1)) || q[c] + kRound > hend[c]) fast = false;
+                if ((q[c] & (kFixRound - 1)) || q[c] + kFixRound > hend[c]) fast = false;
             }
         }
         if (!any) break;
         if (fast) {
-            uint4 v[K][kVec];
+            uint4 v[K][kFixVec];
             uint32_t e0[K], mx[K], ckn[K];
 #pragma unroll
             for (int c = 0; c < K; ++c) {
@@ -621,24 +635,24 @@ __global__ __launch_bounds__(kFixThreads / K) void dfa_split_fix_kernel(
                 ckn[c] = ~0u;  // no checkpoint at the end of this round
                 if (act[c]) {
                     // the checkpoint the round ends on, loaded while the round is walked
-                    const int64_t r_end = q[c] + kRound - g[c] * a.seg;
+                    const int64_t r_end = q[c] + kFixRound - g[c] * a.seg;
                     if ((r_end & rmask) == 0)
                         ckn[c] = static_cast<uint32_t>(
                                      ckpt[((r_end >> a.rshift) - 1) * a.segments + g[c]]) & ~1u;
 #pragma unroll
-                    for (int j = 0; j < kVec; ++j)
+                    for (int j = 0; j < kFixVec; ++j)
                         v[c][j] = have[c] ? nv[c][j] : ld16(a.data + q[c] + 16 * j);
-                    have[c] = q[c] + 2 * kRound <= hend[c];
+                    have[c] = q[c] + 2 * kFixRound <= hend[c];
                     if (have[c]) {
 #pragma unroll
-                        for (int j = 0; j < kVec; ++j) nv[c][j] = ld16(a.data + q[c] + kRound + 16 * j);
+                        for (int j = 0; j < kFixVec; ++j) nv[c][j] = ld16(a.data + q[c] + kFixRound + 16 * j);
                     }
                 } else {
 #pragma unroll
-                    for (int j = 0; j < kVec; ++j) v[c][j] = zero;
+                    for (int j = 0; j < kFixVec; ++j) v[c][j] = zero;
                 }
             }
-            step_round<T, K>(tab, col, e, mx, v);
+            step_round<T, K, kFixRound>(tab, col, e, mx, v);
 #pragma unroll
             for (int c = 0; c < K; ++c) {
                 if (!act[c]) {
@@ -646,10 +660,10 @@ __global__ __launch_bounds__(kFixThreads / K) void dfa_split_fix_kernel(
                     continue;
                 }
                 const int64_t s0 = g[c] * a.seg;
-                q[c] += kRound;
+                q[c] += kFixRound;
                 if (mx[c] >= acc) {
-                    const int32_t k = find_round<T>(tab, col, e0[c], v[c], acc);
-                    finish(c, static_cast<int32_t>(q[c] - kRound + k - s0), e[c]);
+                    const int32_t k = find_round<T, kFixRound>(tab, col, e0[c], v[c], acc);
+                    finish(c, static_cast<int32_t>(q[c] - kFixRound + k - s0), e[c]);
                     continue;
                 }
                 const int64_t r = q[c] - s0;
@@ -665,7 +679,7 @@ __global__ __launch_bounds__(kFixThreads / K) void dfa_split_fix_kernel(
         } else {
 #pragma unroll
             for (int c = 0; c < K; ++c) {
-                if (!act[c] || !((q[c] & (kRound - 1)) || q[c] + kRound > hend[c])) continue;
+                if (!act[c] || !((q[c] & (kFixRound - 1)) || q[c] + kFixRound > hend[c])) continue;
                 // the last partial round of a head: byte by byte to its end
                 const int64_t s0 = g[c] * a.seg;
                 int32_t hit = -1;
@@ -683,6 +697,105 @@ __global__ __launch_bounds__(kFixThreads / K) void dfa_split_fix_kernel(
     }
 }
 
+// ---- full transition maps of unresolved heads ---------------------------------------------
+//
+// A head whose assumed entry was wrong, and whose repair would be a single-lane walk of up to
+// a whole segment (the tail of the split walk), is resolved in parallel instead: its
+// transition map for every entry state. The head is cut into 32 intervals, handled by 4
+// blocks of 8 intervals each (one CU per block, so one head's lookups spread over 4 CUs).
+// Each (interval, entry) pair is walked by one lane, two entries per lane walked together,
+// recording the first accepting step; then one lane per entry composes the block's 8
+// interval maps into a part map. The finish pass chains the 4 part maps of the head for its
+// one true entry. That is n times the lookups of one walk, spread over 4096 lanes with at
+// most interval-length dependent steps, which beats the single walk when few heads need it.
+constexpr int kMapIntervals = 32;
+constexpr int kMapParts = 4;
+constexpr int kMapPartIntervals = kMapIntervals / kMapParts;  // 8
+constexpr int kMapThreads = 1024;                              // 8 intervals x 128 lanes
+constexpr uint16_t kNoHit = 0xffffu;
+
+__global__ __launch_bounds__(kMapThreads) void dfa_split_map_kernel(const uint16_t *__restrict__ gtab,
+                                                                    SplitArgs a) {
+    extern __shared__ uint4 lds_raw[];
+    const char *tab = stage_table<uint16_t, true>(gtab, a.table_bytes, lds_raw);
+    const int n = a.n;  // at most 256: two entries per lane
+    uint16_t *imap = reinterpret_cast<uint16_t *>(reinterpret_cast<char *>(lds_raw) + a.table_bytes);
+    uint16_t *ihit = imap + kMapPartIntervals * n;
+    const uint32_t col = a.col, acc = a.acc;
+    const unsigned int flagged = *a.map_count;
+    const unsigned int cnt = flagged < static_cast<unsigned int>(a.map_cap)
+                                 ? flagged : static_cast<unsigned int>(a.map_cap);
+    const int64_t I = a.seg / kMapIntervals;  // a multiple of 16
+    const int kk = threadIdx.x >> 7, sub = threadIdx.x & 127;
+    for (unsigned int w = blockIdx.x; w < cnt * kMapParts; w += gridDim.x) {
+        const unsigned int m = w / kMapParts, part = w % kMapParts;
+        const int64_t g = a.map_list[m];
+        const int64_t s0 = g * a.seg;
+        const int32_t in = split_find_input(a.off, a.count, s0);
+        int64_t hend = s0 + a.seg < a.total ? s0 + a.seg : a.total;
+        if (a.off[in + 1] < hend) hend = a.off[in + 1];
+        const int k = static_cast<int>(part) * kMapPartIntervals + kk;  // interval in the head
+        const int64_t lo = s0 + k * I;
+        const int64_t hi = (lo + I) < hend ? (lo + I) : hend;
+        uint32_t e[2], hit[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int x = sub + 128 * j;  // entries past n walk state 0 and are not stored
+            e[j] = static_cast<uint32_t>(x < n ? x : 0) * 2u;
+            hit[j] = kNoHit;
+        }
+        int64_t q = lo;
+        for (; q + 16 <= hi; q += 16) {
+            const uint4 v = ld16(a.data + q);
+            const uint32_t wv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int b = 0; b < 16; ++b) {
+                const uint32_t c = (wv[b >> 2] >> ((b & 3) * 8)) & 0xffu;
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    e[j] = dstep<uint16_t>(tab, col, e[j], c);
+                    if (hit[j] == kNoHit && e[j] >= acc) hit[j] = static_cast<uint32_t>(q - lo + b);
+                }
+            }
+        }
+        for (; q < hi; ++q) {
+            const uint32_t c = a.data[q];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                e[j] = dstep<uint16_t>(tab, col, e[j], c);
+                if (hit[j] == kNoHit && e[j] >= acc) hit[j] = static_cast<uint32_t>(q - lo);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int x = sub + 128 * j;
+            if (x < n) {
+                imap[kk * n + x] = static_cast<uint16_t>(e[j]);
+                ihit[kk * n + x] = static_cast<uint16_t>(hit[j]);
+            }
+        }
+        __syncthreads();
+        // compose the block's intervals: the part map, first accept as an offset in the head
+        for (int x = threadIdx.x; x < n; x += blockDim.x) {
+            uint32_t st = static_cast<uint32_t>(x), ev = static_cast<uint32_t>(x) * 2u;
+            int32_t first = -1;
+            for (int j = 0; j < kMapPartIntervals; ++j) {
+                if (first < 0) {
+                    const uint32_t h = ihit[j * n + st];
+                    if (h != kNoHit)
+                        first = static_cast<int32_t>((part * kMapPartIntervals + j) * I + h);
+                }
+                ev = imap[j * n + st];
+                st = ev >> 1;
+            }
+            const int64_t at = static_cast<int64_t>(w) * n + x;  // w = m * kMapParts + part
+            a.map_exit[at] = static_cast<uint16_t>(ev);
+            a.map_acc[at] = first;
+        }
+        __syncthreads();
+    }
+}
+
 __global__ __launch_bounds__(256) void dfa_split_event_kernel(SplitArgs a) {
     const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
     for (int64_t g = 1 + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -693,9 +806,23 @@ __global__ __launch_bounds__(256) void dfa_split_event_kernel(SplitArgs a) {
         const uint32_t prev =
             a.off[i] >= (g - 1) * a.seg ? a.tail_exit[g - 1] : a.fixed_exit[g - 1];
         unsigned long long key = kNoEvent;
-        if (a.fixed_entry[g] != prev) key = 2ull * static_cast<unsigned long long>(g);
+        const bool wrong = a.fixed_entry[g] != prev;
+        if (wrong) key = 2ull * static_cast<unsigned long long>(g);
         else if (a.fixed_acc[g] >= 0) key = 2ull * static_cast<unsigned long long>(g) + 1ull;
         if (key != kNoEvent) atomicMin(&a.first_event[i], key);
+        if (a.map_idx) {
+            // a head whose assumed entry differs from its predecessor's exit gets its full
+            // transition map, so that the finish pass resolves it with one lookup
+            int32_t m = -1;
+            if (wrong) {
+                const unsigned int slot = atomicAdd(a.map_count, 1u);
+                if (slot < static_cast<unsigned int>(a.map_cap)) {
+                    m = static_cast<int32_t>(slot);
+                    a.map_list[slot] = static_cast<int32_t>(g);
+                }
+            }
+            a.map_idx[g] = m;
+        }
     }
 }
 
@@ -733,6 +860,18 @@ __global__ __launch_bounds__(256) void dfa_split_finish_kernel(const T *__restri
         if (a.fixed_entry[g] == cur) {
             acc_at = a.fixed_acc[g];
             ex = a.fixed_exit[g];
+        } else if (a.map_idx && a.map_idx[g] >= 0) {
+            // chain the head's part maps for this entry
+            const int64_t base = static_cast<int64_t>(a.map_idx[g]) * kMapParts;
+            uint32_t st = cur / sizeof(T);
+            acc_at = -1;
+            ex = cur;
+            for (int p = 0; p < kMapParts; ++p) {
+                const int64_t at = (base + p) * a.n + st;
+                if (acc_at < 0) acc_at = a.map_acc[at];
+                ex = a.map_exit[at];
+                st = ex / sizeof(T);
+            }
         } else {
             const int64_t hend = (s0 + a.seg) < ie ? (s0 + a.seg) : ie;
             split_merge_walk<T>(tab, a, g, s0, hend, cur, CkView<T>{ckpt + g, a.segments},
@@ -839,12 +978,21 @@ DfaSplitPlan dfa_split_plan(const DfaDev &d, int64_t total_bytes, int32_t count,
         while ((1ll << p.ckpt_shift) * 16 < c) ++p.ckpt_shift;
     }
     p.segments = (total_bytes + p.seg_bytes - 1) / p.seg_bytes;
+    // maps need the table and 32 interval maps in LDS (n <= 256) and 16-byte intervals
+    p.maps = lds_fits(d) && d.n <= 256 && p.seg_bytes >= 512 && p.chains == 1;
+    p.map_cap = p.maps ? 1024 : 0;
     const size_t S = static_cast<size_t>(p.segments > 0 ? p.segments : 1);
     const size_t nck = static_cast<size_t>(p.seg_bytes >> p.ckpt_shift);
     const size_t ent = d.wide ? 4 : 2;
+    const size_t cap = static_cast<size_t>(p.map_cap);
+    const size_t nn = static_cast<size_t>(d.n);
     p.scratch_bytes = align256(static_cast<size_t>(count) * 4) +
                       align256(static_cast<size_t>(count) * 8) + 5 * align256(S * 4) +
-                      align256(S * 8) + align256(S * nck * ent);
+                      align256(S * 8) + align256(S * nck * ent) +
+                      (p.maps ? align256(S * 4) + align256(cap * 4) + 256 +
+                                    align256(cap * kMapParts * nn * 2) +
+                                    align256(cap * kMapParts * nn * 4)
+                              : 0);
     return p;
 }
 
@@ -862,7 +1010,8 @@ static void split_launch_passes(const DfaDev &d, SplitArgs &a, hipStream_t strea
     const int32_t cus = device_cu_count();
     if (kLds) {
         allow_big_lds(&dfa_split_walk_kernel<T, kLds, kChains>, lds);
-        allow_big_lds(&dfa_split_fix_kernel<T, kLds, kChains>, lds);
+        allow_big_lds(&dfa_split_fix_kernel<T, kLds, kChains, 64>, lds);
+        allow_big_lds(&dfa_split_fix_kernel<T, kLds, kChains, 32>, lds);
         allow_big_lds(&dfa_split_finish_kernel<T, kLds>, lds);
     }
     // spread the segments over every CU, kChains per lane
@@ -893,14 +1042,36 @@ static void split_launch_passes(const DfaDev &d, SplitArgs &a, hipStream_t strea
     dfa_split_walk_kernel<T, kLds, kChains><<<walk.blocks, walk.threads, lds, stream>>>(tab, a);
     GFSM_HIP(hipGetLastError());
     if (a.segments > 1) {
-        dfa_split_fix_kernel<T, kLds, kChains><<<fix.blocks, fix.threads, lds, stream>>>(tab, a, 0);
-        GFSM_HIP(hipGetLastError());
-        dfa_split_fix_kernel<T, kLds, kChains><<<fix.blocks, fix.threads, lds, stream>>>(tab, a, 1);
-        GFSM_HIP(hipGetLastError());
+        // 64-byte rounds when the fix blocks fit their VGPR budget, else 32-byte rounds
+        auto launch_fix = [&](int32_t repair) {
+            if (fix.threads <= fix_threads_for(64) / kChains)
+                dfa_split_fix_kernel<T, kLds, kChains, 64><<<fix.blocks, fix.threads, lds, stream>>>(tab, a, repair);
+            else
+                dfa_split_fix_kernel<T, kLds, kChains, 32><<<fix.blocks, fix.threads, lds, stream>>>(tab, a, repair);
+            GFSM_HIP(hipGetLastError());
+        };
+        launch_fix(0);
+        // The repair pass re-walks heads whose assumed entry was wrong, one lane each. With
+        // maps and segments of 2 KiB or more there are few such heads (the speculative walk
+        // of the previous segment failed to merge), and maps resolve them in parallel.
+        if (!a.map_idx || a.seg < 2048) {
+            launch_fix(1);
+        }
         const int64_t eblocks64 = (a.segments + 255) / 256;
         const int32_t eblocks = static_cast<int32_t>(eblocks64 < 65536 ? eblocks64 : 65536);
         dfa_split_event_kernel<<<eblocks, 256, 0, stream>>>(a);
         GFSM_HIP(hipGetLastError());
+        if constexpr (kLds && sizeof(T) == 2) {
+            if (a.map_idx) {
+                const size_t mlds = static_cast<size_t>(a.table_bytes) +
+                                    static_cast<size_t>(kMapPartIntervals) * a.n * 4;
+                ensure_dynamic_lds(reinterpret_cast<const void *>(&dfa_split_map_kernel), mlds);
+                const int32_t mblocks = cus;
+                dfa_split_map_kernel<<<mblocks, kMapThreads, mlds, stream>>>(
+                    reinterpret_cast<const uint16_t *>(d.table), a);
+                GFSM_HIP(hipGetLastError());
+            }
+        }
         dfa_split_finish_kernel<T, kLds><<<(a.count + 255) / 256, 256, lds, stream>>>(tab, a);
         GFSM_HIP(hipGetLastError());
     }
@@ -946,8 +1117,24 @@ void launch_dfa_split(const DfaDev &d, const DfaSplitPlan &plan, const uint8_t *
     a.flagmask = reinterpret_cast<unsigned long long *>(base);
     base += align256(S * 8);
     a.ckpt = base;
+    base += align256(S * static_cast<size_t>(plan.seg_bytes >> plan.ckpt_shift) * (d.wide ? 4 : 2));
+    a.n = d.n;
+    if (plan.maps) {
+        const size_t cap = static_cast<size_t>(plan.map_cap), nn = static_cast<size_t>(d.n);
+        a.map_cap = plan.map_cap;
+        a.map_idx = reinterpret_cast<int32_t *>(base);
+        base += align256(S * 4);
+        a.map_list = reinterpret_cast<int32_t *>(base);
+        base += align256(cap * 4);
+        a.map_count = reinterpret_cast<unsigned int *>(base);
+        base += 256;
+        a.map_exit = reinterpret_cast<uint16_t *>(base);
+        base += align256(cap * kMapParts * nn * 2);
+        a.map_acc = reinterpret_cast<int32_t *>(base);
+    }
     split_init_kernel<<<(count + 255) / 256, 256, 0, stream>>>(flags, lens, a.done,
-                                                                a.first_event, count);
+                                                                a.first_event, count,
+                                                                a.map_count);
     GFSM_HIP(hipGetLastError());
     if (plan.segments <= 0) return;
     const bool lds = lds_fits(d);

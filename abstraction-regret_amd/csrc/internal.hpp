@@ -117,6 +117,8 @@ struct DfaSplitPlan {
     int32_t ckpt_shift = 0;    // log2 of the checkpoint interval R
     int64_t segments = 0;      // ceil(total_bytes / C)
     int32_t chains = 1;        // interleaved chains per lane
+    bool maps = false;         // unresolved heads get full transition maps (LDS tables)
+    int32_t map_cap = 0;       // most heads mapped per launch (the rest are re-walked)
     size_t scratch_bytes = 0;  // device scratch the launch needs
 };
 // seg_bytes: requested segment length (a power of two in [256, 1 MiB]); 0 = default

@@ -108,7 +108,7 @@ def _time_steps(torch, fn, steps: int, warmup: int):
 KERNELS = {
     "gather": "dfa_walk_kernel<gather>",
     "lds": "dfa_walk_kernel<lds>",
-    "split": "dfa_split (init, walk, fix x2, event, finish)",
+    "split": "dfa_split (init, walk, fix, repair below 2 KiB segments, event, map, finish)",
 }
 
 
