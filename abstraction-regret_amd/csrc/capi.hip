@@ -6,6 +6,7 @@
 // status codes plus a thread-local message.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cstring>
 #include <functional>
 #include <string>
@@ -327,8 +328,12 @@ void run_host(const uint8_t *data, const int64_t *offsets, int32_t count, int32_
     if (count == 0) return;
     const int64_t total = offsets[count] - offsets[0];
     if (offsets[0] != 0) throw Error(GFSM_EINVAL, "offsets[0] must be 0");
-    for (int32_t i = 0; i < count; ++i)
+    for (int32_t i = 0; i < count; ++i) {
         if (offsets[i + 1] < offsets[i]) throw Error(GFSM_EINVAL, "offsets are not non-decreasing");
+        if (offsets[i + 1] - offsets[i] > INT32_MAX)
+            throw Error(GFSM_EINVAL, "input " + std::to_string(i) +
+                                         " is longer than 2^31 - 1 bytes (match_len is int32)");
+    }
     if (total > 0 && !data) throw Error(GFSM_EINVAL, "null data");
     hipStream_t s = host_stream();
     const size_t off_bytes = static_cast<size_t>(count + 1) * sizeof(int64_t);

@@ -857,6 +857,9 @@ void launch_nfa_chain(const NfaChainDev &d, const NfaTileDev &tiles, const uint8
                       bool paired, hipStream_t stream) {
     if (count <= 0) return;
     const int64_t chunk = effective_chunk(d, chunk_bytes);
+    // item numbers are 32-bit
+    if (item_capacity(d, count, total_bytes, chunk_bytes) >= (size_t(1) << 31))
+        throw Error(1, "chain: more than 2^31 (input, chunk) items; raise chunk_bytes");
     const size_t c = static_cast<size_t>(count);
     ChainScratch sc;
     sc.K = round_capacity(total_bytes, chunk);

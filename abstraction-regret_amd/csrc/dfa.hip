@@ -1083,6 +1083,8 @@ void launch_dfa_split(const DfaDev &d, const DfaSplitPlan &plan, const uint8_t *
     if (count <= 0) return;
     if (reinterpret_cast<uintptr_t>(data) & 15)
         throw Error(3, "split needs the input buffer 16-byte aligned");
+    if (plan.segments >= (int64_t(1) << 31))  // segment numbers are stored as int32
+        throw Error(1, "split: more than 2^31 segments; raise segment_bytes");
     SplitArgs a{};
     a.data = data;
     a.off = off;

@@ -29,6 +29,9 @@ def _as_batch(data: np.ndarray, offsets: np.ndarray) -> tuple[np.ndarray, np.nda
     count = int(offsets.size) - 1
     if count < 0:
         raise ValueError("offsets must have num_strings + 1 entries")
+    # the library copies data[0:offsets[-1]]: offsets past the buffer would over-read it
+    if count > 0 and not (0 <= int(offsets[-1]) <= data.size):
+        raise ValueError(f"offsets end at {int(offsets[-1])}, past the {data.size}-byte data")
     if data.size == 0:
         data = np.zeros(1, dtype=np.uint8)
     return data, offsets, count

@@ -127,7 +127,8 @@ int gfsm_nfa_analyze(int32_t num_states, int32_t start_state, const uint8_t *acc
 /* NFA batch, host buffers, synchronous. Replaces run_worklist / run_worklist_warp /
  * run_multistream / ... (api.hpp:47-97): same inputs and outputs, bytes instead of int32
  * symbols and int64 offsets[num_strings + 1]. kernel_ms (nullable) is the device time of
- * the launch(es), measured with HIP events as the upstream entry points do. */
+ * the launch(es), measured with HIP events as the upstream entry points do. Each input is
+ * at most 2^31 - 1 bytes (match_len is int32); the batch itself may exceed 2 GiB. */
 int gfsm_nfa_run_host(gfsm_nfa *nfa, int32_t technique, const uint8_t *data,
                       const int64_t *offsets, int32_t num_strings, int32_t *flags,
                       int32_t *lens, float *kernel_ms);
@@ -135,7 +136,9 @@ int gfsm_nfa_run_host(gfsm_nfa *nfa, int32_t technique, const uint8_t *data,
 /* NFA batch, device buffers, asynchronous on `stream`. No upstream counterpart (upstream
  * always copies from the host); it is the resident form the batched and served paths use.
  * total_bytes must equal d_offsets[num_strings] (the host knows it; reading it back would
- * cost a synchronisation): techniques that cut inputs into segments size scratch from it. */
+ * cost a synchronisation): techniques that cut inputs into segments size scratch from it.
+ * The offsets are not read back for checking: they must start at 0, never decrease, and
+ * describe inputs of at most 2^31 - 1 bytes, as for the host form. */
 int gfsm_nfa_run_device(gfsm_nfa *nfa, int32_t technique, const uint8_t *d_data,
                         const int64_t *d_offsets, int32_t num_strings, int64_t total_bytes,
                         int32_t *d_flags, int32_t *d_lens, void *stream);

@@ -175,3 +175,18 @@ def test_dfa_create_validates_before_touching_the_device():
     rc = lib.gfsm_dfa_create(2, 0, _lib.ptr(acc, ctypes.c_uint8), _lib.ptr(trans, ctypes.c_int32),
                              ctypes.byref(h))
     assert rc == _lib.EINVAL and b"out of range" in lib.gfsm_last_error() and not h
+
+
+def test_batch_offsets_must_stay_inside_the_data():
+    """The host form copies data[0:offsets[-1]]; offsets past the buffer are refused before
+    anything reaches the library."""
+    from gpufsm.backends.hip.handles import _as_batch
+
+    data = np.zeros(10, dtype=np.uint8)
+    _as_batch(data, np.array([0, 4, 10], dtype=np.int64))
+    with pytest.raises(ValueError, match="past"):
+        _as_batch(data, np.array([0, 4, 11], dtype=np.int64))
+    with pytest.raises(ValueError, match="num_strings"):
+        _as_batch(data, np.array([], dtype=np.int64))
+    d, o, n = _as_batch(np.zeros(0, dtype=np.uint8), np.array([0], dtype=np.int64))
+    assert n == 0 and d.size == 1

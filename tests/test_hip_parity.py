@@ -126,6 +126,27 @@ def test_edge_cases():
     assert _verdicts(run_batch(dfa, [b""], backend="hip")) == [simulate_dfa(dfa, b"")]
 
 
+def test_inputs_longer_than_int32_are_refused():
+    """match_len is int32, so the host form refuses an input of 2^31 bytes or more with
+    EINVAL. The offsets are checked before the data is read, so a 1-byte buffer will do."""
+    import ctypes
+
+    nfa, _ = EXAMPLES["ab_star_c_plus_d"]()
+    data = np.zeros(1, dtype=np.uint8)
+    off = np.array([0, 1, 1 + (1 << 31)], dtype=np.int64)
+    flags = np.zeros(2, dtype=np.int32)
+    lens = np.zeros(2, dtype=np.int32)
+    lib = _lib.load()
+    with DeviceNFA(nfa) as dev:
+        rc = lib.gfsm_nfa_run_host(
+            dev.handle, _lib.NFA_AUTO, data.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+            off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), 2,
+            flags.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+            lens.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), None)
+    assert rc == _lib.EINVAL
+    assert b"2^31" in lib.gfsm_last_error()
+
+
 @pytest.mark.parametrize("n", [16, 256, 300, 4096, 40000])
 @pytest.mark.parametrize("tech", DFA_TECHS)
 def test_random_dfa_ragged_unaligned(n, tech):
