@@ -194,6 +194,15 @@ def bench_dfa(args, torch) -> dict:
         if time.perf_counter() - t0 > 2.0 or reps >= 8:
             break
     cpu_s = (time.perf_counter() - t0) / reps
+    # the reference CPU backend's own algorithm (Python, one core), on whole inputs
+    from oracle.reference import simulate_dfa
+
+    extras["cpu_python_reference"] = _python_cpu_baseline(
+        "reference", lambda x: simulate_dfa(dfa, x),
+        (data[offsets[i]:offsets[i + 1]].tobytes() for i in range(count)), 2.0,
+        f"whole {length}-byte inputs of the headline batch, in order, about 2 s")
+    _log(f"cpu python reference: {extras['cpu_python_reference']['value'] * 1e3:.2f} MB/s "
+         f"on {extras['cpu_python_reference']['inputs']} inputs")
 
     # plain per-call drop-in (gpufsm.run_batch builds an executor, copies, launches, copies back)
     if not args.profile:
@@ -256,6 +265,39 @@ def bench_dfa(args, torch) -> dict:
     }
 
 
+def _python_cpu_baseline(technique: str, fn, inputs, budget_s: float, what: str) -> dict:
+    """The reference CPU backend's algorithm, restated in Python, on one core.
+
+    `src/gpufsm/backends/cpu.py:28-67` times each input with ``time.perf_counter``; so does
+    this, over a bounded sample (inputs in order until ``budget_s`` has passed, at least 4),
+    and reports examined bytes per second with a bootstrap CI95 over the sampled inputs
+    (as `bench/timing.py:21-76` reports its CIs). ``fn(bytes) -> (accepted, match_len)``.
+    Examined bytes are match_len for an accepted input, else the whole input (none of the
+    timed automata can die: the DFAs are complete, the pattern unions keep their start).
+    """
+    times, nbytes = [], []
+    t_end = time.perf_counter() + budget_s
+    for x in inputs:
+        t0 = time.perf_counter()
+        accepted, match_len = fn(x)
+        times.append(time.perf_counter() - t0)
+        nbytes.append(match_len if accepted else len(x))
+        if len(times) >= 4 and time.perf_counter() > t_end:
+            break
+    t, b = np.asarray(times), np.asarray(nbytes, dtype=np.float64)
+    rng = np.random.default_rng(0)
+    boot = []
+    for _ in range(1000):
+        idx = rng.integers(0, t.size, t.size)
+        boot.append(b[idx].sum() / t[idx].sum() / 1e9)
+    lo, hi = np.percentile(boot, [2.5, 97.5])
+    return {"value": float(b.sum() / t.sum() / 1e9), "unit": "GB/s", "cores": 1,
+            "kind": "reference algorithm restated in Python", "technique": technique,
+            "inputs": int(t.size), "examined_bytes": int(b.sum()),
+            "median_ms_per_input": float(np.median(t) * 1e3),
+            "ci95": [float(lo), float(hi)], "sample": what}
+
+
 def _examined_upper(flags: np.ndarray, lens: np.ndarray, offsets: np.ndarray) -> int:
     """Bytes up to each first match (whole input when nothing matched)."""
     return int(np.where(flags != 0, lens, np.diff(offsets)).sum())
@@ -289,6 +331,23 @@ def bench_ragged(args, torch) -> dict:
     examined, credited = int(wex.sum()), int(off[-1])
     _log(f"ragged nfa: oracle {cpu_s:.1f} s on {threads} threads, {int(wf.sum())} accepted, "
          f"{examined / 1e6:.1f} MB examined")
+
+    # the reference CPU backend's faster technique (bitmap, Python, one core) on prefixes:
+    # the union's start state has thousands of edges, about 1 ms per byte in Python
+    py_cpu = None
+    if not args.profile:
+        from oracle.bitmap import BitmapProgram
+
+        t0 = time.perf_counter()
+        prog = BitmapProgram(pu.nfa)
+        compile_s = time.perf_counter() - t0
+        py_cpu = _python_cpu_baseline(
+            "bitmap", prog.run,
+            (data[off[i]:min(off[i + 1], off[i] + 512)].tobytes() for i in range(count)), 3.0,
+            "first 512 bytes of the batch's inputs, in order, about 3 s; the per-automaton "
+            f"compile ({compile_s:.1f} s) is not counted")
+        _log(f"ragged nfa: cpu python bitmap {py_cpu['value'] * 1e6:.1f} KB/s on "
+             f"{py_cpu['inputs']} prefixes")
 
     # batched: resident automaton + batch, autotuned, graph-captured
     s = torch.cuda.current_stream()
@@ -366,6 +425,7 @@ def bench_ragged(args, torch) -> dict:
                            "credited_GBps": credited / plain_s / 1e9,
                            "kernel_ms": res[0].kernel_ms},
         "batched_over_plain": plain_s / batched_s,
+        "cpu_python_bitmap": py_cpu,
         "resident_bytes": resident_bytes,
         "count": count,
         "states": pu.nfa.num_states,
