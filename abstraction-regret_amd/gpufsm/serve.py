@@ -45,7 +45,7 @@ class Request:
     data: bytes
     arrival: float
     rid: int = 0
-    status: str = "queued"        # queued | admitted | running | ok | rejected | timeout
+    status: str = "queued"        # queued | admitted | running | ok | rejected | timeout | error
     accepted: bool = False
     match_len: int = 0
     done_at: float = 0.0
@@ -166,7 +166,7 @@ class ServingLoop:
     def __init__(self, automata: dict, *, budget_bytes: int = 32 << 30, slots: int = 4,
                  max_batch_bytes: int = 16 << 20, max_batch_count: int = 4096,
                  queue_timeout_s: float | None = None, technique: str = "auto",
-                 engine=None, backend=None) -> None:
+                 engine=None, backend=None, keep_completed: int = 1 << 20) -> None:
         self.automata = dict(automata)
         self.max_batch_bytes = int(max_batch_bytes)
         self.max_batch_count = int(max_batch_count)
@@ -187,8 +187,11 @@ class ServingLoop:
         self._ids = itertools.count()
         self._stop = False
         self._thread: threading.Thread | None = None
-        self.completed: list[Request] = []
+        # finished requests, newest last, for stats(); bounded so a long-lived loop does
+        # not grow without limit
+        self.completed: collections.deque[Request] = collections.deque(maxlen=keep_completed)
         self.launches = 0
+        self.error: BaseException | None = None  # what stopped the scheduler, if anything
 
     # -- client side --------------------------------------------------------------------
     def submit(self, name: str, data: bytes) -> Request:
@@ -196,6 +199,8 @@ class ServingLoop:
             raise KeyError(f"unknown automaton {name!r}; have {sorted(self.automata)}")
         r = Request(name, bytes(data), time.perf_counter(), next(self._ids))
         with self._cv:
+            if self.error is not None:
+                raise RuntimeError("the serving loop stopped on an error") from self.error
             self._waiting.append(r)
             self._cv.notify()
         return r
@@ -237,10 +242,11 @@ class ServingLoop:
                 continue
             flags, lens = self.slots.results(i)
             t = time.perf_counter()
-            for r, f, n in zip(reqs, flags, lens):
-                self.admission.release(len(r.data))
-                r.finish("ok", t, bool(f), int(n))
-                self.completed.append(r)
+            with self._cv:  # completed is read by stats() from other threads
+                for r, f, n in zip(reqs, flags, lens):
+                    self.admission.release(len(r.data))
+                    r.finish("ok", t, bool(f), int(n))
+                    self.completed.append(r)
             del self._running[i]
             self._free.append(i)
             busy = True
@@ -275,25 +281,46 @@ class ServingLoop:
             data = np.frombuffer(b"".join(r.data for r in batch), dtype=np.uint8)
             for r in batch:
                 r.status = "running"
+            self._running[i] = batch  # before the launch, so a failing launch fails these too
             self.slots.launch(i, batch[0].name, data, offsets)
-            self._running[i] = batch
             self.launches += 1
             busy = True
         return busy
 
     def _run(self) -> None:
-        while True:
-            with self._cv:
-                if self._stop:
-                    return
-            if not self.step():
+        try:
+            while True:
                 with self._cv:
-                    if not self._waiting and not self._stop:
-                        self._cv.wait(0.0005 if self._running else 0.01)
+                    if self._stop:
+                        return
+                if not self.step():
+                    with self._cv:
+                        if not self._waiting and not self._stop:
+                            self._cv.wait(0.0005 if self._running else 0.01)
+        except BaseException as exc:  # a launch or device error: fail everything pending
+            self._fail_all(exc)
+
+    def _fail_all(self, exc: BaseException) -> None:
+        """Finish every unfinished request with status "error", so no caller waits forever,
+        and refuse new ones. The exception is kept in :attr:`error`."""
+        now = time.perf_counter()
+        with self._cv:
+            self.error = exc
+            pending = list(self._waiting) + list(self._admitted)
+            pending += [r for reqs in self._running.values() for r in reqs]
+            self._waiting.clear()
+            self._admitted.clear()
+            self._running.clear()
+            for r in pending:
+                if not r._event.is_set():
+                    r.finish("error", now)
+                    self.completed.append(r)
 
     # -- reporting ----------------------------------------------------------------------
     def stats(self, since: float = 0.0) -> dict:
-        done = [r for r in self.completed if r.arrival >= since]
+        with self._cv:  # the scheduler appends while another thread may be asking
+            snapshot = list(self.completed)
+        done = [r for r in snapshot if r.arrival >= since]
         ok = [r for r in done if r.status == "ok"]
         lat = np.array([r.latency for r in ok]) if ok else np.zeros(0)
         span = (max(r.done_at for r in ok) - min(r.arrival for r in ok)) if ok else 0.0
@@ -304,6 +331,7 @@ class ServingLoop:
             "ok": len(ok),
             "rejected": sum(r.status == "rejected" for r in done),
             "timeout": sum(r.status == "timeout" for r in done),
+            "error": sum(r.status == "error" for r in done),
             "p50_ms": float(np.percentile(lat, 50) * 1e3) if ok else math.nan,
             "p99_ms": float(np.percentile(lat, 99) * 1e3) if ok else math.nan,
             "credited_GBps": credited / span / 1e9 if span > 0 else math.nan,

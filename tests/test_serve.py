@@ -187,3 +187,40 @@ def test_unknown_automaton_and_concurrent_submitters():
             assert r.wait(30)
     for k, (name, data) in enumerate(reqs):
         assert (got[k].accepted, got[k].match_len) == _alone(automata, name, data)
+
+
+class FailingSlots(HostSlots):
+    """Host slots whose third launch raises, as a device error would."""
+
+    def launch(self, i, name, data, offsets):
+        if len(self.launch_log) == 2:
+            self.launch_log.append("boom")
+            raise RuntimeError("device fault (simulated on the host)")
+        super().launch(i, name, data, offsets)
+
+
+def test_a_failing_launch_fails_every_pending_request_instead_of_hanging():
+    dfa = W.dense_dfa(16, 0.0, seed=3)
+    automata = {"d": dfa}
+    slots = FailingSlots(automata, 1, delay=0.05)
+    loop = ServingLoop(automata, budget_bytes=1 << 20, backend=slots, max_batch_count=1)
+    with loop:
+        reqs = [loop.submit("d", bytes([k]) * 100) for k in range(6)]
+        assert all(r.wait(10) for r in reqs), "a request was left waiting"
+        assert isinstance(loop.error, RuntimeError)
+        with pytest.raises(RuntimeError, match="stopped on an error"):
+            loop.submit("d", b"x")
+    statuses = [r.status for r in reqs]
+    assert statuses[:2] == ["ok", "ok"] and set(statuses[2:]) == {"error"}
+    assert loop.stats()["error"] == 4
+
+
+def test_completed_history_is_bounded():
+    dfa = W.dense_dfa(16, 0.0, seed=4)
+    automata = {"d": dfa}
+    loop = ServingLoop(automata, budget_bytes=1 << 20, backend=HostSlots(automata, 2),
+                       keep_completed=5)
+    with loop:
+        reqs = [loop.submit("d", b"ab" * 10) for _ in range(12)]
+        assert all(r.wait(10) for r in reqs)
+    assert len(loop.completed) == 5 and loop.stats()["ok"] == 5
