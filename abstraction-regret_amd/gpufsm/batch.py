@@ -44,10 +44,15 @@ def _torch():
 
 
 class ResidentCache:
-    """Device-resident automaton handles keyed by identity, LRU under ``budget_bytes``."""
+    """Device-resident automaton handles keyed by identity, LRU under ``budget_bytes``.
 
-    def __init__(self, budget_bytes: int | None = None) -> None:
+    ``on_evict(handle)`` runs before an evicted handle's device memory is freed (the engine
+    drops the graphs captured on it).
+    """
+
+    def __init__(self, budget_bytes: int | None = None, on_evict=None) -> None:
         self.budget_bytes = budget_bytes
+        self.on_evict = on_evict
         self._items: collections.OrderedDict[int, tuple[object, object]] = collections.OrderedDict()
         self._lock = threading.Lock()
         self.uploads = 0
@@ -58,19 +63,22 @@ class ResidentCache:
 
     def get(self, automaton):
         key = id(automaton)
+        # one lock over lookup and upload: two threads missing on the same automaton must
+        # not both upload it
         with self._lock:
             hit = self._items.get(key)
             if hit is not None and hit[0] is automaton:
                 self._items.move_to_end(key)
                 return hit[1]
-        kind = Kind.of(automaton)
-        handle = DeviceNFA(automaton) if kind is Kind.NFA else DeviceDFA(automaton)
-        with self._lock:
+            kind = Kind.of(automaton)
+            handle = DeviceNFA(automaton) if kind is Kind.NFA else DeviceDFA(automaton)
             self.uploads += 1
             self._items[key] = (automaton, handle)  # holds the automaton: id stays unique
             if self.budget_bytes is not None:
                 while len(self._items) > 1 and self.bytes > self.budget_bytes:
                     _, (_, old) = self._items.popitem(last=False)
+                    if self.on_evict is not None:
+                        self.on_evict(old)
                     old.close()
         return handle
 
@@ -106,30 +114,50 @@ class DeviceBatch:
         self._h_off = torch.empty(self.cap_count + 1, dtype=torch.int64, pin_memory=True)
         self._h_out = torch.empty(2 * self.cap_count, dtype=torch.int32, pin_memory=True)
         self.workspace = None
+        self._retired = []   # superseded workspaces: graphs captured on them still point there
         self.count = 0
         self.total = 0
+        self._staged = None  # event after the last upload from the pinned buffers
         self.load(data, offsets, stream=stream)
 
     def load(self, data: np.ndarray, offsets: np.ndarray, stream=None) -> None:
         """Stage a packed batch (asynchronously on ``stream``) into the resident buffers."""
         torch = self._torch
+        offsets = np.asarray(offsets, dtype=np.int64)
         count = int(offsets.size) - 1
+        if count < 0:
+            raise ValueError("offsets must have count + 1 entries")
         total = int(offsets[-1])
+        # the kernels trust the offsets: check them here, where it costs O(count) on the host
+        lengths = np.diff(offsets)
+        if offsets[0] != 0 or (lengths < 0).any():
+            raise ValueError("offsets must start at 0 and never decrease")
+        if count and int(lengths.max()) > np.iinfo(np.int32).max:
+            raise ValueError("an input is longer than 2^31 - 1 bytes (match_len is int32)")
+        if len(data) < total:
+            raise ValueError(f"offsets end at {total}, past the {len(data)}-byte data")
         if count > self.cap_count or total > self.cap_bytes:
             raise ValueError(f"batch of {count} inputs / {total} bytes exceeds the capacity "
                              f"{self.cap_count} / {self.cap_bytes}")
         stream = stream or torch.cuda.current_stream()
+        # the previous upload from the pinned buffers may still be in flight
+        if self._staged is not None:
+            self._staged.synchronize()
         self._h_data[:total].numpy()[:] = np.asarray(data[:total], dtype=np.uint8)
-        self._h_off[: count + 1].numpy()[:] = np.asarray(offsets, dtype=np.int64)
+        self._h_off[: count + 1].numpy()[:] = offsets
         with torch.cuda.stream(stream):
             self.data[:total].copy_(self._h_data[:total], non_blocking=True)
             self.offsets[: count + 1].copy_(self._h_off[: count + 1], non_blocking=True)
+            self._staged = torch.cuda.Event()
+            self._staged.record(stream)
         self.count, self.total = count, total
 
     def ensure_workspace(self, nbytes: int) -> int:
         """Device pointer of a workspace of at least ``nbytes`` (256-byte aligned)."""
         torch = self._torch
         if self.workspace is None or self.workspace.numel() < nbytes:
+            if self.workspace is not None:
+                self._retired.append(self.workspace)
             self.workspace = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device="cuda")
         return self.workspace.data_ptr()
 
@@ -156,12 +184,17 @@ class BatchEngine:
     def __init__(self, *, budget_bytes: int | None = None, tuner: Autotuner | None | bool = True,
                  arch: str | None = None) -> None:
         self._torch = _torch()
-        self.cache = ResidentCache(budget_bytes)
+        self.cache = ResidentCache(budget_bytes, on_evict=self._drop_graphs_of)
         if tuner is True:
             tuner = Autotuner()
         self.tuner = tuner or None
         self.arch = arch or _lib.device_arch(0)
         self._graphs: dict[tuple, object] = {}
+
+    def _drop_graphs_of(self, handle) -> None:
+        """Forget the graphs captured on ``handle``: its device memory is about to be freed."""
+        for key in [k for k in self._graphs if k[0] == id(handle)]:
+            del self._graphs[key]
 
     # -- technique selection ------------------------------------------------------------
     def choose(self, automaton, batch: DeviceBatch, technique: str = "auto",

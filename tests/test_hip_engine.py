@@ -136,6 +136,48 @@ def test_split_segment_sizes(engine, segment):
         np.testing.assert_array_equal(m, wn, err_msg=f"n={n}")
 
 
+def test_graphs_survive_workspace_growth_and_eviction():
+    """A graph keeps pointing at the workspace it was captured with, so a later, larger
+    workspace on the same batch must not free it; an evicted automaton's graphs must be
+    dropped, and the next graph run recaptures on the re-uploaded automaton."""
+    import torch
+
+    eng = BatchEngine(tuner=None, budget_bytes=(2 << 20) + (64 << 10))  # one 2 MiB table
+    try:
+        s = torch.cuda.current_stream()
+        small = W.dense_dfa(256, 0.0005, seed=81)
+        d, o = W.uniform_batch(32, 1 << 15, seed=82)
+        wf, wn = _oracle(small, d, o)
+        batch = DeviceBatch(d, o, stream=s)
+        # 256-byte segments: no transition maps, a small workspace
+        eng.enqueue(small, batch, "split:segment=256", stream=s, graph=True)
+        f, n = batch.fetch(s)
+        np.testing.assert_array_equal(f, wf)
+        np.testing.assert_array_equal(n, wn)
+        before = batch.workspace.data_ptr()
+        eng.enqueue(small, batch, "split:segment=4096", stream=s)  # map scratch: more bytes
+        f, n = batch.fetch(s)
+        np.testing.assert_array_equal(f, wf)
+        assert batch.workspace.data_ptr() != before, "the workspace did not grow"
+        for rep in range(3):  # the first graph replays on its own, retained workspace
+            batch.flags.fill_(-1)
+            eng.enqueue(small, batch, "split:segment=256", stream=s, graph=True)
+            f, n = batch.fetch(s)
+            np.testing.assert_array_equal(f, wf, err_msg=f"replay {rep}")
+            np.testing.assert_array_equal(n, wn, err_msg=f"replay {rep}")
+        big = W.dense_dfa(4096, 0.1, seed=83)   # 2 MiB table: evicts the other automaton
+        eng.run_packed(big, d, o)
+        assert eng.cache.uploads == 2 and not eng._graphs  # its graph went with it
+        batch.flags.fill_(-1)
+        eng.enqueue(small, batch, "split:segment=256", stream=s, graph=True)  # re-uploaded
+        f, n = batch.fetch(s)
+        np.testing.assert_array_equal(f, wf)
+        np.testing.assert_array_equal(n, wn)
+        assert eng.cache.uploads == 3
+    finally:
+        eng.close()
+
+
 def test_resident_cache_reuses_and_evicts():
     eng = BatchEngine(tuner=None, budget_bytes=3 << 20)
     try:
