@@ -347,44 +347,52 @@ void run_host(const uint8_t *data, const int64_t *offsets, int32_t count, int32_
     char *base = static_cast<char *>(dev.p);
     thread_local HostStage stage;
     const bool staged = static_cast<size_t>(total) <= kStageLimit;
-    if (staged) {
-        char *h = stage.get(o_out > out_bytes ? o_out : out_bytes);
-        std::memcpy(h + o_off, offsets, off_bytes);
-        if (total > 0) std::memcpy(h + o_data, data, static_cast<size_t>(total));
-        GFSM_HIP(hipMemcpyAsync(base, h, o_data + static_cast<size_t>(total),
-                                hipMemcpyHostToDevice, s));
-    } else {
-        GFSM_HIP(hipMemcpyAsync(base + o_off, offsets, off_bytes, hipMemcpyHostToDevice, s));
-        GFSM_HIP(hipMemcpyAsync(base + o_data, data, static_cast<size_t>(total),
-                                hipMemcpyHostToDevice, s));
+    // Past this point copies from the thread's pinned staging buffer may be in flight: an
+    // error must not return before they finish, or the next call would overwrite the
+    // buffer under them.
+    try {
+        if (staged) {
+            char *h = stage.get(o_out > out_bytes ? o_out : out_bytes);
+            std::memcpy(h + o_off, offsets, off_bytes);
+            if (total > 0) std::memcpy(h + o_data, data, static_cast<size_t>(total));
+            GFSM_HIP(hipMemcpyAsync(base, h, o_data + static_cast<size_t>(total),
+                                    hipMemcpyHostToDevice, s));
+        } else {
+            GFSM_HIP(hipMemcpyAsync(base + o_off, offsets, off_bytes, hipMemcpyHostToDevice, s));
+            GFSM_HIP(hipMemcpyAsync(base + o_data, data, static_cast<size_t>(total),
+                                    hipMemcpyHostToDevice, s));
+        }
+        int32_t *d_flags = reinterpret_cast<int32_t *>(base + o_out);
+        int32_t *d_lens = d_flags + count;
+        thread_local hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (!e0) GFSM_HIP(hipEventCreate(&e0));
+        if (!e1) GFSM_HIP(hipEventCreate(&e1));
+        GFSM_HIP(hipEventRecord(e0, s));
+        enqueue(reinterpret_cast<const uint8_t *>(base + o_data),
+                reinterpret_cast<const int64_t *>(base + o_off), total, d_flags, d_lens,
+                ws ? static_cast<void *>(base + o_ws) : nullptr, ws, s);
+        GFSM_HIP(hipEventRecord(e1, s));
+        if (staged) {
+            char *h = stage.p;  // large enough: o_out >= out_bytes or sized for it above
+            GFSM_HIP(hipMemcpyAsync(h, d_flags, out_bytes, hipMemcpyDeviceToHost, s));
+            GFSM_HIP(hipStreamSynchronize(s));
+            std::memcpy(flags, h, static_cast<size_t>(count) * sizeof(int32_t));
+            std::memcpy(lens, h + static_cast<size_t>(count) * sizeof(int32_t),
+                        static_cast<size_t>(count) * sizeof(int32_t));
+        } else {
+            GFSM_HIP(hipMemcpyAsync(flags, d_flags, static_cast<size_t>(count) * sizeof(int32_t),
+                                    hipMemcpyDeviceToHost, s));
+            GFSM_HIP(hipMemcpyAsync(lens, d_lens, static_cast<size_t>(count) * sizeof(int32_t),
+                                    hipMemcpyDeviceToHost, s));
+            GFSM_HIP(hipStreamSynchronize(s));
+        }
+        float ms = 0.0f;
+        GFSM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (kernel_ms) *kernel_ms = ms;
+    } catch (...) {
+        (void)hipStreamSynchronize(s);
+        throw;
     }
-    int32_t *d_flags = reinterpret_cast<int32_t *>(base + o_out);
-    int32_t *d_lens = d_flags + count;
-    thread_local hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (!e0) GFSM_HIP(hipEventCreate(&e0));
-    if (!e1) GFSM_HIP(hipEventCreate(&e1));
-    GFSM_HIP(hipEventRecord(e0, s));
-    enqueue(reinterpret_cast<const uint8_t *>(base + o_data),
-            reinterpret_cast<const int64_t *>(base + o_off), total, d_flags, d_lens,
-            ws ? static_cast<void *>(base + o_ws) : nullptr, ws, s);
-    GFSM_HIP(hipEventRecord(e1, s));
-    if (staged) {
-        char *h = stage.p;  // large enough: o_out >= out_bytes or sized for it above
-        GFSM_HIP(hipMemcpyAsync(h, d_flags, out_bytes, hipMemcpyDeviceToHost, s));
-        GFSM_HIP(hipStreamSynchronize(s));
-        std::memcpy(flags, h, static_cast<size_t>(count) * sizeof(int32_t));
-        std::memcpy(lens, h + static_cast<size_t>(count) * sizeof(int32_t),
-                    static_cast<size_t>(count) * sizeof(int32_t));
-    } else {
-        GFSM_HIP(hipMemcpyAsync(flags, d_flags, static_cast<size_t>(count) * sizeof(int32_t),
-                                hipMemcpyDeviceToHost, s));
-        GFSM_HIP(hipMemcpyAsync(lens, d_lens, static_cast<size_t>(count) * sizeof(int32_t),
-                                hipMemcpyDeviceToHost, s));
-        GFSM_HIP(hipStreamSynchronize(s));
-    }
-    float ms = 0.0f;
-    GFSM_HIP(hipEventElapsedTime(&ms, e0, e1));
-    if (kernel_ms) *kernel_ms = ms;
 }
 
 // Lays out every device array of an NFA handle in `pk` (and fills the scalar fields).
