@@ -105,9 +105,8 @@ template <typename T, bool kLds>
 __device__ __forceinline__ const char *stage_table(const T *gtab, uint32_t bytes,
                                                    uint4 *lds_raw) {
     if constexpr (kLds) {
-        const uint4 *src = reinterpret_cast<const uint4 *>(gtab);
-        const uint32_t vecs = bytes / 16;  // tables are n * 256 entries: a multiple of 16 B
-        for (uint32_t k = threadIdx.x; k < vecs; k += blockDim.x) lds_raw[k] = src[k];
+        // tables are n * 256 entries: a multiple of 16 bytes
+        copy_to_lds(reinterpret_cast<const uint4 *>(gtab), lds_raw, bytes / 16);
         __syncthreads();
         return reinterpret_cast<const char *>(lds_raw);
     } else {
@@ -745,8 +744,10 @@ __global__ __launch_bounds__(kMapThreads) void dfa_split_map_kernel(const uint16
             hit[j] = kNoHit;
         }
         int64_t q = lo;
+        uint4 nv = q + 16 <= hi ? ld16(a.data + q) : make_uint4(0, 0, 0, 0);
         for (; q + 16 <= hi; q += 16) {
-            const uint4 v = ld16(a.data + q);
+            const uint4 v = nv;
+            if (q + 32 <= hi) nv = ld16(a.data + q + 16);  // read while this vector is walked
             const uint32_t wv[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
             for (int b = 0; b < 16; ++b) {
@@ -826,63 +827,110 @@ __global__ __launch_bounds__(256) void dfa_split_event_kernel(SplitArgs a) {
     }
 }
 
+constexpr int kAhead = 8;  // heads whose fix results the finish pass loads together
+
+// Finish-pass resolution of heads g..g1 of input i from the true entry `cur`, without the
+// table: a head whose fix assumed `cur` keeps its fix result (those of kAhead heads are
+// loaded together, so a long consistent run costs one memory round trip per kAhead heads),
+// a head with a transition map chains its part maps. Writes the verdict and returns false
+// at an accept or at the input's end; returns true, with g and cur set, at a head that
+// needs a merge walk.
+template <typename T>
+__device__ bool split_resolve(const SplitArgs &a, int32_t i, int64_t ib, int64_t &g, int64_t g1,
+                              uint32_t &cur) {
+    while (g <= g1) {
+        uint32_t fe[kAhead], fx[kAhead];
+        int32_t fa[kAhead];
+#pragma unroll
+        for (int k = 0; k < kAhead; ++k) {
+            const int64_t h = (g + k <= g1) ? g + k : g1;  // only heads of this input
+            fe[k] = a.fixed_entry[h];
+            fa[k] = a.fixed_acc[h];
+            fx[k] = a.fixed_exit[h];
+        }
+        const int64_t w = (g1 - g + 1) < kAhead ? (g1 - g + 1) : kAhead;
+        int64_t k = 0;  // heads of the window that were consistent so far
+#pragma unroll
+        for (int kk = 0; kk < kAhead; ++kk) {
+            if (kk == k && kk < w && fe[kk] == cur) {
+                if (fa[kk] >= 0) {
+                    a.flags[i] = 1;
+                    a.lens[i] = static_cast<int32_t>((g + kk) * a.seg + fa[kk] - ib + 1);
+                    return false;
+                }
+                cur = fx[kk];
+                k = kk + 1;
+            }
+        }
+        g += k;
+        if (k == w) continue;
+        // head g: its fix assumed another entry
+        if (!a.map_idx || a.map_idx[g] < 0) return true;
+        const int64_t base = static_cast<int64_t>(a.map_idx[g]) * kMapParts;
+        uint32_t st = cur / sizeof(T);
+        int32_t acc_at = -1;
+        for (int p = 0; p < kMapParts; ++p) {  // chain the head's part maps for this entry
+            const int64_t at = (base + p) * a.n + st;
+            if (acc_at < 0) acc_at = a.map_acc[at];
+            cur = a.map_exit[at];
+            st = cur / sizeof(T);
+        }
+        if (acc_at >= 0) {
+            a.flags[i] = 1;
+            a.lens[i] = static_cast<int32_t>(g * a.seg + acc_at - ib + 1);
+            return false;
+        }
+        ++g;
+    }
+    return false;
+}
+
 template <typename T, bool kLds>
 __global__ __launch_bounds__(256) void dfa_split_finish_kernel(const T *__restrict__ gtab,
                                                                SplitArgs a) {
     extern __shared__ uint4 lds_raw[];
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t ib = 0, ie = 0, g = 0, g1 = -1;
+    uint32_t cur = 0;
+    if (i < a.count && !a.done[i]) {
+        const unsigned long long key = a.first_event[i];
+        if (key != kNoEvent) {  // else: consistent chain and no accept, (0, 0)
+            ib = a.off[i];
+            ie = a.off[i + 1];
+            g = static_cast<int64_t>(key >> 1);
+            if (key & 1ull) {
+                a.flags[i] = 1;
+                a.lens[i] = static_cast<int32_t>(g * a.seg + a.fixed_acc[g] - ib + 1);
+            } else {
+                // the entry the fix pass assumed for head g is wrong: resolve from there
+                cur = ib >= (g - 1) * a.seg ? a.tail_exit[g - 1] : a.fixed_exit[g - 1];
+                g1 = (ie - 1) / a.seg;
+            }
+        }
+    }
+    bool walk = g <= g1 && split_resolve<T>(a, i, ib, g, g1, cur);
     const char *tab = reinterpret_cast<const char *>(gtab);
     if constexpr (kLds) {
-        // stage the table only when some input of the block still needs a re-walk (the
+        // the table is staged only when some input of the block needs a merge walk (the
         // condition is block-uniform, so every thread reaches the barrier inside)
-        const bool need = i < a.count && !a.done[i] && a.first_event[i] != kNoEvent &&
-                          (a.first_event[i] & 1ull) == 0ull;
-        if (__syncthreads_or(need ? 1 : 0)) tab = stage_table<T, kLds>(gtab, a.table_bytes, lds_raw);
+        if (__syncthreads_or(walk ? 1 : 0)) tab = stage_table<T, kLds>(gtab, a.table_bytes, lds_raw);
     }
-    if (i >= a.count || a.done[i]) return;
-    const unsigned long long key = a.first_event[i];
-    if (key == kNoEvent) return;  // consistent chain and no accept: (0, 0)
-    const int64_t ib = a.off[i], ie = a.off[i + 1];
-    int64_t g = static_cast<int64_t>(key >> 1);
-    if (key & 1ull) {
-        a.flags[i] = 1;
-        a.lens[i] = static_cast<int32_t>(g * a.seg + a.fixed_acc[g] - ib + 1);
-        return;
-    }
-    // the entry pass 2 assumed for segment g is wrong: continue sequentially from there
     const T *ckpt = static_cast<const T *>(a.ckpt);
-    uint32_t cur = ib >= (g - 1) * a.seg ? a.tail_exit[g - 1] : a.fixed_exit[g - 1];
-    const int64_t g1 = (ie - 1) / a.seg;
-    for (; g <= g1; ++g) {
+    while (walk) {
         const int64_t s0 = g * a.seg;
+        const int64_t hend = (s0 + a.seg) < ie ? (s0 + a.seg) : ie;
         int32_t acc_at;
         uint32_t ex;
-        if (a.fixed_entry[g] == cur) {
-            acc_at = a.fixed_acc[g];
-            ex = a.fixed_exit[g];
-        } else if (a.map_idx && a.map_idx[g] >= 0) {
-            // chain the head's part maps for this entry
-            const int64_t base = static_cast<int64_t>(a.map_idx[g]) * kMapParts;
-            uint32_t st = cur / sizeof(T);
-            acc_at = -1;
-            ex = cur;
-            for (int p = 0; p < kMapParts; ++p) {
-                const int64_t at = (base + p) * a.n + st;
-                if (acc_at < 0) acc_at = a.map_acc[at];
-                ex = a.map_exit[at];
-                st = ex / sizeof(T);
-            }
-        } else {
-            const int64_t hend = (s0 + a.seg) < ie ? (s0 + a.seg) : ie;
-            split_merge_walk<T>(tab, a, g, s0, hend, cur, CkView<T>{ckpt + g, a.segments},
-                                a.head_exit[g], acc_at, ex);
-        }
+        split_merge_walk<T>(tab, a, g, s0, hend, cur, CkView<T>{ckpt + g, a.segments},
+                            a.head_exit[g], acc_at, ex);
         if (acc_at >= 0) {
             a.flags[i] = 1;
             a.lens[i] = static_cast<int32_t>(s0 + acc_at - ib + 1);
-            return;
+            break;
         }
         cur = ex;
+        ++g;
+        walk = g <= g1 && split_resolve<T>(a, i, ib, g, g1, cur);
     }
 }
 

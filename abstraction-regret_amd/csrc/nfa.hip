@@ -229,7 +229,8 @@ __global__ __launch_bounds__(kBpThreads) void nfa_bitpacked_kernel(
     const int32_t G = gridDim.x / d.tiles;
     {
         const uint64_t *src = d.mask + static_cast<size_t>(tile) * 256 * kTileWords;
-        for (int32_t i = threadIdx.x; i < 256 * kTileWords; i += blockDim.x) lds[i] = src[i];
+        copy_to_lds(reinterpret_cast<const uint4 *>(src), reinterpret_cast<uint4 *>(lds),
+                    256 * kTileWords / 2);
         if (threadIdx.x < kTileWords) {
             lds[256 * kTileWords + threadIdx.x] = d.first[tile * kTileWords + threadIdx.x];
             lds[257 * kTileWords + threadIdx.x] = d.hassucc[tile * kTileWords + threadIdx.x];
