@@ -57,6 +57,13 @@ __device__ __forceinline__ int32_t lane_rank(uint64_t m) {
         static_cast<uint32_t>(m >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u)));
 }
 
+// Rank of the lane among the set lanes of m in its half of the wave: lo_keep is ~0 for
+// lanes 0-31 and 0 for lanes 32-63 (whose low-half bits all count as below them).
+__device__ __forceinline__ int32_t lane_rank_half(uint64_t m, uint32_t lo_keep) {
+    return static_cast<int32_t>(__builtin_amdgcn_mbcnt_hi(
+        static_cast<uint32_t>(m >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m) & lo_keep, 0u)));
+}
+
 // Scratch of one launch. Work items are handed out chunk-major: round k holds chunk k of
 // every input that has more than k chunks, so the chunks of one input run one round apart
 // and a later chunk usually starts after an earlier one has found the input's first match
@@ -598,32 +605,39 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_pair_kernel(
         // ---- the bytes: both halves step together
         const int32_t ilo = min(__builtin_amdgcn_readlane(lo, 0), __builtin_amdgcn_readlane(lo, 32));
         const int32_t ihi = max(__builtin_amdgcn_readlane(hi, 0), __builtin_amdgcn_readlane(hi, 32));
-        uint2 e = info[ilo];
-        for (int32_t i = ilo; i < ihi; ++i) {
-            const uint2 en = info[i + 1];  // the next byte's values, read ahead
+        // word offset of 32-bit word wd of a byte's plane of the continuation table
+        auto next_off = [&](uint32_t pk, uint32_t wd) -> uint32_t {
+            return kSmall ? (((pk >> kPlaneShift) & 0xfffffu) + wd) << 2
+                          : ((pk >> kPlaneShift) & 0xffu) * (plane_words << 2) + (wd << 2);
+        };
+        auto acc_word = [&](uint32_t wd) -> uint32_t {
+            return kSmall ? s_acc[wd] : *reinterpret_cast<const uint32_t *>(acc_base + (wd << 2));
+        };
+        auto list_entry = [&](uint32_t k) -> int32_t {
+            return kSmall ? static_cast<int32_t>(*reinterpret_cast<const uint16_t *>(slot_base + (k << 1)))
+                          : *reinterpret_cast<const int32_t *>(slot_base + (k << 2));
+        };
+
+        // One byte of both halves; returns true when neither half has an item left to step.
+        // The step issues about 30 VALU and 10 SALU instructions per wave and is bound by
+        // instruction issue at full occupancy rather than by the latency of its loads:
+        // stepping two bytes per memory round trip (the second byte's continuation loaded
+        // speculatively, its additions found by cross-lane reads of read-ahead pair lists)
+        // was measured 20% slower.
+        auto one_byte = [&](int32_t i) -> bool {
+            const uint2 e = info[i];
             const uint32_t po = e.x, pk = e.y;
             const bool inr = static_cast<int32_t>(pk) < 0;
             const bool was_free = slot < 0;
-            const uint64_t freem = __ballot(was_free);
-            const uint32_t flo = static_cast<uint32_t>(freem) & lo_keep;
-            const uint32_t fhi = static_cast<uint32_t>(freem >> 32);
-            const int32_t frank = static_cast<int32_t>(
-                __builtin_amdgcn_mbcnt_hi(fhi, __builtin_amdgcn_mbcnt_lo(flo, 0u)));
+            const int32_t frank = lane_rank_half(__ballot(was_free), lo_keep);
             // additions go to the half's free lanes in lane order; every lane reads an entry
             // (at most 31 past the list, inside the padding) and only taking lanes use it
-            const uint32_t k = po + static_cast<uint32_t>(frank);
-            const int32_t cand =
-                kSmall ? static_cast<int32_t>(*reinterpret_cast<const uint16_t *>(slot_base + (k << 1)))
-                       : *reinterpret_cast<const int32_t *>(slot_base + (k << 2));
+            const int32_t cand = list_entry(po + static_cast<uint32_t>(frank));
             // continuation of held states; free lanes look up slot 0, whose bit is always 0
             const uint32_t t = static_cast<uint32_t>(slot + 1);
             const uint32_t wd = t >> 5;
-            const uint32_t noff =
-                kSmall ? (((pk >> kPlaneShift) & 0xfffffu) + wd) << 2
-                       : ((pk >> kPlaneShift) & 0xffu) * (plane_words << 2) + (wd << 2);
-            const uint32_t nw = *reinterpret_cast<const uint32_t *>(next_base + noff);
-            const uint32_t aw = kSmall ? s_acc[wd]
-                                       : *reinterpret_cast<const uint32_t *>(acc_base + (wd << 2));
+            const uint32_t nw = *reinterpret_cast<const uint32_t *>(next_base + next_off(pk, wd));
+            const uint32_t aw = acc_word(wd);
             // all loads in flight together: without this the compiler moves the pair list
             // load under the "state died" branch, after the continuation load has returned
             asm volatile("" ::"v"(nw), "v"(cand), "v"(aw));
@@ -637,51 +651,43 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_pair_kernel(
             const bool short_now = hl == 31 && cnt > nfree;
             const int32_t nslot = alive ? static_cast<int32_t>(t) : (take ? cand : -1);
             slot = inr ? nslot : slot;
-            e = en;
-            if (__ballot(hit || (pk & kStartAcc) || short_now) != 0) {
-                // more additions than lanes free before the byte (rare): the rest go to the
-                // lanes whose state died on this byte, in lane order; only when those are
-                // too few as well does the item overflow to the LDS tier
-                const uint64_t sm = __ballot(short_now);
-                const int32_t F = __shfl(nfree, hbase + 31);
-                const bool died = inr && !was_free && !alive;
-                const uint64_t dm = __ballot(died);
-                const uint32_t dlo = static_cast<uint32_t>(dm) & lo_keep;
-                const uint32_t dhi = static_cast<uint32_t>(dm >> 32);
-                const int32_t drank = static_cast<int32_t>(
-                    __builtin_amdgcn_mbcnt_hi(dhi, __builtin_amdgcn_mbcnt_lo(dlo, 0u)));
-                const int32_t ndied = __shfl(drank + (died ? 1 : 0), hbase + 31);
-                const bool short_half = static_cast<uint32_t>(sm >> hbase) != 0;
-                if (short_half && died && F + drank < cnt) {
-                    const uint32_t k2 = po + static_cast<uint32_t>(F + drank);
-                    slot = kSmall ? static_cast<int32_t>(
-                                        *reinterpret_cast<const uint16_t *>(slot_base + (k2 << 1)))
-                                  : *reinterpret_cast<const int32_t *>(slot_base + (k2 << 2));
+            if (__ballot(hit || (pk & kStartAcc) || short_now) == 0) return false;
+            // more additions than lanes free before the byte (rare): the rest go to the
+            // lanes whose state died on this byte, in lane order; only when those are too
+            // few as well does the item overflow to the LDS tier
+            const uint64_t sm = __ballot(short_now);
+            const int32_t F = __shfl(nfree, hbase + 31);
+            const bool died = inr && !was_free && !alive;
+            const int32_t drank = lane_rank_half(__ballot(died), lo_keep);
+            const int32_t ndied = __shfl(drank + (died ? 1 : 0), hbase + 31);
+            const bool short_half = static_cast<uint32_t>(sm >> hbase) != 0;
+            if (short_half && died && F + drank < cnt)
+                slot = list_entry(po + static_cast<uint32_t>(F + drank));
+            const bool ovf_half = short_half && cnt > F + ndied;
+            const uint64_t em = __ballot(hit || (pk & kStartAcc) || ovf_half);
+            if (static_cast<uint32_t>(em >> hbase) != 0) {
+                if (ovf_half) {
+                    found = -1;
+                    stopped = i + 1;
+                } else if (i >= hit_from) {
+                    found = static_cast<int32_t>(w0 + i - it.ib + 1);
+                    stopped = i + 1;
                 }
-                const bool ovf_half = short_half && cnt > F + ndied;
-                const uint64_t em = __ballot(hit || (pk & kStartAcc) || ovf_half);
-                if (static_cast<uint32_t>(em >> hbase) != 0) {
-                    if (ovf_half) {
-                        found = -1;
-                        stopped = i + 1;
-                    } else if (i >= hit_from) {
-                        found = static_cast<int32_t>(w0 + i - it.ib + 1);
-                        stopped = i + 1;
-                    }
-                    if (stopped) {
-                        slot = -1;
+                if (stopped) {
+                    slot = -1;
 #pragma unroll
-                        for (int q = 0; q < 4; ++q) {
-                            const int32_t r = kHalf * q + hl;
-                            if (r > i) info[r].y = 0u;
-                        }
+                    for (int q = 0; q < 4; ++q) {
+                        const int32_t r = kHalf * q + hl;
+                        if (r > i) info[r].y = 0u;
                     }
                 }
-                wave_sync();
-                e = info[i + 1];
-                if (__ballot(running && stopped == 0) == 0) break;
             }
-        }
+            wave_sync();
+            return __ballot(running && stopped == 0) == 0;
+        };
+
+        for (int32_t i = ilo; i < ihi; ++i)
+            if (one_byte(i)) break;
 
         // ---- window end: carry the start side, finish items
         d1 = __shfl(d1_after, hbase + 31);
