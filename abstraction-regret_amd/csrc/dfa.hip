@@ -171,6 +171,7 @@ struct SplitArgs {
     int64_t total, seg, nck, segments;
     uint32_t start, acc, col, table_bytes;
     int32_t *flags, *lens, *done;
+    int32_t *seg_input;  // [segments] input holding the segment's first byte (walk pass)
     unsigned long long *first_event;
     uint32_t *head_exit, *tail_exit, *fixed_entry, *fixed_exit;
     int32_t *fixed_acc;
@@ -319,6 +320,7 @@ __global__ __launch_bounds__(kWalkThreads / K) void dfa_split_walk_kernel(
         const int64_t s0 = g[c] * a.seg;
         const int64_t s1 = (s0 + a.seg) < a.total ? (s0 + a.seg) : a.total;
         const int32_t i = split_find_input(a.off, a.count, s0);
+        a.seg_input[g[c]] = i;
         in[c] = i;
         q[c] = s0;
         pend[c] = a.off[i + 1] < s1 ? a.off[i + 1] : s1;
@@ -583,7 +585,7 @@ __global__ __launch_bounds__(fix_threads_for(kFixRound) / K) void dfa_split_fix_
             next_g += lanes;
             if (gg == 0) continue;
             const int64_t s0 = gg * a.seg;
-            const int32_t i = split_find_input(a.off, a.count, s0);
+            const int32_t i = a.seg_input[gg];
             if (a.off[i] >= s0 || a.done[i]) continue;
             const bool started_before = a.off[i] >= (gg - 1) * a.seg;
             uint32_t en;
@@ -730,7 +732,7 @@ __global__ __launch_bounds__(kMapThreads) void dfa_split_map_kernel(const uint16
         const unsigned int m = w / kMapParts, part = w % kMapParts;
         const int64_t g = a.map_list[m];
         const int64_t s0 = g * a.seg;
-        const int32_t in = split_find_input(a.off, a.count, s0);
+        const int32_t in = a.seg_input[g];
         int64_t hend = s0 + a.seg < a.total ? s0 + a.seg : a.total;
         if (a.off[in + 1] < hend) hend = a.off[in + 1];
         const int k = static_cast<int>(part) * kMapPartIntervals + kk;  // interval in the head
@@ -802,7 +804,7 @@ __global__ __launch_bounds__(256) void dfa_split_event_kernel(SplitArgs a) {
     for (int64_t g = 1 + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
          g < a.segments; g += stride) {
         const int64_t s0 = g * a.seg;
-        const int32_t i = split_find_input(a.off, a.count, s0);
+        const int32_t i = a.seg_input[g];
         if (a.off[i] >= s0 || a.done[i]) continue;
         const uint32_t prev =
             a.off[i] >= (g - 1) * a.seg ? a.tail_exit[g - 1] : a.fixed_exit[g - 1];
@@ -827,60 +829,78 @@ __global__ __launch_bounds__(256) void dfa_split_event_kernel(SplitArgs a) {
     }
 }
 
-constexpr int kAhead = 8;  // heads whose fix results the finish pass loads together
+constexpr int kAhead = 8;          // heads whose fix results the finish pass loads together
+constexpr int32_t kMapUnknown = -2;  // finish pass: head g's map not looked up yet
+
+// Chain the part maps of map m for entry cur: cur becomes the head's exit; returns the
+// first accept (offset in the head) or -1.
+template <typename T>
+__device__ __forceinline__ int32_t split_map_chain(const SplitArgs &a, int32_t m, uint32_t &cur) {
+    const int64_t base = static_cast<int64_t>(m) * kMapParts;
+    uint32_t st = cur / sizeof(T);
+    int32_t acc_at = -1;
+    for (int p = 0; p < kMapParts; ++p) {
+        const int64_t at = (base + p) * a.n + st;
+        if (acc_at < 0) acc_at = a.map_acc[at];
+        cur = a.map_exit[at];
+        st = cur / sizeof(T);
+    }
+    return acc_at;
+}
 
 // Finish-pass resolution of heads g..g1 of input i from the true entry `cur`, without the
-// table: a head whose fix assumed `cur` keeps its fix result (those of kAhead heads are
-// loaded together, so a long consistent run costs one memory round trip per kAhead heads),
-// a head with a transition map chains its part maps. Writes the verdict and returns false
-// at an accept or at the input's end; returns true, with g and cur set, at a head that
-// needs a merge walk.
+// table. A head whose fix assumed `cur` keeps its fix result: those of kAhead heads are
+// loaded together (with their map indexes), so a consistent run costs one memory round
+// trip per kAhead heads. A head whose fix assumed another entry chains its part maps. `m`
+// is head g's map index when the caller already knows that head g's entry was wrong (-1:
+// it has none), else kMapUnknown. Writes the verdict and returns false at an accept or at
+// the input's end; returns true, with g and cur set, at a head that needs a merge walk.
 template <typename T>
 __device__ bool split_resolve(const SplitArgs &a, int32_t i, int64_t ib, int64_t &g, int64_t g1,
-                              uint32_t &cur) {
+                              uint32_t &cur, int32_t m) {
     while (g <= g1) {
-        uint32_t fe[kAhead], fx[kAhead];
-        int32_t fa[kAhead];
+        if (m == kMapUnknown) {
+            uint32_t fe[kAhead], fx[kAhead];
+            int32_t fa[kAhead], mi[kAhead];
 #pragma unroll
-        for (int k = 0; k < kAhead; ++k) {
-            const int64_t h = (g + k <= g1) ? g + k : g1;  // only heads of this input
-            fe[k] = a.fixed_entry[h];
-            fa[k] = a.fixed_acc[h];
-            fx[k] = a.fixed_exit[h];
-        }
-        const int64_t w = (g1 - g + 1) < kAhead ? (g1 - g + 1) : kAhead;
-        int64_t k = 0;  // heads of the window that were consistent so far
-#pragma unroll
-        for (int kk = 0; kk < kAhead; ++kk) {
-            if (kk == k && kk < w && fe[kk] == cur) {
-                if (fa[kk] >= 0) {
-                    a.flags[i] = 1;
-                    a.lens[i] = static_cast<int32_t>((g + kk) * a.seg + fa[kk] - ib + 1);
-                    return false;
-                }
-                cur = fx[kk];
-                k = kk + 1;
+            for (int k = 0; k < kAhead; ++k) {
+                const int64_t h = (g + k <= g1) ? g + k : g1;  // only heads of this input
+                fe[k] = a.fixed_entry[h];
+                fa[k] = a.fixed_acc[h];
+                fx[k] = a.fixed_exit[h];
+                mi[k] = a.map_idx ? a.map_idx[h] : -1;
             }
+            const int64_t w = (g1 - g + 1) < kAhead ? (g1 - g + 1) : kAhead;
+            int64_t k = 0;  // heads of the window that were consistent so far
+#pragma unroll
+            for (int kk = 0; kk < kAhead; ++kk) {
+                if (kk == k && kk < w) {
+                    if (fe[kk] == cur) {
+                        if (fa[kk] >= 0) {
+                            a.flags[i] = 1;
+                            a.lens[i] = static_cast<int32_t>((g + kk) * a.seg + fa[kk] - ib + 1);
+                            return false;
+                        }
+                        cur = fx[kk];
+                        k = kk + 1;
+                    } else {
+                        m = mi[kk];
+                    }
+                }
+            }
+            g += k;
+            if (k == w) continue;
         }
-        g += k;
-        if (k == w) continue;
         // head g: its fix assumed another entry
-        if (!a.map_idx || a.map_idx[g] < 0) return true;
-        const int64_t base = static_cast<int64_t>(a.map_idx[g]) * kMapParts;
-        uint32_t st = cur / sizeof(T);
-        int32_t acc_at = -1;
-        for (int p = 0; p < kMapParts; ++p) {  // chain the head's part maps for this entry
-            const int64_t at = (base + p) * a.n + st;
-            if (acc_at < 0) acc_at = a.map_acc[at];
-            cur = a.map_exit[at];
-            st = cur / sizeof(T);
-        }
+        if (m < 0) return true;
+        const int32_t acc_at = split_map_chain<T>(a, m, cur);
         if (acc_at >= 0) {
             a.flags[i] = 1;
             a.lens[i] = static_cast<int32_t>(g * a.seg + acc_at - ib + 1);
             return false;
         }
         ++g;
+        m = kMapUnknown;
     }
     return false;
 }
@@ -892,23 +912,29 @@ __global__ __launch_bounds__(256) void dfa_split_finish_kernel(const T *__restri
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     int64_t ib = 0, ie = 0, g = 0, g1 = -1;
     uint32_t cur = 0;
-    if (i < a.count && !a.done[i]) {
+    int32_t m = kMapUnknown;
+    if (i < a.count) {
+        const int32_t dn = a.done[i];
         const unsigned long long key = a.first_event[i];
-        if (key != kNoEvent) {  // else: consistent chain and no accept, (0, 0)
+        if (!dn && key != kNoEvent) {  // else: done, or consistent with no accept, (0, 0)
+            g = static_cast<int64_t>(key >> 1);
             ib = a.off[i];
             ie = a.off[i + 1];
-            g = static_cast<int64_t>(key >> 1);
             if (key & 1ull) {
+                const int32_t fa = a.fixed_acc[g];
                 a.flags[i] = 1;
-                a.lens[i] = static_cast<int32_t>(g * a.seg + a.fixed_acc[g] - ib + 1);
+                a.lens[i] = static_cast<int32_t>(g * a.seg + fa - ib + 1);
             } else {
-                // the entry the fix pass assumed for head g is wrong: resolve from there
-                cur = ib >= (g - 1) * a.seg ? a.tail_exit[g - 1] : a.fixed_exit[g - 1];
+                // the entry the fix pass assumed for head g is wrong (that is the event):
+                // resolve from the true entry, starting with head g's map
+                const uint32_t te = a.tail_exit[g - 1], fxp = a.fixed_exit[g - 1];
+                m = a.map_idx ? a.map_idx[g] : -1;
+                cur = ib >= (g - 1) * a.seg ? te : fxp;
                 g1 = (ie - 1) / a.seg;
             }
         }
     }
-    bool walk = g <= g1 && split_resolve<T>(a, i, ib, g, g1, cur);
+    bool walk = g <= g1 && split_resolve<T>(a, i, ib, g, g1, cur, m);
     const char *tab = reinterpret_cast<const char *>(gtab);
     if constexpr (kLds) {
         // the table is staged only when some input of the block needs a merge walk (the
@@ -930,7 +956,7 @@ __global__ __launch_bounds__(256) void dfa_split_finish_kernel(const T *__restri
         }
         cur = ex;
         ++g;
-        walk = g <= g1 && split_resolve<T>(a, i, ib, g, g1, cur);
+        walk = g <= g1 && split_resolve<T>(a, i, ib, g, g1, cur, kMapUnknown);
     }
 }
 
@@ -1035,7 +1061,7 @@ DfaSplitPlan dfa_split_plan(const DfaDev &d, int64_t total_bytes, int32_t count,
     const size_t cap = static_cast<size_t>(p.map_cap);
     const size_t nn = static_cast<size_t>(d.n);
     p.scratch_bytes = align256(static_cast<size_t>(count) * 4) +
-                      align256(static_cast<size_t>(count) * 8) + 5 * align256(S * 4) +
+                      align256(static_cast<size_t>(count) * 8) + 6 * align256(S * 4) +
                       align256(S * 8) + align256(S * nck * ent) +
                       (p.maps ? align256(S * 4) + align256(cap * 4) + 256 +
                                     align256(cap * kMapParts * nn * 2) +
@@ -1163,6 +1189,8 @@ void launch_dfa_split(const DfaDev &d, const DfaSplitPlan &plan, const uint8_t *
     a.fixed_exit = reinterpret_cast<uint32_t *>(base);
     base += align256(S * 4);
     a.fixed_acc = reinterpret_cast<int32_t *>(base);
+    base += align256(S * 4);
+    a.seg_input = reinterpret_cast<int32_t *>(base);
     base += align256(S * 4);
     a.flagmask = reinterpret_cast<unsigned long long *>(base);
     base += align256(S * 8);
