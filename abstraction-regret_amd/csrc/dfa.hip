@@ -381,23 +381,38 @@ __global__ __launch_bounds__(kWalkThreads / K) void dfa_split_walk_kernel(
         if (fast) {
             uint4 v[K][kVec];
             uint32_t e0[K];
+            int32_t dn[K];
+            // this round's bytes: read ahead by the previous round, or loaded now (the first
+            // round of a part)
+#pragma unroll
+            for (int c = 0; c < K; ++c) {
+                if (mode[c] && !have[c]) {
+#pragma unroll
+                    for (int j = 0; j < kVec; ++j) nv[c][j] = ld16(a.data + q[c] + 16 * j);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < K; ++c)
+#pragma unroll
+                for (int j = 0; j < kVec; ++j) v[c][j] = mode[c] ? nv[c][j] : zero;
+            // The next round's bytes and the input's done flag, loaded while this round is
+            // walked. The loads are unconditional (a chain with no next round re-reads this
+            // round's bytes; an idle chain reads the batch's first bytes, which exist since
+            // another chain of the lane walks a full round), so the wait for this round's
+            // bytes counts only the loads issued after them instead of draining every load.
 #pragma unroll
             for (int c = 0; c < K; ++c) {
                 e0[c] = e[c];
-                if (mode[c]) {
+                dn[c] = __atomic_load_n(&a.done[mode[c] ? in[c] : 0], __ATOMIC_RELAXED);
+                const bool next = mode[c] && q[c] + 2 * kRound <= pend[c];
+                const int64_t at = next ? q[c] + kRound : (mode[c] ? q[c] : 0);
 #pragma unroll
-                    for (int j = 0; j < kVec; ++j)
-                        v[c][j] = have[c] ? nv[c][j] : ld16(a.data + q[c] + 16 * j);
-                    have[c] = q[c] + 2 * kRound <= pend[c];
-                    if (have[c]) {
-#pragma unroll
-                        for (int j = 0; j < kVec; ++j) nv[c][j] = ld16(a.data + q[c] + kRound + 16 * j);
-                    }
-                } else {
-#pragma unroll
-                    for (int j = 0; j < kVec; ++j) v[c][j] = zero;
-                }
+                for (int j = 0; j < kVec; ++j) nv[c][j] = ld16(a.data + at + 16 * j);
+                have[c] = next;
             }
+            // keep the loads above ahead of the round (the scheduler would otherwise sink them
+            // to where their registers free up, near the round's end)
+            __builtin_amdgcn_sched_barrier(0);
             step_round<T, K>(tab, col, e, mx, v);
 #pragma unroll
             for (int c = 0; c < K; ++c) {
@@ -422,7 +437,7 @@ __global__ __launch_bounds__(kWalkThreads / K) void dfa_split_walk_kernel(
                         if (mx[c] >= acc) fm[c] |= 1ull << (((r >> a.rshift) - 1) & 63);
                         mx[c] = 0;
                     }
-                    if ((r & 255) == 0 && __atomic_load_n(&a.done[in[c]], __ATOMIC_RELAXED)) {
+                    if ((r & 255) == 0 && dn[c]) {  // the flag as of this round's start
                         q[c] = pend[c];
                         part_end(c, false);
                         continue;
@@ -628,31 +643,39 @@ __global__ __launch_bounds__(fix_threads_for(kFixRound) / K) void dfa_split_fix_
         if (!any) break;
         if (fast) {
             uint4 v[K][kFixVec];
-            uint32_t e0[K], mx[K], ckn[K];
+            uint32_t e0[K], mx[K], ckr[K];
+            bool at_ck[K];
+            // this round's bytes: read ahead by the previous round, or loaded now (the first
+            // round of a head)
+#pragma unroll
+            for (int c = 0; c < K; ++c) {
+                if (act[c] && !have[c]) {
+#pragma unroll
+                    for (int j = 0; j < kFixVec; ++j) nv[c][j] = ld16(a.data + q[c] + 16 * j);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < K; ++c)
+#pragma unroll
+                for (int j = 0; j < kFixVec; ++j) v[c][j] = act[c] ? nv[c][j] : zero;
+            // The checkpoint the round ends on and the next round's bytes, loaded while the
+            // round is walked. Unconditional loads (clamped to valid addresses, see the walk
+            // kernel) keep the wait for this round's bytes a count of the later loads.
 #pragma unroll
             for (int c = 0; c < K; ++c) {
                 e0[c] = e[c];
                 mx[c] = 0;
-                ckn[c] = ~0u;  // no checkpoint at the end of this round
-                if (act[c]) {
-                    // the checkpoint the round ends on, loaded while the round is walked
-                    const int64_t r_end = q[c] + kFixRound - g[c] * a.seg;
-                    if ((r_end & rmask) == 0)
-                        ckn[c] = static_cast<uint32_t>(
-                                     ckpt[((r_end >> a.rshift) - 1) * a.segments + g[c]]) & ~1u;
+                const int64_t r_end = act[c] ? q[c] + kFixRound - g[c] * a.seg : 0;
+                at_ck[c] = act[c] && (r_end & rmask) == 0;
+                ckr[c] = static_cast<uint32_t>(
+                    ckpt[at_ck[c] ? ((r_end >> a.rshift) - 1) * a.segments + g[c] : 0]);
+                const bool next = act[c] && q[c] + 2 * kFixRound <= hend[c];
+                const int64_t at = next ? q[c] + kFixRound : (act[c] ? q[c] : 0);
 #pragma unroll
-                    for (int j = 0; j < kFixVec; ++j)
-                        v[c][j] = have[c] ? nv[c][j] : ld16(a.data + q[c] + 16 * j);
-                    have[c] = q[c] + 2 * kFixRound <= hend[c];
-                    if (have[c]) {
-#pragma unroll
-                        for (int j = 0; j < kFixVec; ++j) nv[c][j] = ld16(a.data + q[c] + kFixRound + 16 * j);
-                    }
-                } else {
-#pragma unroll
-                    for (int j = 0; j < kFixVec; ++j) v[c][j] = zero;
-                }
+                for (int j = 0; j < kFixVec; ++j) nv[c][j] = ld16(a.data + at + 16 * j);
+                have[c] = next;
             }
+            __builtin_amdgcn_sched_barrier(0);  // the loads stay ahead of the round
             step_round<T, K, kFixRound>(tab, col, e, mx, v);
 #pragma unroll
             for (int c = 0; c < K; ++c) {
@@ -660,6 +683,9 @@ __global__ __launch_bounds__(fix_threads_for(kFixRound) / K) void dfa_split_fix_
                     e[c] = a.start;
                     continue;
                 }
+                // the state the head's speculative path had at the round's end, if the round
+                // ends on a checkpoint
+                const uint32_t ckn = at_ck[c] ? (ckr[c] & ~1u) : ~0u;
                 const int64_t s0 = g[c] * a.seg;
                 q[c] += kFixRound;
                 if (mx[c] >= acc) {
@@ -669,7 +695,7 @@ __global__ __launch_bounds__(fix_threads_for(kFixRound) / K) void dfa_split_fix_
                 }
                 const int64_t r = q[c] - s0;
                 const CkView<T> ck{ckpt + g[c], a.segments};
-                if (ckn[c] == e[c]) {
+                if (ckn == e[c]) {
                     const int32_t after = split_accept_after<T>(tab, a, g[c], s0, hend[c], ck,
                                                                 (r >> a.rshift) - 1);
                     finish(c, after, a.head_exit[g[c]]);
