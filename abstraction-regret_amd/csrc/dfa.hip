@@ -78,6 +78,31 @@ __device__ __forceinline__ int64_t walk_exact(const char *tab, uint32_t col, uin
         e = dstep<T>(tab, col, e, data[q]);
         if (e >= acc) return q;
     }
+    // 64-byte rounds with the next round's bytes in flight: 64 dependent steps outlast a
+    // load, 16 do not. The read-ahead is unconditional (the last round re-reads its own
+    // bytes) and held ahead of the round by a scheduling barrier, so the wait for the
+    // current bytes counts the later loads instead of draining them.
+    if (q + 64 <= b) {
+        uint4 nv[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) nv[j] = ld16(data + q + 16 * j);
+        for (; q + 64 <= b; q += 64) {
+            uint4 v[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[j] = nv[j];
+            const int64_t at = q + 128 <= b ? q + 64 : q;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) nv[j] = ld16(data + at + 16 * j);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t e0 = e;
+                uint32_t mx = 0;
+                e = step16<T>(tab, col, e, v[j], mx);
+                if (mx >= acc) return q + 16 * j + locate16<T>(tab, col, e0, v[j], acc);
+            }
+        }
+    }
     for (; q + 16 <= b; q += 16) {
         const uint4 v = ld16(data + q);
         const uint32_t e0 = e;
@@ -771,11 +796,15 @@ __global__ __launch_bounds__(kMapThreads) void dfa_split_map_kernel(const uint16
             e[j] = static_cast<uint32_t>(x < n ? x : 0) * 2u;
             hit[j] = kNoHit;
         }
+        // the interval's bytes, one vector ahead: its loads are issued together and only the
+        // vector being walked is waited for (unconditional loads, the last one re-read)
         int64_t q = lo;
-        uint4 nv = q + 16 <= hi ? ld16(a.data + q) : make_uint4(0, 0, 0, 0);
-        for (; q + 16 <= hi; q += 16) {
+        const int nvec = static_cast<int>((hi - lo) >> 4);
+        uint4 nv = ld16(a.data + (nvec > 0 ? lo : 0));
+        for (int k = 0; k < nvec; ++k, q += 16) {
             const uint4 v = nv;
-            if (q + 32 <= hi) nv = ld16(a.data + q + 16);  // read while this vector is walked
+            nv = ld16(a.data + (k + 1 < nvec ? q + 16 : q));
+            __builtin_amdgcn_sched_barrier(0);
             const uint32_t wv[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
             for (int b = 0; b < 16; ++b) {
