@@ -145,12 +145,27 @@ class DeviceBatch:
             self._staged.synchronize()
         self._h_data[:total].numpy()[:] = np.asarray(data[:total], dtype=np.uint8)
         self._h_off[: count + 1].numpy()[:] = offsets
-        with torch.cuda.stream(stream):
-            self.data[:total].copy_(self._h_data[:total], non_blocking=True)
-            self.offsets[: count + 1].copy_(self._h_off[: count + 1], non_blocking=True)
-            self._staged = torch.cuda.Event()
-            self._staged.record(stream)
+        # asynchronous copies on the stream's raw handle (no stream context switch)
+        s = stream.cuda_stream
+        _lib.memcpy_async(self.data.data_ptr(), self._h_data.data_ptr(), total, _lib.MEMCPY_H2D, s)
+        _lib.memcpy_async(self.offsets.data_ptr(), self._h_off.data_ptr(), 8 * (count + 1),
+                          _lib.MEMCPY_H2D, s)
+        self._staged = torch.cuda.Event()
+        self._staged.record(stream)
         self.count, self.total = count, total
+
+    def download(self, stream) -> None:
+        """Enqueue the copy of the verdicts into the pinned output buffer on ``stream``."""
+        c, s = self.count, stream.cuda_stream
+        h = self._h_out.data_ptr()
+        _lib.memcpy_async(h, self.flags.data_ptr(), 4 * c, _lib.MEMCPY_D2H, s)
+        _lib.memcpy_async(h + 4 * self.cap_count, self.lens.data_ptr(), 4 * c, _lib.MEMCPY_D2H, s)
+
+    def results(self) -> tuple[np.ndarray, np.ndarray]:
+        """The verdicts of the last :meth:`download` (once its stream has passed it)."""
+        out = self._h_out.numpy()
+        c = self.count
+        return out[:c].copy(), out[self.cap_count:self.cap_count + c].copy()
 
     def ensure_workspace(self, nbytes: int) -> int:
         """Device pointer of a workspace of at least ``nbytes`` (256-byte aligned)."""
@@ -163,15 +178,10 @@ class DeviceBatch:
 
     def fetch(self, stream=None) -> tuple[np.ndarray, np.ndarray]:
         """Copy the verdicts back (waits for ``stream``)."""
-        torch = self._torch
-        stream = stream or torch.cuda.current_stream()
-        c = self.count
-        with torch.cuda.stream(stream):
-            self._h_out[:c].copy_(self.flags[:c], non_blocking=True)
-            self._h_out[self.cap_count:self.cap_count + c].copy_(self.lens[:c], non_blocking=True)
+        stream = stream or self._torch.cuda.current_stream()
+        self.download(stream)
         stream.synchronize()
-        out = self._h_out.numpy()
-        return out[:c].copy(), out[self.cap_count:self.cap_count + c].copy()
+        return self.results()
 
 
 def _techniques(kind: Kind) -> dict[str, int]:

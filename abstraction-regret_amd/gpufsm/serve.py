@@ -110,7 +110,12 @@ def form_batch(admitted: collections.deque, max_bytes: int, max_count: int) -> l
 
 
 class GpuSlots:
-    """Launch slots on the GPU: each has a stream, a resident batch buffer and an event."""
+    """Launch slots on the GPU: each has a stream, a resident batch buffer and an event.
+
+    A waiter thread per slot blocks on the slot's completion event (without the GIL) and
+    calls the notify callback the loop registers, so the scheduler sleeps until a request
+    arrives or a launch completes instead of polling the events.
+    """
 
     def __init__(self, engine, automata: dict, slots: int, capacity_bytes: int,
                  capacity_count: int, technique: str = "auto") -> None:
@@ -131,6 +136,15 @@ class GpuSlots:
         for a in automata.values():  # resident before the first request
             engine.cache.get(a)
         torch.cuda.synchronize()
+        self._done = [False] * slots
+        self._error: list[BaseException | None] = [None] * slots
+        self._armed = [threading.Event() for _ in range(slots)]
+        self._notify = None
+        self._closing = False
+        self._waiters = [threading.Thread(target=self._wait_slot, args=(i,), daemon=True,
+                                          name=f"gpufsm-slot{i}") for i in range(slots)]
+        for t in self._waiters:
+            t.start()
 
     def __len__(self) -> int:
         return len(self.streams)
@@ -138,32 +152,59 @@ class GpuSlots:
     def resident_bytes(self) -> int:
         return self.engine.cache.bytes
 
+    def set_notify(self, callback) -> None:
+        """``callback()`` is called from a waiter thread after a slot's launch completes."""
+        self._notify = callback
+
+    def _wait_slot(self, i: int) -> None:
+        while True:
+            self._armed[i].wait()
+            self._armed[i].clear()
+            if self._closing:
+                return
+            try:
+                self.events[i].synchronize()  # releases the GIL while it blocks
+            except BaseException as exc:  # surfaced by results() on the scheduler thread
+                self._error[i] = exc
+            self._done[i] = True
+            if self._notify is not None:
+                self._notify()
+
     def launch(self, i: int, name: str, data: np.ndarray, offsets: np.ndarray) -> None:
         s, b = self.streams[i], self.batches[i]
         b.load(data, offsets, stream=s)
         self.engine.enqueue(self.automata[name], b, self.technique, stream=s)
-        c = b.count
-        with self.torch.cuda.stream(s):
-            b._h_out[:c].copy_(b.flags[:c], non_blocking=True)
-            b._h_out[b.cap_count:b.cap_count + c].copy_(b.lens[:c], non_blocking=True)
-            ev = self.torch.cuda.Event()
-            ev.record(s)
+        b.download(s)
+        ev = self.torch.cuda.Event()
+        ev.record(s)
         self.events[i] = ev
+        self._done[i] = False
+        self._armed[i].set()
 
     def ready(self, i: int) -> bool:
-        return self.events[i] is not None and self.events[i].query()
+        return self._done[i]
 
     def results(self, i: int) -> tuple[np.ndarray, np.ndarray]:
-        b = self.batches[i]
-        out = b._h_out.numpy()
         self.events[i] = None
-        return out[:b.count].copy(), out[b.cap_count:b.cap_count + b.count].copy()
+        self._done[i] = False
+        err, self._error[i] = self._error[i], None
+        if err is not None:
+            raise err
+        return self.batches[i].results()
+
+    def close(self) -> None:
+        """Stop the waiter threads (a waiter inside a wait first sees its launch finish)."""
+        self._closing = True
+        for e in self._armed:
+            e.set()
+        for t in self._waiters:
+            t.join(10.0)
 
 
 class ServingLoop:
     """Continuous batching over device-resident automata; see the module docstring."""
 
-    def __init__(self, automata: dict, *, budget_bytes: int = 32 << 30, slots: int = 4,
+    def __init__(self, automata: dict, *, budget_bytes: int = 32 << 30, slots: int = 8,
                  max_batch_bytes: int = 16 << 20, max_batch_count: int = 4096,
                  queue_timeout_s: float | None = None, technique: str = "auto",
                  engine=None, backend=None, keep_completed: int = 1 << 20) -> None:
@@ -192,6 +233,15 @@ class ServingLoop:
         self.completed: collections.deque[Request] = collections.deque(maxlen=keep_completed)
         self.launches = 0
         self.error: BaseException | None = None  # what stopped the scheduler, if anything
+        # a backend that reports completions lets the scheduler sleep between events; one
+        # that does not is polled every half millisecond while launches are in flight
+        self._notified = hasattr(backend, "set_notify")
+        if self._notified:
+            backend.set_notify(self._wake)
+
+    def _wake(self) -> None:
+        with self._cv:
+            self._cv.notify()
 
     # -- client side --------------------------------------------------------------------
     def submit(self, name: str, data: bytes) -> Request:
@@ -225,6 +275,8 @@ class ServingLoop:
         if self._thread is not None:
             self._thread.join(timeout)
             self._thread = None
+        if hasattr(self.slots, "close"):
+            self.slots.close()
 
     def __enter__(self):
         return self.start()
@@ -295,8 +347,14 @@ class ServingLoop:
                         return
                 if not self.step():
                     with self._cv:
-                        if not self._waiting and not self._stop:
-                            self._cv.wait(0.0005 if self._running else 0.01)
+                        # a completion reported before this check is seen here; one after it
+                        # notifies the wait below (the waiter needs the lock to notify)
+                        done = any(self.slots.ready(i) for i in self._running)
+                        if not self._waiting and not self._stop and not done:
+                            if self._notified:
+                                self._cv.wait(0.05)
+                            else:
+                                self._cv.wait(0.0005 if self._running else 0.01)
         except BaseException as exc:  # a launch or device error: fail everything pending
             self._fail_all(exc)
 

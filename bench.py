@@ -491,8 +491,8 @@ def bench_serving(args, torch) -> dict:
     from oracle import gate, native
 
     automata, unions = W.serving_automata()
-    loop = ServingLoop(automata, budget_bytes=32 << 30, slots=4, max_batch_bytes=16 << 20,
-                       max_batch_count=1024)
+    loop = ServingLoop(automata, budget_bytes=32 << 30, slots=args.serve_slots,
+                       max_batch_bytes=16 << 20, max_batch_count=1024)
     sweep = []
     with loop:
         warm = W.serving_requests(automata, unions, 64, seed=99)
@@ -506,6 +506,16 @@ def bench_serving(args, torch) -> dict:
             st["qps_offered"] = qps
             st["seconds"] = time.perf_counter() - t_start
             st["mean_request_bytes"] = float(np.mean([len(d) for _, d in reqs]))
+            # per automaton: the large random DFAs are walked serially (a random n-state
+            # DFA needs segments of several n bytes to split), so they set the tail
+            per = {}
+            for name in sorted(automata):
+                lat = np.array([r.latency for r in done if r.name == name and r.status == "ok"])
+                if lat.size:
+                    per[name] = {"requests": int(lat.size),
+                                 "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                                 "p99_ms": float(np.percentile(lat, 99) * 1e3)}
+            st["by_automaton"] = per
             sweep.append(st)
             _log(f"serving qps={qps}: p50 {st['p50_ms']:.2f} ms p99 {st['p99_ms']:.2f} ms, "
                  f"{st['credited_GBps']:.2f} GB/s credited, {st['rejected']} rejected")
@@ -521,8 +531,9 @@ def bench_serving(args, torch) -> dict:
                                     f"served {name} request")
     return {"workload": "8 resident automata (DFAs 256/1K/4K/16K states, pattern unions "
                         "1K/4K/16K/64K states), Poisson arrivals, log-normal request "
-                        "lengths (median 16 KiB, clipped 4 MiB), HBM budget 32 GiB, 4 slots",
-            "sweep": sweep}
+                        f"lengths (median 16 KiB, clipped 4 MiB), HBM budget 32 GiB, "
+                        f"{args.serve_slots} slots",
+            "slots": args.serve_slots, "sweep": sweep}
 
 
 def bench_divergent(args, torch) -> dict:
@@ -899,6 +910,8 @@ def main() -> int:
     ap.add_argument("--ragged-technique", default="auto",
                     help='"auto" (autotuned) or a fixed label such as "chain:chunk=2048"')
     ap.add_argument("--serve-qps", type=float, nargs="+", default=[1000.0, 4000.0, 16000.0])
+    ap.add_argument("--serve-slots", type=int, default=8,
+                    help="serving loop launch slots (streams with their own batch buffers)")
     ap.add_argument("--div-count", type=int, default=1024)
     ap.add_argument("--div-length", type=int, default=1 << 20)
     ap.add_argument("--div-sweep", type=float, nargs="+", default=[0.3, 0.6, 0.95])
