@@ -224,3 +224,53 @@ def test_completed_history_is_bounded():
         reqs = [loop.submit("d", b"ab" * 10) for _ in range(12)]
         assert all(r.wait(10) for r in reqs)
     assert len(loop.completed) == 5 and loop.stats()["ok"] == 5
+
+
+class NotifyingSlots(HostSlots):
+    """Host slots that report each completion from a timer thread, as GpuSlots' waiters do,
+    and count how often the scheduler asked ready() (it sleeps between notifications)."""
+
+    def __init__(self, automata, n, delay):
+        super().__init__(automata, n, delay)
+        self.notify = None
+        self.done = [False] * n
+        self.polls = 0
+
+    def set_notify(self, callback):
+        self.notify = callback
+
+    def launch(self, i, name, data, offsets):
+        super().launch(i, name, data, offsets)
+        self.done[i] = False
+
+        def complete():
+            self.done[i] = True
+            self.notify()
+
+        threading.Timer(self.delay, complete).start()
+
+    def ready(self, i):
+        self.polls += 1
+        return self.done[i]
+
+    def results(self, i):
+        self.done[i] = False
+        return super().results(i)
+
+
+def test_completion_notifications_wake_the_scheduler():
+    """With a backend that notifies completions the scheduler sleeps until one arrives (no
+    polling) and still finishes every request with its lone verdict."""
+    automata, pu = _automata()
+    slots = NotifyingSlots(automata, 2, delay=0.02)
+    loop = ServingLoop(automata, budget_bytes=1 << 30, backend=slots, max_batch_count=4)
+    reqs = _requests(automata, pu, 24, seed=11)
+    with loop:
+        done = [loop.submit(name, data) for name, data in reqs]
+        assert all(r.wait(10) for r in done), "a notified completion was missed"
+    for (name, data), r in zip(reqs, done):
+        assert r.status == "ok"
+        assert (r.accepted, r.match_len) == _alone(automata, name, data)
+    # at most a few checks per launch: one per wake plus the pre-sleep check, never a
+    # half-millisecond poll across each 20 ms launch
+    assert slots.polls < 20 * len(slots.launch_log), (slots.polls, len(slots.launch_log))
