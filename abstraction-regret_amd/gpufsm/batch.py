@@ -117,8 +117,19 @@ class DeviceBatch:
         self._retired = []   # superseded workspaces: graphs captured on them still point there
         self.count = 0
         self.total = 0
-        self._staged = None  # event after the last upload from the pinned buffers
+        # event after the last copy from or into the pinned buffers: the copies are raw
+        # hipMemcpyAsync calls, which PyTorch's allocators do not track, so it is waited for
+        # before a pinned buffer is rewritten and before the buffers are released
+        self._staged = None
         self.load(data, offsets, stream=stream)
+
+    def __del__(self) -> None:
+        ev = getattr(self, "_staged", None)
+        if ev is not None:
+            try:
+                ev.synchronize()
+            except Exception:  # interpreter shutdown or a failed device: nothing to wait on
+                pass
 
     def load(self, data: np.ndarray, offsets: np.ndarray, stream=None) -> None:
         """Stage a packed batch (asynchronously on ``stream``) into the resident buffers."""
@@ -140,7 +151,7 @@ class DeviceBatch:
             raise ValueError(f"batch of {count} inputs / {total} bytes exceeds the capacity "
                              f"{self.cap_count} / {self.cap_bytes}")
         stream = stream or torch.cuda.current_stream()
-        # the previous upload from the pinned buffers may still be in flight
+        # the previous copy from or into the pinned buffers may still be in flight
         if self._staged is not None:
             self._staged.synchronize()
         self._h_data[:total].numpy()[:] = np.asarray(data[:total], dtype=np.uint8)
@@ -154,12 +165,16 @@ class DeviceBatch:
         self._staged.record(stream)
         self.count, self.total = count, total
 
-    def download(self, stream) -> None:
-        """Enqueue the copy of the verdicts into the pinned output buffer on ``stream``."""
+    def download(self, stream):
+        """Enqueue the copy of the verdicts into the pinned output buffer on ``stream``;
+        returns the event recorded after it (the verdicts are in place once it completes)."""
         c, s = self.count, stream.cuda_stream
         h = self._h_out.data_ptr()
         _lib.memcpy_async(h, self.flags.data_ptr(), 4 * c, _lib.MEMCPY_D2H, s)
         _lib.memcpy_async(h + 4 * self.cap_count, self.lens.data_ptr(), 4 * c, _lib.MEMCPY_D2H, s)
+        self._staged = self._torch.cuda.Event()
+        self._staged.record(stream)
+        return self._staged
 
     def results(self) -> tuple[np.ndarray, np.ndarray]:
         """The verdicts of the last :meth:`download` (once its stream has passed it)."""

@@ -171,13 +171,12 @@ class GpuSlots:
                 self._notify()
 
     def launch(self, i: int, name: str, data: np.ndarray, offsets: np.ndarray) -> None:
+        if self._closing:  # no waiter would ever report this launch
+            raise RuntimeError("launch on closed GPU slots")
         s, b = self.streams[i], self.batches[i]
         b.load(data, offsets, stream=s)
         self.engine.enqueue(self.automata[name], b, self.technique, stream=s)
-        b.download(s)
-        ev = self.torch.cuda.Event()
-        ev.record(s)
-        self.events[i] = ev
+        self.events[i] = b.download(s)
         self._done[i] = False
         self._armed[i].set()
 
