@@ -73,6 +73,8 @@ struct ChainScratch {
     int32_t *best;             // [count] first accept (1-based), INT_MAX = none yet
     int32_t *ovf2;             // [count] inputs the LDS tier hands to the tiles
     int32_t *ovf_items;        // items the register tier hands to the LDS tier
+    int32_t *deferred;         // [count] an item of the input overflowed the register tier:
+                               // its later items go to the LDS tier without a register run
     int32_t *nch;              // [count] chunks per input
     int32_t *perm;             // [count] inputs by chunk count, descending
     int32_t *hist;             // [K] inputs with exactly c chunks
@@ -115,6 +117,7 @@ __global__ __launch_bounds__(256) void chain_prep_kernel(const int64_t *__restri
     if (i < count) {
         sc.best[i] = INT_MAX;
         sc.ovf2[i] = 0;
+        sc.deferred[i] = 0;
         const int64_t len = off[i + 1] - off[i];
         const int32_t c = static_cast<int32_t>((len + chunk - 1) / chunk);
         sc.nch[i] = c;
@@ -304,9 +307,12 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_reg_kernel(
     Item it;
     while (next_item(&sc.counter[0], nullptr, sc, off, maxk, items, chunk, overlap, lane, it)) {
         bool stop = settled(sc.best, it, it.cs);  // wave-uniform throughout
+        // an item of this input overflowed already: this one goes to the LDS tier unrun
+        const bool defer = !stop && __atomic_load_n(&sc.deferred[it.in], __ATOMIC_RELAXED) != 0;
+        stop = stop || defer;
         int32_t slot = -1;                        // this lane's listed state, -1 = free
         uint64_t occ = 0;                         // lanes holding a state
-        bool s0 = true, ovf = false;              // start state active before the window
+        bool s0 = true, ovf = defer;              // start state active before the window
         int32_t d1 = 256;                         // byte of the depth-1 set; 256 = none
         int32_t found = 0;
         const int64_t abase =
@@ -427,7 +433,10 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_reg_kernel(
         }
         if (lane == 0) {
             if (found > 0) atomicMin(&sc.best[it.in], found);
-            if (ovf) sc.ovf_items[atomicAdd(&sc.counter[2], 1u)] = static_cast<int32_t>(it.id);
+            if (ovf) {
+                sc.ovf_items[atomicAdd(&sc.counter[2], 1u)] = static_cast<int32_t>(it.id);
+                __atomic_store_n(&sc.deferred[it.in], 1, __ATOMIC_RELAXED);
+            }
         }
     }
 }
@@ -519,7 +528,14 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_pair_kernel(
             if (!running) {
                 if (static_cast<int64_t>(item) < items) {
                     decode_item(item, sc, off, maxk, chunk, overlap, it);
-                    if (!settled(sc.best, it, it.cs)) {
+                    const bool skip = settled(sc.best, it, it.cs);
+                    // an item of this input overflowed already (a dense input overflows in
+                    // every chunk): this one goes to the LDS tier unrun
+                    const bool defer =
+                        !skip && __atomic_load_n(&sc.deferred[it.in], __ATOMIC_RELAXED) != 0;
+                    if (defer && hl == 0)
+                        sc.ovf_items[atomicAdd(&sc.counter[2], 1u)] = static_cast<int32_t>(it.id);
+                    if (!skip && !defer) {
                         running = true;
                         w0 = it.ps - static_cast<int64_t>(
                                          (reinterpret_cast<uintptr_t>(data) + it.ps) & 3u);
@@ -702,7 +718,10 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_pair_kernel(
         if (running && stopped != 0) {
             if (hl == 0) {
                 if (found > 0) atomicMin(&sc.best[it.in], found);
-                if (found < 0) sc.ovf_items[atomicAdd(&sc.counter[2], 1u)] = static_cast<int32_t>(it.id);
+                if (found < 0) {
+                    sc.ovf_items[atomicAdd(&sc.counter[2], 1u)] = static_cast<int32_t>(it.id);
+                    __atomic_store_n(&sc.deferred[it.in], 1, __ATOMIC_RELAXED);
+                }
             }
             running = false;
             slot = -1;
@@ -728,7 +747,9 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_lds_kernel(
     const int64_t overlap = d.max_chain > 0 ? d.max_chain - 1 : 0;
     Item it;
     while (next_item(&sc.counter[1], list, sc, off, maxk, items, chunk, overlap, lane, it)) {
-        bool stop = settled(sc.best, it, it.cs);  // wave-uniform
+        // wave-uniform; an input already handed to the tile tier is rerun there whole
+        bool stop = settled(sc.best, it, it.cs) ||
+                    __atomic_load_n(&ovf_out[it.in], __ATOMIC_RELAXED) != 0;
         int32_t *A = lists[wv][0];
         int32_t *B = lists[wv][1];
         int32_t k = 0;
@@ -853,7 +874,7 @@ size_t nfa_chain_scratch_bytes(const NfaChainDev &d, int32_t count, int64_t tota
                                int32_t chunk_bytes) {
     const size_t c = static_cast<size_t>(count);
     const size_t K = static_cast<size_t>(round_capacity(total_bytes, effective_chunk(d, chunk_bytes)));
-    return 4 * align256(c * 4) + align256(item_capacity(d, count, total_bytes, chunk_bytes) * 4) +
+    return 5 * align256(c * 4) + align256(item_capacity(d, count, total_bytes, chunk_bytes) * 4) +
            3 * align256(K * 4) + align256(K * 8) + 256 + 256;
 }
 
@@ -880,6 +901,7 @@ void launch_nfa_chain(const NfaChainDev &d, const NfaTileDev &tiles, const uint8
     sc.ovf_items = reinterpret_cast<int32_t *>(
         take(item_capacity(d, count, total_bytes, chunk_bytes) * 4));
     sc.ovf2 = reinterpret_cast<int32_t *>(take(c * 4));
+    sc.deferred = reinterpret_cast<int32_t *>(take(c * 4));
     sc.nch = reinterpret_cast<int32_t *>(take(c * 4));
     sc.perm = reinterpret_cast<int32_t *>(take(c * 4));
     sc.hist = reinterpret_cast<int32_t *>(take(K * 4));

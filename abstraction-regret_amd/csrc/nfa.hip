@@ -263,11 +263,14 @@ __global__ __launch_bounds__(kBpThreads) void nfa_bitpacked_kernel(
         // each byte is broadcast with a uniform readlane.
         const int64_t abase = base & ~static_cast<int64_t>(3);
         int64_t wstart = abase;
-        const int64_t stop = base + limit;
+        int64_t stop = base + limit;  // lowered when another tile reports an earlier match
         uint32_t word = bp_load_word(data, wstart + 4 * lane, stop, data_end);
         while (wstart < stop && !found) {
             const int64_t next = wstart + 256;
             const uint32_t nword = bp_load_word(data, next + 4 * lane, stop, data_end);
+            // the input's best match so far (other tiles, other tiers), read with the next
+            // window and applied after this one: the load has returned by then
+            const int32_t bnow = __atomic_load_n(&best[in], __ATOMIC_RELAXED);
             for (int32_t j = 0; j < 64 && !found; ++j) {
                 const uint32_t w4 = __builtin_amdgcn_readlane(word, j);
 #pragma unroll
@@ -289,10 +292,9 @@ __global__ __launch_bounds__(kBpThreads) void nfa_bitpacked_kernel(
             }
             word = nword;
             wstart = next;
-            if (!found && ((wstart - abase) & 4095) == 0) {
-                const int32_t bst = __atomic_load_n(&best[in], __ATOMIC_RELAXED);
-                if (bst != INT_MAX && base + bst - 1 < stop) break;
-            }
+            // a match ending at byte base + bnow - 1 is known: only an earlier one matters
+            // here, so the scan limit drops to it (this tile's bytes before it still run)
+            if (bnow != INT_MAX && base + bnow - 1 < stop) stop = base + bnow - 1;
         }
         if (found > 0 && lane == 0) atomicMin(&best[in], found);
     }
