@@ -45,6 +45,9 @@ namespace gfsm {
 namespace {
 
 constexpr int kChainWaves = 4;  // waves per block (all chain kernels)
+// The register tiers are held to 8 waves per SIMD (amdgpu_waves_per_eu, at most 64 VGPRs):
+// their step waits on a dependent load per byte, and an allocation a few registers past 64
+// would cost an eighth of the waves that hide it.
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -244,17 +247,38 @@ __device__ __forceinline__ void decode_item(uint32_t item, const ChainScratch &s
     it.ps = (it.cs - overlap) > it.ib ? (it.cs - overlap) : it.ib;
 }
 
+// A wave's block of items taken from a counter: [next, end). Wave-uniform.
+struct ItemPool {
+    uint32_t next = 0, end = 0;
+};
+
+// Items a wave takes per atomic: 1 unless the list is long against the waves in the grid.
+// Each hand-out is one atomic on one address, and those serialize (about 11 ns each), so a
+// list of many short items (items of settled inputs are skipped at once) costs that much per
+// item however little each does. Blocks of up to 16 divide that cost; a list of a few items
+// per wave keeps single hand-out, so long items do not queue behind each other.
+__device__ __forceinline__ uint32_t pool_block(int64_t items, int64_t waves) {
+    const int64_t q = items / (waves * 4);
+    return q < 1 ? 1u : (q > 16 ? 16u : static_cast<uint32_t>(q));
+}
+
 // Takes the next item for the whole wave; false when none are left. Items are numbered
-// 0..n-1, or are list[0..n-1] when a list is given. Every lane takes part in the atomic
-// (lane 0 adds 1, the others 0) and the value is read from lane 0 under a full mask: no
-// branch precedes the broadcast. (A lane-0-only branch there lets the compiler peel lane 0
-// off into a loop of its own, after which the other lanes never see a new item.)
-__device__ __forceinline__ bool next_item(unsigned int *counter, const int32_t *list,
-                                          const ChainScratch &sc, const int64_t *__restrict__ off,
-                                          int32_t maxk, int64_t n, int64_t chunk, int64_t overlap,
-                                          int lane, Item &it) {
-    const unsigned int mine = atomicAdd(counter, lane == 0 ? 1u : 0u);
-    const unsigned int k = __builtin_amdgcn_readfirstlane(mine);
+// 0..n-1, or are list[0..n-1] when a list is given. A wave whose block is used up takes the
+// next `per` items with one atomic: every lane takes part (lane 0 adds `per`, the others 0)
+// and the value is read from lane 0 under a full mask, so no branch precedes the broadcast.
+// (A lane-0-only branch there lets the compiler peel lane 0 off into a loop of its own,
+// after which the other lanes never see a new item.) The counter ends above n by at most
+// `per` per wave.
+__device__ __forceinline__ bool next_item(unsigned int *counter, uint32_t per, ItemPool &pool,
+                                          const int32_t *list, const ChainScratch &sc,
+                                          const int64_t *__restrict__ off, int32_t maxk, int64_t n,
+                                          int64_t chunk, int64_t overlap, int lane, Item &it) {
+    if (pool.next == pool.end) {
+        const unsigned int mine = atomicAdd(counter, lane == 0 ? per : 0u);
+        pool.next = __builtin_amdgcn_readfirstlane(mine);
+        pool.end = pool.next + per;
+    }
+    const uint32_t k = pool.next++;
     if (static_cast<int64_t>(k) >= n) return false;
     const uint32_t item = list ? static_cast<uint32_t>(list[k]) : k;
     decode_item(item, sc, off, maxk, chunk, overlap, it);
@@ -282,7 +306,7 @@ constexpr int kSmallTiles = 16;
 // step then reads three values with readlanes, issues its (at most three) loads together,
 // and has one branch, taken only on an event (an accept, an empty set, a list overflow).
 template <bool kAlways, bool kSmall>
-__global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_reg_kernel(
+__global__ __launch_bounds__(64 * kChainWaves) __attribute__((amdgpu_waves_per_eu(8, 8))) void nfa_chain_reg_kernel(
     NfaChainDev d, const uint8_t *__restrict__ data, const int64_t *__restrict__ off,
     int32_t count, ChainScratch sc, int64_t chunk) {
     __shared__ uint32_t s_flags[256];
@@ -305,7 +329,10 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_reg_kernel(
     const char *__restrict__ slot_base = kSmall ? reinterpret_cast<const char *>(d.pair_slot16)
                                                 : reinterpret_cast<const char *>(d.pair_slot);
     Item it;
-    while (next_item(&sc.counter[0], nullptr, sc, off, maxk, items, chunk, overlap, lane, it)) {
+    ItemPool pool;
+    const uint32_t per = pool_block(items, static_cast<int64_t>(gridDim.x) * kChainWaves);
+    while (next_item(&sc.counter[0], per, pool, nullptr, sc, off, maxk, items, chunk, overlap,
+                     lane, it)) {
         bool stop = settled(sc.best, it, it.cs);  // wave-uniform throughout
         // an item of this input overflowed already: this one goes to the LDS tier unrun
         const bool defer = !stop && __atomic_load_n(&sc.deferred[it.in], __ATOMIC_RELAXED) != 0;
@@ -465,7 +492,7 @@ constexpr uint32_t kStartAcc = 0x80u;       // bit 7; bits 0-6: pair count, clip
 constexpr int kPlaneShift = 9;
 
 template <bool kAlways, bool kSmall>
-__global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_pair_kernel(
+__global__ __launch_bounds__(64 * kChainWaves) __attribute__((amdgpu_waves_per_eu(8, 8))) void nfa_chain_pair_kernel(
     NfaChainDev d, const uint8_t *__restrict__ data, const int64_t *__restrict__ off,
     int32_t count, ChainScratch sc, int64_t chunk) {
     __shared__ uint32_t s_flags[256];
@@ -473,6 +500,9 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_pair_kernel(
     // {pair list offset, packed byte info} per byte; one spare entry per half, read by the
     // prefetch after the window's last byte
     __shared__ uint2 s_info[kChainWaves][2][kHalfWin + 1];
+    // deferred items of each wave, appended to the overflow list 64 at a time: one append
+    // atomic per item serializes like the hand-out does
+    __shared__ int32_t s_defer[kChainWaves][64];
     for (int i = threadIdx.x; i < 256; i += blockDim.x) s_flags[i] = d.bflags[i];
     if (kSmall) {
         const uint32_t *acc32 = reinterpret_cast<const uint32_t *>(d.acc);
@@ -513,28 +543,49 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_pair_kernel(
     const uint32_t halves = gridDim.x * (kChainWaves * 2);
     const uint32_t first_item = (blockIdx.x * kChainWaves + (threadIdx.x >> 6)) * 2 + half;
     bool first_round = true;  // wave-uniform
+    // later items come from the wave's block of counter values (ItemPool), one half served
+    // per pass of the loop below
+    ItemPool pool;
+    const uint32_t per = pool_block(items, static_cast<int64_t>(gridDim.x) * kChainWaves);
+    int32_t *dbuf = s_defer[threadIdx.x >> 6];
+    uint32_t dcount = 0;  // wave-uniform: items staged in dbuf
+    // appends the staged items with one atomic (every lane takes part, lane 0 adds)
+    auto flush_deferred = [&]() {
+        const unsigned int mine = atomicAdd(&sc.counter[2], lane == 0 ? dcount : 0u);
+        const unsigned int base = __builtin_amdgcn_readfirstlane(mine);
+        if (static_cast<uint32_t>(lane) < dcount) sc.ovf_items[base + lane] = dbuf[lane];
+        dcount = 0;
+    };
     for (;;) {
         // halves without an item take the next one (all lanes take part in the atomic and
         // the broadcast; items already settled are skipped here)
         while (!exhausted && __ballot(!running) != 0) {
+            if (dcount > 62) flush_deferred();  // room for both halves' items
             uint32_t item = first_item;
+            bool take = !running;  // this half takes an item in this pass
             if (!first_round) {
-                const unsigned int mine =
-                    atomicAdd(&sc.counter[0], (hl == 0 && !running) ? 1u : 0u);
-                item = static_cast<uint32_t>(__shfl(static_cast<int>(mine), hbase)) + halves;
+                // the lower half that needs an item takes the pool's next one
+                const uint64_t need = __ballot(!running);
+                const int taker = (need & 0xffffffffull) ? 0 : 1;  // wave-uniform
+                take = !running && half == taker;
+                if (pool.next == pool.end) {
+                    const unsigned int mine = atomicAdd(&sc.counter[0], lane == 0 ? per : 0u);
+                    pool.next = __builtin_amdgcn_readfirstlane(mine);
+                    pool.end = pool.next + per;
+                }
+                item = pool.next + halves;
+                ++pool.next;
             }
             first_round = false;
             bool none = false;
-            if (!running) {
+            bool defer = false;  // the item taken goes to the LDS tier unrun
+            if (take) {
                 if (static_cast<int64_t>(item) < items) {
                     decode_item(item, sc, off, maxk, chunk, overlap, it);
                     const bool skip = settled(sc.best, it, it.cs);
                     // an item of this input overflowed already (a dense input overflows in
                     // every chunk): this one goes to the LDS tier unrun
-                    const bool defer =
-                        !skip && __atomic_load_n(&sc.deferred[it.in], __ATOMIC_RELAXED) != 0;
-                    if (defer && hl == 0)
-                        sc.ovf_items[atomicAdd(&sc.counter[2], 1u)] = static_cast<int32_t>(it.id);
+                    defer = !skip && __atomic_load_n(&sc.deferred[it.in], __ATOMIC_RELAXED) != 0;
                     if (!skip && !defer) {
                         running = true;
                         w0 = it.ps - static_cast<int64_t>(
@@ -549,6 +600,11 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_pair_kernel(
                     none = true;
                 }
             }
+            // stage deferred items: lane 0 of each deferring half, the lower half first
+            const uint64_t dm = __ballot(defer && hl == 0);
+            if (defer && hl == 0)
+                dbuf[dcount + ((half && (dm & 1ull)) ? 1u : 0u)] = static_cast<int32_t>(it.id);
+            dcount += static_cast<uint32_t>(__popcll(dm));
             exhausted = __ballot(none) != 0;
         }
         if (__ballot(running) == 0) break;  // nothing left to hand out and nothing running
@@ -727,6 +783,7 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_pair_kernel(
             slot = -1;
         }
     }
+    if (dcount != 0) flush_deferred();  // wave-uniform; the LDS tier launches after this kernel
 }
 
 // ------------------------------------------------------------------------- lds tier ----
@@ -746,7 +803,10 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_lds_kernel(
     const int64_t data_end = off[count];
     const int64_t overlap = d.max_chain > 0 ? d.max_chain - 1 : 0;
     Item it;
-    while (next_item(&sc.counter[1], list, sc, off, maxk, items, chunk, overlap, lane, it)) {
+    ItemPool pool;
+    const uint32_t per = pool_block(items, static_cast<int64_t>(gridDim.x) * kChainWaves);
+    while (next_item(&sc.counter[1], per, pool, list, sc, off, maxk, items, chunk, overlap,
+                     lane, it)) {
         // wave-uniform; an input already handed to the tile tier is rerun there whole
         bool stop = settled(sc.best, it, it.cs) ||
                     __atomic_load_n(&ovf_out[it.in], __ATOMIC_RELAXED) != 0;
