@@ -311,6 +311,7 @@ __global__ __launch_bounds__(64 * kChainWaves) __attribute__((amdgpu_waves_per_e
     int32_t count, ChainScratch sc, int64_t chunk) {
     __shared__ uint32_t s_flags[256];
     __shared__ uint32_t s_acc[kSmall ? (kSmallTiles + 1) * kTileWords * 2 : 1];
+    __shared__ int32_t s_defer[kChainWaves][64];  // deferred items, appended 64 at a time
     for (int i = threadIdx.x; i < 256; i += blockDim.x) s_flags[i] = d.bflags[i];
     if (kSmall) {
         const uint32_t *acc32 = reinterpret_cast<const uint32_t *>(d.acc);
@@ -323,6 +324,15 @@ __global__ __launch_bounds__(64 * kChainWaves) __attribute__((amdgpu_waves_per_e
     const int64_t items = sc.rbase[maxk];
     const int64_t data_end = off[count];
     const int64_t overlap = d.max_chain > 0 ? d.max_chain - 1 : 0;
+    int32_t *dbuf = s_defer[threadIdx.x >> 6];
+    uint32_t dcount = 0;  // wave-uniform: items staged in dbuf
+    // appends the staged items with one atomic (every lane takes part, lane 0 adds)
+    auto flush_deferred = [&]() {
+        const unsigned int mine = atomicAdd(&sc.counter[2], lane == 0 ? dcount : 0u);
+        const unsigned int base = __builtin_amdgcn_readfirstlane(mine);
+        if (static_cast<uint32_t>(lane) < dcount) sc.ovf_items[base + lane] = dbuf[lane];
+        dcount = 0;
+    };
     const char *__restrict__ next_base = reinterpret_cast<const char *>(d.next);
     const uint32_t plane = static_cast<uint32_t>(d.tiles + 1) * (kTileWords * 8);  // bytes per byte plane
     const char *__restrict__ acc_base = reinterpret_cast<const char *>(d.acc);
@@ -334,12 +344,17 @@ __global__ __launch_bounds__(64 * kChainWaves) __attribute__((amdgpu_waves_per_e
     while (next_item(&sc.counter[0], per, pool, nullptr, sc, off, maxk, items, chunk, overlap,
                      lane, it)) {
         bool stop = settled(sc.best, it, it.cs);  // wave-uniform throughout
-        // an item of this input overflowed already: this one goes to the LDS tier unrun
-        const bool defer = !stop && __atomic_load_n(&sc.deferred[it.in], __ATOMIC_RELAXED) != 0;
-        stop = stop || defer;
+        // an item of this input overflowed already: this one goes to the LDS tier unrun,
+        // staged and appended with others
+        if (!stop && __atomic_load_n(&sc.deferred[it.in], __ATOMIC_RELAXED) != 0) {
+            if (dcount == 64) flush_deferred();
+            if (lane == 0) dbuf[dcount] = static_cast<int32_t>(it.id);
+            ++dcount;
+            continue;
+        }
         int32_t slot = -1;                        // this lane's listed state, -1 = free
         uint64_t occ = 0;                         // lanes holding a state
-        bool s0 = true, ovf = defer;              // start state active before the window
+        bool s0 = true, ovf = false;              // start state active before the window
         int32_t d1 = 256;                         // byte of the depth-1 set; 256 = none
         int32_t found = 0;
         const int64_t abase =
@@ -466,6 +481,7 @@ __global__ __launch_bounds__(64 * kChainWaves) __attribute__((amdgpu_waves_per_e
             }
         }
     }
+    if (dcount != 0) flush_deferred();  // wave-uniform; the LDS tier launches after this kernel
 }
 
 // ---------------------------------------------------------------- paired reg tier ----

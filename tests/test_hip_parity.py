@@ -336,6 +336,83 @@ def test_chain_medium_active_sets(mode, monkeypatch):
     assert wf.any() and not wf.all()
 
 
+def _wildcard_chains(chains: int = 6, depth: int = 15):
+    """Start state on every byte, then `chains` chains of `depth` positions; every position
+    takes any byte except the last, which takes byte 7 + chain. After depth - 1 bytes about
+    chains * (depth - 1) states are live on every byte."""
+    b = NFABuilder()
+    s0 = b.add_state()
+    b.set_start(s0)
+    b.add_transition(s0, 256, s0)
+    for chain in range(chains):
+        prev = s0
+        for pos in range(depth):
+            s = b.add_state(accept=(pos == depth - 1))
+            b.add_transition(prev, 256 if pos < depth - 1 else 7 + chain, s)
+            prev = s
+    return b.build()
+
+
+def _wildcard_chains_verdicts(data, off, chains: int = 6, depth: int = 15):
+    """Closed form for `_wildcard_chains`: the first accept ends on the first byte of
+    7 .. 7 + chains - 1 at an input position p >= depth - 1, with match length p + 1."""
+    idx = np.flatnonzero((data >= 7) & (data < 7 + chains))
+    count = off.size - 1
+    flags = np.zeros(count, np.int32)
+    lens = np.zeros(count, np.int32)
+    for i in range(count):
+        k = np.searchsorted(idx, off[i] + depth - 1)
+        if k < idx.size and idx[k] < off[i + 1]:
+            flags[i], lens[i] = 1, idx[k] - off[i] + 1
+    return flags, lens
+
+
+def _wildcard_chain_batch(rng, count, length, late_hits):
+    """`count` inputs of `length` bytes of 0x41; every other input gets one final-position
+    byte in its last `late_hits` bytes, so its earlier chunks all have to run."""
+    data = np.full(count * length, 0x41, dtype=np.uint8)
+    off = np.arange(count + 1, dtype=np.int64) * length
+    for i in range(0, count, 2):
+        data[off[i] + length - 1 - rng.integers(0, late_hits)] = 7 + rng.integers(0, 6)
+    return data, off
+
+
+def test_wildcard_chain_closed_form_matches_the_oracle():
+    """Pins the closed form used at full size below against the oracle, on inputs with the
+    final-position bytes anywhere, including before position depth - 1."""
+    nfa = _wildcard_chains()
+    rng = np.random.default_rng(21)
+    lens = rng.integers(0, 600, size=48)
+    data = np.where(rng.random(int(lens.sum())) < 0.995, 0x41,
+                    rng.integers(5, 15, size=int(lens.sum()))).astype(np.uint8)
+    off = np.zeros(49, dtype=np.int64)
+    np.cumsum(lens, out=off[1:])
+    wf, wn, _ = native.nfa_batch(nfa, data, off)
+    f, n = _wildcard_chains_verdicts(data, off)
+    np.testing.assert_array_equal(f, wf)
+    np.testing.assert_array_equal(n, wn)
+    assert wf.any() and not wf.all()
+
+
+@pytest.mark.parametrize("mode", ["pairs", "wave64"])
+def test_chain_many_deferred_items_per_wave(mode, monkeypatch):
+    """About 84 live states on every byte overflow the register tier in both modes, so every
+    later chunk of an input goes to the LDS tier unrun. 64 inputs of 1 MiB in 64-byte chunks
+    are a million such items, over a hundred per wave of the persistent grid: the register
+    tier's staged appends fill and flush their 64-entry buffers many times, and the LDS tier
+    takes its items in blocks. Checked against the closed form pinned above."""
+    _chain_mode(monkeypatch, mode)
+    monkeypatch.setenv("GFSM_CHAIN_CHUNK", "64")
+    nfa = _wildcard_chains()
+    data, off = _wildcard_chain_batch(np.random.default_rng(22), 64, 1 << 20, 4096)
+    with DeviceNFA(nfa) as dev:
+        f, n, _ = dev.run_host(data, off, _lib.NFA_TECHNIQUES["chain"])
+    wf, wn = _wildcard_chains_verdicts(data, off)
+    np.testing.assert_array_equal(f, wf)
+    np.testing.assert_array_equal(n, wn)
+    assert int(wf.sum()) == 32
+
+
 @pytest.mark.parametrize("mode", ["pairs", "wave64"])
 def test_chain_additions_into_lanes_freed_on_the_same_byte(mode, monkeypatch):
     """After A B...B, 32 states are listed (32 chains A B B ... B); on byte X all of them die
