@@ -130,12 +130,15 @@ def _measure_dfa(torch, batch, dfa, technique: str, data, offsets, steps: int, w
         gate.require_arrays(f, m, wf, wm, f"hip dfa {technique} ({what})")
         wall, ks = _time_steps(torch, lambda: batch.run(dev, tech, stream), steps, warmup)
         entry_bytes = dev.props.entry_bytes
+        names = {v: k for k, v in _lib.DFA_TECHNIQUES.items()}
+        resolved = names[dev.resolve(batch.count, batch.total, tech)]
     examined = int(wex.sum())
     credited = int(offsets[-1])
     step_s = wall / steps
     out = {
         "workload": what,
         "technique": technique,
+        "resolved_technique": resolved,
         "accepted": int(f.sum()),
         "examined_bytes": examined,
         "credited_bytes": credited,
@@ -220,10 +223,7 @@ def bench_dfa(args, torch) -> dict:
     compulsory = head["table_bytes"] + 8 * (count + 1) + 8 * count
     kernel_ms = head["kernel_ms"]
     achieved = (examined + compulsory) / (kernel_ms * 1e-3) / 1e9
-    tech = args.technique
-    if tech == "auto":
-        tech = "split" if length >= 4 * (1024 if n <= 320 else max(1024, 4 * n)) else (
-            "lds" if n <= 320 else "gather")
+    tech = head["resolved_technique"]  # what `auto` picked for this shape
     kernel_name = KERNELS.get(tech, tech)
     workload_name = f"dfa{n}_{count}x{length}_nonlatching"
     return {

@@ -161,6 +161,18 @@ def test_random_dfa_ragged_unaligned(n, tech):
     assert got == _oracle_dfa(dfa, batch)
 
 
+def test_dfa_auto_resolution_by_batch_shape():
+    """auto segments a batch of few long inputs, and walks one lane per input when the
+    inputs alone fill every CU (far more than 256 per CU here) or are short."""
+    lds, split = _lib.DFA_TECHNIQUES["lds"], _lib.DFA_TECHNIQUES["split"]
+    with DeviceDFA(random_dfa(256, accept_prob=0.0, seed=5)) as dev:
+        assert dev.resolve(4096, 4096 << 16) == split        # 4096 x 64 KiB
+        assert dev.resolve(1 << 20, (1 << 20) << 14) == lds  # 1 M x 16 KiB
+        assert dev.resolve(4096, 4096 << 12) == lds          # 4096 x 4 KiB
+    with DeviceDFA(random_dfa(4096, accept_prob=0.0, seed=5)) as dev:  # table in L2
+        assert dev.resolve(1 << 20, (1 << 20) << 14) == _lib.DFA_TECHNIQUES["gather"]
+
+
 @pytest.mark.parametrize("tech", DFA_TECHS)
 def test_dfa_non_latching_long_inputs(tech):
     """No accepting state: every byte is walked, across 16-byte vector boundaries."""
