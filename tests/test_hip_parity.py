@@ -161,6 +161,24 @@ def test_random_dfa_ragged_unaligned(n, tech):
     assert got == _oracle_dfa(dfa, batch)
 
 
+@pytest.mark.parametrize("count", [131072, 262144])
+def test_dfa_lds_many_inputs_per_cu(count):
+    """Batches with 512 and 1024 inputs per CU (on 256 CUs) run the per-input walk in
+    larger blocks; ragged short inputs, some empty, some latching."""
+    dfa = random_dfa(256, accept_prob=0.02, seed=count)
+    rng = np.random.default_rng(count)
+    lens = rng.integers(0, 97, size=count)
+    data = rng.integers(0, 256, size=int(lens.sum()), dtype=np.uint8)
+    off = np.zeros(count + 1, dtype=np.int64)
+    np.cumsum(lens, out=off[1:])
+    with DeviceDFA(dfa) as dev:
+        f, n, _ = dev.run_host(data, off, _lib.DFA_TECHNIQUES["lds"])
+    wf, wn, _ = native.dfa_batch(dfa, data, off)
+    np.testing.assert_array_equal(f, wf)
+    np.testing.assert_array_equal(n, wn)
+    assert wf.any() and not wf.all()
+
+
 def test_dfa_auto_resolution_by_batch_shape():
     """auto segments a batch of few long inputs, and walks one lane per input when the
     inputs alone fill every CU (far more than 256 per CU here) or are short."""
