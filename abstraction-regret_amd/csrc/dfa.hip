@@ -69,7 +69,9 @@ __device__ __forceinline__ uint4 ld16(const uint8_t *p) {
 
 // Exact walk of [a, b) from e: absolute position of the first byte after which an
 // accepting state is active, or -1 (then e is the state after b). Never reads outside.
-template <typename T>
+// R: bytes per round. 128 reads a whole cache line per round, so a line is not evicted
+// between the rounds that read its halves when thousands of lanes stream at once.
+template <typename T, int R = 64>
 __device__ __forceinline__ int64_t walk_exact(const char *tab, uint32_t col, uint32_t acc,
                                               uint32_t &e, const uint8_t *__restrict__ data,
                                               int64_t a, int64_t b) {
@@ -78,24 +80,25 @@ __device__ __forceinline__ int64_t walk_exact(const char *tab, uint32_t col, uin
         e = dstep<T>(tab, col, e, data[q]);
         if (e >= acc) return q;
     }
-    // 64-byte rounds with the next round's bytes in flight: 64 dependent steps outlast a
+    // R-byte rounds with the next round's bytes in flight: 64 dependent steps outlast a
     // load, 16 do not. The read-ahead is unconditional (the last round re-reads its own
     // bytes) and held ahead of the round by a scheduling barrier, so the wait for the
     // current bytes counts the later loads instead of draining them.
-    if (q + 64 <= b) {
-        uint4 nv[4];
+    constexpr int kV = R / 16;
+    if (q + R <= b) {
+        uint4 nv[kV];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) nv[j] = ld16(data + q + 16 * j);
-        for (; q + 64 <= b; q += 64) {
-            uint4 v[4];
+        for (int j = 0; j < kV; ++j) nv[j] = ld16(data + q + 16 * j);
+        for (; q + R <= b; q += R) {
+            uint4 v[kV];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) v[j] = nv[j];
-            const int64_t at = q + 128 <= b ? q + 64 : q;
+            for (int j = 0; j < kV; ++j) v[j] = nv[j];
+            const int64_t at = q + 2 * R <= b ? q + R : q;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) nv[j] = ld16(data + at + 16 * j);
+            for (int j = 0; j < kV; ++j) nv[j] = ld16(data + at + 16 * j);
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
+            for (int j = 0; j < kV; ++j) {
                 const uint32_t e0 = e;
                 uint32_t mx = 0;
                 e = step16<T>(tab, col, e, v[j], mx);
@@ -157,7 +160,7 @@ __global__ __launch_bounds__(1024) void dfa_walk_kernel(const T *__restrict__ gt
          i += gridDim.x * blockDim.x) {
         const int64_t ib = off[i];
         uint32_t e = start;
-        const int64_t hit = walk_exact<T>(tab, col, acc, e, data, ib, off[i + 1]);
+        const int64_t hit = walk_exact<T, kLds ? 128 : 64>(tab, col, acc, e, data, ib, off[i + 1]);
         flags[i] = hit >= 0;
         lens[i] = hit >= 0 ? static_cast<int32_t>(hit - ib + 1) : 0;
     }
