@@ -6,6 +6,7 @@
 // status codes plus a thread-local message.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <cstring>
 #include <functional>
@@ -23,6 +24,11 @@ struct gfsm_nfa {
     int64_t nnz = 0, nne = 0;
     bool start_accepts = false;
     NfaWorklistDev wl{};
+    // general layout (epsilon closure folded): uploaded for automata that are not
+    // chain-structured and for any automaton of at most 512 states
+    bool general_ok = false;
+    std::string general_why;
+    NfaGeneralDev gen{};
     bool chain_ok = false;
     NfaTileDev tile{};
     NfaChainDev chain{};
@@ -145,22 +151,47 @@ void check_batch_args(const void *data, const int64_t *offsets, int32_t count,
     (void)data;
 }
 
-int32_t resolve_nfa(const gfsm_nfa *h, int32_t technique) {
+// The general layout's worklist (ballot compaction) runs when the layout exists and its
+// bitmaps fit LDS; otherwise the worklist walks the unfolded automaton with its own
+// epsilon closure.
+bool use_wl2(const gfsm_nfa *h) { return h->general_ok && h->n <= 65536; }
+
+// The bit-parallel step on the general layout (automata that are not chain-structured).
+bool gather_ok(const gfsm_nfa *h) { return h->general_ok && nfa_gather_fits(h->gen); }
+
+int32_t resolve_nfa(const gfsm_nfa *h, int32_t technique, int32_t count) {
     switch (technique) {
         case GFSM_NFA_AUTO:
             // chain-structured automata: the sparse chain walk (its own tiers fall back to
-            // the tiled pass on inputs whose active sets outgrow the lists)
-            return h->chain_ok ? GFSM_NFA_CHAIN : GFSM_NFA_WORKLIST;
+            // the tiled pass on inputs whose active sets outgrow the lists); small general
+            // automata over batches that fill the lanes: one lane per input; otherwise the
+            // wave-cooperative worklist
+            if (h->chain_ok) return GFSM_NFA_CHAIN;
+            if (h->general_ok && h->gen.nwords <= kRegMaxWords && count >= 1024)
+                return GFSM_NFA_REGISTER;
+            return GFSM_NFA_WORKLIST;
         case GFSM_NFA_WORKLIST:
             return technique;
         case GFSM_NFA_BITPACKED:
+            if (h->chain_ok || gather_ok(h)) return technique;
+            throw Error(GFSM_EUNSUPPORTED,
+                        "bitpacked needs a chain-structured NFA (tiled shift step) or a "
+                        "general layout whose bitsets fit LDS: " +
+                            (h->general_ok ? std::to_string(h->n) + " states"
+                                           : h->general_why));
         case GFSM_NFA_CHAIN:
             if (!h->chain_ok)
                 throw Error(GFSM_EUNSUPPORTED,
-                            std::string(technique == GFSM_NFA_CHAIN ? "chain" : "bitpacked") +
-                                " needs a chain-structured NFA (no epsilon edges, at most "
-                                "one predecessor and successor per non-start state, homogeneous "
-                                "labels, chains of at most 4096 states)");
+                            "chain needs a chain-structured NFA (no epsilon edges, at most "
+                            "one predecessor and successor per non-start state, homogeneous "
+                            "labels, chains of at most 4096 states)");
+            return technique;
+        case GFSM_NFA_REGISTER:
+            if (!h->general_ok || h->gen.nwords > kRegMaxWords)
+                throw Error(GFSM_EUNSUPPORTED,
+                            "register needs at most " + std::to_string(kRegMaxWords * 64) +
+                                " states with a foldable epsilon closure" +
+                                (h->general_ok ? std::string() : ": " + h->general_why));
             return technique;
         default:
             throw Error(GFSM_EINVAL, "unknown NFA technique " + std::to_string(technique));
@@ -169,19 +200,26 @@ int32_t resolve_nfa(const gfsm_nfa *h, int32_t technique) {
 
 struct Tuning {
     int32_t chunk = 0, segment = 0, flags = 0;
+    int32_t max_len = 0;  // longest input, 0 = not known
 };
 
 Tuning read_tuning(const gfsm_tuning *t) {
     Tuning r;
     if (t) {
-        if (t->chunk_bytes < 0 || t->segment_bytes < 0)
+        if (t->chunk_bytes < 0 || t->segment_bytes < 0 || t->max_input_bytes < 0)
             throw Error(GFSM_EINVAL, "tuning sizes must be >= 0");
         r.chunk = t->chunk_bytes;
         r.segment = t->segment_bytes;
         r.flags = t->flags;
+        r.max_len = t->max_input_bytes;
     }
     return r;
 }
+
+// A one-lane-per-input walk lasts as long as its longest input (about 25 ns per byte for a
+// lane alone), the segmented walk about total / 1.3 TB/s: an input longer than
+// total / 32768 bytes (and than four segments) sends the batch to the segmented walk.
+constexpr int64_t kLaneWalkSkew = 32768;
 
 int32_t resolve_dfa(const gfsm_dfa *h, int32_t technique, int32_t count, int64_t total,
                     const Tuning &tu = Tuning()) {
@@ -191,10 +229,17 @@ int32_t resolve_dfa(const gfsm_dfa *h, int32_t technique, int32_t count, int64_t
             // would serialise each of them into one long dependent chain. With 256 inputs
             // per CU or more, one lane per input already keeps the LDS busy and needs no
             // repair passes (65,536 x 16 KiB, 256 states: 1.86 TB/s against 1.69 split).
+            // A known longest input that is long against both the segment and the batch
+            // is segmented whatever the other inputs are: one lane per input would walk it
+            // alone. The count rule needs the longest input for the same reason.
             const DfaSplitPlan plan = dfa_split_plan(h->dev, total, count, tu.segment);
             const int64_t avg = count > 0 ? total / count : 0;
-            if (h->lds_ok && count >= 256 * static_cast<int64_t>(device_cu_count()))
-                return GFSM_DFA_LDS;
+            if (tu.max_len > 0) {
+                const int64_t bound = std::max<int64_t>(4 * plan.seg_bytes, total / kLaneWalkSkew);
+                if (tu.max_len > bound) return GFSM_DFA_SPLIT;
+                if (h->lds_ok && count >= 256 * static_cast<int64_t>(device_cu_count()))
+                    return GFSM_DFA_LDS;
+            }
             if (avg >= 4 * plan.seg_bytes) return GFSM_DFA_SPLIT;
             return h->lds_ok ? GFSM_DFA_LDS : GFSM_DFA_GATHER;
         }
@@ -216,10 +261,15 @@ size_t nfa_workspace(const gfsm_nfa *h, int32_t tech, int32_t count, int64_t tot
     if (count <= 0 || h->start_accepts) return 0;
     if (tech == GFSM_NFA_CHAIN)
         return nfa_chain_scratch_bytes(h->chain, count, total, nfa_chain_chunk_bytes(tu.chunk));
-    if (tech == GFSM_NFA_BITPACKED) return align256(static_cast<size_t>(count) * sizeof(int32_t));
+    if (tech == GFSM_NFA_BITPACKED)
+        return h->chain_ok ? align256(static_cast<size_t>(count) * sizeof(int32_t)) : 0;
+    if (tech == GFSM_NFA_REGISTER) return 0;
     int32_t grid = 1;
     int64_t scratch_elems = 0;
-    nfa_worklist_geometry(h->wl, count, &grid, &scratch_elems);
+    if (use_wl2(h))
+        nfa_wl2_geometry(h->gen, count, &grid, &scratch_elems);
+    else
+        nfa_worklist_geometry(h->wl, count, &grid, &scratch_elems);
     return align256(static_cast<size_t>(scratch_elems) * sizeof(int32_t));
 }
 
@@ -250,12 +300,25 @@ void nfa_enqueue(gfsm_nfa *h, int32_t tech, const Tuning &tu, const uint8_t *d_d
         return;
     }
     if (tech == GFSM_NFA_BITPACKED) {
-        launch_nfa_bitpacked(h->tile, d_data, d_off, count, d_flags, d_lens,
-                             static_cast<int32_t *>(ws), s);
+        if (h->chain_ok)
+            launch_nfa_bitpacked(h->tile, d_data, d_off, count, d_flags, d_lens,
+                                 static_cast<int32_t *>(ws), s);
+        else
+            launch_nfa_gather(h->gen, d_data, d_off, count, d_flags, d_lens, s);
+        return;
+    }
+    if (tech == GFSM_NFA_REGISTER) {
+        launch_nfa_register(h->gen, d_data, d_off, count, d_flags, d_lens, s);
         return;
     }
     int32_t grid = 1;
     int64_t scratch_elems = 0;
+    if (use_wl2(h)) {
+        nfa_wl2_geometry(h->gen, count, &grid, &scratch_elems);
+        launch_nfa_wl2(h->gen, d_data, d_off, count, d_flags, d_lens,
+                       scratch_elems > 0 ? static_cast<int32_t *>(ws) : nullptr, grid, s);
+        return;
+    }
     nfa_worklist_geometry(h->wl, count, &grid, &scratch_elems);
     launch_nfa_worklist(h->wl, d_data, d_off, count, d_flags, d_lens,
                         scratch_elems > 0 ? static_cast<int32_t *>(ws) : nullptr, grid, s);
@@ -299,6 +362,18 @@ void check_ws(const void *ws, int64_t ws_bytes) {
     if (ws_bytes > 0 && !ws) throw Error(GFSM_EINVAL, "null workspace");
     if (reinterpret_cast<uintptr_t>(ws) & 255u)
         throw Error(GFSM_EINVAL, "workspace must be 256-byte aligned");
+}
+
+// Longest input of a host batch (the offsets are checked by run_host before any launch;
+// malformed ones give a value that the check then rejects). 1 for a batch of empty inputs,
+// so the value always counts as known.
+int32_t longest_input(const int64_t *offsets, int32_t count) {
+    int64_t m = 1;
+    for (int32_t i = 0; i < count; ++i) {
+        const int64_t l = offsets[i + 1] - offsets[i];
+        if (l > m) m = l;
+    }
+    return m > INT32_MAX ? INT32_MAX : static_cast<int32_t>(m);
 }
 
 // Pinned host staging of the calling thread (grows, kept for the life of the thread).
@@ -399,9 +474,17 @@ void run_host(const uint8_t *data, const int64_t *offsets, int32_t count, int32_
     }
 }
 
+// The general layout is built for automata that are not chain-structured (their techniques
+// all run on it) and for any automaton of at most 512 states (the register technique; the
+// fold costs little there). Chain-structured unions above that size run the chain, tiled
+// and unfolded worklist walks, and skip the fold's host time in the per-call path.
+bool wants_general(int32_t num_states, const TileHost &t) {
+    return !t.ok || num_states <= kRegMaxWords * 64;
+}
+
 // Lays out every device array of an NFA handle in `pk` (and fills the scalar fields).
 void pack_nfa(gfsm_nfa *h, int32_t num_states, const WorklistHost &w, const TileHost &t,
-              std::vector<uint16_t> &s16, Packer &pk) {
+              const GeneralHost &g, std::vector<uint16_t> &s16, Packer &pk) {
     pk.add(&h->wl.row, w.row);
     pk.add(&h->wl.edge, w.edge);
     pk.add(&h->wl.bigidx, w.bigidx);
@@ -414,6 +497,26 @@ void pack_nfa(gfsm_nfa *h, int32_t num_states, const WorklistHost &w, const Tile
     h->wl.nwords = w.nwords;
     h->wl.k0 = static_cast<int32_t>(w.start_list.size());
     h->wl.has_eps = w.has_eps ? 1 : 0;
+    h->general_ok = g.ok;
+    h->general_why = g.ok ? std::string() : g.why;
+    if (g.ok) {
+        const WorklistHost &r = g.rows;
+        pk.add(&h->gen.row, r.row);
+        pk.add(&h->gen.edge, r.edge);
+        pk.add(&h->gen.bigidx, r.bigidx);
+        pk.add(&h->gen.bigoff, r.bigoff);
+        pk.add(&h->gen.accw, r.accw);
+        pk.add(&h->gen.start_list, r.start_list);
+        pk.add(&h->gen.start_bits, g.start_bits);
+        if (g.gather_ok) {
+            pk.add(&h->gen.boff, g.boff);
+            pk.add(&h->gen.bedge, g.bedge);
+            h->gen.nedges = static_cast<int32_t>(g.bedge.size());
+        }
+        h->gen.n = num_states;
+        h->gen.nwords = r.nwords;
+        h->gen.k0 = static_cast<int32_t>(r.start_list.size());
+    }
     h->chain_ok = t.ok;
     if (t.ok) {
         pk.add(&h->tile.mask, t.mask);
@@ -480,6 +583,9 @@ int gfsm_nfa_create(int32_t num_states, int32_t start_state, const uint8_t *acce
         validate_nfa(a);
         WorklistHost w = build_worklist(a);
         TileHost t = build_tiles(a);
+        GeneralHost g;
+        if (wants_general(num_states, t)) g = build_general(a);
+        else g.why = "not built for chain-structured automata above 512 states";
         auto *h = new gfsm_nfa();
         try {
             h->n = num_states;
@@ -489,7 +595,7 @@ int gfsm_nfa_create(int32_t num_states, int32_t start_state, const uint8_t *acce
             h->start_accepts = w.start_accepts;
             Packer pk;
             std::vector<uint16_t> s16;
-            pack_nfa(h, num_states, w, t, s16, pk);
+            pack_nfa(h, num_states, w, t, g, s16, pk);
             pk.commit(h->blocks, h->device_bytes);
         } catch (...) {
             free_blocks(h->blocks);
@@ -521,6 +627,7 @@ int gfsm_nfa_query(const gfsm_nfa *nfa, gfsm_nfa_props *out) {
         out->num_tiles = nfa->chain_ok ? nfa->tile.tiles : 0;
         out->start_always = nfa->chain_ok ? nfa->tile.start_always : 0;
         out->max_chain = nfa->max_chain;
+        out->general_ok = nfa->general_ok;
         out->device_bytes = nfa->device_bytes;
     });
 }
@@ -536,6 +643,8 @@ int gfsm_nfa_analyze(int32_t num_states, int32_t start_state, const uint8_t *acc
         validate_nfa(a);
         WorklistHost w = build_worklist(a);
         TileHost t = build_tiles(a);
+        GeneralHost g;
+        if (wants_general(num_states, t)) g = build_general(a);
         std::memset(out, 0, sizeof(*out));
         out->num_states = num_states;
         out->start_state = start_state;
@@ -546,10 +655,11 @@ int gfsm_nfa_analyze(int32_t num_states, int32_t start_state, const uint8_t *acc
         out->num_tiles = t.ok ? t.tiles : 0;
         out->start_always = t.ok ? t.start_always : 0;
         out->max_chain = t.max_chain;
+        out->general_ok = g.ok;
         gfsm_nfa tmp;
         Packer pk;
         std::vector<uint16_t> s16;
-        pack_nfa(&tmp, num_states, w, t, s16, pk);
+        pack_nfa(&tmp, num_states, w, t, g, s16, pk);
         const int64_t bytes = static_cast<int64_t>(pk.total());
         out->device_bytes = bytes;
     });
@@ -561,7 +671,7 @@ int gfsm_nfa_run_host(gfsm_nfa *nfa, int32_t technique, const uint8_t *data,
     return guarded([&] {
         if (!nfa) throw Error(GFSM_EINVAL, "null handle");
         check_batch_args(data, offsets, num_strings, flags, lens);
-        const int32_t tech = resolve_nfa(nfa, technique);
+        const int32_t tech = resolve_nfa(nfa, technique, num_strings);
         run_host(data, offsets, num_strings, flags, lens, kernel_ms,
                  [&](int64_t tot) { return nfa_workspace(nfa, tech, num_strings, tot, Tuning()); },
                  [&](const uint8_t *dd, const int64_t *doff, int64_t tot, int32_t *df,
@@ -579,7 +689,7 @@ int gfsm_nfa_run_device(gfsm_nfa *nfa, int32_t technique, const uint8_t *d_data,
         if (!nfa) throw Error(GFSM_EINVAL, "null handle");
         check_batch_args(d_data, d_offsets, num_strings, d_flags, d_lens);
         if (total_bytes < 0) throw Error(GFSM_EINVAL, "total_bytes must be >= 0");
-        const int32_t tech = resolve_nfa(nfa, technique);
+        const int32_t tech = resolve_nfa(nfa, technique, num_strings);
         nfa_enqueue(nfa, tech, Tuning(), d_data, d_offsets, num_strings, total_bytes, d_flags,
                     d_lens, nullptr, 0, static_cast<hipStream_t>(stream));
     });
@@ -593,7 +703,8 @@ int gfsm_nfa_workspace_size(gfsm_nfa *nfa, int32_t technique, int32_t num_string
         if (num_strings < 0 || total_bytes < 0) throw Error(GFSM_EINVAL, "negative size");
         const Tuning tu = read_tuning(tuning);
         *out_bytes = static_cast<int64_t>(
-            nfa_workspace(nfa, resolve_nfa(nfa, technique), num_strings, total_bytes, tu));
+            nfa_workspace(nfa, resolve_nfa(nfa, technique, num_strings), num_strings,
+                          total_bytes, tu));
     });
 }
 
@@ -608,7 +719,7 @@ int gfsm_nfa_run_device_ws(gfsm_nfa *nfa, int32_t technique, const gfsm_tuning *
         if (total_bytes < 0) throw Error(GFSM_EINVAL, "total_bytes must be >= 0");
         check_ws(d_workspace, workspace_bytes);
         const Tuning tu = read_tuning(tuning);
-        const int32_t tech = resolve_nfa(nfa, technique);
+        const int32_t tech = resolve_nfa(nfa, technique, num_strings);
         const size_t need = nfa_workspace(nfa, tech, num_strings, total_bytes, tu);
         if (static_cast<size_t>(workspace_bytes) < need)
             throw Error(GFSM_EINVAL, "workspace of " + std::to_string(workspace_bytes) +
@@ -619,13 +730,13 @@ int gfsm_nfa_run_device_ws(gfsm_nfa *nfa, int32_t technique, const gfsm_tuning *
     });
 }
 
-int gfsm_nfa_resolve(gfsm_nfa *nfa, int32_t technique, int32_t num_strings,
-                     int64_t total_bytes, int32_t *out_technique) {
+int gfsm_nfa_resolve(gfsm_nfa *nfa, int32_t technique, const gfsm_tuning *tuning,
+                     int32_t num_strings, int64_t total_bytes, int32_t *out_technique) {
     return guarded([&] {
         if (!nfa || !out_technique) throw Error(GFSM_EINVAL, "null argument");
-        (void)num_strings;
-        (void)total_bytes;
-        *out_technique = resolve_nfa(nfa, technique);
+        if (num_strings < 0 || total_bytes < 0) throw Error(GFSM_EINVAL, "negative size");
+        (void)read_tuning(tuning);
+        *out_technique = resolve_nfa(nfa, technique, num_strings);
     });
 }
 
@@ -692,12 +803,14 @@ int gfsm_dfa_run_host(gfsm_dfa *dfa, int32_t technique, const uint8_t *data,
         if (!dfa) throw Error(GFSM_EINVAL, "null handle");
         check_batch_args(data, offsets, num_strings, flags, lens);
         const int64_t total = num_strings > 0 ? offsets[num_strings] : 0;
-        const int32_t tech = resolve_dfa(dfa, technique, num_strings, total);
+        Tuning tu;
+        tu.max_len = longest_input(offsets, num_strings);
+        const int32_t tech = resolve_dfa(dfa, technique, num_strings, total, tu);
         run_host(data, offsets, num_strings, flags, lens, kernel_ms,
-                 [&](int64_t tot) { return dfa_workspace(dfa, tech, num_strings, tot, Tuning()); },
+                 [&](int64_t tot) { return dfa_workspace(dfa, tech, num_strings, tot, tu); },
                  [&](const uint8_t *dd, const int64_t *doff, int64_t tot, int32_t *df,
                      int32_t *dl, void *ws, size_t wsb, hipStream_t s) {
-                     dfa_enqueue(dfa, tech, Tuning(), dd, doff, num_strings, tot, df, dl, ws,
+                     dfa_enqueue(dfa, tech, tu, dd, doff, num_strings, tot, df, dl, ws,
                                  wsb, s);
                  });
     });
@@ -750,12 +863,13 @@ int gfsm_dfa_run_device_ws(gfsm_dfa *dfa, int32_t technique, const gfsm_tuning *
     });
 }
 
-int gfsm_dfa_resolve(gfsm_dfa *dfa, int32_t technique, int32_t num_strings,
-                     int64_t total_bytes, int32_t *out_technique) {
+int gfsm_dfa_resolve(gfsm_dfa *dfa, int32_t technique, const gfsm_tuning *tuning,
+                     int32_t num_strings, int64_t total_bytes, int32_t *out_technique) {
     return guarded([&] {
         if (!dfa || !out_technique) throw Error(GFSM_EINVAL, "null argument");
         if (num_strings < 0 || total_bytes < 0) throw Error(GFSM_EINVAL, "negative size");
-        *out_technique = resolve_dfa(dfa, technique, num_strings, total_bytes);
+        *out_technique =
+            resolve_dfa(dfa, technique, num_strings, total_bytes, read_tuning(tuning));
     });
 }
 

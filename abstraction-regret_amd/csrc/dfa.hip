@@ -17,6 +17,7 @@
 //           a 4096-state table is 2 MiB and stays in each XCD's 4 MB L2);
 //   lds     the same walk with the table staged in LDS (tables up to 160 KiB: 320 states);
 //   split   in-input parallel walk over fixed segments of the batch (see below).
+#include "device.hpp"
 #include "internal.hpp"
 
 #include <climits>

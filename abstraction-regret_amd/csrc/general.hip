@@ -1,0 +1,458 @@
+// NFA step kernels for general automata (any labels, wildcards, epsilon edges) on gfx950.
+//
+// Semantics: `src/gpufsm/reference.py:36-65` (simulate), latch-first-match. Every kernel
+// here runs the automaton with its epsilon closure folded into the symbol edges on the host
+// (layout.cpp, fold_epsilon): the set after byte i is the union of the folded successors of
+// the set before it, which is closure(successors) of the reference, and the start set is
+// closure(start). An accepting start closure never reaches these kernels; an empty set ends
+// the input with (0, 0).
+//
+// gather    one wave64 per input; the state set is a bitset in LDS, one 64-bit word per
+//           lane (lane j owns words j, j + 64, ...). A step on byte c walks the folded
+//           edges labelled c or ANY, stored byte-major as (source << 16 | target), 64 at a
+//           time: an edge whose source bit is set ORs its target bit into the next set
+//           (ds_or_b64, no return). The work per byte is the number of edges labelled c,
+//           whatever the size of the active set, so it is the technique for dense active
+//           sets. Accept and "set empty" are one 64-bit ballot each. When the lists and the
+//           accept words fit, the block stages them in LDS once. Upstream counterparts: the
+//           bit-packed kernels `simulate_one<NW>` / `bitpacked_multistream_kernel` /
+//           `bitpacked_multistream_shared_kernel` (`native/bitpacked.cu:12-166`), which scan
+//           every state's CSR row per byte on one thread (up to 512 states).
+//
+// worklist  one wave64 per input over the compacted list of active states. Each lane takes
+//           list entries, finds the state's edges on c (label-sorted rows; rows longer than
+//           16 edges carry a per-label index) and ORs the targets into an LDS bitmap
+//           (ds_or_b64, no return). The bitmap is then compacted into the next list: every
+//           lane owns a run of words, counts their bits, a wave prefix sum gives its offset,
+//           and it writes its state ids there and clears the words, so the bitmap is zero
+//           again for the next byte. The accept test is fused into the compaction. Upstream
+//           counterparts: `worklist_warp_kernel`, `worklist_shared_kernel` and
+//           `worklist_compact_kernel` (`native/worklist_global.cu:128-342`).
+//
+// register  one lane per input, the state set in registers (NW 64-bit words, up to 512
+//           states). The lane walks the set bits of its set and the label-sorted rows of
+//           those states. Upstream counterpart: `worklist_multistream_kernel<NW>`
+//           (`native/worklist_register.cu:16-128`).
+#include "internal.hpp"
+
+namespace gfsm {
+
+namespace {
+
+constexpr int kGenWaves = 4;  // waves per block (gather, worklist)
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Bytes [a, a + 4) of the batch as one little-endian word (a 4-aligned); bytes outside
+// [0, data_end) read as zero.
+__device__ __forceinline__ uint32_t gen_load_word(const uint8_t *__restrict__ data, int64_t a,
+                                                  int64_t data_end) {
+    if (a >= 0 && a + 4 <= data_end) return *reinterpret_cast<const uint32_t *>(data + a);
+    uint32_t w = 0;
+    for (int q = 0; q < 4; ++q)
+        if (a + q >= 0 && a + q < data_end) w |= static_cast<uint32_t>(data[a + q]) << (8 * q);
+    return w;
+}
+
+// Byte r (0..255) of a 256-byte window whose 4-byte words are spread over the lanes.
+__device__ __forceinline__ uint32_t gen_window_byte(uint32_t word, int32_t r) {
+    return (static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(word), r >> 2)) >>
+            ((r & 3) << 3)) & 0xffu;
+}
+
+// Inclusive prefix sum over the 64 lanes.
+__device__ __forceinline__ int32_t wave_inclusive_sum(int32_t x, int lane) {
+#pragma unroll
+    for (int dlt = 1; dlt < 64; dlt <<= 1) {
+        const int32_t y = __shfl_up(x, dlt, 64);
+        if (lane >= dlt) x += y;
+    }
+    return x;
+}
+
+// The wave's input bytes: a 256-byte window per 64 lanes, the next window in flight.
+struct Window {
+    int64_t wstart;   // absolute position of the window's first byte (4-aligned)
+    uint32_t word, nword;
+};
+
+__device__ __forceinline__ void window_open(Window &w, const uint8_t *data, int64_t base,
+                                            int64_t data_end, int lane) {
+    w.wstart = base & ~static_cast<int64_t>(3);
+    w.word = gen_load_word(data, w.wstart + 4 * lane, data_end);
+    w.nword = gen_load_word(data, w.wstart + 256 + 4 * lane, data_end);
+}
+
+// Byte at absolute position `at` (>= the window start, wave-uniform and increasing).
+__device__ __forceinline__ uint32_t window_at(Window &w, const uint8_t *data, int64_t at,
+                                              int64_t data_end, int lane) {
+    while (at >= w.wstart + 256) {
+        w.wstart += 256;
+        w.word = w.nword;
+        w.nword = gen_load_word(data, w.wstart + 256 + 4 * lane, data_end);
+    }
+    return gen_window_byte(w.word, static_cast<int32_t>(at - w.wstart));
+}
+
+// Calls f(t) for every folded edge of state s labelled b or ANY.
+template <typename F>
+__device__ __forceinline__ void for_edges(const NfaGeneralDev &d, int32_t s, uint32_t b, F &&f) {
+    const int32_t bi = d.bigidx[s];
+    if (bi >= 0) {
+        const int32_t *bo = d.bigoff + static_cast<int64_t>(bi) * kBigStride;
+        for (int32_t e = bo[b], ee = bo[b + 1]; e < ee; ++e) f(d.edge[e] >> kLabelBits);
+        for (int32_t e = bo[kAnySymbol], ee = bo[kAnySymbol + 1]; e < ee; ++e)
+            f(d.edge[e] >> kLabelBits);
+    } else {
+        for (int32_t e = d.row[s], ee = d.row[s + 1]; e < ee; ++e) {
+            const uint32_t v = d.edge[e];
+            const uint32_t lab = v & ((1u << kLabelBits) - 1u);
+            if (lab == b || lab == static_cast<uint32_t>(kAnySymbol)) f(v >> kLabelBits);
+        }
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- gather (bit-parallel) --
+
+// LDS of a gather block: [staged: boff (258 int32, padded to 8-byte words) | accw | bedge]
+// then two bitsets of nwords per wave.
+__host__ __device__ static int64_t gather_stage_words(const NfaGeneralDev &d) {
+    return (258 + 1) / 2 + d.nwords + (static_cast<int64_t>(d.nedges) + 1) / 2;
+}
+
+__global__ __launch_bounds__(64 * kGenWaves) void nfa_gather_kernel(
+    NfaGeneralDev d, const uint8_t *__restrict__ data, const int64_t *__restrict__ off,
+    int32_t count, int32_t *flags, int32_t *lens, int32_t staged) {
+    extern __shared__ uint64_t lds[];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int32_t nwords = d.nwords;
+    const int32_t *boff = d.boff;
+    const uint32_t *bedge = d.bedge;
+    const uint64_t *accw = d.accw;
+    int64_t stage = 0;
+    if (staged) {
+        int32_t *sb = reinterpret_cast<int32_t *>(lds);
+        uint64_t *sa = lds + (258 + 1) / 2;
+        uint32_t *se = reinterpret_cast<uint32_t *>(sa + nwords);
+        for (int32_t j = threadIdx.x; j < 258; j += blockDim.x) sb[j] = d.boff[j];
+        for (int32_t j = threadIdx.x; j < nwords; j += blockDim.x) sa[j] = d.accw[j];
+        for (int32_t j = threadIdx.x; j < d.nedges; j += blockDim.x) se[j] = d.bedge[j];
+        __syncthreads();
+        boff = sb;
+        accw = sa;
+        bedge = se;
+        stage = gather_stage_words(d);
+    }
+    uint64_t *cur = lds + stage + static_cast<int64_t>(wave) * 2 * nwords;
+    uint64_t *nxt = cur + nwords;
+    const int64_t data_end = off[count];
+    const int32_t any0 = boff[kAnySymbol], any1 = boff[kAnySymbol + 1];
+
+    for (int32_t in = blockIdx.x * kGenWaves + wave; in < count; in += gridDim.x * kGenWaves) {
+        const int64_t base = off[in];
+        const int64_t len = off[in + 1] - base;
+        for (int32_t w = lane; w < nwords; w += 64) cur[w] = d.start_bits[w];
+        wave_sync();
+        int32_t res = 0;
+        Window win;
+        window_open(win, data, base, data_end, lane);
+        for (int64_t pos = 0; pos < len; ++pos) {
+            const uint32_t c = window_at(win, data, base + pos, data_end, lane);
+            for (int32_t w = lane; w < nwords; w += 64) nxt[w] = 0;
+            wave_sync();
+            const int32_t e0 = boff[c];
+            const int32_t nc = boff[c + 1] - e0;
+            const int32_t total = nc + (any1 - any0);
+            for (int32_t j = lane; j < total; j += 64) {
+                const uint32_t v = bedge[j < nc ? e0 + j : any0 + (j - nc)];
+                const uint32_t s = v >> 16, t = v & 0xffffu;
+                if ((cur[s >> 6] >> (s & 63)) & 1ull)
+                    atomicOr(reinterpret_cast<unsigned long long *>(&nxt[t >> 6]),
+                             1ull << (t & 63));
+            }
+            wave_sync();
+            uint64_t any = 0, acc = 0;
+            for (int32_t w = lane; w < nwords; w += 64) {
+                const uint64_t x = nxt[w];
+                any |= x;
+                acc |= x & accw[w];
+            }
+            uint64_t *t = cur;
+            cur = nxt;
+            nxt = t;
+            if (__any(acc != 0ull)) {
+                res = static_cast<int32_t>(pos + 1);
+                break;
+            }
+            if (!__any(any != 0ull)) break;
+        }
+        if (lane == 0) {
+            flags[in] = res > 0;
+            lens[in] = res;
+        }
+        wave_sync();  // every lane is done with cur/nxt before the next input's start set
+    }
+}
+
+static constexpr size_t kGatherStageBudget = 96 * 1024;
+static constexpr size_t kGatherLdsMax = 160 * 1024;
+
+bool nfa_gather_fits(const NfaGeneralDev &d) {
+    return d.boff != nullptr &&
+           static_cast<size_t>(kGenWaves) * 2 * d.nwords * 8 <= kGatherLdsMax;
+}
+
+void launch_nfa_gather(const NfaGeneralDev &d, const uint8_t *data, const int64_t *off,
+                       int32_t count, int32_t *flags, int32_t *lens, hipStream_t stream) {
+    if (count <= 0) return;
+    if (!nfa_gather_fits(d))
+        throw Error(3, "bitpacked (general): the bitsets of " + std::to_string(d.n) +
+                           " states do not fit LDS");
+    const size_t sets = static_cast<size_t>(kGenWaves) * 2 * d.nwords * 8;
+    const size_t stage = static_cast<size_t>(gather_stage_words(d)) * 8;
+    const bool staged = stage <= kGatherStageBudget && stage + sets <= kGatherLdsMax;
+    const size_t bytes = sets + (staged ? stage : 0);
+    ensure_dynamic_lds(reinterpret_cast<const void *>(&nfa_gather_kernel), bytes);
+    int32_t grid = (count + kGenWaves - 1) / kGenWaves;
+    if (grid > 65536) grid = 65536;  // waves loop over the inputs beyond that
+    nfa_gather_kernel<<<grid, 64 * kGenWaves, bytes, stream>>>(d, data, off, count, flags, lens,
+                                                               staged ? 1 : 0);
+    GFSM_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- worklist (ballot) ------
+
+// LDS of a worklist block: per wave the bitmap (nwords words), then, when they fit, the two
+// lists (n int32 each).
+static size_t wl2_wave_words(const NfaGeneralDev &d, bool lds_lists) {
+    return static_cast<size_t>(d.nwords) + (lds_lists ? static_cast<size_t>(d.n) : 0);
+}
+
+__global__ __launch_bounds__(64 * kGenWaves) void nfa_wl2_kernel(
+    NfaGeneralDev d, const uint8_t *__restrict__ data, const int64_t *__restrict__ off,
+    int32_t count, int32_t *flags, int32_t *lens, int32_t *scratch, int32_t lds_lists) {
+    extern __shared__ uint64_t lds[];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int32_t nwords = d.nwords, n = d.n;
+    const size_t per_wave = static_cast<size_t>(nwords) + (lds_lists ? static_cast<size_t>(n) : 0);
+    uint64_t *bits = lds + static_cast<size_t>(wave) * per_wave;
+    int32_t *la = lds_lists ? reinterpret_cast<int32_t *>(bits + nwords)
+                            : scratch + (static_cast<int64_t>(blockIdx.x) * kGenWaves + wave) *
+                                            2 * static_cast<int64_t>(n);
+    int32_t *lb = la + n;
+    for (int32_t w = lane; w < nwords; w += 64) bits[w] = 0;
+    // each lane compacts a contiguous run of words
+    const int32_t wpl = (nwords + 63) >> 6;
+    const int32_t w0 = lane * wpl;
+    const int32_t w1 = (w0 + wpl) < nwords ? (w0 + wpl) : nwords;
+    const int64_t data_end = off[count];
+    wave_sync();
+
+    for (int32_t in = blockIdx.x * kGenWaves + wave; in < count; in += gridDim.x * kGenWaves) {
+        const int64_t base = off[in];
+        const int64_t len = off[in + 1] - base;
+        int32_t *cur = la, *nxt = lb;
+        for (int32_t j = lane; j < d.k0; j += 64) cur[j] = d.start_list[j];
+        int32_t k = d.k0;
+        int32_t res = 0;
+        Window win;
+        window_open(win, data, base, data_end, lane);
+        wave_sync();
+        for (int64_t pos = 0; pos < len; ++pos) {
+            const uint32_t c = window_at(win, data, base + pos, data_end, lane);
+            for (int32_t j = lane; j < k; j += 64) {
+                for_edges(d, cur[j], c, [&](uint32_t t) {
+                    atomicOr(reinterpret_cast<unsigned long long *>(&bits[t >> 6]),
+                             1ull << (t & 63));
+                });
+            }
+            wave_sync();
+            int32_t cnt = 0;
+            uint64_t acc = 0;
+            for (int32_t w = w0; w < w1; ++w) {
+                const uint64_t x = bits[w];
+                cnt += __popcll(x);
+                acc |= x & d.accw[w];
+            }
+            const int32_t incl = wave_inclusive_sum(cnt, lane);
+            const int32_t total = __builtin_amdgcn_readlane(incl, 63);
+            int32_t at = incl - cnt;
+            for (int32_t w = w0; w < w1; ++w) {
+                uint64_t x = bits[w];
+                if (!x) continue;
+                bits[w] = 0;
+                while (x) {
+                    nxt[at++] = (w << 6) + __builtin_ctzll(x);
+                    x &= x - 1;
+                }
+            }
+            wave_sync();
+            if (__any(acc != 0ull)) {
+                res = static_cast<int32_t>(pos + 1);
+                break;
+            }
+            if (total == 0) break;
+            int32_t *t = cur;
+            cur = nxt;
+            nxt = t;
+            k = total;
+        }
+        if (lane == 0) {
+            flags[in] = res > 0;
+            lens[in] = res;
+        }
+        wave_sync();
+    }
+}
+
+static constexpr size_t kWl2LdsBudget = 64 * 1024;           // per block
+static constexpr int64_t kWl2ScratchBudget = 256ll << 20;    // bytes of global lists
+
+void nfa_wl2_geometry(const NfaGeneralDev &d, int32_t count, int32_t *grid,
+                      int64_t *scratch_elems) {
+    const bool lds_lists = wl2_wave_words(d, true) * 8 * kGenWaves <= kWl2LdsBudget;
+    int64_t g = (static_cast<int64_t>(count) + kGenWaves - 1) / kGenWaves;
+    if (!lds_lists) {
+        const int64_t per_block = static_cast<int64_t>(d.n) * 2 * kGenWaves;
+        int64_t cap = kWl2ScratchBudget / (per_block * 4);
+        if (cap < 64) cap = 64;
+        if (g > cap) g = cap;
+        *scratch_elems = g * per_block;
+    } else {
+        if (g > 65536) g = 65536;
+        *scratch_elems = 0;
+    }
+    if (g < 1) g = 1;
+    *grid = static_cast<int32_t>(g);
+}
+
+void launch_nfa_wl2(const NfaGeneralDev &d, const uint8_t *data, const int64_t *off,
+                    int32_t count, int32_t *flags, int32_t *lens, int32_t *scratch,
+                    int32_t grid, hipStream_t stream) {
+    if (count <= 0) return;
+    const bool lds_lists = scratch == nullptr;
+    const size_t bytes = wl2_wave_words(d, lds_lists) * 8 * kGenWaves;
+    if (bytes > kGatherLdsMax)
+        throw Error(3, "worklist: the bitmap of " + std::to_string(d.n) +
+                           " states does not fit LDS");
+    ensure_dynamic_lds(reinterpret_cast<const void *>(&nfa_wl2_kernel), bytes);
+    nfa_wl2_kernel<<<grid, 64 * kGenWaves, bytes, stream>>>(d, data, off, count, flags, lens,
+                                                            scratch, lds_lists ? 1 : 0);
+    GFSM_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- register (lane/input) --
+
+constexpr int kRegThreads = 256;
+
+template <int NW>
+__device__ __forceinline__ void reg_set(uint64_t (&x)[NW], uint32_t t) {
+    const uint32_t w = t >> 6;
+    const uint64_t bit = 1ull << (t & 63);
+#pragma unroll
+    for (int q = 0; q < NW; ++q) x[q] |= (w == static_cast<uint32_t>(q)) ? bit : 0ull;
+}
+
+template <int NW>
+__global__ __launch_bounds__(kRegThreads) void nfa_reg_kernel(
+    NfaGeneralDev d, const uint8_t *__restrict__ data, const int64_t *__restrict__ off,
+    int32_t count, int32_t *flags, int32_t *lens) {
+    uint64_t start[NW], accw[NW];
+#pragma unroll
+    for (int q = 0; q < NW; ++q) {
+        start[q] = q < d.nwords ? d.start_bits[q] : 0ull;
+        accw[q] = q < d.nwords ? d.accw[q] : 0ull;
+    }
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += gridDim.x * blockDim.x) {
+        const int64_t base = off[i];
+        const int64_t len = off[i + 1] - base;
+        const uint8_t *p = data + base;
+        uint64_t cur[NW];
+#pragma unroll
+        for (int q = 0; q < NW; ++q) cur[q] = start[q];
+        int32_t res = 0;
+        // bytes until the lane's position is 4-aligned, then aligned 4-byte words, then the
+        // tail bytewise
+        const int64_t lead = (4 - (base & 3)) & 3;
+        const int64_t head = lead < len ? lead : len;
+        int64_t pos = 0;
+        bool stop = false;
+        while (pos < len && !stop) {
+            uint32_t word;
+            int nb;
+            if (pos >= head && pos + 4 <= len) {
+                word = *reinterpret_cast<const uint32_t *>(p + pos);
+                nb = 4;
+            } else {
+                word = p[pos];
+                nb = 1;
+            }
+            for (int q = 0; q < nb && !stop; ++q, ++pos) {
+                const uint32_t c = (word >> (8 * q)) & 0xffu;
+                uint64_t nxt[NW];
+#pragma unroll
+                for (int w = 0; w < NW; ++w) nxt[w] = 0ull;
+#pragma unroll
+                for (int w = 0; w < NW; ++w) {
+                    uint64_t b = cur[w];
+                    while (b) {
+                        const int32_t st = w * 64 + __builtin_ctzll(b);
+                        b &= b - 1;
+                        for_edges(d, st, c, [&](uint32_t t) { reg_set<NW>(nxt, t); });
+                    }
+                }
+                uint64_t any = 0, acc = 0;
+#pragma unroll
+                for (int w = 0; w < NW; ++w) {
+                    any |= nxt[w];
+                    acc |= nxt[w] & accw[w];
+                    cur[w] = nxt[w];
+                }
+                if (acc) {
+                    res = static_cast<int32_t>(pos + 1);
+                    stop = true;
+                } else if (!any) {
+                    stop = true;
+                }
+            }
+        }
+        flags[i] = res > 0;
+        lens[i] = res;
+    }
+}
+
+void launch_nfa_register(const NfaGeneralDev &d, const uint8_t *data, const int64_t *off,
+                         int32_t count, int32_t *flags, int32_t *lens, hipStream_t stream) {
+    if (count <= 0) return;
+    const int32_t nw = d.nwords;
+    if (nw > kRegMaxWords)
+        throw Error(3, "register needs at most " + std::to_string(kRegMaxWords * 64) +
+                           " states, the automaton has " + std::to_string(d.n));
+    int32_t grid = (count + kRegThreads - 1) / kRegThreads;
+    if (grid > 65536) grid = 65536;
+#define GFSM_REG(NW) \
+    nfa_reg_kernel<NW><<<grid, kRegThreads, 0, stream>>>(d, data, off, count, flags, lens)
+    switch (nw) {
+        case 1: GFSM_REG(1); break;
+        case 2: GFSM_REG(2); break;
+        case 3: GFSM_REG(3); break;
+        case 4: GFSM_REG(4); break;
+        case 5: GFSM_REG(5); break;
+        case 6: GFSM_REG(6); break;
+        case 7: GFSM_REG(7); break;
+        default: GFSM_REG(8); break;
+    }
+#undef GFSM_REG
+    GFSM_HIP(hipGetLastError());
+}
+
+}  // namespace gfsm

@@ -30,29 +30,6 @@ struct Error : std::runtime_error {
     Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
 };
 
-// Block-wide copy of `vecs` 16-byte vectors from global memory into LDS (no barrier). Each
-// thread keeps 8 loads in flight before its LDS writes: a loop that waits for every load in
-// turn pays one memory round trip per vector per thread. Loads and writes are unconditional:
-// indexes past the end are clamped to the last vector, which is then written again with its
-// own value (a guarded load gets sunk into its branch and waited for there). vecs >= 1.
-__device__ __forceinline__ void copy_to_lds(const uint4 *__restrict__ src, uint4 *dst,
-                                            uint32_t vecs) {
-    constexpr int kBatch = 8;
-    for (uint32_t k0 = threadIdx.x; k0 < vecs; k0 += kBatch * blockDim.x) {
-        uint4 v[kBatch];
-#pragma unroll
-        for (int j = 0; j < kBatch; ++j) {
-            const uint32_t k = k0 + j * blockDim.x;
-            v[j] = src[k < vecs ? k : vecs - 1];
-        }
-#pragma unroll
-        for (int j = 0; j < kBatch; ++j) {
-            const uint32_t k = k0 + j * blockDim.x;
-            dst[k < vecs ? k : vecs - 1] = v[j];
-        }
-    }
-}
-
 #define GFSM_HIP(call)                                                                    \
     do {                                                                                  \
         hipError_t err_ = (call);                                                         \
@@ -77,6 +54,24 @@ struct NfaWorklistDev {
     const int32_t *start_list; // [k0] start closure
     int32_t n, nwords, k0, has_eps;
 };
+
+// General layout (any NFA, epsilon closure folded on the host; see layout.hpp): label-sorted
+// folded rows with the worklist's per-label index, the closure of the start state, and the
+// byte-major gather lists of the bit-parallel step (null above 65536 states).
+struct NfaGeneralDev {
+    const int32_t *row;        // [n + 1]
+    const uint32_t *edge;      // [nnz'] target << 9 | label, label-sorted per row
+    const int32_t *bigidx;     // [n]
+    const int32_t *bigoff;     // [nbig * 258]
+    const uint64_t *accw;      // [nwords]
+    const int32_t *start_list; // [k0] closure(start)
+    const uint64_t *start_bits;// [nwords] closure(start)
+    const int32_t *boff;       // [258] or null: edges labelled c at [boff[c], boff[c + 1]),
+                               // ANY edges at [boff[256], boff[257])
+    const uint32_t *bedge;     // [nedges] source << 16 | target
+    int32_t n, nwords, k0, nedges;
+};
+constexpr int kRegMaxWords = 8;  // register technique: at most 512 states
 
 // Bitpacked chain layout: states renumbered into tiles of 4096 so that every chain is
 // contiguous inside one tile and the successor of bit i is bit i + 1.
@@ -160,6 +155,18 @@ void ensure_dynamic_lds(const void *kernel, size_t bytes);
 int persistent_blocks(const void *kernel, int threads);
 // Positive integer from the environment (tuning overrides), else fallback.
 int64_t env_int(const char *name, int64_t fallback);
+
+// General techniques (general.hip).
+bool nfa_gather_fits(const NfaGeneralDev &d);
+void launch_nfa_gather(const NfaGeneralDev &d, const uint8_t *data, const int64_t *off,
+                       int32_t count, int32_t *flags, int32_t *lens, hipStream_t stream);
+void nfa_wl2_geometry(const NfaGeneralDev &d, int32_t count, int32_t *grid,
+                      int64_t *scratch_elems);
+void launch_nfa_wl2(const NfaGeneralDev &d, const uint8_t *data, const int64_t *off,
+                    int32_t count, int32_t *flags, int32_t *lens, int32_t *scratch,
+                    int32_t grid, hipStream_t stream);
+void launch_nfa_register(const NfaGeneralDev &d, const uint8_t *data, const int64_t *off,
+                         int32_t count, int32_t *flags, int32_t *lens, hipStream_t stream);
 
 void launch_nfa_worklist(const NfaWorklistDev &d, const uint8_t *data, const int64_t *off,
                          int32_t count, int32_t *flags, int32_t *lens, int32_t *scratch,

@@ -323,6 +323,118 @@ TileHost build_tiles(const NfaInput &a) {
     return h;
 }
 
+FoldedHost fold_epsilon(const NfaInput &a, int64_t cap) {
+    FoldedHost h;
+    const int32_t n = a.n;
+    const int32_t nnz = a.srp[n], nne = a.erp[n];
+    if (cap <= 0) cap = std::max<int64_t>(4ll * nnz, int64_t(1) << 22);
+    // closure(t) for every state, depth-first over epsilon edges (seeds included); the
+    // total of the closure sizes is bounded by the cap too
+    std::vector<int64_t> coff(static_cast<size_t>(n) + 1, 0);
+    std::vector<int32_t> clos;
+    std::vector<int32_t> stamp(static_cast<size_t>(n), -1), stack;
+    for (int32_t t = 0; t < n; ++t) {
+        coff[t] = static_cast<int64_t>(clos.size());
+        clos.push_back(t);
+        if (nne > 0) {
+            stamp[t] = t;
+            stack.assign(1, t);
+            while (!stack.empty()) {
+                const int32_t s = stack.back();
+                stack.pop_back();
+                for (int32_t k = a.erp[s]; k < a.erp[s + 1]; ++k) {
+                    const int32_t u = a.et[k];
+                    if (stamp[u] != t) {
+                        stamp[u] = t;
+                        clos.push_back(u);
+                        stack.push_back(u);
+                    }
+                }
+            }
+        }
+        if (static_cast<int64_t>(clos.size()) > cap + n) {
+            h.why = "epsilon closures exceed " + std::to_string(cap) + " states in total";
+            return h;
+        }
+    }
+    coff[n] = static_cast<int64_t>(clos.size());
+    int64_t bound = 0;
+    for (int32_t k = 0; k < nnz; ++k) bound += coff[a.st[k] + 1] - coff[a.st[k]];
+    if (bound > cap) {
+        h.why = "folding the epsilon closure gives up to " + std::to_string(bound) +
+                " edges (cap " + std::to_string(cap) + ")";
+        return h;
+    }
+    // folded rows: for each (s, label) the union of the closures of its targets, each
+    // target once (first occurrence order)
+    h.srp.assign(static_cast<size_t>(n) + 1, 0);
+    h.st.reserve(static_cast<size_t>(bound));
+    h.ss.reserve(static_cast<size_t>(bound));
+    // The row's edges are visited in label order, so all edges of one (s, label) pair are
+    // consecutive and one stamp per target (key s * 257 + label) drops every duplicate.
+    std::vector<int64_t> seen(static_cast<size_t>(n), -1);
+    std::vector<int32_t> order;
+    for (int32_t s = 0; s < n; ++s) {
+        order.resize(static_cast<size_t>(a.srp[s + 1] - a.srp[s]));
+        std::iota(order.begin(), order.end(), a.srp[s]);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int32_t x, int32_t y) { return a.ss[x] < a.ss[y]; });
+        for (int32_t k : order) {
+            const int32_t lab = a.ss[k];
+            const int64_t key = static_cast<int64_t>(s) * (kAnySymbol + 1) + lab;
+            const int32_t t = a.st[k];
+            for (int64_t q = coff[t]; q < coff[t + 1]; ++q) {
+                const int32_t u = clos[q];
+                if (seen[u] == key) continue;
+                seen[u] = key;
+                h.st.push_back(u);
+                h.ss.push_back(lab);
+            }
+        }
+        h.srp[s + 1] = static_cast<int32_t>(h.st.size());
+    }
+    for (int64_t q = coff[a.start]; q < coff[a.start + 1]; ++q) {
+        h.start_set.push_back(clos[q]);
+        if (a.accept[clos[q]]) h.start_accepts = true;
+    }
+    h.ok = true;
+    return h;
+}
+
+GeneralHost build_general(const NfaInput &a) {
+    GeneralHost g;
+    FoldedHost f = fold_epsilon(a);
+    if (!f.ok) {
+        g.why = f.why;
+        return g;
+    }
+    const int32_t n = a.n;
+    std::vector<int32_t> no_eps(static_cast<size_t>(n) + 1, 0);
+    NfaInput b{n, a.start, a.accept, f.srp.data(), f.st.data(), f.ss.data(), no_eps.data(),
+               nullptr};
+    g.rows = build_worklist(b);
+    g.rows.start_list = f.start_set;
+    g.rows.start_accepts = f.start_accepts;
+    g.start_bits.assign(static_cast<size_t>(g.rows.nwords), 0ull);
+    for (int32_t s : f.start_set) g.start_bits[s >> 6] |= 1ull << (s & 63);
+    if (n <= 65536) {
+        // counting sort of the folded edges by label (ANY last)
+        const int32_t m = static_cast<int32_t>(f.st.size());
+        g.boff.assign(kAnySymbol + 2, 0);
+        for (int32_t k = 0; k < m; ++k) g.boff[f.ss[k] + 1]++;
+        for (int c = 0; c <= kAnySymbol; ++c) g.boff[c + 1] += g.boff[c];
+        std::vector<int32_t> cur(g.boff.begin(), g.boff.end() - 1);
+        g.bedge.resize(static_cast<size_t>(m));
+        for (int32_t s = 0; s < n; ++s)
+            for (int32_t k = f.srp[s]; k < f.srp[s + 1]; ++k)
+                g.bedge[cur[f.ss[k]]++] = (static_cast<uint32_t>(s) << 16) |
+                                          static_cast<uint32_t>(f.st[k]);
+        g.gather_ok = true;
+    }
+    g.ok = true;
+    return g;
+}
+
 DfaHost build_dfa(int32_t n, int32_t start, const uint8_t *accept, const int32_t *trans) {
     if (n < 1) bad("num_states must be >= 1");
     if (start < 0 || start >= n) bad("start_state out of range");

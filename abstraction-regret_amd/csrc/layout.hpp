@@ -64,6 +64,39 @@ void validate_nfa(const NfaInput &a);
 WorklistHost build_worklist(const NfaInput &a);
 TileHost build_tiles(const NfaInput &a);
 
+// The same automaton with its epsilon closure folded into the symbol edges: an edge
+// (s, c, t) becomes (s, c, u) for every u in closure(t), duplicates dropped, and the start
+// set is closure(start). Simulating it without any epsilon pass gives the verdicts of
+// `reference.py:36-65` at every byte: closure(union of targets) is the union of the
+// targets' closures, and a successor set is empty exactly when its closure is.
+struct FoldedHost {
+    bool ok = false;                    // false: the folded edges would exceed the cap
+    std::string why;
+    std::vector<int32_t> srp, st, ss;   // CSR of the folded symbol edges (no epsilon edges)
+    std::vector<int32_t> start_set;     // closure(start), start first
+    bool start_accepts = false;
+};
+// cap: most folded edges accepted (0 = max(4 * nnz, 2^22)).
+FoldedHost fold_epsilon(const NfaInput &a, int64_t cap = 0);
+
+// Device layout of the general techniques (any NFA), built from the folded automaton.
+//  * rows: label-sorted folded rows with the worklist's per-label index (WorklistHost of
+//    the folded automaton; start_list = closure(start));
+//  * byte-major gather lists (n <= 65536): the folded edges labelled c as
+//    (source << 16 | target) at [boff[c], boff[c + 1]), the ANY edges at
+//    [boff[256], boff[257]);
+//  * start_bits: closure(start) as a bitset.
+struct GeneralHost {
+    bool ok = false;                    // folded (and the rows built)
+    std::string why;
+    WorklistHost rows;
+    bool gather_ok = false;             // n <= 65536: the byte-major lists exist
+    std::vector<int32_t> boff;          // [258]
+    std::vector<uint32_t> bedge;
+    std::vector<uint64_t> start_bits;   // [nwords]
+};
+GeneralHost build_general(const NfaInput &a);
+
 // DFA device layout. States are renumbered so that the accepting ones come last; the
 // table is stored by byte (column c holds the successor of every state on byte c) and each
 // entry is the successor's byte offset inside a column (state * entry size). A step is then

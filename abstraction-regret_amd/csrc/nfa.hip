@@ -25,6 +25,7 @@
 //             position over its tiles (global atomicMin, also used to stop tiles early).
 //             Upstream counterpart: the bit-packed multistream kernels
 //             (`native/bitpacked.cu:12-166`), which OR per-state masks word by word.
+#include "device.hpp"
 #include "internal.hpp"
 
 #include <climits>

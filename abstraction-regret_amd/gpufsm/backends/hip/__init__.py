@@ -5,7 +5,8 @@ Registers ``Backend.HIP`` techniques with the unchanged registry
 (`src/gpufsm/backends/cuda/__init__.py:19-28`):
 
 * NFA: ``auto`` (default; ``chain`` when the automaton is chain-structured, else
-  ``worklist``), ``worklist``, ``bitpacked``, ``chain``;
+  ``register`` for at most 512 states over batches of 1024 inputs or more, else
+  ``worklist``), ``worklist``, ``bitpacked``, ``chain``, ``register``;
 * DFA: ``auto`` (default; ``split`` when inputs are long, else ``lds`` for tables of at
   most 320 states, else ``gather``), ``gather``, ``lds``, ``split``.
 

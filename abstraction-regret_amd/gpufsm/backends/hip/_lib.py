@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 
 # status codes and technique ids (gpufsm_hip.h)
 OK, EINVAL, ENOMEM, EUNSUPPORTED, EDEVICE = 0, 1, 2, 3, 4
-NFA_AUTO, NFA_WORKLIST, NFA_BITPACKED, NFA_CHAIN = 0, 1, 2, 3
+NFA_AUTO, NFA_WORKLIST, NFA_BITPACKED, NFA_CHAIN, NFA_REGISTER = 0, 1, 2, 3, 4
 DFA_AUTO, DFA_GATHER, DFA_LDS, DFA_SPLIT = 0, 1, 2, 3
 
 NFA_TECHNIQUES = {
@@ -24,6 +24,7 @@ NFA_TECHNIQUES = {
     "worklist": NFA_WORKLIST,
     "bitpacked": NFA_BITPACKED,
     "chain": NFA_CHAIN,
+    "register": NFA_REGISTER,
 }
 DFA_TECHNIQUES = {"auto": DFA_AUTO, "gather": DFA_GATHER, "lds": DFA_LDS, "split": DFA_SPLIT}
 
@@ -78,7 +79,7 @@ class NfaProps(ctypes.Structure):
         ("num_tiles", ctypes.c_int32),
         ("start_always", ctypes.c_int32),
         ("max_chain", ctypes.c_int32),
-        ("pad", ctypes.c_int32),
+        ("general_ok", ctypes.c_int32),
         ("device_bytes", ctypes.c_int64),
     ]
 
@@ -102,7 +103,7 @@ class Tuning(ctypes.Structure):
         ("chunk_bytes", ctypes.c_int32),
         ("segment_bytes", ctypes.c_int32),
         ("flags", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("max_input_bytes", ctypes.c_int32),
     ]
 
 
@@ -135,12 +136,12 @@ _SIGNATURES = {
                                            _i64p]),
     "gfsm_nfa_run_device_ws": (_c.c_int, [_vp, _i32, _c.POINTER(Tuning), _vp, _vp, _i32,
                                           _c.c_int64, _vp, _vp, _vp, _c.c_int64, _vp]),
-    "gfsm_nfa_resolve": (_c.c_int, [_vp, _i32, _i32, _c.c_int64, _i32p]),
+    "gfsm_nfa_resolve": (_c.c_int, [_vp, _i32, _c.POINTER(Tuning), _i32, _c.c_int64, _i32p]),
     "gfsm_dfa_workspace_size": (_c.c_int, [_vp, _i32, _i32, _c.c_int64, _c.POINTER(Tuning),
                                            _i64p]),
     "gfsm_dfa_run_device_ws": (_c.c_int, [_vp, _i32, _c.POINTER(Tuning), _vp, _vp, _i32,
                                           _c.c_int64, _vp, _vp, _vp, _c.c_int64, _vp]),
-    "gfsm_dfa_resolve": (_c.c_int, [_vp, _i32, _i32, _c.c_int64, _i32p]),
+    "gfsm_dfa_resolve": (_c.c_int, [_vp, _i32, _c.POINTER(Tuning), _i32, _c.c_int64, _i32p]),
 }
 
 _lib: ctypes.CDLL | None = None

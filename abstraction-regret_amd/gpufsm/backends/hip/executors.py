@@ -27,12 +27,17 @@ class HIPNFAExecutor:
         self.technique = technique
         self._tech = _lib.NFA_TECHNIQUES[technique]
         self._dev = DeviceNFA(nfa)
-        if technique in ("bitpacked", "chain") and not self._dev.props.chain_ok:
-            self._dev.close()
-            raise ValueError(
-                f"HIP technique {technique!r} needs a chain-structured NFA (pattern union: no "
-                "epsilon edges, one predecessor and successor per non-start state, "
-                "homogeneous labels); use 'worklist' or 'auto'")
+        props = self._dev.props
+        if technique != "auto":
+            # unsupported sizes are ValueError at construction, as upstream's executors do
+            # (`triton/worklist.py:118-122`); the library gives the reason
+            try:
+                self._dev.resolve(1, 0, self._tech)
+            except _lib.HIPError as exc:
+                self._dev.close()
+                raise ValueError(f"HIP technique {technique!r} cannot run this NFA: {exc}"
+                                 f" (chain-structured: {bool(props.chain_ok)}, "
+                                 f"{nfa.num_states} states)") from exc
 
     def run(self, input_bytes: bytes) -> Result:
         return self.run_batch([input_bytes])[0]

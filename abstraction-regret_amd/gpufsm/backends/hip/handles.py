@@ -49,11 +49,17 @@ def analyze_nfa(nfa: NFA) -> NfaProps:
     return props
 
 
-def make_tuning(chunk_bytes: int = 0, segment_bytes: int = 0, flags: int = 0):
-    """A ``gfsm_tuning`` (None when every field is the library's choice)."""
-    if not (chunk_bytes or segment_bytes or flags):
+def make_tuning(chunk_bytes: int = 0, segment_bytes: int = 0, flags: int = 0,
+                max_input_bytes: int = 0):
+    """A ``gfsm_tuning`` (None when every field is the library's choice).
+
+    ``max_input_bytes`` is the batch's longest input (0 = not known); the DFA ``auto`` rule
+    walks one lane per input only when it is known and short against the batch.
+    """
+    if not (chunk_bytes or segment_bytes or flags or max_input_bytes):
         return None
-    return _lib.Tuning(int(chunk_bytes), int(segment_bytes), int(flags), 0)
+    return _lib.Tuning(int(chunk_bytes), int(segment_bytes), int(flags),
+                       min(int(max_input_bytes), 2**31 - 1))
 
 
 class _Handle:
@@ -66,11 +72,12 @@ class _Handle:
     def _fn(self, name: str):
         return getattr(_lib.load(), f"{self._prefix}_{name}")
 
-    def resolve(self, count: int, total_bytes: int, technique: int = 0) -> int:
+    def resolve(self, count: int, total_bytes: int, technique: int = 0, tuning=None) -> int:
         """The concrete technique id ``technique`` (auto = 0) resolves to for this shape."""
         out = ctypes.c_int32(0)
-        check(self._fn("resolve")(self.handle, technique, int(count), int(total_bytes),
-                                  ctypes.byref(out)))
+        check(self._fn("resolve")(self.handle, technique,
+                                  ctypes.byref(tuning) if tuning is not None else None,
+                                  int(count), int(total_bytes), ctypes.byref(out)))
         return int(out.value)
 
     def workspace_size(self, count: int, total_bytes: int, technique: int = 0,

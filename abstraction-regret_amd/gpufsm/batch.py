@@ -22,8 +22,10 @@ work itself is the C ABI of ``libgpufsm_hip.so``.
 from __future__ import annotations
 
 import collections
+import itertools
 import threading
 import time
+import weakref
 
 import numpy as np
 
@@ -89,18 +91,24 @@ class ResidentCache:
             self._items.clear()
 
 
+_batch_serials = itertools.count(1)
+
+
 class DeviceBatch:
     """Packed inputs, outputs and launch workspace resident in HBM.
 
     ``count`` inputs at ``data[offsets[i]:offsets[i+1]]``. The buffers keep their addresses
     for the life of the batch (a graph captured on them stays valid); :meth:`load` replaces
-    the contents with another batch that fits the capacity.
+    the contents with another batch that fits the capacity. ``serial`` is unique for the life
+    of the process (``id()`` of a collected batch is reused by the next one), so graphs
+    captured on a batch are keyed by it and never replayed on another batch's buffers.
     """
 
     def __init__(self, data: np.ndarray, offsets: np.ndarray, *, capacity_bytes: int = 0,
                  capacity_count: int = 0, stream=None) -> None:
         torch = _torch()
         self._torch = torch
+        self.serial = next(_batch_serials)
         count = int(offsets.size) - 1
         total = int(offsets[-1]) if count >= 0 else 0
         self.cap_bytes = max(int(capacity_bytes), total, 1)
@@ -117,6 +125,7 @@ class DeviceBatch:
         self._retired = []   # superseded workspaces: graphs captured on them still point there
         self.count = 0
         self.total = 0
+        self.max_len = 0     # longest input of the loaded batch
         # event after the last copy from or into the pinned buffers: the copies are raw
         # hipMemcpyAsync calls, which PyTorch's allocators do not track, so it is waited for
         # before a pinned buffer is rewritten and before the buffers are released
@@ -164,6 +173,7 @@ class DeviceBatch:
         self._staged = torch.cuda.Event()
         self._staged.record(stream)
         self.count, self.total = count, total
+        self.max_len = int(lengths.max()) if count else 0
 
     def download(self, stream):
         """Enqueue the copy of the verdicts into the pinned output buffer on ``stream``;
@@ -214,12 +224,23 @@ class BatchEngine:
             tuner = Autotuner()
         self.tuner = tuner or None
         self.arch = arch or _lib.device_arch(0)
+        # (id(handle), batch serial, choice, count, total, longest input) -> graph; the cache holds every
+        # handle it names, and a batch's graphs are dropped when the batch is collected
         self._graphs: dict[tuple, object] = {}
+        self._graphs_lock = threading.Lock()
+        self.captures = 0
 
     def _drop_graphs_of(self, handle) -> None:
         """Forget the graphs captured on ``handle``: its device memory is about to be freed."""
-        for key in [k for k in self._graphs if k[0] == id(handle)]:
-            del self._graphs[key]
+        with self._graphs_lock:
+            for key in [k for k in self._graphs if k[0] == id(handle)]:
+                del self._graphs[key]
+
+    def _drop_graphs_of_batch(self, serial: int) -> None:
+        """Forget the graphs captured on the batch with ``serial`` (it has been collected)."""
+        with self._graphs_lock:
+            for key in [k for k in self._graphs if k[1] == serial]:
+                del self._graphs[key]
 
     # -- technique selection ------------------------------------------------------------
     def choose(self, automaton, batch: DeviceBatch, technique: str = "auto",
@@ -238,8 +259,7 @@ class BatchEngine:
             torch = self._torch
             if offsets is None:
                 offsets = batch.offsets[: batch.count + 1].cpu().numpy()
-            k = int(np.searchsorted(offsets, self.tuner.sample_bytes, side="right")) - 1
-            k = max(1, min(batch.count, k))
+            k = self.tuner.sample_count(kind.value, offsets, batch.count)
             s = stream or torch.cuda.current_stream()
             w0 = torch.cuda.Event(enable_timing=True)
             w1 = torch.cuda.Event(enable_timing=True)
@@ -271,11 +291,12 @@ class BatchEngine:
     def _launch(self, dev, kind: Kind, batch: DeviceBatch, choice: Choice, stream,
                 count: int | None = None) -> None:
         tech = _techniques(kind)[choice.technique]
-        tuning = choice.tuning()
         c = batch.count if count is None else count
         total = int(batch.total) if count is None else None
         if total is None:
             total = int(batch.offsets[c].item())
+        # the longest input of a prefix is at most the batch's
+        tuning = choice.tuning(max(batch.max_len, 1))
         need = dev.workspace_size(c, total, tech, tuning)
         ws = batch.ensure_workspace(need)
         dev.run_device_ws(batch.data.data_ptr(), batch.offsets.data_ptr(), c, total,
@@ -299,8 +320,9 @@ class BatchEngine:
         if not graph:
             self._launch(dev, kind, batch, choice, stream)
             return choice
-        key = (id(dev), id(batch), choice, batch.count, batch.total)
-        g = self._graphs.get(key)
+        key = (id(dev), batch.serial, choice, batch.count, batch.total, batch.max_len)
+        with self._graphs_lock:
+            g = self._graphs.get(key)
         if g is None:
             # one eager run first: it sizes the workspace and sets kernel attributes, so the
             # capture records kernel launches only
@@ -311,7 +333,12 @@ class BatchEngine:
             side.wait_stream(stream)
             with torch.cuda.graph(g, stream=side):
                 self._launch(dev, kind, batch, choice, torch.cuda.current_stream())
-            self._graphs[key] = g
+            with self._graphs_lock:
+                first = not any(k[1] == batch.serial for k in self._graphs)
+                self._graphs[key] = g
+            if first:
+                weakref.finalize(batch, _drop_batch_graphs, weakref.ref(self), batch.serial)
+            self.captures += 1
             stream.wait_stream(side)
         with torch.cuda.stream(stream):
             g.replay()
@@ -342,5 +369,12 @@ class BatchEngine:
         return batch_results(flags, lens, kernel_ms, transfer_ms)
 
     def close(self) -> None:
-        self._graphs.clear()
+        with self._graphs_lock:
+            self._graphs.clear()
         self.cache.close()
+
+
+def _drop_batch_graphs(engine_ref, serial: int) -> None:
+    engine = engine_ref()
+    if engine is not None:
+        engine._drop_graphs_of_batch(serial)

@@ -4,7 +4,8 @@ Every technique is bit-exact (the parity suite checks each against the oracle), 
 choice only moves time. The tuner times each candidate on a bounded sample of the real batch
 (a prefix of its inputs, run in place: same device buffers, smaller count), keeps the
 fastest, and records every candidate's measured examined-byte rate next to the choice, in a
-JSON table keyed by a shape signature:
+JSON table keyed by a shape signature (the sample is described in
+:meth:`Autotuner.sample_count`):
 
     kind | device arch | states (power-of-two bucket) | structure flags | mean input length
     (bucket) | batch size (bucket)
@@ -17,7 +18,10 @@ Candidates
   per wave in the register tier (for active sets beyond 32 listed states), ``chain``
   without the byte-pair lists, ``bitpacked`` (dense active sets) and, up to 4096 states,
   ``worklist``.
-  NFA, general: ``worklist`` only.
+  NFA, general: ``worklist`` (sparse active sets), ``bitpacked`` (the bit-parallel step,
+  dense active sets; up to 65,536 states).
+  Any NFA of at most 512 states with the folded layout: also ``register`` (one lane per
+  input).
   DFA: ``gather``; ``lds`` when the table fits LDS; ``split`` (segment 1, 2 and 4 KiB when
   the table fits LDS, the library's choice otherwise) when inputs average at least 1 KiB.
 
@@ -57,8 +61,10 @@ class Choice:
             parts.append("wave64")
         return ":".join(parts)
 
-    def tuning(self):
-        return make_tuning(self.chunk_bytes, self.segment_bytes, self.flags)
+    def tuning(self, max_input_bytes: int = 0):
+        """The ``gfsm_tuning`` of this choice for a batch whose longest input is
+        ``max_input_bytes`` (0 = not known)."""
+        return make_tuning(self.chunk_bytes, self.segment_bytes, self.flags, max_input_bytes)
 
     @classmethod
     def parse(cls, label: str) -> Choice:
@@ -98,15 +104,20 @@ def default_table_path() -> str:
 
 def candidates(kind: str, props, mean_len: float) -> list[Choice]:
     if kind == "nfa":
+        lanes = ([Choice("register")] if props.general_ok and props.num_states <= 512
+                 else [])
         if not props.chain_ok:
-            return [Choice("worklist")]
+            out = [Choice("worklist")]
+            if props.general_ok and props.num_states <= 65536:
+                out.append(Choice("bitpacked"))
+            return out + lanes
         out = [Choice("chain", chunk_bytes=c) for c in (2048, 4096, 8192)]
         out += [Choice("chain", chunk_bytes=4096, flags=_lib.TUNE_CHAIN_WAVE64),
                 Choice("chain", chunk_bytes=4096, flags=_lib.TUNE_CHAIN_FIRST),
                 Choice("bitpacked")]
         if props.num_states <= 4096:  # the general list walk only competes on small unions
             out.append(Choice("worklist"))
-        return out
+        return out + lanes
     out = [Choice("gather")]
     if props.lds_ok:
         out.append(Choice("lds"))
@@ -122,9 +133,11 @@ class Autotuner:
     """Chooses a :class:`Choice` per shape class; see the module docstring."""
 
     def __init__(self, path: str | None = None, sample_bytes: int = 16 << 20,
-                 repeats: int = 3, persist: bool = True) -> None:
+                 repeats: int = 3, persist: bool = True,
+                 dfa_sample_bytes: int = 1 << 30) -> None:
         self.path = path or default_table_path()
         self.sample_bytes = int(sample_bytes)
+        self.dfa_sample_bytes = int(dfa_sample_bytes)
         self.repeats = int(repeats)
         self.persist = persist
         self._lock = threading.Lock()
@@ -136,11 +149,27 @@ class Autotuner:
             except (OSError, ValueError):
                 self.table = {}
 
+    def sample_count(self, kind: str, offsets, count: int) -> int:
+        """How many leading inputs of a batch a candidate is timed on.
+
+        NFA: the prefix of at most ``sample_bytes`` (at least one input). DFA: the whole
+        batch up to ``dfa_sample_bytes``. The per-input DFA walk runs one lane per input and
+        its rate grows with inputs per CU (256 to 1024), while the segmented walk fills the
+        device from any count; a byte-bounded prefix of a batch of short inputs would leave
+        the per-input walk a few lanes per CU and misjudge it.
+        """
+        import numpy as np
+
+        limit = self.dfa_sample_bytes if kind == "dfa" else self.sample_bytes
+        k = int(np.searchsorted(offsets, limit, side="right")) - 1
+        return max(1, min(count, k))
+
     @staticmethod
     def signature(kind: str, arch: str, props, count: int, total: int) -> str:
         mean = total // count if count else 0
         if kind == "nfa":
-            struct = f"chain{props.chain_ok}always{props.start_always}"
+            struct = (f"chain{props.chain_ok}always{props.start_always}"
+                      f"general{props.general_ok}")
         else:
             struct = f"w{props.entry_bytes}lds{props.lds_ok}"
         return (f"{kind}|{arch}|n{_bucket(props.num_states)}|{struct}|len{_bucket(mean)}"

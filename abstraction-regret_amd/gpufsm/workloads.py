@@ -145,6 +145,36 @@ def pattern_union(num_patterns: int, len_range: tuple[int, int] = (5, 40),
                         class_bytes=class_bytes)
 
 
+def optional_positions(pu: PatternUnion, p_optional: float = 0.25, seed: int = 0) -> NFA:
+    """The union with optional positions: with probability ``p_optional`` a pattern position
+    s that is not its pattern's last gets an epsilon edge s -> s + 1, so reaching s also
+    counts as having matched s + 1 (the regex ``c?`` for the class of s + 1).
+
+    The epsilon edges make the automaton general (not chain-structured), which is the
+    non-chain control for the general NFA techniques; the active sets stay as large as the
+    union's, or larger.
+    """
+    rng = np.random.default_rng(seed)
+    nfa = pu.nfa
+    n = int(nfa.num_states)
+    first = np.zeros(pu.lengths.size, dtype=np.int64)
+    np.cumsum(pu.lengths[:-1], out=first[1:])
+    first += 1
+    last = first + pu.lengths - 1
+    is_last = np.zeros(n, dtype=bool)
+    is_last[last] = True
+    cand = np.flatnonzero(~is_last[1:]) + 1
+    src = cand[rng.random(cand.size) < p_optional]
+    counts = np.zeros(n, dtype=np.int64)
+    counts[src] = 1
+    erow = np.zeros(n + 1, dtype=np.int32)
+    np.cumsum(counts, out=erow[1:])
+    return NFA(num_states=n, start_state=0, accept=np.asarray(nfa.accept, dtype=bool).copy(),
+               sym_row_ptr=nfa.sym_row_ptr, sym_targets=nfa.sym_targets,
+               sym_symbols=nfa.sym_symbols, eps_row_ptr=erow,
+               eps_targets=(src + 1).astype(np.int32))
+
+
 def lognormal_lengths(count: int, median: int, sigma: float, lo: int, hi: int,
                       seed: int = 0) -> np.ndarray:
     rng = np.random.default_rng(seed)

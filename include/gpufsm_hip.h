@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define GFSM_ABI_VERSION 2
+#define GFSM_ABI_VERSION 3
 
 /* status codes */
 #define GFSM_OK 0
@@ -45,15 +45,22 @@ extern "C" {
 #define GFSM_EDEVICE 4       /* a HIP runtime call failed */
 
 /* NFA techniques (upstream names in parentheses are the CUDA kernels each one replaces) */
-#define GFSM_NFA_AUTO 0       /* pick from the automaton's structure */
-#define GFSM_NFA_WORKLIST 1   /* one wave64 per input, compacted active list + LDS dedup
-                                 bitmap (worklist_warp / worklist_shared / worklist_compact) */
-#define GFSM_NFA_BITPACKED 2  /* bit-parallel chain step, 4096-state tiles, class masks in
-                                 LDS (bitpacked / multistream); chain-structured NFAs only */
+#define GFSM_NFA_AUTO 0       /* pick from the automaton's structure and batch size */
+#define GFSM_NFA_WORKLIST 1   /* one wave64 per input over the compacted active list:
+                                 successors ORed into an LDS bitmap, compacted by popcount
+                                 and a wave prefix sum (worklist_warp / worklist_shared /
+                                 worklist_compact); any NFA */
+#define GFSM_NFA_BITPACKED 2  /* bit-parallel step (bitpacked / multistream / shared): for
+                                 chain-structured NFAs the shift step over 4096-state tiles
+                                 with class masks in LDS, for any other NFA a bitset per
+                                 input in LDS updated from the byte's edge list (up to 65,536
+                                 states; epsilon closure folded on the host) */
 #define GFSM_NFA_CHAIN 3      /* sparse chain step: persistent waves over (input, chunk)
                                  items, active slots listed in LDS and compacted by ballot;
                                  long inputs cut into overlapping chunks when the start state
                                  loops on every byte; chain-structured NFAs only */
+#define GFSM_NFA_REGISTER 4   /* one lane per input, the state set in registers, up to 512
+                                 states (worklist_multistream, the register worklist) */
 
 /* DFA techniques */
 #define GFSM_DFA_AUTO 0
@@ -74,7 +81,11 @@ typedef struct {
     int32_t chunk_bytes;      /* chain: bytes per work item when inputs are cut (>= 64) */
     int32_t segment_bytes;    /* dfa split: segment length, a power of two in [256, 1 MiB] */
     int32_t flags;            /* GFSM_TUNE_* bits */
-    int32_t reserved;
+    int32_t max_input_bytes;  /* longest input of the batch, 0 = not known. The DFA auto
+                                 rule walks one lane per input only when it is known and
+                                 short against the batch (a lane walks its input alone; one
+                                 long input among many short ones would serialise the run);
+                                 the host forms compute it themselves */
 } gfsm_tuning;
 
 typedef struct {
@@ -87,7 +98,9 @@ typedef struct {
     int32_t num_tiles;        /* 4096-state tiles of the bitpacked layout (0 if !chain_ok) */
     int32_t start_always;     /* start state has a wildcard self-loop */
     int32_t max_chain;        /* longest chain (bytes a match can span) if chain_ok */
-    int32_t pad;
+    int32_t general_ok;       /* the epsilon-folded layout of the general techniques exists
+                                 (automata that are not chain-structured, and any automaton
+                                 of at most 512 states, when the fold stays under its cap) */
     int64_t device_bytes;     /* HBM held by the handle */
 } gfsm_nfa_props;
 
@@ -159,10 +172,11 @@ int gfsm_nfa_run_device_ws(gfsm_nfa *nfa, int32_t technique, const gfsm_tuning *
                            int32_t *d_lens, void *d_workspace, int64_t workspace_bytes,
                            void *stream);
 
-/* The concrete technique GFSM_NFA_AUTO (or `technique`) resolves to for this batch shape;
- * GFSM_EUNSUPPORTED if the technique cannot run the automaton. No upstream counterpart. */
-int gfsm_nfa_resolve(gfsm_nfa *nfa, int32_t technique, int32_t num_strings,
-                     int64_t total_bytes, int32_t *out_technique);
+/* The concrete technique GFSM_NFA_AUTO (or `technique`) resolves to for this batch shape
+ * and tuning (nullable); GFSM_EUNSUPPORTED if the technique cannot run the automaton. No
+ * upstream counterpart. */
+int gfsm_nfa_resolve(gfsm_nfa *nfa, int32_t technique, const gfsm_tuning *tuning,
+                     int32_t num_strings, int64_t total_bytes, int32_t *out_technique);
 
 /* DFA handles. Replaces the per-call trans/accept upload of run_dfa (api.hpp:101-104);
  * layout of `src/gpufsm/core/dfa.py:25-37` (trans[s * 256 + c]). */
@@ -192,8 +206,8 @@ int gfsm_dfa_run_device_ws(gfsm_dfa *dfa, int32_t technique, const gfsm_tuning *
                            int32_t num_strings, int64_t total_bytes, int32_t *d_flags,
                            int32_t *d_lens, void *d_workspace, int64_t workspace_bytes,
                            void *stream);
-int gfsm_dfa_resolve(gfsm_dfa *dfa, int32_t technique, int32_t num_strings,
-                     int64_t total_bytes, int32_t *out_technique);
+int gfsm_dfa_resolve(gfsm_dfa *dfa, int32_t technique, const gfsm_tuning *tuning,
+                     int32_t num_strings, int64_t total_bytes, int32_t *out_technique);
 
 #ifdef __cplusplus
 }

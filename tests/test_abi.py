@@ -162,7 +162,7 @@ def test_null_handles_are_errors_not_crashes():
     assert lib.gfsm_nfa_workspace_size(None, 0, 1, 1, None, ctypes.byref(out)) == _lib.EINVAL
     assert lib.gfsm_dfa_workspace_size(None, 0, 1, 1, None, ctypes.byref(out)) == _lib.EINVAL
     tech = ctypes.c_int32(0)
-    assert lib.gfsm_nfa_resolve(None, 0, 1, 1, ctypes.byref(tech)) == _lib.EINVAL
+    assert lib.gfsm_nfa_resolve(None, 0, None, 1, 1, ctypes.byref(tech)) == _lib.EINVAL
     assert lib.gfsm_dfa_run_device_ws(None, 0, None, None, None, 0, 0, None, None, None, 0,
                                       None) == _lib.EINVAL
 

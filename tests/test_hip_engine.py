@@ -245,3 +245,66 @@ def test_serving_mix_equals_alone():
     finally:
         alone.close()
     assert loop.launches < len(reqs)
+
+
+def test_graph_cache_is_keyed_per_batch_not_by_id():
+    """Batches of one shape built and freed in a loop reuse ids; each must capture its own
+    graph (replaying another batch's graph would read freed buffers) and match the oracle,
+    and a collected batch's graphs must leave the engine."""
+    import gc
+
+    import torch
+
+    eng = BatchEngine(tuner=None)
+    try:
+        s = torch.cuda.current_stream()
+        dfa = W.dense_dfa(256, 0.01, seed=91)
+        seen_ids = set()
+        for it in range(6):
+            d, o = W.uniform_batch(48, 3000, seed=100 + it)   # same shape, new bytes
+            wf, wn = _oracle(dfa, d, o)
+            batch = DeviceBatch(d, o, stream=s)
+            seen_ids.add(id(batch))
+            before = eng.captures
+            for rep in range(2):
+                batch.flags.fill_(-1)
+                eng.enqueue(dfa, batch, "split:segment=256", stream=s, graph=True)
+                f, n = batch.fetch(s)
+                np.testing.assert_array_equal(f, wf, err_msg=f"iteration {it} replay {rep}")
+                np.testing.assert_array_equal(n, wn, err_msg=f"iteration {it} replay {rep}")
+            assert eng.captures == before + 1, "a new batch replayed an older batch's graph"
+            del batch
+            gc.collect()
+            assert not eng._graphs, "a collected batch left its graph in the engine"
+        assert len(seen_ids) < 6  # ids were reused, which is the case this guards
+    finally:
+        eng.close()
+
+
+def test_tuner_picks_the_per_input_walk_for_many_short_inputs():
+    """262,144 x 4 KiB on the 256-state table: the per-input walk runs 1024 lanes per CU
+    on the whole batch and beats the segmented walk, so the tuner, timing candidates on the
+    whole batch, must choose it; the engine's verdicts match the oracle."""
+    import torch
+
+    eng = BatchEngine(tuner=Autotuner(persist=False))
+    try:
+        s = torch.cuda.current_stream()
+        dfa = W.dense_dfa(256, 0.0, seed=92)
+        # one accepting state entered from 4 of the 65,536 (state, byte) pairs: most inputs
+        # walk every byte, a few hundred latch
+        trans = np.asarray(dfa.trans).reshape(256, 256)
+        trans[trans == 11] = 12
+        trans[[3, 70, 140, 210], [1, 2, 3, 4]] = 11
+        dfa.trans[:] = trans.reshape(-1)
+        dfa.accept[11] = True
+        d, o = W.uniform_batch(262144, 4096, seed=93)
+        batch = DeviceBatch(d, o, stream=s)
+        choice = eng.enqueue(dfa, batch, "auto", stream=s)
+        f, n = batch.fetch(s)
+        wf, wn = _oracle(dfa, d, o)
+        np.testing.assert_array_equal(f, wf)
+        np.testing.assert_array_equal(n, wn)
+        assert choice.technique == "lds", choice
+    finally:
+        eng.close()

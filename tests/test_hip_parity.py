@@ -20,14 +20,14 @@ from gpufsm.core.dfa import random_dfa
 from gpufsm.core.nfa import NFABuilder
 from gpufsm.core.packing import pack_bytes
 from gpufsm.core.registry import unavailable_reason
-from gpufsm.backends.hip.handles import DeviceDFA, DeviceNFA
+from gpufsm.backends.hip.handles import DeviceDFA, DeviceNFA, make_tuning
 from gpufsm.backends.hip import _lib
 from oracle import native
 from oracle.reference import simulate, simulate_dfa
 
 pytestmark = pytest.mark.gpu
 
-NFA_TECHS = ["auto", "worklist"]
+NFA_TECHS = ["auto", "worklist", "bitpacked", "register"]  # every one runs any small NFA
 DFA_TECHS = ["auto", "gather", "lds", "split"]
 
 
@@ -103,13 +103,20 @@ def test_fuzz_batches(tech):
         assert got == [simulate(nfa, d) for d in batch], trial
 
 
-def test_multiword_and_large_automata():
+@pytest.mark.parametrize("tech", ["worklist", "bitpacked", "register"])
+def test_multiword_and_large_automata(tech):
+    """Random epsilon-NFAs of 65 to 20,000 states (multi-word sets: one to 313 words; the
+    register technique runs up to 512 states)."""
     rng = random.Random(23)
-    for n in (65, 130, 300, 1000, 5000, 20000):
+    for n in (65, 130, 300, 500, 1000, 5000, 20000):
         nfa = random_nfa(rng, n, b"abcde", max_sym=3, max_eps=1, p_accept=0.01, p_any=0.02)
         batch = [bytes(rng.choice(b"abcdef") for _ in range(rng.randint(0, 200)))
                  for _ in range(50)]
-        got = _verdicts(run_batch(nfa, batch, backend="hip", technique="worklist"))
+        if tech == "register" and n > 512:
+            with pytest.raises(ValueError, match="512"):
+                run_batch(nfa, batch, backend="hip", technique=tech)
+            continue
+        got = _verdicts(run_batch(nfa, batch, backend="hip", technique=tech))
         assert got == _oracle_nfa(nfa, batch), n
 
 
@@ -181,14 +188,64 @@ def test_dfa_lds_many_inputs_per_cu(count):
 
 def test_dfa_auto_resolution_by_batch_shape():
     """auto segments a batch of few long inputs, and walks one lane per input when the
-    inputs alone fill every CU (far more than 256 per CU here) or are short."""
+    inputs alone fill every CU (far more than 256 per CU here) and the longest input is
+    known to be short against the batch, or when inputs are short."""
     lds, split = _lib.DFA_TECHNIQUES["lds"], _lib.DFA_TECHNIQUES["split"]
+
+    def longest(n):
+        return make_tuning(max_input_bytes=n)
+
     with DeviceDFA(random_dfa(256, accept_prob=0.0, seed=5)) as dev:
         assert dev.resolve(4096, 4096 << 16) == split        # 4096 x 64 KiB
-        assert dev.resolve(1 << 20, (1 << 20) << 14) == lds  # 1 M x 16 KiB
+        # 1 M x 16 KiB: one lane per input once the longest input is known
+        assert dev.resolve(1 << 20, (1 << 20) << 14, tuning=longest(1 << 14)) == lds
+        assert dev.resolve(1 << 20, (1 << 20) << 14) == split  # longest not known
         assert dev.resolve(4096, 4096 << 12) == lds          # 4096 x 4 KiB
+        # 65,535 inputs of 8 KiB plus one of 512 MiB: the long one would be walked by one
+        # lane alone, so the batch is segmented even though the inputs fill the CUs
+        total = 65535 * 8192 + (512 << 20)
+        assert dev.resolve(65536, total, tuning=longest(512 << 20)) == split
+        # 262,144 x 4 KiB, longest known: one lane per input
+        assert dev.resolve(262144, 262144 << 12, tuning=longest(4096)) == lds
+        # the host form computes the longest input itself (a skewed batch: split)
+        lens = np.full(65536, 64, dtype=np.int64)
+        lens[7] = 3 << 20
+        off = np.zeros(lens.size + 1, dtype=np.int64)
+        np.cumsum(lens, out=off[1:])
+        data = np.random.default_rng(1).integers(0, 256, size=int(off[-1]), dtype=np.uint8)
+        f, n, _ = dev.run_host(data, off, _lib.DFA_AUTO)
+        wf, wn, _ = native.dfa_batch(dev.dfa, data, off)
+        np.testing.assert_array_equal(f, wf)
+        np.testing.assert_array_equal(n, wn)
     with DeviceDFA(random_dfa(4096, accept_prob=0.0, seed=5)) as dev:  # table in L2
-        assert dev.resolve(1 << 20, (1 << 20) << 14) == _lib.DFA_TECHNIQUES["gather"]
+        assert dev.resolve(1 << 20, (1 << 20) << 14, tuning=longest(1 << 14)) == \
+            _lib.DFA_TECHNIQUES["gather"]
+
+
+@pytest.mark.parametrize("count", [131072, 262144])
+def test_dfa_lds_many_lanes_long_inputs(count):
+    """The 512- and 1024-lane per-input walks on inputs long enough to run whole 128-byte
+    rounds: lengths up to 700 bytes, unaligned starts, and first accepts after byte 128 and
+    inside the read-ahead round (one accepting state entered from 8 of the 65,536 (state,
+    byte) pairs, so first matches land at every depth)."""
+    dfa = random_dfa(256, accept_prob=0.0, seed=count)
+    trans = np.asarray(dfa.trans).reshape(256, 256)
+    trans[trans == 9] = 10
+    trans[[1, 40, 80, 120, 160, 200, 240, 255], [5, 6, 7, 8, 9, 10, 11, 12]] = 9
+    dfa.trans[:] = trans.reshape(-1)
+    dfa.accept[9] = True
+    rng = np.random.default_rng(count + 1)
+    lens = rng.integers(0, 701, size=count)
+    data = rng.integers(0, 256, size=int(lens.sum()), dtype=np.uint8)
+    off = np.zeros(count + 1, dtype=np.int64)
+    np.cumsum(lens, out=off[1:])
+    with DeviceDFA(dfa) as dev:
+        f, n, _ = dev.run_host(data, off, _lib.DFA_TECHNIQUES["lds"])
+    wf, wn, _ = native.dfa_batch(dfa, data, off)
+    np.testing.assert_array_equal(f, wf)
+    np.testing.assert_array_equal(n, wn)
+    assert (wn[wf.astype(bool)] > 128).sum() > 1000   # accepts past the first round
+    assert ((wn > 128) & (wn < 256)).any() and (wn > 512).any()
 
 
 @pytest.mark.parametrize("tech", DFA_TECHS)

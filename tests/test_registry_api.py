@@ -60,7 +60,8 @@ def test_cpu_oracle_backend_registered():
 
 
 def test_hip_techniques_registered():
-    assert list_techniques(Backend.HIP, Kind.NFA) == ["auto", "worklist", "bitpacked", "chain"]
+    assert list_techniques(Backend.HIP, Kind.NFA) == ["auto", "worklist", "bitpacked", "chain",
+                                                      "register"]
     assert list_techniques(Backend.HIP, Kind.DFA) == ["auto", "gather", "lds", "split"]
     assert _DEFAULTS[(Kind.NFA, Backend.HIP)] == "auto"
     assert _DEFAULTS[(Kind.DFA, Backend.HIP)] == "auto"
