@@ -248,9 +248,10 @@ def test_serving_mix_equals_alone():
 
 
 def test_graph_cache_is_keyed_per_batch_not_by_id():
-    """Batches of one shape built and freed in a loop reuse ids; each must capture its own
-    graph (replaying another batch's graph would read freed buffers) and match the oracle,
-    and a collected batch's graphs must leave the engine."""
+    """Batches of one shape built and freed in a loop (CPython may give a new batch the id of
+    a collected one): each must capture its own graph (replaying another batch's graph would
+    read freed buffers) and match the oracle, and a collected batch's graphs must leave the
+    engine."""
     import gc
 
     import torch
@@ -259,12 +260,10 @@ def test_graph_cache_is_keyed_per_batch_not_by_id():
     try:
         s = torch.cuda.current_stream()
         dfa = W.dense_dfa(256, 0.01, seed=91)
-        seen_ids = set()
         for it in range(6):
             d, o = W.uniform_batch(48, 3000, seed=100 + it)   # same shape, new bytes
             wf, wn = _oracle(dfa, d, o)
             batch = DeviceBatch(d, o, stream=s)
-            seen_ids.add(id(batch))
             before = eng.captures
             for rep in range(2):
                 batch.flags.fill_(-1)
@@ -276,7 +275,6 @@ def test_graph_cache_is_keyed_per_batch_not_by_id():
             del batch
             gc.collect()
             assert not eng._graphs, "a collected batch left its graph in the engine"
-        assert len(seen_ids) < 6  # ids were reused, which is the case this guards
     finally:
         eng.close()
 
