@@ -754,18 +754,24 @@ int gfsm_dfa_create(int32_t num_states, int32_t start_state, const uint8_t *acce
             Packer pk;
             const uint32_t *t32 = nullptr;
             const uint16_t *t16 = nullptr;
-            if (t.wide)
+            const uint8_t *t8 = nullptr;
+            if (t.esz == 4)
                 pk.add(&t32, t.t32);
-            else
+            else if (t.esz == 2)
                 pk.add(&t16, t.t16);
+            else
+                pk.add(&t8, t.t8);
             pk.commit(h->blocks, h->device_bytes);
-            h->dev.table = t.wide ? static_cast<const void *>(t32) : static_cast<const void *>(t16);
+            h->dev.table = t.esz == 4   ? static_cast<const void *>(t32)
+                           : t.esz == 2 ? static_cast<const void *>(t16)
+                                        : static_cast<const void *>(t8);
             h->dev.n = num_states;
             h->dev.wide = t.wide ? 1 : 0;
+            h->dev.esz = static_cast<int32_t>(t.esz);
             h->dev.start_entry = t.start_entry;
             h->dev.accept_from = t.accept_from;
             h->dev.col_bytes = t.col_bytes;
-            h->lds_ok = !t.wide && dfa_lds_bytes(num_states) <= 160 * 1024;
+            h->lds_ok = !t.wide && dfa_table_bytes(h->dev) <= 160 * 1024;
         } catch (...) {
             free_blocks(h->blocks);
             delete h;
@@ -790,7 +796,7 @@ int gfsm_dfa_query(const gfsm_dfa *dfa, gfsm_dfa_props *out) {
         out->num_states = dfa->n;
         out->start_state = dfa->start;
         out->start_accepts = dfa->start_accepts;
-        out->entry_bytes = dfa->dev.wide ? 4 : 2;
+        out->entry_bytes = dfa->dev.esz;
         out->lds_ok = dfa->lds_ok;
         out->device_bytes = dfa->device_bytes;
     });

@@ -30,13 +30,31 @@ namespace gfsm {
 
 // ---------------------------------------------------------------- step primitives -------
 
+// One step from entry e on byte b. Byte tables (at most 256 states, 256-entry columns):
+// the entry is the state id and the successor sits at (b << 8) | e. Wider tables: the entry
+// is the successor's byte offset in a column, at b * col + e.
 template <typename T>
 __device__ __forceinline__ uint32_t dstep(const char *tab, uint32_t col, uint32_t e,
                                           uint32_t b) {
-    if constexpr (sizeof(T) == 2)
+    if constexpr (sizeof(T) == 1)
+        return *reinterpret_cast<const uint8_t *>(tab + ((b << 8) | e));
+    else if constexpr (sizeof(T) == 2)
         return *reinterpret_cast<const T *>(tab + (__umul24(b, col) + e));
     else
         return *reinterpret_cast<const T *>(tab + (b * col + e));
+}
+
+// One step on byte k (0..3, a constant) of the input word w. For a byte table the address
+// is one v_perm_b32: byte k of w into bits 8-15 and the state (byte 0 of e, whose other
+// bytes are zero) into bits 0-7; the other bytes select zero, so the address is below 64 KiB.
+template <typename T>
+__device__ __forceinline__ uint32_t dstepw(const char *tab, uint32_t col, uint32_t e,
+                                           uint32_t w, int k) {
+    if constexpr (sizeof(T) == 1)
+        return *reinterpret_cast<const uint8_t *>(
+            tab + __builtin_amdgcn_perm(w, e, 0x0c0c0400u + (static_cast<uint32_t>(k) << 8)));
+    else
+        return dstep<T>(tab, col, e, (w >> (k * 8)) & 0xffu);
 }
 
 // 16 bytes of a vector, tracking the largest entry reached.
@@ -46,7 +64,7 @@ __device__ __forceinline__ uint32_t step16(const char *tab, uint32_t col, uint32
     const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-        e = dstep<T>(tab, col, e, (w[k >> 2] >> ((k & 3) * 8)) & 0xffu);
+        e = dstepw<T>(tab, col, e, w[k >> 2], k & 3);
         mx = e > mx ? e : mx;
     }
     return e;
@@ -57,12 +75,30 @@ template <typename T>
 __device__ __forceinline__ int32_t locate16(const char *tab, uint32_t col, uint32_t e,
                                             const uint4 &v, uint32_t acc) {
     const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
     for (int k = 0; k < 16; ++k) {
-        e = dstep<T>(tab, col, e, (w[k >> 2] >> ((k & 3) * 8)) & 0xffu);
+        e = dstepw<T>(tab, col, e, w[k >> 2], k & 3);
         if (e >= acc) return k;
     }
     return 15;  // unreachable when the block's maximum was >= acc
 }
+
+// Checkpoints: the state an interval ends in, with bit 0 set when an accepting state was
+// entered in the interval. Entries of 2- and 4-byte tables are even (byte offsets of 2- or
+// 4-byte entries), so bit 0 is free; byte-table entries are state ids, stored shifted left
+// by one in a 16-bit checkpoint.
+template <typename T>
+struct Ck {
+    using type = T;
+    __device__ static uint32_t encode(uint32_t e, bool flag) { return e | (flag ? 1u : 0u); }
+    __device__ static uint32_t state(uint32_t c) { return c & ~1u; }
+};
+template <>
+struct Ck<uint8_t> {
+    using type = uint16_t;
+    __device__ static uint32_t encode(uint32_t e, bool flag) { return (e << 1) | (flag ? 1u : 0u); }
+    __device__ static uint32_t state(uint32_t c) { return c >> 1; }
+};
 
 __device__ __forceinline__ uint4 ld16(const uint8_t *p) {
     return *reinterpret_cast<const uint4 *>(p);
@@ -265,7 +301,7 @@ __device__ __forceinline__ void step_round(const char *tab, uint32_t col, uint32
             const uint4 &x = v[c][k >> 4];
             const int kk = k & 15;
             const uint32_t w = kk < 4 ? x.x : (kk < 8 ? x.y : (kk < 12 ? x.z : x.w));
-            e[c] = dstep<T>(tab, col, e[c], (w >> ((kk & 3) * 8)) & 0xffu);
+            e[c] = dstepw<T>(tab, col, e[c], w, kk & 3);
             mx[c] = e[c] > mx[c] ? e[c] : mx[c];
         }
     }
@@ -276,8 +312,9 @@ template <typename T>
 __device__ __forceinline__ int32_t find16(const char *tab, uint32_t col, uint32_t &e,
                                           const uint4 &v, uint32_t acc) {
     const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
     for (int k = 0; k < 16; ++k) {
-        e = dstep<T>(tab, col, e, (w[k >> 2] >> ((k & 3) * 8)) & 0xffu);
+        e = dstepw<T>(tab, col, e, w[k >> 2], k & 3);
         if (e >= acc) return k;
     }
     return -1;
@@ -298,11 +335,12 @@ __device__ __forceinline__ int32_t find_round(const char *tab, uint32_t col, uin
 // which hold consecutive segments, write and read whole contiguous lines.
 template <typename T>
 struct CkView {
-    const T *p;     // &ckpt[segment]
-    int64_t step;   // number of segments
+    const typename Ck<T>::type *p;  // &ckpt[segment]
+    int64_t step;                   // number of segments
     __device__ __forceinline__ uint32_t operator[](int64_t k) const {
         return static_cast<uint32_t>(p[k * step]);
     }
+    __device__ __forceinline__ uint32_t state(int64_t k) const { return Ck<T>::state((*this)[k]); }
 };
 
 // Threads per block of the walk (one block per CU: the table fills the LDS). The walk keeps
@@ -321,7 +359,8 @@ __global__ __launch_bounds__(kWalkThreads / K) void dfa_split_walk_kernel(
     const char *tab = stage_table<T, kLds>(gtab, a.table_bytes, lds_raw);
     const uint32_t col = a.col, acc = a.acc, start = a.start;
     const int64_t rmask = (1ll << a.rshift) - 1;
-    T *ckpt = static_cast<T *>(a.ckpt);
+    using CkT = typename Ck<T>::type;
+    CkT *ckpt = static_cast<CkT *>(a.ckpt);
     const int64_t lanes = static_cast<int64_t>(gridDim.x) * blockDim.x;
     int64_t next_g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const uint4 zero = make_uint4(0, 0, 0, 0);
@@ -364,7 +403,7 @@ __global__ __launch_bounds__(kWalkThreads / K) void dfa_split_walk_kernel(
                 const int64_t hl = pend[c] - s0;
                 if (hl & rmask) {
                     ckpt[(hl >> a.rshift) * a.segments + g[c]] =
-                        static_cast<T>(e[c] | (mx[c] >= acc ? 1u : 0u));
+                        static_cast<CkT>(Ck<T>::encode(e[c], mx[c] >= acc));
                     if (mx[c] >= acc) fm[c] |= 1ull << ((hl >> a.rshift) & 63);
                 }
                 if (masks) a.flagmask[g[c]] = fm[c];
@@ -462,7 +501,7 @@ __global__ __launch_bounds__(kWalkThreads / K) void dfa_split_walk_kernel(
                     const int64_t r = q[c] - g[c] * a.seg;
                     if ((r & rmask) == 0) {
                         ckpt[((r >> a.rshift) - 1) * a.segments + g[c]] =
-                            static_cast<T>(e[c] | (mx[c] >= acc ? 1u : 0u));
+                            static_cast<CkT>(Ck<T>::encode(e[c], mx[c] >= acc));
                         if (mx[c] >= acc) fm[c] |= 1ull << (((r >> a.rshift) - 1) & 63);
                         mx[c] = 0;
                     }
@@ -548,7 +587,7 @@ __device__ __forceinline__ void split_merge_walk(const char *tab, const SplitArg
         }
         q += 16;
         const int64_t r = q - s0;
-        if ((r & rmask) == 0 && (ck[(r >> a.rshift) - 1] & ~1u) == e) {
+        if ((r & rmask) == 0 && ck.state((r >> a.rshift) - 1) == e) {
             merged = (r >> a.rshift) - 1;
             break;
         }
@@ -564,14 +603,14 @@ __device__ __forceinline__ void split_merge_walk(const char *tab, const SplitArg
         }
         exit_out = e;
         const int64_t hl = hend - s0;
-        if (!(hl & rmask) || (ck[hl >> a.rshift] & ~1u) != e) return;
+        if (!(hl & rmask) || ck.state(hl >> a.rshift) != e) return;
         merged = hl >> a.rshift;  // met at the final partial interval: nothing after it
     }
     // paths coincide after checkpoint `merged`: take the first flagged interval after it
     const int64_t nk = ((hend - s0) + rmask) >> a.rshift;
     const int64_t k = split_next_flagged<T>(a, g, ck, merged, nk);
     if (k < nk) {
-        uint32_t c = ck[k - 1] & ~1u;
+        uint32_t c = ck.state(k - 1);
         const int64_t lo = s0 + (k << a.rshift);
         const int64_t hi = (lo + rmask + 1) < hend ? (lo + rmask + 1) : hend;
         const int64_t hit = walk_exact<T>(tab, col, acc, c, data, lo, hi);
@@ -590,7 +629,7 @@ __device__ __forceinline__ int32_t split_accept_after(const char *tab, const Spl
     const int64_t nk = ((hend - s0) + rmask) >> a.rshift;
     const int64_t k = split_next_flagged<T>(a, g, ck, merged, nk);
     if (k >= nk) return -1;
-    uint32_t c = ck[k - 1] & ~1u;
+    uint32_t c = ck.state(k - 1);
     const int64_t lo = s0 + (k << a.rshift);
     const int64_t hi = (lo + rmask + 1) < hend ? (lo + rmask + 1) : hend;
     const int64_t hit = walk_exact<T>(tab, a.col, a.acc, c, a.data, lo, hi);
@@ -609,7 +648,7 @@ __global__ __launch_bounds__(fix_threads_for(kFixRound) / K) void dfa_split_fix_
     const char *tab = stage_table<T, kLds>(gtab, a.table_bytes, lds_raw);
     const uint32_t col = a.col, acc = a.acc;
     const int64_t rmask = (1ll << a.rshift) - 1;
-    const T *ckpt = static_cast<const T *>(a.ckpt);
+    const typename Ck<T>::type *ckpt = static_cast<const typename Ck<T>::type *>(a.ckpt);
     const int64_t lanes = static_cast<int64_t>(gridDim.x) * blockDim.x;
     int64_t next_g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const uint4 zero = make_uint4(0, 0, 0, 0);
@@ -714,7 +753,7 @@ __global__ __launch_bounds__(fix_threads_for(kFixRound) / K) void dfa_split_fix_
                 }
                 // the state the head's speculative path had at the round's end, if the round
                 // ends on a checkpoint
-                const uint32_t ckn = at_ck[c] ? (ckr[c] & ~1u) : ~0u;
+                const uint32_t ckn = at_ck[c] ? Ck<T>::state(ckr[c]) : ~0u;
                 const int64_t s0 = g[c] * a.seg;
                 q[c] += kFixRound;
                 if (mx[c] >= acc) {
@@ -770,10 +809,11 @@ constexpr int kMapPartIntervals = kMapIntervals / kMapParts;  // 8
 constexpr int kMapThreads = 1024;                              // 8 intervals x 128 lanes
 constexpr uint16_t kNoHit = 0xffffu;
 
-__global__ __launch_bounds__(kMapThreads) void dfa_split_map_kernel(const uint16_t *__restrict__ gtab,
+template <typename T>
+__global__ __launch_bounds__(kMapThreads) void dfa_split_map_kernel(const T *__restrict__ gtab,
                                                                     SplitArgs a) {
     extern __shared__ uint4 lds_raw[];
-    const char *tab = stage_table<uint16_t, true>(gtab, a.table_bytes, lds_raw);
+    const char *tab = stage_table<T, true>(gtab, a.table_bytes, lds_raw);
     const int n = a.n;  // at most 256: two entries per lane
     uint16_t *imap = reinterpret_cast<uint16_t *>(reinterpret_cast<char *>(lds_raw) + a.table_bytes);
     uint16_t *ihit = imap + kMapPartIntervals * n;
@@ -797,7 +837,7 @@ __global__ __launch_bounds__(kMapThreads) void dfa_split_map_kernel(const uint16
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             const int x = sub + 128 * j;  // entries past n walk state 0 and are not stored
-            e[j] = static_cast<uint32_t>(x < n ? x : 0) * 2u;
+            e[j] = static_cast<uint32_t>(x < n ? x : 0) * static_cast<uint32_t>(sizeof(T));
             hit[j] = kNoHit;
         }
         // the interval's bytes, one vector ahead: its loads are issued together and only the
@@ -812,10 +852,9 @@ __global__ __launch_bounds__(kMapThreads) void dfa_split_map_kernel(const uint16
             const uint32_t wv[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
             for (int b = 0; b < 16; ++b) {
-                const uint32_t c = (wv[b >> 2] >> ((b & 3) * 8)) & 0xffu;
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
-                    e[j] = dstep<uint16_t>(tab, col, e[j], c);
+                    e[j] = dstepw<T>(tab, col, e[j], wv[b >> 2], b & 3);
                     if (hit[j] == kNoHit && e[j] >= acc) hit[j] = static_cast<uint32_t>(q - lo + b);
                 }
             }
@@ -824,7 +863,7 @@ __global__ __launch_bounds__(kMapThreads) void dfa_split_map_kernel(const uint16
             const uint32_t c = a.data[q];
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
-                e[j] = dstep<uint16_t>(tab, col, e[j], c);
+                e[j] = dstep<T>(tab, col, e[j], c);
                 if (hit[j] == kNoHit && e[j] >= acc) hit[j] = static_cast<uint32_t>(q - lo);
             }
         }
@@ -839,7 +878,7 @@ __global__ __launch_bounds__(kMapThreads) void dfa_split_map_kernel(const uint16
         __syncthreads();
         // compose the block's intervals: the part map, first accept as an offset in the head
         for (int x = threadIdx.x; x < n; x += blockDim.x) {
-            uint32_t st = static_cast<uint32_t>(x), ev = static_cast<uint32_t>(x) * 2u;
+            uint32_t st = static_cast<uint32_t>(x), ev = static_cast<uint32_t>(x) * sizeof(T);
             int32_t first = -1;
             for (int j = 0; j < kMapPartIntervals; ++j) {
                 if (first < 0) {
@@ -848,7 +887,7 @@ __global__ __launch_bounds__(kMapThreads) void dfa_split_map_kernel(const uint16
                         first = static_cast<int32_t>((part * kMapPartIntervals + j) * I + h);
                 }
                 ev = imap[j * n + st];
-                st = ev >> 1;
+                st = ev / sizeof(T);
             }
             const int64_t at = static_cast<int64_t>(w) * n + x;  // w = m * kMapParts + part
             a.map_exit[at] = static_cast<uint16_t>(ev);
@@ -1000,7 +1039,7 @@ __global__ __launch_bounds__(256) void dfa_split_finish_kernel(const T *__restri
         // condition is block-uniform, so every thread reaches the barrier inside)
         if (__syncthreads_or(walk ? 1 : 0)) tab = stage_table<T, kLds>(gtab, a.table_bytes, lds_raw);
     }
-    const T *ckpt = static_cast<const T *>(a.ckpt);
+    const typename Ck<T>::type *ckpt = static_cast<const typename Ck<T>::type *>(a.ckpt);
     while (walk) {
         const int64_t s0 = g * a.seg;
         const int64_t hend = (s0 + a.seg) < ie ? (s0 + a.seg) : ie;
@@ -1077,9 +1116,9 @@ int persistent_blocks(const void *kernel, int threads) {
     return cus * per_cu;
 }
 
-size_t dfa_lds_bytes(int32_t n) { return static_cast<size_t>(n) * kAlphabet * sizeof(uint16_t); }
+size_t dfa_table_bytes(const DfaDev &d) { return static_cast<size_t>(d.col_bytes) * kAlphabet; }
 
-static bool lds_fits(const DfaDev &d) { return !d.wide && dfa_lds_bytes(d.n) <= 160 * 1024; }
+static bool lds_fits(const DfaDev &d) { return !d.wide && dfa_table_bytes(d) <= 160 * 1024; }
 
 // Tuning overrides for sweeps: GFSM_SPLIT_SEGMENT (bytes, power of two >= 256) and
 // GFSM_SPLIT_CHAINS (1, 2 or 4 interleaved chains per lane).
@@ -1116,7 +1155,7 @@ DfaSplitPlan dfa_split_plan(const DfaDev &d, int64_t total_bytes, int32_t count,
     p.map_cap = p.maps ? 1024 : 0;
     const size_t S = static_cast<size_t>(p.segments > 0 ? p.segments : 1);
     const size_t nck = static_cast<size_t>(p.seg_bytes >> p.ckpt_shift);
-    const size_t ent = d.wide ? 4 : 2;
+    const size_t ent = d.wide ? 4 : 2;  // checkpoint entry (16 bits for byte and uint16 tables)
     const size_t cap = static_cast<size_t>(p.map_cap);
     const size_t nn = static_cast<size_t>(d.n);
     p.scratch_bytes = align256(static_cast<size_t>(count) * 4) +
@@ -1194,14 +1233,13 @@ static void split_launch_passes(const DfaDev &d, SplitArgs &a, hipStream_t strea
         const int32_t eblocks = static_cast<int32_t>(eblocks64 < 65536 ? eblocks64 : 65536);
         dfa_split_event_kernel<<<eblocks, 256, 0, stream>>>(a);
         GFSM_HIP(hipGetLastError());
-        if constexpr (kLds && sizeof(T) == 2) {
+        if constexpr (kLds && sizeof(T) <= 2) {
             if (a.map_idx) {
                 const size_t mlds = static_cast<size_t>(a.table_bytes) +
                                     static_cast<size_t>(kMapPartIntervals) * a.n * 4;
-                ensure_dynamic_lds(reinterpret_cast<const void *>(&dfa_split_map_kernel), mlds);
+                ensure_dynamic_lds(reinterpret_cast<const void *>(&dfa_split_map_kernel<T>), mlds);
                 const int32_t mblocks = cus;
-                dfa_split_map_kernel<<<mblocks, kMapThreads, mlds, stream>>>(
-                    reinterpret_cast<const uint16_t *>(d.table), a);
+                dfa_split_map_kernel<T><<<mblocks, kMapThreads, mlds, stream>>>(tab, a);
                 GFSM_HIP(hipGetLastError());
             }
         }
@@ -1277,6 +1315,10 @@ void launch_dfa_split(const DfaDev &d, const DfaSplitPlan &plan, const uint8_t *
     const bool lds = lds_fits(d);
     if (d.wide) {
         split_launch_passes<uint32_t, false, 1>(d, a, stream);
+    } else if (d.esz == 1) {
+        if (plan.chains == 4) split_launch_passes<uint8_t, true, 4>(d, a, stream);
+        else if (plan.chains == 2) split_launch_passes<uint8_t, true, 2>(d, a, stream);
+        else split_launch_passes<uint8_t, true, 1>(d, a, stream);
     } else if (lds) {
         if (plan.chains == 4) split_launch_passes<uint16_t, true, 4>(d, a, stream);
         else if (plan.chains == 1) split_launch_passes<uint16_t, true, 1>(d, a, stream);
@@ -1305,6 +1347,10 @@ void launch_dfa_gather(const DfaDev &d, const uint8_t *data, const int64_t *off,
         dfa_walk_kernel<uint32_t, false><<<grid, 256, 0, stream>>>(
             static_cast<const uint32_t *>(d.table), tb, d.col_bytes, d.start_entry,
             d.accept_from, data, off, count, flags, lens);
+    else if (d.esz == 1)
+        dfa_walk_kernel<uint8_t, false><<<grid, 256, 0, stream>>>(
+            static_cast<const uint8_t *>(d.table), tb, d.col_bytes, d.start_entry,
+            d.accept_from, data, off, count, flags, lens);
     else
         dfa_walk_kernel<uint16_t, false><<<grid, 256, 0, stream>>>(
             static_cast<const uint16_t *>(d.table), tb, d.col_bytes, d.start_entry,
@@ -1318,6 +1364,7 @@ void launch_dfa_lds(const DfaDev &d, const uint8_t *data, const int64_t *off, in
     if (!lds_fits(d)) throw Error(3, "lds: the table does not fit in LDS");
     const uint32_t tb = d.col_bytes * kAlphabet;
     allow_big_lds(&dfa_walk_kernel<uint16_t, true>, tb);
+    allow_big_lds(&dfa_walk_kernel<uint8_t, true>, tb);
     // One block per CU holds the table, so the block is the lanes a CU gets: 256, or more
     // when the batch has that many inputs per CU (more reads in flight per CU; 131,072 x
     // 2 KiB inputs ran 1.7 TB/s at 256 lanes, latency-bound).
@@ -1327,9 +1374,14 @@ void launch_dfa_lds(const DfaDev &d, const uint8_t *data, const int64_t *off, in
     int grid = (count + threads - 1) / threads;
     const int cap = 2 * cus;
     if (grid > cap) grid = cap;
-    dfa_walk_kernel<uint16_t, true><<<grid, threads, tb, stream>>>(
-        static_cast<const uint16_t *>(d.table), tb, d.col_bytes, d.start_entry, d.accept_from,
-        data, off, count, flags, lens);
+    if (d.esz == 1)
+        dfa_walk_kernel<uint8_t, true><<<grid, threads, tb, stream>>>(
+            static_cast<const uint8_t *>(d.table), tb, d.col_bytes, d.start_entry,
+            d.accept_from, data, off, count, flags, lens);
+    else
+        dfa_walk_kernel<uint16_t, true><<<grid, threads, tb, stream>>>(
+            static_cast<const uint16_t *>(d.table), tb, d.col_bytes, d.start_entry,
+            d.accept_from, data, off, count, flags, lens);
     GFSM_HIP(hipGetLastError());
 }
 

@@ -110,11 +110,13 @@ struct NfaChainDev {
 
 // ----- DFA ------------------------------------------------------------------------------
 
-// Table by byte, entries = successor's byte offset in a column (see layout.hpp): the step
-// is `e = *(T *)(table + byte * col_bytes + e)`, and e >= accept_from means accepting.
+// Table by byte (see layout.hpp): byte entries (state ids, 256-entry columns) up to 256
+// states, else entries = successor's byte offset in a column, so the step is
+// `e = *(T *)(table + byte * col_bytes + e)`; e >= accept_from means accepting.
 struct DfaDev {
-    const void *table;         // uint16 [256][n] or uint32 [256][n]
+    const void *table;         // uint8 [256][256], uint16 [256][n] or uint32 [256][n]
     int32_t n, wide;           // wide: uint32 entries
+    int32_t esz;               // entry size: 1, 2 or 4
     uint32_t start_entry, accept_from, col_bytes;
 };
 
@@ -127,7 +129,8 @@ void launch_dfa_gather(const DfaDev &d, const uint8_t *data, const int64_t *off,
                        int32_t *flags, int32_t *lens, hipStream_t stream);
 void launch_dfa_lds(const DfaDev &d, const uint8_t *data, const int64_t *off, int32_t count,
                     int32_t *flags, int32_t *lens, hipStream_t stream);
-size_t dfa_lds_bytes(int32_t n);
+// Bytes of the device table (what an LDS walk stages).
+size_t dfa_table_bytes(const DfaDev &d);
 
 // Split DFA walk (exact, in-input parallel): geometry and scratch of one launch.
 struct DfaSplitPlan {

@@ -445,8 +445,9 @@ DfaHost build_dfa(int32_t n, int32_t start, const uint8_t *accept, const int32_t
         if (trans[k] < 0 || trans[k] >= n)
             bad("trans entry out of range at index " + std::to_string(k));
     DfaHost h;
-    h.wide = n > 0x7fff;
-    const uint32_t esz = h.wide ? 4u : 2u;
+    h.esz = n <= 256 ? 1u : (n <= 0x7fff ? 2u : 4u);
+    h.wide = h.esz == 4;
+    const uint32_t esz = h.esz;
     // non-accepting states first, accepting states last (original order kept in each)
     std::vector<uint32_t> id(static_cast<size_t>(n));
     uint32_t next = 0;
@@ -457,15 +458,21 @@ DfaHost build_dfa(int32_t n, int32_t start, const uint8_t *accept, const int32_t
         if (accept[s]) id[s] = next++;
     h.accept_from = first_accepting * esz;
     h.start_entry = id[start] * esz;
-    h.col_bytes = static_cast<uint32_t>(n) * esz;
+    const size_t colen = esz == 1 ? 256 : static_cast<size_t>(n);  // entries per column
+    h.col_bytes = static_cast<uint32_t>(colen) * esz;
     h.start_accepts = accept[start] != 0;
-    if (h.wide) h.t32.resize(total); else h.t16.resize(total);
+    const size_t cells = colen * kAlphabet;
+    if (esz == 1) h.t8.assign(cells, 0);
+    else if (esz == 2) h.t16.resize(cells);
+    else h.t32.resize(cells);
     for (int32_t s = 0; s < n; ++s) {
         const uint32_t row = id[s];
         for (int c = 0; c < kAlphabet; ++c) {
             const uint32_t e = id[trans[static_cast<size_t>(s) * kAlphabet + c]] * esz;
-            const size_t at = static_cast<size_t>(c) * n + row;
-            if (h.wide) h.t32[at] = e; else h.t16[at] = static_cast<uint16_t>(e);
+            const size_t at = static_cast<size_t>(c) * colen + row;
+            if (esz == 1) h.t8[at] = static_cast<uint8_t>(e);
+            else if (esz == 2) h.t16[at] = static_cast<uint16_t>(e);
+            else h.t32[at] = e;
         }
     }
     return h;

@@ -98,17 +98,23 @@ struct GeneralHost {
 GeneralHost build_general(const NfaInput &a);
 
 // DFA device layout. States are renumbered so that the accepting ones come last; the
-// table is stored by byte (column c holds the successor of every state on byte c) and each
-// entry is the successor's byte offset inside a column (state * entry size). A step is then
-// one multiply-add and one load, and "the successor accepts" is "entry >= accept_from".
+// table is stored by byte (column c holds the successor of every state on byte c).
+//  * Up to 256 states: byte entries, the successor's id, in columns of 256 (padded), so the
+//    successor of entry e on byte c is at (c << 8) | e: 64 KiB.
+//  * Up to 32767 states: 16-bit entries, above that 32-bit, each the successor's byte
+//    offset inside its column (state * entry size), so a step is one multiply-add and one
+//    load.
+// Either way "the successor accepts" is "entry >= accept_from".
 struct DfaHost {
-    std::vector<uint16_t> t16;          // [256][n] when n <= 32767
+    std::vector<uint8_t> t8;            // [256][256] when n <= 256
+    std::vector<uint16_t> t16;          // [256][n] when 256 < n <= 32767
     std::vector<uint32_t> t32;          // [256][n] otherwise
-    bool wide = false;
+    uint32_t esz = 2;                   // entry size in bytes: 1, 2 or 4
+    bool wide = false;                  // esz == 4
     bool start_accepts = false;
     uint32_t start_entry = 0;           // renumbered start state, scaled
     uint32_t accept_from = 0;           // scaled id of the first accepting state
-    uint32_t col_bytes = 0;             // n * entry size
+    uint32_t col_bytes = 0;             // bytes per column: 256 (byte entries) or n * esz
 };
 DfaHost build_dfa(int32_t n, int32_t start, const uint8_t *accept, const int32_t *trans);
 

@@ -267,7 +267,8 @@ void check_dfa(Rng &r, int32_t n, double p_acc) {
     for (auto &x : acc) x = r.unit() < p_acc;
     const int32_t start = r.below(n);
     DfaHost h = build_dfa(n, start, acc.data(), trans.data());
-    if (h.wide != (n > 0x7fff)) fail("DFA entry width", n, 0);
+    if (h.esz != (n <= 256 ? 1u : (n <= 0x7fff ? 2u : 4u)) || h.wide != (h.esz == 4))
+        fail("DFA entry width", n, 0);
     const int inputs = 64;
     std::vector<uint8_t> data;
     std::vector<int64_t> off{0};
@@ -282,7 +283,8 @@ void check_dfa(Rng &r, int32_t n, double p_acc) {
     if (oracle_dfa_batch(n, start, acc.data(), trans.data(), data.data(), off.data(), inputs,
                          f.data(), m.data(), ex.data(), 2) != 0)
         fail("oracle_dfa_batch failed");
-    const uint32_t esz = h.wide ? 4u : 2u;
+    const uint32_t esz = h.esz;
+    const size_t colen = esz == 1 ? 256 : static_cast<size_t>(n);
     for (int k = 0; k < inputs; ++k) {
         // the kernels' walk: e = table[byte * col + e], accepting iff e >= accept_from
         uint32_t e = h.start_entry;
@@ -291,8 +293,8 @@ void check_dfa(Rng &r, int32_t n, double p_acc) {
             got = {1, 0};
         } else {
             for (int64_t q = off[k]; q < off[k + 1]; ++q) {
-                const size_t at = static_cast<size_t>(data[q]) * n + e / esz;
-                e = h.wide ? h.t32[at] : h.t16[at];
+                const size_t at = static_cast<size_t>(data[q]) * colen + e / esz;
+                e = esz == 1 ? h.t8[at] : (esz == 2 ? h.t16[at] : h.t32[at]);
                 if (e >= h.accept_from) {
                     got = {1, static_cast<int>(q - off[k] + 1)};
                     break;
