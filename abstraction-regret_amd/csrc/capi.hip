@@ -10,6 +10,8 @@
 #include <climits>
 #include <cstring>
 #include <functional>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -45,6 +47,11 @@ struct gfsm_dfa {
     std::vector<void *> blocks;
     int64_t device_bytes = 0;
 };
+
+// Scratch of the allocation-free entry point (gfsm_dfa_run_device_static): one array per
+// GPU, held until the process ends. In host code `scratch` names a host-side stand-in, not
+// the device array: its device address comes from hipGetSymbolAddress, per device.
+static __device__ __align__(256) unsigned char scratch[GFSM_STATIC_SCRATCH_BYTES];
 
 namespace {
 
@@ -355,6 +362,26 @@ void dfa_enqueue(gfsm_dfa *h, int32_t tech, const Tuning &tu, const uint8_t *d_d
         launch_dfa_lds(h->dev, d_data, d_off, count, d_flags, d_lens, s);
     else
         launch_dfa_gather(h->dev, d_data, d_off, count, d_flags, d_lens, s);
+}
+
+// Device address of `scratch` on the current device: looked up on the first call on each
+// device (with that device current, as hipGetSymbolAddress requires) and kept.
+std::mutex g_static_mu;
+std::map<int, void *> g_static_ptr;
+
+void *static_scratch() {
+    int dev = 0;
+    GFSM_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_static_mu);
+    auto it = g_static_ptr.find(dev);
+    if (it != g_static_ptr.end()) return it->second;
+    void *ptr = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (rc != hipSuccess || !ptr)
+        throw Error(GFSM_EDEVICE, std::string("hipGetSymbolAddress of the static scratch failed: ") +
+                                      hipGetErrorString(rc));
+    g_static_ptr[dev] = ptr;
+    return ptr;
 }
 
 void check_ws(const void *ws, int64_t ws_bytes) {
@@ -832,6 +859,30 @@ int gfsm_dfa_run_device(gfsm_dfa *dfa, int32_t technique, const uint8_t *d_data,
         const int32_t tech = resolve_dfa(dfa, technique, num_strings, total_bytes);
         dfa_enqueue(dfa, tech, Tuning(), d_data, d_offsets, num_strings, total_bytes, d_flags,
                     d_lens, nullptr, 0, static_cast<hipStream_t>(stream));
+    });
+}
+
+int gfsm_dfa_run_device_static(gfsm_dfa *dfa, int32_t technique, const gfsm_tuning *tuning,
+                               const uint8_t *d_data, const int64_t *d_offsets,
+                               int32_t num_strings, int64_t total_bytes, int32_t *d_flags,
+                               int32_t *d_lens, void *stream) {
+    return guarded([&] {
+        if (!dfa) throw Error(GFSM_EINVAL, "null handle");
+        check_batch_args(d_data, d_offsets, num_strings, d_flags, d_lens);
+        if (total_bytes < 0) throw Error(GFSM_EINVAL, "total_bytes must be >= 0");
+        const Tuning tu = read_tuning(tuning);
+        const int32_t tech = resolve_dfa(dfa, technique, num_strings, total_bytes, tu);
+        const size_t need = dfa_workspace(dfa, tech, num_strings, total_bytes, tu);
+        if (need > static_cast<size_t>(GFSM_STATIC_SCRATCH_BYTES))
+            throw Error(GFSM_EUNSUPPORTED,
+                        "the run needs " + std::to_string(need) + " bytes of scratch; the "
+                        "static scratch holds " + std::to_string(GFSM_STATIC_SCRATCH_BYTES) +
+                        " (use gfsm_dfa_run_device_ws with a larger workspace, or a smaller "
+                        "batch)");
+        void *ws = static_scratch();  // recorded on the first call on this device
+        dfa_enqueue(dfa, tech, tu, d_data, d_offsets, num_strings, total_bytes, d_flags, d_lens,
+                    ws, static_cast<size_t>(GFSM_STATIC_SCRATCH_BYTES),
+                    static_cast<hipStream_t>(stream));
     });
 }
 

@@ -50,6 +50,7 @@ EXPORTS = (
     "gfsm_dfa_workspace_size",
     "gfsm_dfa_run_device_ws",
     "gfsm_dfa_resolve",
+    "gfsm_dfa_run_device_static",
 )
 
 TUNE_CHAIN_FIRST = 1
@@ -142,7 +143,11 @@ _SIGNATURES = {
     "gfsm_dfa_run_device_ws": (_c.c_int, [_vp, _i32, _c.POINTER(Tuning), _vp, _vp, _i32,
                                           _c.c_int64, _vp, _vp, _vp, _c.c_int64, _vp]),
     "gfsm_dfa_resolve": (_c.c_int, [_vp, _i32, _c.POINTER(Tuning), _i32, _c.c_int64, _i32p]),
+    "gfsm_dfa_run_device_static": (_c.c_int, [_vp, _i32, _c.POINTER(Tuning), _vp, _vp, _i32,
+                                              _c.c_int64, _vp, _vp, _vp]),
 }
+
+STATIC_SCRATCH_BYTES = 32 << 20  # GFSM_STATIC_SCRATCH_BYTES
 
 _lib: ctypes.CDLL | None = None
 _load_error: str | None = None

@@ -209,6 +209,22 @@ int gfsm_dfa_run_device_ws(gfsm_dfa *dfa, int32_t technique, const gfsm_tuning *
 int gfsm_dfa_resolve(gfsm_dfa *dfa, int32_t technique, const gfsm_tuning *tuning,
                      int32_t num_strings, int64_t total_bytes, int32_t *out_technique);
 
+/* Allocation-free DFA run. No counterpart in the upstream project: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same inputs, outputs and asynchrony as
+ * gfsm_dfa_run_device_ws, but the technique's scratch (the split walk's segment state) is a
+ * library array of GFSM_STATIC_SCRATCH_BYTES in each GPU's memory, held until the process
+ * ends; the call makes no device allocation (its first call on a device looks up that
+ * device's array). A run needing more scratch returns GFSM_EUNSUPPORTED before enqueuing
+ * anything (gfsm_dfa_workspace_size gives the need; about 19 MiB for 4096 x 64 KiB on a
+ * 256-state table). Every call on one GPU shares the array: calls whose work overlaps on
+ * one GPU (other streams, other threads) race and give wrong results; serialise them. */
+#define GFSM_STATIC_SCRATCH_BYTES (32 << 20)
+int gfsm_dfa_run_device_static(gfsm_dfa *dfa, int32_t technique, const gfsm_tuning *tuning,
+                               const uint8_t *d_data, const int64_t *d_offsets,
+                               int32_t num_strings, int64_t total_bytes, int32_t *d_flags,
+                               int32_t *d_lens, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

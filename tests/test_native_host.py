@@ -26,7 +26,11 @@ ROOT = Path(__file__).resolve().parent.parent
 CSRC = ROOT / "abstraction-regret_amd" / "csrc"
 SAN = ["-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined",
        "-fno-omit-frame-pointer"]
-HIP_HOST = ["-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", f"-I{CSRC}"]
+# host C++ view of the HIP sources: the HIP headers' host declarations, plus the two macros
+# the HIP compiler itself provides (__align__ as the GNU attribute, HIP_SYMBOL as the
+# symbol's address)
+HIP_HOST = ["-D__HIP_PLATFORM_AMD__", "-D__align__(x)=__attribute__((aligned(x)))",
+            "-DHIP_SYMBOL(x)=(&(x))", "-I/opt/rocm/include", f"-I{CSRC}"]
 KERNEL_OBJECTS = ["dfa.o", "nfa.o", "chain.o", "general.o"]
 
 pytestmark = pytest.mark.skipif(shutil.which("g++") is None or shutil.which("gcc") is None,
