@@ -22,6 +22,7 @@ work itself is the C ABI of ``libgpufsm_hip.so``.
 from __future__ import annotations
 
 import collections
+import concurrent.futures
 import itertools
 import threading
 import time
@@ -93,6 +94,32 @@ class ResidentCache:
 
 _batch_serials = itertools.count(1)
 
+# Host copies into pinned staging are split over a few threads above this size: one thread
+# copies at a fraction of what the host's memory and the PCIe link carry, and numpy drops
+# the GIL inside a contiguous copy.
+_COPY_SPLIT_BYTES = 16 << 20
+_COPY_THREADS = 4
+_copy_pool: concurrent.futures.ThreadPoolExecutor | None = None
+_copy_pool_lock = threading.Lock()
+
+
+def _copy_into(dst: np.ndarray, src: np.ndarray) -> None:
+    """``dst[:] = src`` for 1-D uint8 arrays, split over threads when large."""
+    global _copy_pool
+    n = int(src.size)
+    if n < _COPY_SPLIT_BYTES:
+        dst[:] = src
+        return
+    with _copy_pool_lock:
+        if _copy_pool is None:
+            _copy_pool = concurrent.futures.ThreadPoolExecutor(_COPY_THREADS,
+                                                               thread_name_prefix="gpufsm-copy")
+    step = -(-n // _COPY_THREADS)
+    futs = [_copy_pool.submit(np.copyto, dst[a:a + step], src[a:a + step])
+            for a in range(0, n, step)]
+    for f in futs:
+        f.result()
+
 
 class DeviceBatch:
     """Packed inputs, outputs and launch workspace resident in HBM.
@@ -163,7 +190,7 @@ class DeviceBatch:
         # the previous copy from or into the pinned buffers may still be in flight
         if self._staged is not None:
             self._staged.synchronize()
-        self._h_data[:total].numpy()[:] = np.asarray(data[:total], dtype=np.uint8)
+        _copy_into(self._h_data[:total].numpy(), np.asarray(data[:total], dtype=np.uint8))
         self._h_off[: count + 1].numpy()[:] = offsets
         # asynchronous copies on the stream's raw handle (no stream context switch)
         s = stream.cuda_stream
@@ -357,6 +384,42 @@ class BatchEngine:
         e1.record(stream)
         flags, lens = batch.fetch(stream)
         return flags, lens, e0.elapsed_time(e1)
+
+    def run_stream(self, automaton, batches, technique: str = "auto", depth: int = 2,
+                   graph: bool = True, capacity_bytes: int = 0, capacity_count: int = 0):
+        """Verdicts ``(flags, lens)`` of a stream of host batches ``(data, offsets)``, in order.
+
+        ``depth`` slots run at once, each with its own stream, resident batch buffers and
+        captured graph: while the GPU walks batch k, the host copies batch k + 1 into the
+        next slot's pinned staging and its host-to-device copy is enqueued, so the staging
+        cost of a host-resident stream overlaps the kernels instead of adding to them. A
+        slot is reused once its previous verdicts have been taken. Batches must fit the
+        first batch's size unless ``capacity_bytes`` / ``capacity_count`` say more.
+        """
+        torch = self._torch
+        if depth < 1:
+            raise ValueError("depth must be >= 1")
+        slots: list[tuple[DeviceBatch, object]] = []
+        pending: collections.deque = collections.deque()  # (slot, event) in batch order
+        for k, (data, offsets) in enumerate(batches):
+            j = k % depth
+            while pending and any(p[0] == j for p in pending):
+                slot, ev = pending.popleft()
+                ev.synchronize()
+                yield slots[slot][0].results()
+            if j == len(slots):
+                st = torch.cuda.Stream()  # default priority
+                slots.append((DeviceBatch(data, offsets, capacity_bytes=capacity_bytes,
+                                          capacity_count=capacity_count, stream=st), st))
+            else:
+                slots[j][0].load(data, offsets, stream=slots[j][1])
+            batch, st = slots[j]
+            self.enqueue(automaton, batch, technique, stream=st, graph=graph)
+            pending.append((j, batch.download(st)))
+        while pending:
+            slot, ev = pending.popleft()
+            ev.synchronize()
+            yield slots[slot][0].results()
 
     def run_batch(self, automaton, inputs: list[bytes], technique: str = "auto") -> list[Result]:
         """Drop-in ``run_batch`` over the resident path (same results, same Result layout)."""

@@ -307,3 +307,70 @@ def test_tuner_picks_the_per_input_walk_for_many_short_inputs():
         assert choice.technique == "lds", choice
     finally:
         eng.close()
+
+
+def test_serving_configured_mix_equals_alone_and_oracle():
+    """The configured serving mix: 4 random DFAs of 256 to 16,384 states and 4 pattern
+    unions of 1,024 to 65,536 states resident together, Poisson arrivals, log-normal request
+    lengths (median 16 KiB) plus requests of the 4 MiB cap on the largest automata. Every
+    served verdict equals the request's lone run and the oracle."""
+    import torch  # noqa: F401  (the loop's device buffers)
+
+    automata, unions = W.serving_automata(seed=0)
+    reqs = W.serving_requests(automata, unions, 240, seed=3)
+    rng = np.random.default_rng(4)
+    for name in ("dfa16384", "nfa65536", "dfa256", "nfa1024"):
+        n = 4 << 20
+        if name in unions:
+            d, _ = W.pattern_inputs(unions[name], np.array([n], dtype=np.int64), 0.70, seed=5)
+        else:
+            d = rng.integers(0, 256, size=n, dtype=np.uint8)
+        reqs.insert(int(rng.integers(0, len(reqs))), (name, d.tobytes()))
+    assert max(len(x) for _, x in reqs) == 4 << 20
+    loop = ServingLoop(automata, budget_bytes=32 << 30, slots=8)
+    with loop:
+        done = poisson_drive(loop, reqs, qps=4000, seed=6)
+    assert all(r.status == "ok" for r in done)
+    assert loop.launches < len(reqs)  # requests shared launches
+    # the oracle per automaton over all its requests at once; lone runs one by one
+    by_name: dict = {}
+    for k, (name, data) in enumerate(reqs):
+        by_name.setdefault(name, []).append(k)
+    alone = BatchEngine(tuner=None)
+    try:
+        for name, ks in by_name.items():
+            data, off = pack_bytes([reqs[k][1] for k in ks])
+            wf, wn = _oracle(automata[name], data, off)
+            for j, k in enumerate(ks):
+                r = done[k]
+                d = np.frombuffer(reqs[k][1], dtype=np.uint8)
+                f, n, _ = alone.run_packed(automata[name], d, np.array([0, d.size], np.int64))
+                assert (r.accepted, r.match_len) == (bool(f[0]), int(n[0])), (name, k)
+                assert (r.accepted, r.match_len) == (bool(wf[j]), int(wn[j])), (name, k)
+    finally:
+        alone.close()
+
+
+@pytest.mark.parametrize("depth", [1, 2, 3])
+def test_run_stream_matches_oracle_in_order(depth):
+    """A stream of host batches through several slots (their own streams, buffers and
+    graphs): every batch's verdicts come back in order and match the oracle, also when a
+    later batch is smaller than the first."""
+    eng = BatchEngine(tuner=None)
+    try:
+        pu = W.pattern_union(300, (4, 20), (1, 24), seed=61)
+        batches = []
+        for k in range(7):
+            count = 400 if k != 5 else 90
+            lens = W.lognormal_lengths(count, 2000, 1.0, 0, 40000, seed=62 + k)
+            batches.append(W.pattern_inputs(pu, lens, 0.7, seed=70 + k))
+        cap_b = max(int(o[-1]) for _, o in batches)
+        got = list(eng.run_stream(pu.nfa, batches, "auto", depth=depth, capacity_bytes=cap_b,
+                                  capacity_count=400))
+        assert len(got) == len(batches)
+        for k, ((d, o), (f, n)) in enumerate(zip(batches, got)):
+            wf, wn = _oracle(pu.nfa, d, o)
+            np.testing.assert_array_equal(f, wf, err_msg=f"batch {k}")
+            np.testing.assert_array_equal(n, wn, err_msg=f"batch {k}")
+    finally:
+        eng.close()

@@ -97,6 +97,56 @@ def test_closed_forms():
     assert simulate_dfa(dfa, b"") == (False, 0)
 
 
+def test_examined_closed_forms():
+    """Bytes examined, from the automaton's shape alone (Python and C oracles):
+
+    * an anchored literal ``abc``: an input whose set dies on byte k (0-based) examined k + 1;
+      a match examined its match length; an input that ends first examined all of it;
+    * an accepting start: 0 for every input;
+    * a start with an ANY self-loop and nothing accepting: every byte of every input;
+    * an epsilon edge into a state looping on ``a``: ``a`` * m then ``b`` dies on byte m.
+    """
+    b = NFABuilder()
+    s = [b.add_state(accept=(i == 3)) for i in range(4)]
+    b.set_start(s[0])
+    for i, c in enumerate("abc"):
+        b.add_transition(s[i], c, s[i + 1])
+    lit = b.build()
+    cases = {b"xyz": 1, b"ayz": 2, b"abz": 3, b"abcz": 3, b"abc": 3, b"ab": 2, b"": 0,
+             b"ab" + b"q" * 50: 3}
+    b = NFABuilder()
+    b.set_start(b.add_state(accept=True))
+    acc = b.build()
+    b = NFABuilder()
+    b.set_start(b.add_state())
+    b.add_transition(0, ANY_SYMBOL, 0)
+    b.add_transition(0, "a", b.add_state())
+    loop = b.build()
+    b = NFABuilder()
+    t0, t1 = b.add_state(), b.add_state()
+    b.set_start(t0)
+    b.add_epsilon(t0, t1)
+    b.add_transition(t1, "a", t1)
+    eps = b.build()
+    rng = random.Random(5)
+    for m in (0, 1, 7, 300):
+        cases_eps = b"a" * m + b"b" + b"a" * 9
+        assert examined_nfa(eps, cases_eps) == m + 1
+    for nfa, want in ((lit, cases),
+                      (acc, {x: 0 for x in cases}),
+                      (loop, {bytes(rng.randrange(256) for _ in range(n)): n
+                              for n in (0, 1, 5, 999)})):
+        batch = list(want)
+        assert [examined_nfa(nfa, x) for x in batch] == [want[x] for x in batch]
+        data, off = pack_bytes(batch)
+        for indexed in (False, True):
+            _, _, ex = native.nfa_batch(nfa, data, off, indexed=indexed)
+            assert list(ex) == [want[x] for x in batch], indexed
+    data, off = pack_bytes([b"a" * m + b"b" for m in (0, 3, 64)])
+    _, _, ex = native.nfa_batch(eps, data, off)
+    assert list(ex) == [1, 4, 65]
+
+
 _EXAMPLE_REGEX = {"ab_star_c_plus_d": rb"(?:ab*c+)+d", "a_star_b_c_opt_d": rb"a*bc?d"}
 
 
