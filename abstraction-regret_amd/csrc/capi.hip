@@ -267,7 +267,8 @@ size_t nfa_workspace(const gfsm_nfa *h, int32_t tech, int32_t count, int64_t tot
                      const Tuning &tu) {
     if (count <= 0 || h->start_accepts) return 0;
     if (tech == GFSM_NFA_CHAIN)
-        return nfa_chain_scratch_bytes(h->chain, count, total, nfa_chain_chunk_bytes(tu.chunk));
+        return nfa_chain_scratch_bytes(h->chain, count, total,
+                                       nfa_chain_chunk_bytes(tu.chunk, total));
     if (tech == GFSM_NFA_BITPACKED)
         return h->chain_ok ? align256(static_cast<size_t>(count) * sizeof(int32_t)) : 0;
     if (tech == GFSM_NFA_REGISTER) return 0;
@@ -303,7 +304,7 @@ void nfa_enqueue(gfsm_nfa *h, int32_t tech, const Tuning &tu, const uint8_t *d_d
         const bool paired =
             !(tu.flags & GFSM_TUNE_CHAIN_WAVE64) && env_int("GFSM_CHAIN_WAVE64", 0) != 1;
         launch_nfa_chain(h->chain, h->tile, d_data, d_off, count, total, d_flags, d_lens, ws,
-                         nfa_chain_chunk_bytes(tu.chunk), pairs, paired, s);
+                         nfa_chain_chunk_bytes(tu.chunk, total), pairs, paired, s);
         return;
     }
     if (tech == GFSM_NFA_BITPACKED) {

@@ -923,8 +923,15 @@ size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 }  // namespace
 
-int32_t nfa_chain_chunk_bytes(int32_t requested) {
-    const int64_t c = requested > 0 ? requested : env_int("GFSM_CHAIN_CHUNK", 4096);
+int32_t nfa_chain_chunk_bytes(int32_t requested, int64_t total_bytes) {
+    int64_t c = requested > 0 ? requested : env_int("GFSM_CHAIN_CHUNK", 0);
+    if (c <= 0) {
+        // 4 KiB items for large batches; a small batch (a few requests of a serving launch)
+        // is cut finer, down to 512 B, so that it still makes about 4096 items: a request's
+        // latency is the walk of its longest item, about 200 ns per byte
+        c = 4096;
+        while (c > 512 && total_bytes / c < 4096) c >>= 1;
+    }
     return static_cast<int32_t>(c < 64 ? 64 : (c > (1 << 30) ? (1 << 30) : c));
 }
 

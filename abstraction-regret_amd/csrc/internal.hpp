@@ -193,8 +193,9 @@ void launch_nfa_best_finish(const int32_t *best, int32_t count, int32_t *flags, 
 // Sparse chain technique: persistent waves over (input, chunk) work items.
 size_t nfa_chain_scratch_bytes(const NfaChainDev &d, int32_t count, int64_t total_bytes,
                                int32_t chunk_bytes);
-// chunk_bytes when the caller has no preference: GFSM_CHAIN_CHUNK (read per call), or 4096
-int32_t nfa_chain_chunk_bytes(int32_t requested);
+// chunk_bytes when the caller has no preference: GFSM_CHAIN_CHUNK (read per call), else
+// 4096 for batches of 16 MiB or more and down to 512 for smaller ones
+int32_t nfa_chain_chunk_bytes(int32_t requested, int64_t total_bytes);
 // pairs: use the byte-pair lists (register tier first) when the handle has them.
 // paired: the register tier runs two items per wave (32 listed states each) instead of one
 // (64 listed states).

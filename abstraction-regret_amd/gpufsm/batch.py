@@ -157,6 +157,7 @@ class DeviceBatch:
         # hipMemcpyAsync calls, which PyTorch's allocators do not track, so it is waited for
         # before a pinned buffer is rewritten and before the buffers are released
         self._staged = None
+        self._event = None   # one event, re-recorded after each staging copy (see _mark)
         self.load(data, offsets, stream=stream)
 
     def __del__(self) -> None:
@@ -197,10 +198,33 @@ class DeviceBatch:
         _lib.memcpy_async(self.data.data_ptr(), self._h_data.data_ptr(), total, _lib.MEMCPY_H2D, s)
         _lib.memcpy_async(self.offsets.data_ptr(), self._h_off.data_ptr(), 8 * (count + 1),
                           _lib.MEMCPY_H2D, s)
-        self._staged = torch.cuda.Event()
-        self._staged.record(stream)
+        self._mark(stream)
         self.count, self.total = count, total
         self.max_len = int(lengths.max()) if count else 0
+
+    def load_groups(self, data: np.ndarray, offsets: np.ndarray, count: int,
+                    stream=None) -> None:
+        """Stage several sub-batches at once: ``data`` holds them back to back (each at a
+        256-byte aligned start), ``offsets`` their offset arrays back to back (each starting
+        at 0), ``count`` the inputs of all of them. The caller launches each sub-batch over
+        its own slice (:meth:`BatchEngine.enqueue_range`); the verdicts of input k of the
+        whole group land at index k. The caller has checked every sub-batch's offsets."""
+        torch = self._torch
+        total, n_off = int(data.size), int(offsets.size)
+        if total > self.cap_bytes or n_off > self.cap_count + 1 or count > self.cap_count:
+            raise ValueError(f"grouped batch of {count} inputs / {total} bytes exceeds the "
+                             f"capacity {self.cap_count} / {self.cap_bytes}")
+        stream = stream or torch.cuda.current_stream()
+        if self._staged is not None:
+            self._staged.synchronize()
+        _copy_into(self._h_data[:total].numpy(), np.asarray(data, dtype=np.uint8))
+        self._h_off[:n_off].numpy()[:] = offsets
+        s = stream.cuda_stream
+        _lib.memcpy_async(self.data.data_ptr(), self._h_data.data_ptr(), total, _lib.MEMCPY_H2D, s)
+        _lib.memcpy_async(self.offsets.data_ptr(), self._h_off.data_ptr(), 8 * n_off,
+                          _lib.MEMCPY_H2D, s)
+        self._mark(stream)
+        self.count, self.total, self.max_len = count, total, 0
 
     def download(self, stream):
         """Enqueue the copy of the verdicts into the pinned output buffer on ``stream``;
@@ -209,9 +233,17 @@ class DeviceBatch:
         h = self._h_out.data_ptr()
         _lib.memcpy_async(h, self.flags.data_ptr(), 4 * c, _lib.MEMCPY_D2H, s)
         _lib.memcpy_async(h + 4 * self.cap_count, self.lens.data_ptr(), 4 * c, _lib.MEMCPY_D2H, s)
-        self._staged = self._torch.cuda.Event()
-        self._staged.record(stream)
-        return self._staged
+        return self._mark(stream)
+
+    def _mark(self, stream):
+        """Record the batch's event after the copies just enqueued on ``stream``. One event
+        serves every copy: a pinned buffer is rewritten only after the previous copy's event
+        has been waited for, so no one still waits on an older recording."""
+        if self._event is None:
+            self._event = self._torch.cuda.Event()
+        self._event.record(stream)
+        self._staged = self._event
+        return self._event
 
     def results(self) -> tuple[np.ndarray, np.ndarray]:
         """The verdicts of the last :meth:`download` (once its stream has passed it)."""
@@ -329,6 +361,28 @@ class BatchEngine:
         dev.run_device_ws(batch.data.data_ptr(), batch.offsets.data_ptr(), c, total,
                           batch.flags.data_ptr(), batch.lens.data_ptr(), ws,
                           batch.workspace.numel(), stream.cuda_stream, tech, tuning)
+
+    def enqueue_range(self, automaton, batch: DeviceBatch, technique: str, stream,
+                      byte_start: int, off_start: int, in_start: int, count: int, total: int,
+                      max_len: int) -> None:
+        """Enqueue one run over a slice of ``batch`` (one sub-batch of :meth:`DeviceBatch.
+        load_groups`): inputs from data byte ``byte_start`` (256-byte aligned) with offsets
+        from entry ``off_start``, verdicts into index ``in_start`` onward. Runs of one batch
+        on one stream run in order, so they share the batch's workspace. ``technique`` is
+        fixed or "auto" (resolved by the library for the slice's shape; not autotuned)."""
+        if count == 0:
+            return
+        kind = Kind.of(automaton)
+        dev = self.cache.get(automaton)
+        choice = Choice.parse(technique)
+        tech = _techniques(kind)[choice.technique]
+        tuning = choice.tuning(max(int(max_len), 1))
+        need = dev.workspace_size(count, total, tech, tuning)
+        ws = batch.ensure_workspace(need)
+        dev.run_device_ws(batch.data.data_ptr() + byte_start, batch.offsets.data_ptr() + 8 * off_start,
+                          count, total, batch.flags.data_ptr() + 4 * in_start,
+                          batch.lens.data_ptr() + 4 * in_start, ws, batch.workspace.numel(),
+                          stream.cuda_stream, tech, tuning)
 
     def enqueue(self, automaton, batch: DeviceBatch, technique: str = "auto", stream=None,
                 graph: bool = False) -> Choice:

@@ -15,7 +15,8 @@ import numpy as np
 import pytest
 
 from gpufsm import workloads as W
-from gpufsm.serve import Admission, Request, ServingLoop, form_batch, poisson_drive
+from gpufsm.serve import (Admission, Request, ServingLoop, form_batch, form_launch, pack_groups,
+                          poisson_drive)
 from oracle import native
 
 
@@ -57,6 +58,90 @@ class HostSlots:
         out = self.pending[i]
         self.pending[i] = None
         return out
+
+
+class GroupedHostSlots(HostSlots):
+    """HostSlots with the grouped launch of gpufsm.serve.GpuSlots: each group's verdicts
+    from the oracle over its slice of the packed arrays, at the group's output index."""
+
+    def __init__(self, *a, **kw):
+        super().__init__(*a, **kw)
+        self.group_log = []           # automata per launch
+
+    def launch_groups(self, i, data, offsets, table, count):
+        f = np.full(count, -1, np.int32)
+        n = np.full(count, -1, np.int32)
+        for name, byte_at, off_at, in_at, cnt, total, max_len in table:
+            assert byte_at % 256 == 0
+            o = offsets[off_at:off_at + cnt + 1]
+            assert o[0] == 0 and int(o[-1]) == total
+            assert max_len == (int(np.diff(o).max()) if cnt else 0)
+            a = self.automata[name]
+            fn = native.dfa_batch if hasattr(a, "trans") else native.nfa_batch
+            gf, gn, _ = fn(a, data[byte_at:byte_at + total], o)
+            f[in_at:in_at + cnt], n[in_at:in_at + cnt] = gf, gn
+        assert (f >= 0).all()
+        self.pending[i] = (f, n)
+        self.ready_at[i] = time.perf_counter() + self.delay
+        self.group_log.append([t[0] for t in table])
+        busy = sum(p is not None for p in self.pending)
+        self.concurrent_max = max(self.concurrent_max, busy)
+
+
+def test_form_launch_and_pack_groups():
+    mk = lambda name, n: Request(name, bytes((7 * k + n) % 256 for k in range(n)), 0.0)  # noqa: E731
+    q = collections.deque([mk("a", 10), mk("b", 5), mk("a", 300), mk("c", 0), mk("b", 7)])
+    groups = form_launch(q, max_bytes=10 + 5 + 300 + 0 + 7 + 3 * 256, max_count=10)
+    assert [(g, [len(r.data) for r in rs]) for g, rs in groups] == \
+        [("a", [10, 300]), ("b", [5, 7]), ("c", [0])]
+    assert not q
+    data, offs, table, count = pack_groups(groups)
+    assert count == 5 and [t[0] for t in table] == ["a", "b", "c"]
+    for (name, byte_at, off_at, in_at, cnt, total, max_len), (_, rs) in zip(table, groups):
+        assert byte_at % 256 == 0 and offs[off_at] == 0 and int(offs[off_at + cnt]) == total
+        got = [data[byte_at + offs[off_at + k]:byte_at + offs[off_at + k + 1]].tobytes()
+               for k in range(cnt)]
+        assert got == [r.data for r in rs]
+    # the byte cap counts the 256-byte padding of each new automaton; the head always goes
+    q = collections.deque([mk("a", 100), mk("b", 100)])
+    assert [g for g, _ in form_launch(q, max_bytes=300, max_count=10)] == ["a"]
+    q = collections.deque([mk("a", 5000)])
+    assert len(form_launch(q, max_bytes=100, max_count=4)) == 1
+    q = collections.deque([mk("a", 1), mk("a", 1), mk("b", 1)])
+    assert [len(rs) for _, rs in form_launch(q, max_bytes=10**6, max_count=2)] == [2]
+
+
+def test_grouped_launches_mix_results_equal_alone():
+    automata, pu = _automata()
+    reqs = _requests(automata, pu, 150, seed=14)
+    slots = GroupedHostSlots(automata, 2, delay=0.003)
+    loop = ServingLoop(automata, budget_bytes=1 << 30, max_batch_bytes=64 << 10,
+                       max_batch_count=64, backend=slots, group_automata=True)
+    with loop:
+        done = poisson_drive(loop, reqs, qps=20000, seed=15)
+    assert all(r.status == "ok" for r in done)
+    for (name, data), r in zip(reqs, done):
+        assert (r.accepted, r.match_len) == _alone(automata, name, data)
+    # at this rate launches carry several automata at once
+    assert max(len(g) for g in slots.group_log) > 1
+    assert loop.launches < len(reqs)
+
+
+def test_inflight_cap_per_automaton_grows_launches():
+    """One automaton, 4 slots, at most one launch of it in flight: requests that arrive
+    meanwhile wait and leave together, results equal their lone runs."""
+    automata, pu = _automata()
+    reqs = [(n, d) for n, d in _requests(automata, pu, 200, seed=21) if n == "dfa"]
+    slots = HostSlots(automata, 4, delay=0.004)
+    loop = ServingLoop(automata, budget_bytes=1 << 30, max_batch_bytes=1 << 20,
+                       max_batch_count=256, backend=slots, max_inflight_per_automaton=1)
+    with loop:
+        done = poisson_drive(loop, reqs, qps=20000, seed=22)
+    assert all(r.status == "ok" for r in done)
+    for (name, data), r in zip(reqs, done):
+        assert (r.accepted, r.match_len) == _alone(automata, name, data)
+    assert slots.concurrent_max == 1
+    assert loop.launches < len(reqs) / 3
 
 
 def _automata():
