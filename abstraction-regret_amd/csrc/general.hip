@@ -170,7 +170,25 @@ __global__ __launch_bounds__(64 * kGenWaves) void nfa_gather_kernel(
             const int32_t e0 = boff[c];
             const int32_t nc = boff[c + 1] - e0;
             const int32_t total = nc + (any1 - any0);
-            for (int32_t j = lane; j < total; j += 64) {
+            // eight edges per lane per round, their loads issued together (the edge lists of
+            // a dense automaton are read from L2 or the MALL, one round trip per round)
+            int32_t j = lane;
+            for (; j + 448 < total; j += 512) {
+                uint32_t v[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int32_t x = j + 64 * q;
+                    v[q] = bedge[x < nc ? e0 + x : any0 + (x - nc)];
+                }
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const uint32_t s = v[q] >> 16, t = v[q] & 0xffffu;
+                    if ((cur[s >> 6] >> (s & 63)) & 1ull)
+                        atomicOr(reinterpret_cast<unsigned long long *>(&nxt[t >> 6]),
+                                 1ull << (t & 63));
+                }
+            }
+            for (; j < total; j += 64) {
                 const uint32_t v = bedge[j < nc ? e0 + j : any0 + (j - nc)];
                 const uint32_t s = v >> 16, t = v & 0xffffu;
                 if ((cur[s >> 6] >> (s & 63)) & 1ull)

@@ -657,6 +657,92 @@ def bench_divergent(args, torch) -> dict:
     return out
 
 
+def bench_general(args, torch) -> dict:
+    """General (not chain-structured) NFA with dense active sets: the bit-parallel step
+    against the worklist.
+
+    The high-divergence union (65,536 states, 8000 patterns, class widths 16-64) with
+    optional positions (epsilon edges s -> s + 1 on a quarter of the positions, so neither
+    chain nor the tiled shift step applies) and no accepting state: with 95% pattern bytes
+    thousands of states stay active on every byte and every byte is walked. Each technique
+    is gated against the C oracle on a prefix the oracle walks in seconds, then timed on its
+    own batch (the worklist on a smaller one: its work grows with the active set).
+    """
+    from gpufsm import workloads as W
+    from gpufsm.backends.hip import _lib
+    from gpufsm.backends.hip.handles import DeviceNFA
+
+    from oracle import gate, native
+
+    stream = torch.cuda.current_stream().cuda_stream
+    t0 = time.perf_counter()
+    pu = W.pattern_union(8000, (5, 12), (16, 64), seed=0, total_states=65536, accepting=False)
+    nfa = W.optional_positions(pu, 0.25, seed=1)
+    shapes = {"bitpacked": (args.gen_count, args.gen_length),
+              "worklist": (args.gen_wl_count, args.gen_wl_length)}
+    out: dict = {"workload": "65536-state union of 8000 patterns (class widths 16-64) with "
+                             f"{int(nfa.eps_row_ptr[-1])} optional positions (epsilon edges), "
+                             "no accepting state, 95% pattern bytes",
+                 "techniques": {}}
+    threads = _threads()
+    with DeviceNFA(nfa) as dev:
+        assert not dev.props.chain_ok and dev.props.general_ok
+        out["resident_bytes"] = int(dev.props.device_bytes)
+        _log(f"general nfa: {nfa.num_states} states, not chain-structured, folded layout "
+             f"{dev.props.device_bytes / 1e6:.1f} MB ({time.perf_counter() - t0:.1f} s)")
+        for tech, (count, length) in shapes.items():
+            d, o = W.pattern_inputs(pu, np.full(count, length, dtype=np.int64), 0.95, seed=2)
+            batch = DeviceBatch(torch, d, o)
+            # gate on a prefix (8 inputs x 2 KiB)
+            gd, go = W.pattern_inputs(pu, np.full(8, 2048, dtype=np.int64), 0.95, seed=3)
+            gb = DeviceBatch(torch, gd, go)
+            wf, wn, _ = native.nfa_batch(nfa, gd, go, threads=threads)
+            gb.run(dev, _lib.NFA_TECHNIQUES[tech], stream)
+            torch.cuda.synchronize()
+            f, n = gb.results()
+            gate.require_arrays(f, n, wf, wn, f"hip general nfa {tech} (prefix)")
+            wall, ks = _time_steps(torch, lambda: batch.run(dev, _lib.NFA_TECHNIQUES[tech],
+                                                            stream), 2, 1)
+            sec = wall / 2
+            f, _ = batch.results()
+            assert not f.any()
+            out["techniques"][tech] = {"inputs": count, "input_bytes": length,
+                                       "ms_per_batch": 1e3 * sec,
+                                       "kernel_ms": float(np.mean(ks)),
+                                       "examined_GBps": int(o[-1]) / sec / 1e9}
+            _log(f"general nfa {tech}: {count} x {length} B in {1e3 * sec:.2f} ms, "
+                 f"{int(o[-1]) / sec / 1e9:.5f} GB/s examined")
+    r = out["techniques"]
+    out["bitpacked_over_worklist"] = r["bitpacked"]["examined_GBps"] / r["worklist"]["examined_GBps"]
+    return out
+
+
+def general_line(args, torch) -> dict:
+    """The general-NFA control as the line (for profiling its kernels alone)."""
+    r = bench_general(args, torch)
+    t = r["techniques"]["bitpacked"]
+    nbytes = t["inputs"] * t["input_bytes"]
+    algorithmic = nbytes + r["resident_bytes"] + 16 * t["inputs"] + 8
+    return {
+        "metric": "input GB/s batched vs plain run_batch on MI355X; p50/p99 latency at fixed QPS",
+        "value": t["examined_GBps"],
+        "unit": "GB/s",
+        "ms_per_step": t["ms_per_batch"],
+        "dtype": "u8",
+        "config": {"workload": f"general NFA control (resident; value = examined input GB/s of "
+                               f"bitpacked): {r['workload']}, {t['inputs']} inputs x "
+                               f"{t['input_bytes']} bytes; the worklist under extras",
+                   "resolved_technique": "bitpacked"},
+        "roofline": {"bound": "hbm", "achieved": algorithmic / (t["kernel_ms"] * 1e-3) / 1e9,
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": algorithmic / (t["kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+                     "traffic": None, "kernel": "nfa_gather_kernel", "kernel_ms": t["kernel_ms"],
+                     "algorithmic_bytes_per_launch": algorithmic},
+        "cpu_baseline": None,
+        "extras": {"general_nfa": r},
+    }
+
+
 def ragged_line(args, torch) -> dict:
     """The ragged NFA mix as the headline: resident, autotuned, graph-captured batches."""
     r = bench_ragged(args, torch)
@@ -971,7 +1057,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="dfa", choices=["dfa", "ragged", "divergent"],
+    ap.add_argument("--workload", default="dfa",
+                    choices=["dfa", "ragged", "divergent", "general"],
                     help="dfa: the DFA memory-bound face (default, with every extra); "
                          "ragged: the ragged NFA mix as the line; divergent: the "
                          "high-divergence NFA as the line")
@@ -991,9 +1078,13 @@ def main() -> int:
     ap.add_argument("--div-sample-inputs", type=int, default=1024)
     ap.add_argument("--div-sample-len", type=int, default=1 << 14)
     ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--gen-count", type=int, default=1024)
+    ap.add_argument("--gen-length", type=int, default=1 << 14)
+    ap.add_argument("--gen-wl-count", type=int, default=64)
+    ap.add_argument("--gen-wl-length", type=int, default=1 << 12)
     ap.add_argument("--only", default="",
                     help="comma list of extra sections to run (ragged,plain,serving,"
-                         "divergent); default all")
+                         "divergent,general); default all")
     ap.add_argument("--profile", action="store_true",
                     help="headline variant only, plus one device copy of the input bytes "
                          "(the calibration point for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -1026,11 +1117,13 @@ def main() -> int:
         out = ragged_line(args, torch)
     elif args.workload == "divergent":
         out = divergent_line(args, torch)
+    elif args.workload == "general":
+        out = general_line(args, torch)
     else:
         out = bench_dfa(args, torch)
     if not args.profile and args.workload == "dfa":
         want = (set(filter(None, args.only.split(",")))
-                or {"ragged", "plain", "serving", "divergent"})
+                or {"ragged", "plain", "serving", "divergent", "general"})
         if "ragged" in want:
             out["extras"]["ragged_nfa"] = bench_ragged(args, torch)
         if "plain" in want:
@@ -1039,6 +1132,8 @@ def main() -> int:
             out["extras"]["serving"] = bench_serving(args, torch)
         if "divergent" in want:
             out["extras"]["divergent_nfa"] = bench_divergent(args, torch)
+        if "general" in want:
+            out["extras"]["general_nfa"] = bench_general(args, torch)
     line = {
         "metric": out["metric"],
         "value": out["value"],
