@@ -288,8 +288,12 @@ __global__ void split_init_kernel(int32_t *flags, int32_t *lens, int32_t *done,
 // a 32-byte boundary with 32 bytes left in their part (the common case inside long inputs);
 // otherwise the chains that are not aligned advance byte by byte to the next boundary.
 
-constexpr int kRound = 64;               // bytes per chain per round
+constexpr int kRound = 64;               // bytes per chain per round (default)
 constexpr int kVec = kRound / 16;        // 16-byte vectors per round
+// Round of the walk kernel with K chains per lane over entries of esz bytes (the checkpoint
+// interval is a multiple): 128 for byte tables with one chain per lane; the wider steps and
+// the interleaved chains would spill registers at 128.
+constexpr int split_walk_round(int K, int esz) { return K == 1 && esz == 1 ? 128 : 64; }
 
 template <typename T, int K, int R = kRound>
 __device__ __forceinline__ void step_round(const char *tab, uint32_t col, uint32_t (&e)[K],
@@ -356,6 +360,11 @@ template <typename T, bool kLds, int K>
 __global__ __launch_bounds__(kWalkThreads / K) void dfa_split_walk_kernel(
     const T *__restrict__ gtab, SplitArgs a) {
     extern __shared__ uint4 lds_raw[];
+    // bytes per chain per round: a whole 128-byte line for byte tables with one chain per
+    // lane (a line read in two rounds was often evicted from L2 in between when 100,000+
+    // chains stream at once), else 64
+    constexpr int kR = split_walk_round(K, sizeof(T));
+    constexpr int kV = kR / 16;
     const char *tab = stage_table<T, kLds>(gtab, a.table_bytes, lds_raw);
     const uint32_t col = a.col, acc = a.acc, start = a.start;
     const int64_t rmask = (1ll << a.rshift) - 1;
@@ -371,7 +380,7 @@ __global__ __launch_bounds__(kWalkThreads / K) void dfa_split_walk_kernel(
     uint32_t e[K], mx[K];
     unsigned long long fm[K];  // head: flagged intervals so far
     bool have[K];
-    uint4 nv[K][kVec];
+    uint4 nv[K][kV];
 
     // chain c takes the next segment of this lane (or goes idle)
     auto pick = [&](int c) {
@@ -442,12 +451,12 @@ __global__ __launch_bounds__(kWalkThreads / K) void dfa_split_walk_kernel(
         for (int c = 0; c < K; ++c) {
             if (mode[c]) {
                 any = true;
-                if ((q[c] & (kRound - 1)) || q[c] + kRound > pend[c]) fast = false;
+                if ((q[c] & (kR - 1)) || q[c] + kR > pend[c]) fast = false;
             }
         }
         if (!any) break;
         if (fast) {
-            uint4 v[K][kVec];
+            uint4 v[K][kV];
             uint32_t e0[K];
             int32_t dn[K];
             // this round's bytes: read ahead by the previous round, or loaded now (the first
@@ -456,13 +465,13 @@ __global__ __launch_bounds__(kWalkThreads / K) void dfa_split_walk_kernel(
             for (int c = 0; c < K; ++c) {
                 if (mode[c] && !have[c]) {
 #pragma unroll
-                    for (int j = 0; j < kVec; ++j) nv[c][j] = ld16(a.data + q[c] + 16 * j);
+                    for (int j = 0; j < kV; ++j) nv[c][j] = ld16(a.data + q[c] + 16 * j);
                 }
             }
 #pragma unroll
             for (int c = 0; c < K; ++c)
 #pragma unroll
-                for (int j = 0; j < kVec; ++j) v[c][j] = mode[c] ? nv[c][j] : zero;
+                for (int j = 0; j < kV; ++j) v[c][j] = mode[c] ? nv[c][j] : zero;
             // The next round's bytes and the input's done flag, loaded while this round is
             // walked. The loads are unconditional (a chain with no next round re-reads this
             // round's bytes; an idle chain reads the batch's first bytes, which exist since
@@ -472,26 +481,26 @@ __global__ __launch_bounds__(kWalkThreads / K) void dfa_split_walk_kernel(
             for (int c = 0; c < K; ++c) {
                 e0[c] = e[c];
                 dn[c] = __atomic_load_n(&a.done[mode[c] ? in[c] : 0], __ATOMIC_RELAXED);
-                const bool next = mode[c] && q[c] + 2 * kRound <= pend[c];
-                const int64_t at = next ? q[c] + kRound : (mode[c] ? q[c] : 0);
+                const bool next = mode[c] && q[c] + 2 * kR <= pend[c];
+                const int64_t at = next ? q[c] + kR : (mode[c] ? q[c] : 0);
 #pragma unroll
-                for (int j = 0; j < kVec; ++j) nv[c][j] = ld16(a.data + at + 16 * j);
+                for (int j = 0; j < kV; ++j) nv[c][j] = ld16(a.data + at + 16 * j);
                 have[c] = next;
             }
             // keep the loads above ahead of the round (the scheduler would otherwise sink them
             // to where their registers free up, near the round's end)
             __builtin_amdgcn_sched_barrier(0);
-            step_round<T, K>(tab, col, e, mx, v);
+            step_round<T, K, kR>(tab, col, e, mx, v);
 #pragma unroll
             for (int c = 0; c < K; ++c) {
                 if (!mode[c]) {
                     e[c] = start;
                     continue;
                 }
-                q[c] += kRound;
+                q[c] += kR;
                 if (mode[c] == 2) {
                     if (mx[c] >= acc) {
-                        accept_at(c, q[c] - kRound + find_round<T>(tab, col, e0[c], v[c], acc));
+                        accept_at(c, q[c] - kR + find_round<T, kR>(tab, col, e0[c], v[c], acc));
                         q[c] = pend[c];
                         part_end(c, false);
                         continue;
@@ -516,9 +525,9 @@ __global__ __launch_bounds__(kWalkThreads / K) void dfa_split_walk_kernel(
         } else {
 #pragma unroll
             for (int c = 0; c < K; ++c) {
-                if (!mode[c] || !((q[c] & (kRound - 1)) || q[c] + kRound > pend[c])) continue;
+                if (!mode[c] || !((q[c] & (kR - 1)) || q[c] + kR > pend[c])) continue;
                 // byte by byte to the next 32-byte boundary (or the end of the part)
-                const int64_t up = (q[c] | (kRound - 1)) + 1;
+                const int64_t up = (q[c] | (kR - 1)) + 1;
                 const int64_t stop = up < pend[c] ? up : pend[c];
                 bool hit = false;
                 for (; q[c] < stop; ++q[c]) {
@@ -1136,17 +1145,19 @@ DfaSplitPlan dfa_split_plan(const DfaDev &d, int64_t total_bytes, int32_t count,
     p.chains = static_cast<int32_t>(env_int("GFSM_SPLIT_CHAINS", 1));
     if (p.chains != 1 && p.chains != 2 && p.chains != 4) p.chains = 1;
     const bool valid = seg_bytes >= 256 && seg_bytes <= (1 << 20) && !(seg_bytes & (seg_bytes - 1));
+    // the checkpoint interval is a multiple of the walk's round (split_walk_round)
+    const int32_t round_shift = split_walk_round(p.chains, d.esz) == 128 ? 7 : 6;
     if (lds_fits(d)) {
         int64_t c = valid ? seg_bytes : env_int("GFSM_SPLIT_SEGMENT", 2048);
         if (c < 256 || c > (1 << 20) || (c & (c - 1))) c = 2048;
         p.seg_bytes = c;
-        p.ckpt_shift = 6;
+        p.ckpt_shift = round_shift;
     } else {
         int64_t c = 1024;
         while (c < 4ll * d.n && c < 65536) c <<= 1;
         if (valid) c = seg_bytes;
         p.seg_bytes = c;
-        p.ckpt_shift = 6;
+        p.ckpt_shift = round_shift;
         while ((1ll << p.ckpt_shift) * 16 < c) ++p.ckpt_shift;
     }
     p.segments = (total_bytes + p.seg_bytes - 1) / p.seg_bytes;
