@@ -7,10 +7,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cstring>
 #include <functional>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -37,6 +39,16 @@ struct gfsm_nfa {
     int32_t max_chain = 0;
     std::vector<void *> blocks;
     int64_t device_bytes = 0;
+    // Byte-triple lists of the chain technique's lane tier (64 MiB of headers plus the
+    // lists), built on the first run that asks for them (GFSM_TUNE_CHAIN_TRIPLE): the host
+    // keeps the chain layout until then, for automata the lists can apply to.
+    std::mutex tri_mu;
+    int tri_state = 0;  // 0: not built yet, 1: built, -1: not available (tri_why)
+    std::atomic<int64_t> tri_bytes{0};
+    std::string tri_why;
+    NfaTriDev tri{};
+    std::vector<void *> tri_blocks;
+    std::unique_ptr<TileHost> tile_host;
 };
 
 struct gfsm_dfa {
@@ -262,6 +274,44 @@ int32_t resolve_dfa(const gfsm_dfa *h, int32_t technique, int32_t count, int64_t
     }
 }
 
+// The handle's byte-triple lists, built and uploaded on first use (null when the automaton
+// has none: not a pattern union whose start state loops on every byte, or lists over the
+// caps). The build is host work plus a synchronous upload, so it is refused inside stream
+// capture.
+const NfaTriDev *chain_triples(gfsm_nfa *h, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(h->tri_mu);
+    if (h->tri_state == 0) {
+        if (!h->tile_host) {
+            h->tri_state = -1;
+            h->tri_why = "not a pattern union whose start state loops on every byte";
+        } else {
+            hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+            GFSM_HIP(hipStreamIsCapturing(s, &st));
+            if (st != hipStreamCaptureStatusNone)
+                throw Error(GFSM_EINVAL,
+                            "chain byte-triple lists are built by the first run that asks for "
+                            "them (GFSM_TUNE_CHAIN_TRIPLE): run it once outside stream capture");
+            const TripleHost th = build_triples(*h->tile_host);
+            if (th.ok) {
+                Packer pk;
+                pk.add(&h->tri.hdr, th.hdr);
+                pk.add(&h->tri.nonempty, th.nonempty);
+                pk.add(&h->tri.slot, th.slot);
+                pk.add(&h->tri.pair_acc, th.pair_acc);
+                int64_t bytes = 0;
+                pk.commit(h->tri_blocks, bytes);
+                h->tri_bytes.store(bytes);
+                h->tri_state = 1;
+            } else {
+                h->tri_state = -1;
+                h->tri_why = th.why;
+            }
+            h->tile_host.reset();
+        }
+    }
+    return h->tri_state > 0 ? &h->tri : nullptr;
+}
+
 // Device scratch of one NFA run (resolved technique).
 size_t nfa_workspace(const gfsm_nfa *h, int32_t tech, int32_t count, int64_t total,
                      const Tuning &tu) {
@@ -303,8 +353,11 @@ void nfa_enqueue(gfsm_nfa *h, int32_t tech, const Tuning &tu, const uint8_t *d_d
         const bool pairs = !(tu.flags & GFSM_TUNE_CHAIN_FIRST) && env_int("GFSM_CHAIN_FIRST", 0) != 1;
         const bool paired =
             !(tu.flags & GFSM_TUNE_CHAIN_WAVE64) && env_int("GFSM_CHAIN_WAVE64", 0) != 1;
+        const bool triples =
+            pairs && ((tu.flags & GFSM_TUNE_CHAIN_TRIPLE) || env_int("GFSM_CHAIN_TRIPLE", 0) == 1);
+        const NfaTriDev *tri = triples ? chain_triples(h, s) : nullptr;
         launch_nfa_chain(h->chain, h->tile, d_data, d_off, count, total, d_flags, d_lens, ws,
-                         nfa_chain_chunk_bytes(tu.chunk, total), pairs, paired, s);
+                         nfa_chain_chunk_bytes(tu.chunk, total), pairs, paired, tri, s);
         return;
     }
     if (tech == GFSM_NFA_BITPACKED) {
@@ -625,6 +678,19 @@ int gfsm_nfa_create(int32_t num_states, int32_t start_state, const uint8_t *acce
             std::vector<uint16_t> s16;
             pack_nfa(h, num_states, w, t, g, s16, pk);
             pk.commit(h->blocks, h->device_bytes);
+            // the lane tier's lists can apply: keep what their build reads
+            if (t.ok && t.start_always && !t.pair.empty() &&
+                static_cast<int64_t>(t.tiles) * kTileStates <= 65536) {
+                auto keep = std::make_unique<TileHost>();
+                keep->ok = true;
+                keep->start_always = true;
+                keep->tiles = t.tiles;
+                keep->next = std::move(t.next);
+                keep->acc = std::move(t.acc);
+                keep->pair = std::move(t.pair);
+                keep->pair_slot = std::move(t.pair_slot);
+                h->tile_host = std::move(keep);
+            }
         } catch (...) {
             free_blocks(h->blocks);
             delete h;
@@ -638,6 +704,7 @@ int gfsm_nfa_destroy(gfsm_nfa *nfa) {
     return guarded([&] {
         if (!nfa) return;
         free_blocks(nfa->blocks);
+        free_blocks(nfa->tri_blocks);
         delete nfa;
     });
 }
@@ -656,7 +723,8 @@ int gfsm_nfa_query(const gfsm_nfa *nfa, gfsm_nfa_props *out) {
         out->start_always = nfa->chain_ok ? nfa->tile.start_always : 0;
         out->max_chain = nfa->max_chain;
         out->general_ok = nfa->general_ok;
-        out->device_bytes = nfa->device_bytes;
+        // the chain lane tier's byte-triple lists count once a run has built them
+        out->device_bytes = nfa->device_bytes + nfa->tri_bytes.load();
     });
 }
 

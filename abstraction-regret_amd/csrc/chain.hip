@@ -85,7 +85,7 @@ struct ChainScratch {
     int32_t *rcount;           // [K] R_c = inputs with more than c chunks
     int64_t *rbase;            // [K] first item of round k; total items at [maxk]
     int32_t *maxk;             // [1] most chunks of any input
-    unsigned int *counter;     // [4]: reg tier, lds tier, ovf_items fill, unused
+    unsigned int *counter;     // [4]: reg tier, lds tier, ovf_items fill, inputs in ovf2
     int32_t K;                 // capacity of the per-round arrays (> max chunks + 1)
 };
 
@@ -802,6 +802,285 @@ __global__ __launch_bounds__(64 * kChainWaves) __attribute__((amdgpu_waves_per_e
     if (dcount != 0) flush_deferred();  // wave-uniform; the LDS tier launches after this kernel
 }
 
+// ------------------------------------------------------------------------ lane tier ----
+
+// Lane tier: pattern unions whose start state loops on every byte, with byte-triple lists.
+//
+// With the start state always active, the depth-1 set after byte j is a function of b_j,
+// the depth-2 set of (b_j-1, b_j), and every state past that entered at depth 3 from the list
+// tri(b_j-2, b_j-1, b_j) (see TripleHost). Pattern unions hold few such states: on the
+// 2000-pattern ragged mix about 0.5 enter per byte, a state survives a byte with
+// probability about 1/16, and a lane rarely holds more than 3. So each lane runs one
+// (input, chunk) item alone, holding up to kTriHeld states in LDS. Per byte it loads one
+// continuation word per held state, reads the byte's list (whose header it loaded together
+// with those of the other bytes of the 16-byte group), and checks the depth-1, depth-2 and
+// depth-3 accepts against the LDS flags, the pair accept bits and the header. No lane
+// waits for another, so a wave steps 64 items per byte where the paired register tier steps
+// 2, and a byte costs about two scattered loads instead of one per listed state.
+// An item whose held set would exceed kTriHeld goes to the LDS tier unrun (like the register
+// tiers' overflows), and so do the later items of its input.
+constexpr int kTriHeld = 8;
+constexpr int kTriWaves = 4;
+
+// Byte k of a 16-byte group.
+__device__ __forceinline__ uint32_t group_byte(const uint4 &v, int k) {
+    const uint32_t w = k < 4 ? v.x : (k < 8 ? v.y : (k < 12 ? v.z : v.w));
+    return (w >> (8 * (k & 3))) & 0xffu;
+}
+
+// The 16 bytes at [ga, ga + 16) of the batch, ga 16-aligned in absolute address terms; bytes
+// outside [0, data_end) read as zero (byte loads at the batch's two ends).
+__device__ __forceinline__ uint4 tri_load_group(const uint8_t *__restrict__ data, int64_t ga,
+                                                int64_t data_end) {
+    if (ga >= 0 && ga + 16 <= data_end) {
+        // streamed once: non-temporal, so it does not push the lists out of L2
+        const uint32_t *p = reinterpret_cast<const uint32_t *>(data + ga);
+        return make_uint4(__builtin_nontemporal_load(p), __builtin_nontemporal_load(p + 1),
+                          __builtin_nontemporal_load(p + 2), __builtin_nontemporal_load(p + 3));
+    }
+    uint32_t w[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+        if (ga + i >= 0 && ga + i < data_end)
+            w[i >> 2] |= static_cast<uint32_t>(data[ga + i]) << (8 * (i & 3));
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+__global__ __launch_bounds__(64 * kTriWaves) void nfa_chain_tri_kernel(
+    NfaChainDev d, NfaTriDev tri, const uint8_t *__restrict__ data,
+    const int64_t *__restrict__ off, int32_t count, ChainScratch sc, int64_t chunk) {
+    __shared__ uint32_t s_flags[256];
+    __shared__ uint32_t s_acc[(kSmallTiles + 1) * kTileWords * 2];
+    __shared__ uint32_t s_pacc[2048];
+    __shared__ uint4 s_held[64 * kTriWaves];  // kTriHeld 16-bit slots per lane
+    __shared__ uint32_t s_hdr[16][64 * kTriWaves];  // the group's list headers, by byte
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) s_flags[i] = d.bflags[i];
+    const int acc_words = (d.tiles + 1) * kTileWords * 2;
+    const uint32_t *acc32 = reinterpret_cast<const uint32_t *>(d.acc);
+    for (int i = threadIdx.x; i < acc_words; i += blockDim.x) s_acc[i] = acc32[i];
+    for (int i = threadIdx.x; i < 2048; i += blockDim.x) s_pacc[i] = tri.pair_acc[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    uint16_t *held = reinterpret_cast<uint16_t *>(&s_held[threadIdx.x]);
+    const int32_t maxk = *sc.maxk;
+    const int64_t items = sc.rbase[maxk];
+    const int64_t data_end = off[count];
+    const int64_t overlap = d.max_chain > 0 ? d.max_chain - 1 : 0;
+    const uint32_t plane_words = static_cast<uint32_t>(d.tiles + 1) * (kTileWords * 2);
+    const char *__restrict__ next_base = reinterpret_cast<const char *>(d.next);
+    const uint64_t dbase = static_cast<uint64_t>(reinterpret_cast<uintptr_t>(data));
+
+    // the lane's item
+    Item it{0, 0u, 0, 0, 0, 0};
+    bool running = false;
+    bool fresh = false;    // the item's first group is next
+    int64_t q = 0;         // next byte to step
+    int32_t n = 0;         // held states
+    uint32_t x = 0, y = 0; // the bytes at q - 2 and q - 1
+    int32_t hist = 0;      // bytes of the item stepped so far, capped at 2
+    int32_t found = 0;     // > 0: first accept (1-based); -1: overflow to the LDS tier
+    int32_t groups = 0;    // groups stepped in this item
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);  // bytes of the group at q (16-aligned)
+
+    // Hand-out. Every lane's first item is its own index; later ones come from the counter,
+    // offset by the number of lanes, a wave taking kPer numbers per atomic (every lane takes
+    // part in the atomic and the broadcast, see next_item).
+    constexpr uint32_t kPer = 64;
+    const uint32_t lanes_total = gridDim.x * blockDim.x;
+    bool first_round = true;  // wave-uniform
+    bool exhausted = false;   // wave-uniform
+    ItemPool pool;
+
+    for (;;) {
+        while (!exhausted) {
+            const uint64_t need = __ballot(!running);
+            if (need == 0) break;
+            uint32_t item;
+            bool take;
+            if (first_round) {
+                item = blockIdx.x * blockDim.x + threadIdx.x;
+                take = true;
+                first_round = false;
+            } else {
+                if (pool.next == pool.end) {
+                    const unsigned int mine = atomicAdd(&sc.counter[0], lane == 0 ? kPer : 0u);
+                    pool.next = __builtin_amdgcn_readfirstlane(mine);
+                    pool.end = pool.next + kPer;
+                }
+                const uint32_t avail = pool.end - pool.next;
+                const uint32_t r = static_cast<uint32_t>(lane_rank(need));
+                take = !running && r < avail;
+                item = pool.next + lanes_total + r;
+                const uint32_t k = static_cast<uint32_t>(__popcll(need));
+                pool.next += k < avail ? k : avail;
+            }
+            bool none = false;
+            if (take) {
+                if (static_cast<int64_t>(item) < items) {
+                    decode_item(item, sc, off, maxk, chunk, overlap, it);
+                    if (!settled(sc.best, it, it.cs)) {
+                        if (__atomic_load_n(&sc.deferred[it.in], __ATOMIC_RELAXED) != 0) {
+                            // an item of this input overflowed: the LDS tier runs this one
+                            sc.ovf_items[atomicAdd(&sc.counter[2], 1u)] = static_cast<int32_t>(it.id);
+                        } else {
+                            running = true;
+                            fresh = true;
+                            q = it.ps;
+                            n = 0;
+                            hist = 0;
+                            found = 0;
+                            groups = 0;
+                        }
+                    }
+                } else {
+                    none = true;
+                }
+            }
+            if (__ballot(none) != 0) exhausted = true;
+        }
+        if (__ballot(running) == 0) break;  // nothing left to hand out and nothing running
+
+        // ---- one 16-byte group of every running lane
+        const int64_t ga = q - static_cast<int64_t>((dbase + static_cast<uint64_t>(q)) & 15u);
+        const int64_t lim = running ? it.ce : ga;  // bytes below lim are stepped
+        const int32_t s = static_cast<int32_t>(q - ga);  // the group's first byte to step
+        const int32_t e = static_cast<int32_t>(lim - ga < 16 ? lim - ga : 16);  // ... and end
+        if (running && fresh) v = tri_load_group(data, ga, data_end);
+        fresh = false;
+        // the headers of the group's depth-3 lists, kept in LDS by byte: the non-empty bits
+        // first (a table that stays in L2), then the headers of the non-empty lists only
+        {
+            uint32_t idx[16], hdr[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const uint32_t b = group_byte(v, k);
+                const uint32_t y1 = k == 0 ? y : group_byte(v, k - 1);
+                const uint32_t x1 = k == 0 ? x : (k == 1 ? y : group_byte(v, k - 2));
+                const bool ok = running && k >= s && k < e && hist + (k - s) >= 2;
+                idx[k] = ok ? (x1 << 16) | (y1 << 8) | b : ~0u;
+                hdr[k] = ok ? tri.nonempty[idx[k] >> 5] : 0u;
+            }
+#pragma unroll
+            for (int k = 0; k < 16; ++k)
+                hdr[k] = (hdr[k] >> (idx[k] & 31u)) & 1u ? tri.hdr[idx[k]] : 0u;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) s_hdr[k][threadIdx.x] = hdr[k];
+        }
+        // the next group's bytes, read while this one is stepped
+        const bool more = running && e == 16 && ga + 16 < lim;
+        const uint4 vn = more ? tri_load_group(data, ga + 16, data_end) : make_uint4(0u, 0u, 0u, 0u);
+
+        bool stop = false;
+        for (int k = 0; k < 16; ++k) {
+            const bool act = running && !stop && k >= s && k < e;
+            if (__ballot(act) == 0) continue;
+            const uint32_t wk = k < 4 ? v.x : (k < 8 ? v.y : (k < 12 ? v.z : v.w));
+            const uint32_t b = (wk >> (8 * (k & 3))) & 0xffu;
+            const uint32_t wp = k <= 4 ? v.x : (k <= 8 ? v.y : (k <= 12 ? v.z : v.w));
+            const uint32_t yb = k == 0 ? y : ((wp >> (8 * ((k - 1) & 3))) & 0xffu);
+            const int32_t hk = hist + (k - s);  // item bytes stepped before this one
+            // held states: their continuation words on b, loaded together
+            uint4 hv = make_uint4(0u, 0u, 0u, 0u);
+            if (__ballot(act && n > 0) != 0) hv = s_held[threadIdx.x];
+            uint32_t t[kTriHeld], nw[kTriHeld];
+#pragma unroll
+            for (int i = 0; i < kTriHeld; ++i) {
+                const uint32_t hw = i < 2 ? hv.x : (i < 4 ? hv.y : (i < 6 ? hv.z : hv.w));
+                t[i] = ((hw >> (16 * (i & 1))) & 0xffffu) + 1u;
+                nw[i] = 0u;
+                if (__ballot(act && n > i) != 0 && act && n > i)
+                    nw[i] = *reinterpret_cast<const uint32_t *>(
+                        next_base + ((b * plane_words + (t[i] >> 5)) << 2));
+            }
+            // the byte's depth-3 list: inline when it has one entry, else 4 entries a load
+            const uint32_t h = s_hdr[k][threadIdx.x];
+            const uint32_t cnt = act ? (h >> 24) & 127u : 0u;
+            uint2 e4 = make_uint2(0u, 0u);
+            if (__ballot(cnt >= 2) != 0 && cnt >= 2)
+                e4 = *reinterpret_cast<const uint2 *>(tri.slot + (h & 0xffffffu));
+            // survivors, compacted to the front of the lane's list (read before written)
+            int32_t p = 0;
+            bool hit = false;
+#pragma unroll
+            for (int i = 0; i < kTriHeld; ++i) {
+                if (__ballot(act && n > i) != 0) {
+                    const bool alive = act && n > i && ((nw[i] >> (t[i] & 31u)) & 1u);
+                    if (alive) {
+                        if ((s_acc[t[i] >> 5] >> (t[i] & 31u)) & 1u) hit = true;
+                        held[p] = static_cast<uint16_t>(t[i]);
+                        ++p;
+                    }
+                }
+            }
+            // the list's entries join the held set
+            bool ovf = false;
+            if (act && (h >> 31)) hit = true;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                if (__ballot(cnt > static_cast<uint32_t>(m)) != 0 && cnt > static_cast<uint32_t>(m)) {
+                    const uint32_t ew = m < 2 ? e4.x : e4.y;
+                    const uint32_t sl = cnt == 1 ? (h & 0xffffu) : ((ew >> (16 * (m & 1))) & 0xffffu);
+                    if (p < kTriHeld) {
+                        held[p] = static_cast<uint16_t>(sl);
+                        ++p;
+                    } else {
+                        ovf = true;
+                    }
+                }
+            }
+            if (__ballot(cnt > 4u) != 0 && cnt > 4u) {
+                for (uint32_t m = 4; m < cnt; ++m) {
+                    const uint32_t sl = tri.slot[(h & 0xffffffu) + m];
+                    if (p < kTriHeld) {
+                        held[p] = static_cast<uint16_t>(sl);
+                        ++p;
+                    } else {
+                        ovf = true;
+                    }
+                }
+            }
+            // depth 1 (a first position accepts on b) and depth 2 (the pair list accepts)
+            if (act && (s_flags[b] & 2u)) hit = true;
+            if (act && hk >= 1 && ((s_pacc[(yb << 3) | (b >> 5)] >> (b & 31u)) & 1u)) hit = true;
+            if (act) {
+                n = p;
+                if (ovf) {
+                    found = -1;
+                    stop = true;
+                } else if (hit && ga + k >= it.cs) {
+                    found = static_cast<int32_t>(ga + k - it.ib + 1);
+                    stop = true;
+                }
+            }
+        }
+        // ---- group end: carry the last two bytes, move on, finish items
+        if (running && !stop) {
+            const int64_t gend = ga + 16;
+            if (!more) {
+                stop = true;  // the chunk's end: no accept at or after its start
+            } else {
+                const int32_t stepped = 16 - s;
+                x = stepped >= 2 ? group_byte(v, 14) : y;
+                y = group_byte(v, 15);
+                hist = (hist + stepped) < 2 ? hist + stepped : 2;
+                q = gend;
+                v = vn;
+                ++groups;
+                // another chunk of this input may have found an earlier match meanwhile
+                if ((groups & 15) == 0 && settled(sc.best, it, q)) stop = true;
+            }
+        }
+        if (running && stop) {
+            if (found > 0) atomicMin(&sc.best[it.in], found);
+            if (found < 0) {
+                sc.ovf_items[atomicAdd(&sc.counter[2], 1u)] = static_cast<int32_t>(it.id);
+                __atomic_store_n(&sc.deferred[it.in], 1, __ATOMIC_RELAXED);
+            }
+            running = false;
+        }
+    }
+}
+
 // ------------------------------------------------------------------------- lds tier ----
 
 template <bool kPairs, int kCap>
@@ -815,6 +1094,9 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_lds_kernel(
     const int32_t maxk = *sc.maxk;
     // every item, or only those the register tier could not finish
     const int64_t items = from_list ? static_cast<int64_t>(sc.counter[2]) : sc.rbase[maxk];
+    // An empty list is the common case after the register or lane tier: leave before the
+    // hand-out, whose one atomic per wave on one counter would serialize the whole grid.
+    if (items == 0) return;
     const int32_t *list = from_list ? sc.ovf_items : nullptr;
     const int64_t data_end = off[count];
     const int64_t overlap = d.max_chain > 0 ? d.max_chain - 1 : 0;
@@ -914,7 +1196,10 @@ __global__ __launch_bounds__(64 * kChainWaves) void nfa_chain_lds_kernel(
         }
         if (lane == 0) {
             if (found > 0) atomicMin(&sc.best[it.in], found);
-            if (ovf) ovf_out[it.in] = 1;
+            if (ovf) {
+                ovf_out[it.in] = 1;
+                atomicAdd(&sc.counter[3], 1u);  // the tile tier runs only when this is nonzero
+            }
         }
     }
 }
@@ -964,7 +1249,7 @@ size_t nfa_chain_scratch_bytes(const NfaChainDev &d, int32_t count, int64_t tota
 void launch_nfa_chain(const NfaChainDev &d, const NfaTileDev &tiles, const uint8_t *data,
                       const int64_t *off, int32_t count, int64_t total_bytes, int32_t *flags,
                       int32_t *lens, void *scratch, int32_t chunk_bytes, bool pairs,
-                      bool paired, hipStream_t stream) {
+                      bool paired, const NfaTriDev *tri, hipStream_t stream) {
     if (count <= 0) return;
     const int64_t chunk = effective_chunk(d, chunk_bytes);
     // item numbers are 32-bit
@@ -1013,7 +1298,17 @@ void launch_nfa_chain(const NfaChainDev &d, const NfaTileDev &tiles, const uint8
         reinterpret_cast<const void *>(&nfa_chain_lds_kernel<false, 1024>), threads);
     if (d.pair && pairs) {
         const bool small = d.tiles <= kSmallTiles && d.pair_slot16 != nullptr;
-        if (paired && d.start_always && small)
+        if (tri && d.start_always && small) {
+            // GFSM_CHAIN_TRI_LANES caps the lanes (sweeps): fewer lanes than items leave the
+            // later chunks of an input to start after its earlier ones, which may settle it
+            int blocks = persistent_blocks(reinterpret_cast<const void *>(&nfa_chain_tri_kernel),
+                                           64 * kTriWaves);
+            const int64_t cap = (env_int("GFSM_CHAIN_TRI_LANES", 0) + 64 * kTriWaves - 1) / (64 * kTriWaves);
+            if (cap > 0 && cap < blocks) blocks = static_cast<int>(cap);
+            nfa_chain_tri_kernel<<<blocks, 64 * kTriWaves, 0, stream>>>(d, *tri, data, off, count,
+                                                                         sc, chunk);
+        }
+        else if (paired && d.start_always && small)
             nfa_chain_pair_kernel<true, true><<<grid_pair, threads, 0, stream>>>(d, data, off, count, sc, chunk);
         else if (paired && d.start_always)
             nfa_chain_pair_kernel<true, false><<<grid_pair, threads, 0, stream>>>(d, data, off, count, sc, chunk);
@@ -1037,7 +1332,7 @@ void launch_nfa_chain(const NfaChainDev &d, const NfaTileDev &tiles, const uint8
             d, data, off, count, sc, chunk, false, sc.ovf2);
     }
     GFSM_HIP(hipGetLastError());
-    launch_nfa_bitpacked_core(tiles, data, off, count, sc.best, sc.ovf2, stream);
+    launch_nfa_bitpacked_core(tiles, data, off, count, sc.best, sc.ovf2, &sc.counter[3], stream);
     launch_nfa_best_finish(sc.best, count, flags, lens, stream);
 }
 

@@ -23,6 +23,8 @@ constexpr int kBigStride = 258;          // per-label index: 257 label starts + 
 constexpr int kTileStates = 4096;        // bitpacked tile: 64 lanes x 64-bit words
 constexpr int kTileWords = 64;
 constexpr int64_t kMaxPairSlots = int64_t(1) << 24;  // byte-pair list cap (entries)
+// byte-triple lists: entries over all lists, and padded entries (offsets are 24-bit)
+constexpr int64_t kMaxTripleSlots = int64_t(1) << 24;
 
 // Thrown inside the library, converted to a status code at the C boundary.
 struct Error : std::runtime_error {
@@ -182,10 +184,11 @@ void launch_nfa_bitpacked(const NfaTileDev &d, const uint8_t *data, const int64_
                           int32_t count, int32_t *flags, int32_t *lens, int32_t *best,
                           hipStream_t stream);
 // The tiled pass alone over inputs with only[i] != 0 (all when only is null), folding its
-// first-accept positions into best[] (INT_MAX = none yet) with atomicMin.
+// first-accept positions into best[] (INT_MAX = none yet) with atomicMin. only_count (or
+// null): the number of such inputs, when it is known on the device only; 0 skips the pass.
 void launch_nfa_bitpacked_core(const NfaTileDev &d, const uint8_t *data, const int64_t *off,
                                int32_t count, int32_t *best, const int32_t *only,
-                               hipStream_t stream);
+                               const unsigned int *only_count, hipStream_t stream);
 // best[] (1-based first-accept position, INT_MAX = none) -> (flags, lens).
 void launch_nfa_best_finish(const int32_t *best, int32_t count, int32_t *flags, int32_t *lens,
                             hipStream_t stream);
@@ -196,12 +199,21 @@ size_t nfa_chain_scratch_bytes(const NfaChainDev &d, int32_t count, int64_t tota
 // chunk_bytes when the caller has no preference: GFSM_CHAIN_CHUNK (read per call), else
 // 4096 for batches of 16 MiB or more and down to 512 for smaller ones
 int32_t nfa_chain_chunk_bytes(int32_t requested, int64_t total_bytes);
+// Byte-triple lists of a chain automaton (TripleHost on the device).
+struct NfaTriDev {
+    const uint32_t *hdr;       // [1 << 24] headers
+    const uint32_t *nonempty;  // [1 << 19] bit i: hdr[i] has entries
+    const uint16_t *slot;      // lists of two or more entries
+    const uint32_t *pair_acc;  // [2048] accept bits of the byte-pair lists
+};
 // pairs: use the byte-pair lists (register tier first) when the handle has them.
 // paired: the register tier runs two items per wave (32 listed states each) instead of one
 // (64 listed states).
+// tri: the byte-triple lists (null: none); with them and pairs, the lane tier (one item per
+// lane) replaces the register tier.
 void launch_nfa_chain(const NfaChainDev &d, const NfaTileDev &tiles, const uint8_t *data,
                       const int64_t *off, int32_t count, int64_t total_bytes, int32_t *flags,
                       int32_t *lens, void *scratch, int32_t chunk_bytes, bool pairs,
-                      bool paired, hipStream_t stream);
+                      bool paired, const NfaTriDev *tri, hipStream_t stream);
 
 }  // namespace gfsm

@@ -323,6 +323,116 @@ TileHost build_tiles(const NfaInput &a) {
     return h;
 }
 
+TripleHost build_triples(const TileHost &t) {
+    TripleHost h;
+    if (!t.ok) {
+        h.why = "not chain-structured";
+        return h;
+    }
+    if (!t.start_always) {
+        h.why = "the start state does not loop on every byte";
+        return h;
+    }
+    if (t.pair.empty()) {
+        h.why = "no byte-pair lists";
+        return h;
+    }
+    if (static_cast<int64_t>(t.tiles) * kTileStates > 65536) {
+        h.why = "more than 65536 slots";
+        return h;
+    }
+    const size_t T = static_cast<size_t>(t.tiles);
+    const size_t words = (T + 1) * kTileWords;  // one byte's plane of `next`, in u64 words
+    const size_t S = words * 64;                 // slots, the all-zero tile included
+    // bytes on which slot s is entered from slot s - 1 (bit s of next[c]), as a CSR
+    std::vector<int32_t> boff(S + 1, 0);
+    for (int c = 0; c < 256; ++c) {
+        const uint64_t *plane = &t.next[static_cast<size_t>(c) * words];
+        for (size_t w = 0; w < words; ++w)
+            for (uint64_t m = plane[w]; m; m &= m - 1)
+                ++boff[w * 64 + static_cast<size_t>(__builtin_ctzll(m)) + 1];
+    }
+    for (size_t s = 0; s < S; ++s) boff[s + 1] += boff[s];
+    std::vector<uint8_t> bytes(static_cast<size_t>(boff[S]));
+    {
+        std::vector<int32_t> fill(boff.begin(), boff.end() - 1);
+        for (int c = 0; c < 256; ++c) {
+            const uint64_t *plane = &t.next[static_cast<size_t>(c) * words];
+            for (size_t w = 0; w < words; ++w)
+                for (uint64_t m = plane[w]; m; m &= m - 1)
+                    bytes[static_cast<size_t>(fill[w * 64 + static_cast<size_t>(__builtin_ctzll(m))]++)] =
+                        static_cast<uint8_t>(c);
+        }
+    }
+    auto accepting = [&](uint32_t s) { return (t.acc[s >> 6] >> (s & 63)) & 1ull; };
+    // total entries first, so that an automaton over the cap costs no large allocation
+    int64_t total = 0;
+    for (int i = 0; i < 65536; ++i) {
+        const int32_t o = t.pair[2 * i], cnt = t.pair[2 * i + 1] & INT32_MAX;
+        for (int32_t e = o; e < o + cnt; ++e) {
+            const size_t w = static_cast<size_t>(t.pair_slot[static_cast<size_t>(e)]) + 1;
+            total += boff[w + 1] - boff[w];
+        }
+    }
+    if (total > kMaxTripleSlots) {
+        h.why = "byte-triple lists of " + std::to_string(total) + " entries (cap " +
+                std::to_string(kMaxTripleSlots) + ")";
+        return h;
+    }
+    h.entries = total;
+    h.hdr.assign(size_t(1) << 24, 0u);
+    h.pair_acc.assign(2048, 0u);
+    // one pair (c1, c2) at a time: its 256 triples are one contiguous block of headers
+    uint32_t cnt3[256], acc3[256], pos3[256];
+    for (int i = 0; i < 65536; ++i) {
+        const int32_t o = t.pair[2 * i], cnt = t.pair[2 * i + 1] & INT32_MAX;
+        if (t.pair[2 * i + 1] < 0) h.pair_acc[i >> 5] |= 1u << (i & 31);
+        if (cnt == 0) continue;
+        std::fill(cnt3, cnt3 + 256, 0u);
+        std::fill(acc3, acc3 + 256, 0u);
+        for (int32_t e = o; e < o + cnt; ++e) {
+            const uint32_t w = static_cast<uint32_t>(t.pair_slot[static_cast<size_t>(e)]) + 1;
+            for (int32_t k = boff[w]; k < boff[w + 1]; ++k) {
+                ++cnt3[bytes[static_cast<size_t>(k)]];
+                if (accepting(w)) acc3[bytes[static_cast<size_t>(k)]] = 1u;
+            }
+        }
+        for (int c3 = 0; c3 < 256; ++c3) {
+            if (cnt3[c3] > 127) {
+                h = TripleHost();
+                h.why = "a byte-triple list of more than 127 entries";
+                return h;
+            }
+            if (cnt3[c3] >= 2) {
+                pos3[c3] = static_cast<uint32_t>(h.slot.size());
+                h.slot.resize(h.slot.size() + ((cnt3[c3] + 3) & ~3u), 0);
+                if (h.slot.size() > static_cast<size_t>(kMaxTripleSlots)) {
+                    h = TripleHost();
+                    h.why = "padded byte-triple lists above the offset range";
+                    return h;
+                }
+            }
+        }
+        uint32_t *blk = &h.hdr[static_cast<size_t>(i) << 8];
+        for (int c3 = 0; c3 < 256; ++c3)
+            if (cnt3[c3] >= 2) blk[c3] = (acc3[c3] << 31) | (cnt3[c3] << 24) | pos3[c3];
+        for (int32_t e = o; e < o + cnt; ++e) {
+            const uint32_t w = static_cast<uint32_t>(t.pair_slot[static_cast<size_t>(e)]) + 1;
+            for (int32_t k = boff[w]; k < boff[w + 1]; ++k) {
+                const int c3 = bytes[static_cast<size_t>(k)];
+                if (cnt3[c3] == 1) blk[c3] = (acc3[c3] << 31) | (1u << 24) | w;
+                else h.slot[pos3[c3]++] = static_cast<uint16_t>(w);
+            }
+        }
+    }
+    h.slot.resize(h.slot.size() + 8, 0);  // tail padding: the 8-byte loads stay in bounds
+    h.nonempty.assign(size_t(1) << 19, 0u);
+    for (size_t i = 0; i < h.hdr.size(); ++i)
+        if (h.hdr[i] >> 24 & 127u) h.nonempty[i >> 5] |= 1u << (i & 31);
+    h.ok = true;
+    return h;
+}
+
 FoldedHost fold_epsilon(const NfaInput &a, int64_t cap) {
     FoldedHost h;
     const int32_t n = a.n;

@@ -64,6 +64,27 @@ void validate_nfa(const NfaInput &a);
 WorklistHost build_worklist(const NfaInput &a);
 TileHost build_tiles(const NfaInput &a);
 
+// Third positions by byte triple, for chain automata whose start state loops on every byte:
+// tri(c1, c2, c3) = { v + 1 : v in pair(c1, c2), bit v + 1 of next[c3] }, the slots entered
+// at depth 3 on c3 after c1 c2 (the depth-1 and depth-2 sets depend on the last two bytes
+// alone when the start state is always active). One 32-bit header per triple, at index
+// c1 << 16 | c2 << 8 | c3: accepts << 31 | count << 24 | low 24 bits, which hold the slot
+// itself when count == 1 and otherwise the list's offset in `slot`, a multiple of 4 (the
+// first four entries are one 8-byte load). Built only when every slot is below 65536, no
+// list exceeds 127 entries and the lists take at most kMaxTripleSlots entries.
+struct TripleHost {
+    bool ok = false;
+    std::string why;                    // reason when !ok
+    std::vector<uint32_t> hdr;          // [1 << 24]
+    std::vector<uint32_t> nonempty;     // [1 << 19]: bit i set when hdr[i] has entries (2 MiB,
+                                        // small enough to stay in L2 while hdr does not)
+    std::vector<uint16_t> slot;         // lists of two or more entries, + 8 zero entries
+    std::vector<uint32_t> pair_acc;     // [2048]: bit c1 << 8 | c2 set when pair(c1, c2)
+                                        // holds an accepting slot
+    int64_t entries = 0;                // entries over all lists (single ones included)
+};
+TripleHost build_triples(const TileHost &t);
+
 // The same automaton with its epsilon closure folded into the symbol edges: an edge
 // (s, c, t) becomes (s, c, u) for every u in closure(t), duplicates dropped, and the start
 // set is closure(start). Simulating it without any epsilon pass gives the verdicts of

@@ -223,8 +223,12 @@ __device__ __forceinline__ uint64_t bp_step(uint64_t D, uint64_t hs, uint64_t fi
 
 __global__ __launch_bounds__(kBpThreads) void nfa_bitpacked_kernel(
     NfaTileDev d, const uint8_t *__restrict__ data, const int64_t *__restrict__ off,
-    int32_t count, int32_t *best, const int32_t *__restrict__ only) {
+    int32_t count, int32_t *best, const int32_t *__restrict__ only,
+    const unsigned int *only_count) {
     extern __shared__ uint64_t lds[];
+    // no input handed over (the common case behind the chain tiers): leave before staging
+    // the tile's 128 KiB mask table (uniform over the block: every thread reads the count)
+    if (only_count && __atomic_load_n(only_count, __ATOMIC_RELAXED) == 0u) return;
     const int32_t tile = blockIdx.x % d.tiles;
     const int32_t g = blockIdx.x / d.tiles;
     const int32_t G = gridDim.x / d.tiles;
@@ -304,8 +308,8 @@ __global__ __launch_bounds__(kBpThreads) void nfa_bitpacked_kernel(
 // The tiled bit-parallel pass alone (best[] must hold INT_MAX or earlier results): every
 // input, or only those with only[i] != 0.
 void launch_nfa_bitpacked_core(const NfaTileDev &d, const uint8_t *data, const int64_t *off,
-                           int32_t count, int32_t *best, const int32_t *only,
-                           hipStream_t stream) {
+                               int32_t count, int32_t *best, const int32_t *only,
+                               const unsigned int *only_count, hipStream_t stream) {
     const size_t bytes = kBpLdsWords * sizeof(uint64_t);
     ensure_dynamic_lds(reinterpret_cast<const void *>(&nfa_bitpacked_kernel), bytes);
     const int32_t waves_per_block = kBpThreads / 64;
@@ -314,7 +318,7 @@ void launch_nfa_bitpacked_core(const NfaTileDev &d, const uint8_t *data, const i
     if (G > target) G = target;
     if (G < 1) G = 1;
     nfa_bitpacked_kernel<<<G * d.tiles, kBpThreads, bytes, stream>>>(d, data, off, count, best,
-                                                                    only);
+                                                                    only, only_count);
     GFSM_HIP(hipGetLastError());
 }
 
@@ -324,7 +328,7 @@ void launch_nfa_bitpacked(const NfaTileDev &d, const uint8_t *data, const int64_
     if (count <= 0) return;
     bp_init_kernel<<<(count + 255) / 256, 256, 0, stream>>>(best, count);
     GFSM_HIP(hipGetLastError());
-    launch_nfa_bitpacked_core(d, data, off, count, best, nullptr, stream);
+    launch_nfa_bitpacked_core(d, data, off, count, best, nullptr, nullptr, stream);
     launch_nfa_best_finish(best, count, flags, lens, stream);
 }
 

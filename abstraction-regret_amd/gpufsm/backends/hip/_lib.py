@@ -55,6 +55,7 @@ EXPORTS = (
 
 TUNE_CHAIN_FIRST = 1
 TUNE_CHAIN_WAVE64 = 2
+TUNE_CHAIN_TRIPLE = 4
 
 
 class HIPUnavailable(RuntimeError):

@@ -327,7 +327,9 @@ class BatchEngine:
             w1.record(s)
             w1.synchronize()
             times = [w0.elapsed_time(w1) / 1e3]
-            repeats = self.tuner.repeats if give_up is None or times[0] < give_up else 0
+            # at least one timed run after the warm-up, which may include one-time work
+            # (the chain lane tier builds its byte-triple lists on first use)
+            repeats = self.tuner.repeats if give_up is None or times[0] < give_up else 1
             for _ in range(repeats):
                 e0 = torch.cuda.Event(enable_timing=True)
                 e1 = torch.cuda.Event(enable_timing=True)

@@ -59,6 +59,8 @@ class Choice:
             parts.append("first")
         if self.flags & _lib.TUNE_CHAIN_WAVE64:
             parts.append("wave64")
+        if self.flags & _lib.TUNE_CHAIN_TRIPLE:
+            parts.append("triple")
         return ":".join(parts)
 
     def tuning(self, max_input_bytes: int = 0):
@@ -76,6 +78,8 @@ class Choice:
                 kw["flags"] |= _lib.TUNE_CHAIN_FIRST
             elif part == "wave64":
                 kw["flags"] |= _lib.TUNE_CHAIN_WAVE64
+            elif part == "triple":
+                kw["flags"] |= _lib.TUNE_CHAIN_TRIPLE
             elif part.startswith("chunk="):
                 kw["chunk_bytes"] = int(part[6:])
             elif part.startswith("segment="):
@@ -115,6 +119,10 @@ def candidates(kind: str, props, mean_len: float) -> list[Choice]:
         out += [Choice("chain", chunk_bytes=4096, flags=_lib.TUNE_CHAIN_WAVE64),
                 Choice("chain", chunk_bytes=4096, flags=_lib.TUNE_CHAIN_FIRST),
                 Choice("bitpacked")]
+        if props.start_always and props.num_tiles <= 16:
+            # the lane tier (byte-triple lists; one item per lane, so shorter chunks)
+            out += [Choice("chain", chunk_bytes=c, flags=_lib.TUNE_CHAIN_TRIPLE)
+                    for c in (512, 1024, 2048)]
         if props.num_states <= 4096:  # the general list walk only competes on small unions
             out.append(Choice("worklist"))
         return out + lanes
@@ -132,7 +140,7 @@ def candidates(kind: str, props, mean_len: float) -> list[Choice]:
 class Autotuner:
     """Chooses a :class:`Choice` per shape class; see the module docstring."""
 
-    def __init__(self, path: str | None = None, sample_bytes: int = 16 << 20,
+    def __init__(self, path: str | None = None, sample_bytes: int = 128 << 20,
                  repeats: int = 3, persist: bool = True,
                  dfa_sample_bytes: int = 1 << 30) -> None:
         self.path = path or default_table_path()
@@ -152,11 +160,14 @@ class Autotuner:
     def sample_count(self, kind: str, offsets, count: int) -> int:
         """How many leading inputs of a batch a candidate is timed on.
 
-        NFA: the prefix of at most ``sample_bytes`` (at least one input). DFA: the whole
-        batch up to ``dfa_sample_bytes``. The per-input DFA walk runs one lane per input and
-        its rate grows with inputs per CU (256 to 1024), while the segmented walk fills the
-        device from any count; a byte-bounded prefix of a batch of short inputs would leave
-        the per-input walk a few lanes per CU and misjudge it.
+        NFA: the prefix of at most ``sample_bytes`` (at least one input). The chain lane
+        tier runs one (input, chunk) item per lane and needs about as many items as the
+        device has lanes (some 260,000, so 128 MiB of 512-byte chunks) before its rate
+        settles; a smaller prefix would misjudge it against the register tier. DFA: the
+        whole batch up to ``dfa_sample_bytes``. The per-input DFA walk runs one lane per
+        input and its rate grows with inputs per CU (256 to 1024), while the segmented walk
+        fills the device from any count; a byte-bounded prefix of a batch of short inputs
+        would leave the per-input walk a few lanes per CU and misjudge it.
         """
         import numpy as np
 

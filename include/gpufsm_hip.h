@@ -77,6 +77,13 @@ typedef struct gfsm_dfa gfsm_dfa;
 #define GFSM_TUNE_CHAIN_FIRST 1  /* chain: no byte-pair lists (first-position lists only) */
 #define GFSM_TUNE_CHAIN_WAVE64 2 /* chain: one work item per wave in the register tier (64
                                     listed states) instead of two (32 each) */
+#define GFSM_TUNE_CHAIN_TRIPLE 4 /* chain: the lane tier (one work item per lane) over
+                                    byte-triple lists of the states entered at depth 3, for
+                                    pattern unions whose start state loops on every byte and
+                                    whose slots fit 16 bits; ignored for other automata. The
+                                    lists (64 MiB of headers plus the lists) are built and
+                                    uploaded by the first run with this flag, which therefore
+                                    must not be inside stream capture */
 typedef struct {
     int32_t chunk_bytes;      /* chain: bytes per work item when inputs are cut (>= 64) */
     int32_t segment_bytes;    /* dfa split: segment length, a power of two in [256, 1 MiB] */
@@ -101,7 +108,8 @@ typedef struct {
     int32_t general_ok;       /* the epsilon-folded layout of the general techniques exists
                                  (automata that are not chain-structured, and any automaton
                                  of at most 512 states, when the fold stays under its cap) */
-    int64_t device_bytes;     /* HBM held by the handle */
+    int64_t device_bytes;     /* HBM held by the handle (with the chain byte-triple lists
+                                 once a GFSM_TUNE_CHAIN_TRIPLE run has built them) */
 } gfsm_nfa_props;
 
 typedef struct {

@@ -12,10 +12,13 @@
 //     it against the C oracle's simulate_dfa;
 //   * the chain tiles: a pattern union is accepted, every state but the start gets one slot,
 //     and an automaton with an epsilon edge or a state with two successors is refused;
+//   * the chain lane tier's byte-triple lists against a brute-force enumeration, and its walk
+//     against the set simulation;
 //   * validation: malformed CSR input raises the library's error.
 // Exit status 0 and one summary line on success; the first mismatch is printed and exits 1.
 #include <cstdint>
 #include <cstdio>
+#include <algorithm>
 #include <cstdlib>
 #include <set>
 #include <utility>
@@ -373,6 +376,153 @@ void check_tiles(Rng &r) {
     ++g_cases;
 }
 
+// The byte-triple lists of the chain lane tier: every header against a brute-force
+// enumeration from the automaton's own edges (first position u on c1, its successor v on c2,
+// v's successor w on c3 gives slot(w)), the pair accept bits likewise, and a walk of random
+// inputs the way the lane tier walks them (held slots continued through `next`, depth-3
+// arrivals from the lists, depth-1 and depth-2 accepts from the flags) against the set
+// simulation.
+void check_triples(Rng &r) {
+    Nfa a;
+    a.n = 1;
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> out(1);
+    out[0].push_back({0, kAnySymbol});
+    struct Pat {
+        int32_t head;
+        std::vector<std::vector<int>> cls;
+    };
+    std::vector<Pat> pats;
+    auto add_pattern = [&](std::vector<std::vector<int>> cls, bool accepting) {
+        const int32_t head = a.n;
+        const int len = static_cast<int>(cls.size());
+        for (int q = 0; q < len; ++q) out.emplace_back();
+        a.n += len;
+        for (int c : cls[0]) out[0].push_back({head, c});
+        for (int q = 0; q + 1 < len; ++q)
+            for (int c : cls[q + 1]) out[head + q].push_back({head + q + 1, c});
+        a.acc.resize(a.n, 0);
+        if (accepting) a.acc[head + len - 1] = 1;
+        pats.push_back({head, std::move(cls)});
+    };
+    a.acc.assign(1, 0);
+    for (int p = 0; p < 80; ++p) {
+        const int len = 1 + r.below(8);
+        std::vector<std::vector<int>> cls(len);
+        for (auto &c : cls) {
+            std::set<int> bytes;
+            const int w = 1 + r.below(5);
+            while (static_cast<int>(bytes.size()) < w) bytes.insert(r.below(16));
+            c.assign(bytes.begin(), bytes.end());
+        }
+        add_pattern(cls, r.below(4) != 0);
+    }
+    for (int p = 0; p < 9; ++p)  // one list of 9 entries: (1, 2, 3)
+        add_pattern({{1}, {2}, {3}, {4 + p % 3}, {5}}, p % 2 == 0);
+    a.start = 0;
+    a.srp.assign(1, 0);
+    for (int32_t s = 0; s < a.n; ++s) {
+        for (auto [t, l] : out[s]) {
+            a.st.push_back(t);
+            a.ss.push_back(l);
+        }
+        a.srp.push_back(static_cast<int32_t>(a.st.size()));
+    }
+    a.erp.assign(a.n + 1, 0);
+    a.et.resize(1);
+    const TileHost t = build_tiles(a.in());
+    if (!t.ok || !t.start_always || t.pair.empty()) fail("triple test automaton refused");
+    const TripleHost tr = build_triples(t);
+    if (!tr.ok) fail("byte-triple lists refused");
+    // brute force over the patterns
+    std::vector<std::vector<int32_t>> want(size_t(1) << 24);
+    std::vector<uint8_t> want_acc(size_t(1) << 24, 0), want_pacc(65536, 0);
+    int64_t entries = 0;
+    for (const Pat &pt : pats) {
+        const int len = static_cast<int>(pt.cls.size());
+        if (len >= 2)
+            for (int c1 : pt.cls[0])
+                for (int c2 : pt.cls[1])
+                    if (a.acc[pt.head + 1]) want_pacc[c1 * 256 + c2] = 1;
+        if (len < 3) continue;
+        const int32_t w = t.slot_of_state[pt.head + 2];
+        for (int c1 : pt.cls[0])
+            for (int c2 : pt.cls[1])
+                for (int c3 : pt.cls[2]) {
+                    const size_t i = (static_cast<size_t>(c1) << 16) | (c2 << 8) | c3;
+                    want[i].push_back(w);
+                    if (a.acc[pt.head + 2]) want_acc[i] = 1;
+                    ++entries;
+                }
+    }
+    if (entries != tr.entries) fail("triple entry count", static_cast<int>(entries),
+                                    static_cast<int>(tr.entries));
+    for (size_t i = 0; i < want.size(); ++i) {
+        const uint32_t h = tr.hdr[i];
+        const uint32_t cnt = (h >> 24) & 127u;
+        if (cnt != want[i].size() || (h >> 31) != want_acc[i])
+            fail("triple header", static_cast<int>(i), static_cast<int>(cnt));
+        if (((tr.nonempty[i >> 5] >> (i & 31)) & 1u) != (cnt > 0 ? 1u : 0u))
+            fail("triple non-empty bit", static_cast<int>(i));
+        if (cnt == 0) continue;
+        std::vector<int32_t> got;
+        if (cnt == 1) {
+            got.push_back(static_cast<int32_t>(h & 0xffffffu));
+        } else {
+            const uint32_t o = h & 0xffffffu;
+            if (o & 3u) fail("triple list not 4-aligned", static_cast<int>(i));
+            for (uint32_t k = 0; k < cnt; ++k) got.push_back(tr.slot.at(o + k));
+        }
+        std::vector<int32_t> w = want[i];
+        std::sort(w.begin(), w.end());
+        std::sort(got.begin(), got.end());
+        if (got != w) fail("triple list", static_cast<int>(i), static_cast<int>(cnt));
+    }
+    for (int i = 0; i < 65536; ++i)
+        if (((tr.pair_acc[i >> 5] >> (i & 31)) & 1u) != want_pacc[i]) fail("pair accept bit", i);
+    // the lane tier's walk
+    const size_t words = static_cast<size_t>(t.tiles + 1) * kTileWords;
+    auto nextbit = [&](int c, uint32_t s) {
+        return (t.next[static_cast<size_t>(c) * words + (s >> 6)] >> (s & 63)) & 1ull;
+    };
+    auto accbit = [&](uint32_t s) { return (t.acc[s >> 6] >> (s & 63)) & 1ull; };
+    for (int k = 0; k < 400; ++k) {
+        const int len = r.below(200);
+        std::vector<uint8_t> in(static_cast<size_t>(len));
+        for (auto &b : in) b = static_cast<uint8_t>(r.below(17));
+        std::vector<uint32_t> held, nxt;
+        std::pair<int, int> got{0, 0};
+        for (int j = 0; j < len && !got.first; ++j) {
+            const int b = in[static_cast<size_t>(j)];
+            bool hit = false;
+            nxt.clear();
+            for (uint32_t h : held)
+                if (nextbit(b, h + 1)) {
+                    hit |= accbit(h + 1) != 0;
+                    nxt.push_back(h + 1);
+                }
+            if (j >= 2) {
+                const uint32_t h = tr.hdr[(static_cast<size_t>(in[j - 2]) << 16) | (in[j - 1] << 8) | b];
+                const uint32_t cnt = (h >> 24) & 127u;
+                hit |= (h >> 31) != 0;
+                for (uint32_t m = 0; m < cnt; ++m)
+                    nxt.push_back(cnt == 1 ? (h & 0xffffu) : tr.slot[(h & 0xffffffu) + m]);
+            }
+            hit |= (t.bflags[b] & 2u) != 0;
+            if (j >= 1) hit |= ((tr.pair_acc[(in[j - 1] << 3) | (b >> 5)] >> (b & 31)) & 1u) != 0;
+            if (hit) got = {1, j + 1};
+            held.swap(nxt);
+        }
+        if (got != sim_ref(a, in.data(), len)) fail("lane-tier walk", k, got.second);
+        ++g_cases;
+    }
+    // refusals: no ANY self-loop on the start state
+    Nfa no_loop = a;
+    no_loop.ss[0] = 0;
+    const TileHost t2 = build_tiles(no_loop.in());
+    if (t2.ok && build_triples(t2).ok) fail("byte-triple lists without an always-active start");
+    ++g_cases;
+}
+
 void check_validation() {
     Nfa a;
     a.n = 3;
@@ -443,6 +593,7 @@ int main() {
     check_fold_cap();
     for (int32_t n : {1, 2, 255, 256, 300, 4096, 33000}) check_dfa(r, n, n < 300 ? 0.1 : 0.01);
     check_tiles(r);
+    check_triples(r);
     check_validation();
     std::printf("host_check ok: %d cases\n", g_cases);
     return 0;

@@ -329,12 +329,15 @@ def _class_batch(rng, used, count, max_len, p_class=0.8, empty_frac=0.1):
 
 def _chain_mode(monkeypatch, mode):
     """pairs: byte-pair lists, two items per wave in the register tier (the default);
-    wave64: byte-pair lists, one item per wave; first: per-byte lists of first positions."""
+    wave64: byte-pair lists, one item per wave; first: per-byte lists of first positions;
+    triple: the lane tier over byte-triple lists (for automata the lists do not apply to,
+    the pairs mode)."""
     monkeypatch.setenv("GFSM_CHAIN_FIRST", "1" if mode == "first" else "0")
     monkeypatch.setenv("GFSM_CHAIN_WAVE64", "1" if mode == "wave64" else "0")
+    monkeypatch.setenv("GFSM_CHAIN_TRIPLE", "1" if mode == "triple" else "0")
 
 
-@pytest.mark.parametrize("mode", ["pairs", "wave64", "first"])
+@pytest.mark.parametrize("mode", ["pairs", "wave64", "first", "triple"])
 @pytest.mark.parametrize("self_loop", ["any", "all", "partial", "none"])
 @pytest.mark.parametrize("chunk", ["64", "333", "4096"])
 def test_chain_random_automata(self_loop, chunk, mode, monkeypatch):
@@ -353,7 +356,7 @@ def test_chain_random_automata(self_loop, chunk, mode, monkeypatch):
         _check_nfa(nfa, data, off, "chain")
 
 
-@pytest.mark.parametrize("mode", ["pairs", "wave64", "first"])
+@pytest.mark.parametrize("mode", ["pairs", "wave64", "first", "triple"])
 @pytest.mark.parametrize("chunk", ["64", "1000"])
 def test_chain_long_inputs_match_positions(chunk, mode, monkeypatch):
     """Wildcard-start union with a few accepting patterns: first matches fall deep into
@@ -370,7 +373,7 @@ def test_chain_long_inputs_match_positions(chunk, mode, monkeypatch):
     assert len(set((wn[wf.astype(bool)] % int(chunk)).tolist())) > 5
 
 
-@pytest.mark.parametrize("mode", ["pairs", "wave64", "first"])
+@pytest.mark.parametrize("mode", ["pairs", "wave64", "first", "triple"])
 def test_chain_overflow_falls_back_to_tiles(mode, monkeypatch):
     """Active sets of more than 1024 slots: 40 chains of wildcard-like classes, each 40 long,
     so every input overflows the LDS list and is rerun by the tiled pass."""
@@ -393,7 +396,7 @@ def test_chain_overflow_falls_back_to_tiles(mode, monkeypatch):
     assert wf.any()
 
 
-@pytest.mark.parametrize("mode", ["pairs", "wave64"])
+@pytest.mark.parametrize("mode", ["pairs", "wave64", "triple"])
 def test_chain_medium_active_sets(mode, monkeypatch):
     """About 40 listed states at once: 3 chains of 15 positions whose classes hold every
     byte but the last position's. Two items per wave overflow their 32 lanes (the LDS tier
@@ -481,7 +484,7 @@ def test_wildcard_chain_closed_form_matches_the_oracle():
     assert wf.any() and not wf.all()
 
 
-@pytest.mark.parametrize("mode", ["pairs", "wave64"])
+@pytest.mark.parametrize("mode", ["pairs", "wave64", "triple"])
 def test_chain_many_deferred_items_per_wave(mode, monkeypatch):
     """About 84 live states on every byte overflow the register tier in both modes, so every
     later chunk of an input goes to the LDS tier unrun. 64 inputs of 1 MiB in 64-byte chunks
@@ -500,7 +503,7 @@ def test_chain_many_deferred_items_per_wave(mode, monkeypatch):
     assert int(wf.sum()) == 32
 
 
-@pytest.mark.parametrize("mode", ["pairs", "wave64"])
+@pytest.mark.parametrize("mode", ["pairs", "wave64", "triple"])
 def test_chain_additions_into_lanes_freed_on_the_same_byte(mode, monkeypatch):
     """After A B...B, 32 states are listed (32 chains A B B ... B); on byte X all of them die
     while 20 second positions are added (chains B X Y). Two items per wave have no lane
@@ -539,7 +542,7 @@ def test_chain_additions_into_lanes_freed_on_the_same_byte(mode, monkeypatch):
     assert wf.any() and not wf.all()
 
 
-@pytest.mark.parametrize("mode", ["pairs", "wave64", "first"])
+@pytest.mark.parametrize("mode", ["pairs", "wave64", "first", "triple"])
 def test_chain_many_tiles(mode, monkeypatch):
     """90,206 states in 23 tiles: above 65,536 slots the lists hold 32-bit slots and the
     accept words are read from global memory instead of LDS. Accepting third positions of
