@@ -18,7 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 OUT = os.path.join(PKG, "gpufsm", "backends", "hip", "libgpufsm_hip.so")
 BUILD = os.path.join(HERE, "_build")
-SOURCES = ["capi.hip", "dfa.hip", "nfa.hip", "chain.hip", "general.hip", "layout.cpp"]
+SOURCES = ["capi.hip", "dfa.hip", "nfa.hip", "chain.hip", "general.hip", "server.hip",
+           "layout.cpp"]
 ARCH = os.environ.get("GPUFSM_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

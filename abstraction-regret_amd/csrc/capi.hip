@@ -636,6 +636,8 @@ void pack_nfa(gfsm_nfa *h, int32_t num_states, const WorklistHost &w, const Tile
 
 }  // namespace
 
+int gfsm::report_error(int code, const std::string &msg) { return fail(code, msg); }
+
 extern "C" {
 
 int gfsm_abi_version(void) { return GFSM_ABI_VERSION; }

@@ -32,6 +32,10 @@ struct Error : std::runtime_error {
     Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
 };
 
+// Records `msg` as the calling thread's last error (gfsm_last_error) and returns `code`: the
+// conversion every C entry point applies to a failure.
+int report_error(int code, const std::string &msg);
+
 #define GFSM_HIP(call)                                                                    \
     do {                                                                                  \
         hipError_t err_ = (call);                                                         \

@@ -1,0 +1,641 @@
+// Native serving loop: continuous batching of match requests over device-resident automata.
+//
+// Requests (automaton index, bytes) are submitted from any host thread; the library copies
+// the bytes at once, so the caller's buffer is free when gfsm_server_submit returns. One
+// scheduler thread per server, started by gfsm_server_start, runs the loop:
+//
+//   completion  every in-flight launch slot whose completion event has fired hands its
+//               verdicts (already copied back into pinned host memory) to its requests and
+//               gives their bytes back to the budget; waiters are woken;
+//   admission   waiting requests are admitted in arrival order while the HBM byte budget
+//               (resident automata + the bytes of every admitted, unfinished request) has
+//               room; a request that can never fit (longer than a slot, or than the budget
+//               left by the automata) is rejected at once, one that waits longer than the
+//               queue timeout is rejected as timed out;
+//   batching    each free slot takes the oldest admitted request whose automaton has fewer
+//               than max_inflight_per_automaton launches running, plus every later admitted
+//               request of the same automaton that fits the slot (bytes and count), packs
+//               them (uint8 bytes, int64 offsets) into the slot's pinned staging buffers and
+//               enqueues on the slot's own stream: the host-to-device copy, the resident run
+//               (gfsm_*_run_device_ws, technique resolved per launch for its shape) and the
+//               copy back of the verdicts, then records the slot's event.
+//
+// Slots have separate streams, device buffers and workspaces, so launches for different
+// automata (and for one automaton, up to its in-flight cap) overlap on the device while
+// new requests keep arriving. The scheduler polls the events of busy slots (hipEventQuery,
+// a read of a completion signal) and otherwise sleeps on a condition variable that submit
+// and stop signal; no Python and no interpreter lock is involved.
+//
+// No upstream counterpart: upstream has no serving path.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <utility>
+#include <vector>
+
+#include "../../include/gpufsm_hip.h"
+#include "internal.hpp"
+
+using namespace gfsm;
+
+namespace {
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+struct Automaton {
+    gfsm_nfa *nfa = nullptr;
+    gfsm_dfa *dfa = nullptr;
+    int32_t technique = 0;
+    gfsm_tuning tuning{};
+    int64_t resident = 0;  // device bytes of the handle
+    int32_t inflight = 0;  // launches running
+};
+
+struct Req {
+    int64_t id = 0;
+    int32_t automaton = 0;
+    std::unique_ptr<uint8_t[]> data;
+    int64_t len = 0;
+    double arrival = 0.0;
+};
+
+struct Record {
+    gfsm_server_result r{};
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    uint8_t *h_data = nullptr;
+    int64_t *h_off = nullptr;
+    int32_t *h_res = nullptr;  // [2 * count]: flags then lens
+    uint8_t *d_data = nullptr;
+    int64_t *d_off = nullptr;
+    int32_t *d_res = nullptr;
+    void *d_ws = nullptr;
+    int64_t ws_bytes = 0;
+    bool busy = false;
+    int32_t automaton = -1;
+    std::vector<Req> reqs;
+};
+
+constexpr int32_t kPending = GFSM_SERVE_PENDING;
+
+}  // namespace
+
+struct gfsm_server {
+    gfsm_server_config cfg{};
+    int device = 0;
+    std::vector<Automaton> automata;
+    std::vector<Slot> slots;
+
+    std::mutex mu;
+    std::condition_variable cv_sched;  // submit, stop
+    std::condition_variable cv_done;   // completions
+    std::deque<Req> waiting, admitted;
+    std::unordered_map<int64_t, Record> table;  // submitted and not yet collected
+    int64_t next_id = 0;
+    int64_t in_flight_bytes = 0, resident_bytes = 0;
+    gfsm_server_stats stats{};
+    bool started = false, stopping = false, drain = true, failed = false;
+    std::string error;
+    std::thread thread;
+
+    ~gfsm_server();
+    void run();
+    void finish_locked(const Req &q, int32_t status, int32_t accepted, int32_t len, double t);
+    void fail_all_locked(const std::string &msg);
+    void launch(Slot &s);
+};
+
+namespace {
+
+int fail(int code, const std::string &msg) { return gfsm::report_error(code, msg); }
+
+template <typename F>
+int guarded(F &&f) {
+    try {
+        f();
+        return GFSM_OK;
+    } catch (const Error &e) {
+        return fail(e.code, e.what());
+    } catch (const std::bad_alloc &) {
+        return fail(GFSM_ENOMEM, "host allocation failed");
+    } catch (const std::exception &e) {
+        return fail(GFSM_EDEVICE, e.what());
+    }
+}
+
+// A C-ABI call of this library made by the scheduler: failures become exceptions carrying
+// the callee's message.
+void lib_check(int rc, const char *what) {
+    if (rc != GFSM_OK) throw Error(rc, std::string(what) + ": " + gfsm_last_error());
+}
+
+int64_t workspace_need(const Automaton &a, int32_t count, int64_t total,
+                       const gfsm_tuning &tu) {
+    int64_t need = 0;
+    if (a.nfa)
+        lib_check(gfsm_nfa_workspace_size(a.nfa, a.technique, count, total, &tu, &need),
+                  "gfsm_nfa_workspace_size");
+    else
+        lib_check(gfsm_dfa_workspace_size(a.dfa, a.technique, count, total, &tu, &need),
+                  "gfsm_dfa_workspace_size");
+    return need;
+}
+
+void free_slot(Slot &s) {
+    if (s.d_ws && s.stream) {  // stream-ordered allocation: freed on its stream
+        (void)hipFreeAsync(s.d_ws, s.stream);
+        (void)hipStreamSynchronize(s.stream);
+        s.d_ws = nullptr;
+    }
+    if (s.done) (void)hipEventDestroy(s.done);
+    if (s.stream) (void)hipStreamDestroy(s.stream);
+    if (s.h_data) (void)hipHostFree(s.h_data);
+    if (s.h_off) (void)hipHostFree(s.h_off);
+    if (s.h_res) (void)hipHostFree(s.h_res);
+    if (s.d_data) (void)hipFree(s.d_data);
+    if (s.d_off) (void)hipFree(s.d_off);
+    if (s.d_res) (void)hipFree(s.d_res);
+    s = Slot();
+}
+
+}  // namespace
+
+gfsm_server::~gfsm_server() {
+    for (Slot &s : slots) free_slot(s);
+}
+
+void gfsm_server::finish_locked(const Req &q, int32_t status, int32_t accepted, int32_t len,
+                                double t) {
+    auto it = table.find(q.id);
+    if (it == table.end() || it->second.r.status != kPending) return;
+    gfsm_server_result &r = it->second.r;
+    r.status = status;
+    r.accepted = accepted;
+    r.match_len = len;
+    r.done_s = t;
+    switch (status) {
+        case GFSM_SERVE_OK:
+            ++stats.completed;
+            stats.credited_bytes += q.len;
+            break;
+        case GFSM_SERVE_REJECTED: ++stats.rejected; break;
+        case GFSM_SERVE_TIMEOUT: ++stats.timed_out; break;
+        default: ++stats.errors; break;
+    }
+}
+
+void gfsm_server::fail_all_locked(const std::string &msg) {
+    failed = true;
+    error = msg;
+    const double t = now_s();
+    for (const Req &q : waiting) finish_locked(q, GFSM_SERVE_ERROR, 0, 0, t);
+    for (const Req &q : admitted) finish_locked(q, GFSM_SERVE_ERROR, 0, 0, t);
+    for (Slot &s : slots)
+        for (const Req &q : s.reqs) finish_locked(q, GFSM_SERVE_ERROR, 0, 0, t);
+    waiting.clear();
+    admitted.clear();
+    cv_done.notify_all();
+}
+
+// Packs the slot's requests and enqueues copy in, run, copy out and the event. Called
+// without the lock: the slot and its requests belong to the scheduler thread.
+void gfsm_server::launch(Slot &s) {
+    const Automaton &a = automata[static_cast<size_t>(s.automaton)];
+    const int32_t count = static_cast<int32_t>(s.reqs.size());
+    int64_t total = 0, longest = 0;
+    s.h_off[0] = 0;
+    for (int32_t i = 0; i < count; ++i) {
+        const Req &q = s.reqs[static_cast<size_t>(i)];
+        if (q.len) std::memcpy(s.h_data + total, q.data.get(), static_cast<size_t>(q.len));
+        total += q.len;
+        longest = std::max(longest, q.len);
+        s.h_off[i + 1] = total;
+    }
+    gfsm_tuning tu = a.tuning;
+    tu.max_input_bytes = static_cast<int32_t>(std::max<int64_t>(longest, 1));
+    const int64_t need = workspace_need(a, count, total, tu);
+    if (need > s.ws_bytes) {
+        // rare: the workspace was sized for the largest shape at configuration; a larger
+        // need grows it in stream order (no device-wide synchronisation)
+        if (s.d_ws) GFSM_HIP(hipFreeAsync(s.d_ws, s.stream));
+        s.d_ws = nullptr;
+        s.ws_bytes = 0;
+        GFSM_HIP(hipMallocAsync(&s.d_ws, static_cast<size_t>(need), s.stream));
+        s.ws_bytes = need;
+    }
+    if (total)
+        GFSM_HIP(hipMemcpyAsync(s.d_data, s.h_data, static_cast<size_t>(total),
+                                hipMemcpyHostToDevice, s.stream));
+    GFSM_HIP(hipMemcpyAsync(s.d_off, s.h_off, static_cast<size_t>(count + 1) * 8,
+                            hipMemcpyHostToDevice, s.stream));
+    int32_t *d_flags = s.d_res, *d_lens = s.d_res + cfg.max_batch_count;
+    if (a.nfa)
+        lib_check(gfsm_nfa_run_device_ws(a.nfa, a.technique, &tu, s.d_data, s.d_off, count,
+                                         total, d_flags, d_lens, s.d_ws, s.ws_bytes, s.stream),
+                  "gfsm_nfa_run_device_ws");
+    else
+        lib_check(gfsm_dfa_run_device_ws(a.dfa, a.technique, &tu, s.d_data, s.d_off, count,
+                                         total, d_flags, d_lens, s.d_ws, s.ws_bytes, s.stream),
+                  "gfsm_dfa_run_device_ws");
+    GFSM_HIP(hipMemcpyAsync(s.h_res, d_flags, static_cast<size_t>(count) * 4,
+                            hipMemcpyDeviceToHost, s.stream));
+    GFSM_HIP(hipMemcpyAsync(s.h_res + cfg.max_batch_count, d_lens,
+                            static_cast<size_t>(count) * 4, hipMemcpyDeviceToHost, s.stream));
+    GFSM_HIP(hipEventRecord(s.done, s.stream));
+}
+
+void gfsm_server::run() {
+    try {
+        GFSM_HIP(hipSetDevice(device));
+    } catch (const std::exception &e) {
+        std::lock_guard<std::mutex> lk(mu);
+        fail_all_locked(e.what());
+        return;
+    }
+    std::unique_lock<std::mutex> lk(mu);
+    try {
+        for (;;) {
+            const bool any_busy = std::any_of(slots.begin(), slots.end(),
+                                              [](const Slot &s) { return s.busy; });
+            if (stopping && (!drain || (!any_busy && waiting.empty() && admitted.empty())))
+                break;
+            bool progress = false;
+            // completion (event queries outside the lock: they may take microseconds)
+            if (any_busy) {
+                std::vector<size_t> ready;
+                lk.unlock();
+                for (size_t i = 0; i < slots.size(); ++i) {
+                    if (!slots[i].busy) continue;
+                    const hipError_t q = hipEventQuery(slots[i].done);
+                    if (q == hipSuccess)
+                        ready.push_back(i);
+                    else if (q != hipErrorNotReady)
+                        throw Error(GFSM_EDEVICE, std::string("launch slot failed: ") +
+                                                      hipGetErrorString(q));
+                }
+                lk.lock();
+                const double t = now_s();
+                for (size_t i : ready) {
+                    Slot &s = slots[i];
+                    const int32_t *flags = s.h_res, *lens = s.h_res + cfg.max_batch_count;
+                    for (size_t k = 0; k < s.reqs.size(); ++k) {
+                        finish_locked(s.reqs[k], GFSM_SERVE_OK, flags[k] ? 1 : 0, lens[k], t);
+                        in_flight_bytes -= s.reqs[k].len;
+                    }
+                    automata[static_cast<size_t>(s.automaton)].inflight -= 1;
+                    s.reqs.clear();
+                    s.busy = false;
+                    s.automaton = -1;
+                }
+                if (!ready.empty()) {
+                    progress = true;
+                    cv_done.notify_all();
+                }
+            }
+            // admission
+            const double t = now_s();
+            while (!waiting.empty()) {
+                Req &q = waiting.front();
+                if (q.len > cfg.max_batch_bytes ||
+                    resident_bytes + q.len > cfg.budget_bytes) {
+                    finish_locked(q, GFSM_SERVE_REJECTED, 0, 0, t);
+                    waiting.pop_front();
+                    progress = true;
+                    cv_done.notify_all();
+                    continue;
+                }
+                if (resident_bytes + in_flight_bytes + q.len <= cfg.budget_bytes) {
+                    in_flight_bytes += q.len;
+                    admitted.push_back(std::move(q));
+                    waiting.pop_front();
+                    progress = true;
+                    continue;
+                }
+                if (cfg.queue_timeout_s > 0 && t - q.arrival > cfg.queue_timeout_s) {
+                    finish_locked(q, GFSM_SERVE_TIMEOUT, 0, 0, t);
+                    waiting.pop_front();
+                    progress = true;
+                    cv_done.notify_all();
+                    continue;
+                }
+                break;
+            }
+            // batching: one launch per free slot while admitted requests can go
+            for (Slot &s : slots) {
+                if (s.busy || admitted.empty()) continue;
+                auto head = std::find_if(admitted.begin(), admitted.end(), [&](const Req &q) {
+                    return automata[static_cast<size_t>(q.automaton)].inflight <
+                           cfg.max_inflight_per_automaton;
+                });
+                if (head == admitted.end()) break;  // every automaton is at its cap
+                const int32_t ai = head->automaton;
+                std::vector<Req> batch;
+                int64_t used = 0;
+                std::deque<Req> keep;
+                while (!admitted.empty()) {
+                    Req &q = admitted.front();
+                    if (q.automaton == ai &&
+                        static_cast<int32_t>(batch.size()) < cfg.max_batch_count &&
+                        (batch.empty() || used + q.len <= cfg.max_batch_bytes)) {
+                        used += q.len;
+                        batch.push_back(std::move(q));
+                    } else {
+                        keep.push_back(std::move(q));
+                    }
+                    admitted.pop_front();
+                }
+                admitted.swap(keep);
+                automata[static_cast<size_t>(ai)].inflight += 1;
+                stats.launches += 1;
+                // the slot owns its requests from here: a failing launch fails them too
+                s.reqs = std::move(batch);
+                s.automaton = ai;
+                s.busy = true;
+                lk.unlock();
+                launch(s);
+                lk.lock();
+                progress = true;
+            }
+            if (progress) continue;
+            const bool busy_now = std::any_of(slots.begin(), slots.end(),
+                                              [](const Slot &s) { return s.busy; });
+            if (busy_now)
+                cv_sched.wait_for(lk, std::chrono::microseconds(cfg.poll_us));
+            else if (!waiting.empty())  // waiting for budget (timeouts still run)
+                cv_sched.wait_for(lk, std::chrono::microseconds(1000));
+            else if (!stopping)
+                cv_sched.wait(lk);
+        }
+    } catch (const std::exception &e) {
+        if (!lk.owns_lock()) lk.lock();
+        fail_all_locked(std::string("serving loop stopped: ") + e.what());
+    }
+    if (!lk.owns_lock()) lk.lock();
+    // a stop without drain: whatever is still queued will never run
+    const double t = now_s();
+    for (const Req &q : waiting) finish_locked(q, GFSM_SERVE_ERROR, 0, 0, t);
+    for (const Req &q : admitted) finish_locked(q, GFSM_SERVE_ERROR, 0, 0, t);
+    waiting.clear();
+    admitted.clear();
+    cv_done.notify_all();
+    lk.unlock();
+    // launches still in flight finish on the device before their slots are freed
+    for (Slot &s : slots)
+        if (s.busy) (void)hipEventSynchronize(s.done);
+    lk.lock();
+    for (Slot &s : slots) {
+        if (!s.busy) continue;
+        const int32_t *flags = s.h_res, *lens = s.h_res + cfg.max_batch_count;
+        for (size_t k = 0; k < s.reqs.size(); ++k)
+            finish_locked(s.reqs[k], failed ? GFSM_SERVE_ERROR : GFSM_SERVE_OK,
+                          flags[k] ? 1 : 0, lens[k], now_s());
+        s.reqs.clear();
+        s.busy = false;
+    }
+    cv_done.notify_all();
+}
+
+extern "C" {
+
+int gfsm_server_create(const gfsm_server_config *config, gfsm_server **out) {
+    return guarded([&] {
+        if (!out) throw Error(GFSM_EINVAL, "null out");
+        *out = nullptr;
+        gfsm_server_config c{};
+        if (config) c = *config;
+        if (c.slots <= 0) c.slots = 8;
+        if (c.max_batch_count <= 0) c.max_batch_count = 4096;
+        if (c.max_batch_bytes <= 0) c.max_batch_bytes = int64_t(16) << 20;
+        if (c.budget_bytes <= 0) c.budget_bytes = int64_t(32) << 30;
+        if (c.max_inflight_per_automaton <= 0) c.max_inflight_per_automaton = 2;
+        if (c.poll_us <= 0) c.poll_us = 20;
+        if (c.slots > 64) throw Error(GFSM_EINVAL, "at most 64 slots");
+        if (c.max_batch_bytes > (int64_t(1) << 31) - 1)
+            throw Error(GFSM_EINVAL, "max_batch_bytes must be below 2 GiB");
+        auto srv = std::make_unique<gfsm_server>();
+        srv->cfg = c;
+        GFSM_HIP(hipGetDevice(&srv->device));
+        srv->slots.resize(static_cast<size_t>(c.slots));
+        const size_t cnt = static_cast<size_t>(c.max_batch_count);
+        const size_t bytes = static_cast<size_t>(c.max_batch_bytes);
+        for (Slot &s : srv->slots) {
+            GFSM_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+            GFSM_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+            GFSM_HIP(hipHostMalloc(reinterpret_cast<void **>(&s.h_data), bytes + 256,
+                                   hipHostMallocDefault));
+            GFSM_HIP(hipHostMalloc(reinterpret_cast<void **>(&s.h_off), (cnt + 1) * 8,
+                                   hipHostMallocDefault));
+            GFSM_HIP(hipHostMalloc(reinterpret_cast<void **>(&s.h_res), 2 * cnt * 4,
+                                   hipHostMallocDefault));
+            if (hipMalloc(reinterpret_cast<void **>(&s.d_data), bytes + 256) != hipSuccess ||
+                hipMalloc(reinterpret_cast<void **>(&s.d_off), (cnt + 1) * 8) != hipSuccess ||
+                hipMalloc(reinterpret_cast<void **>(&s.d_res), 2 * cnt * 4) != hipSuccess)
+                throw Error(GFSM_ENOMEM, "device buffers of a serving slot");
+        }
+        *out = srv.release();
+    });
+}
+
+static int server_add(gfsm_server *s, gfsm_nfa *nfa, gfsm_dfa *dfa, int32_t technique,
+                      const gfsm_tuning *tuning, int32_t *out_index) {
+    return guarded([&] {
+        if (!s || (!nfa && !dfa) || !out_index) throw Error(GFSM_EINVAL, "null argument");
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->started) throw Error(GFSM_EINVAL, "automata are added before gfsm_server_start");
+        Automaton a;
+        a.nfa = nfa;
+        a.dfa = dfa;
+        a.technique = technique;
+        if (tuning) a.tuning = *tuning;
+        if (nfa) {
+            gfsm_nfa_props p{};
+            lib_check(gfsm_nfa_query(nfa, &p), "gfsm_nfa_query");
+            a.resident = p.device_bytes;
+        } else {
+            gfsm_dfa_props p{};
+            lib_check(gfsm_dfa_query(dfa, &p), "gfsm_dfa_query");
+            a.resident = p.device_bytes;
+        }
+        if (s->resident_bytes + a.resident > s->cfg.budget_bytes)
+            throw Error(GFSM_EINVAL, "the resident automata exceed the HBM budget");
+        // size every slot's workspace for the largest launch shapes of this automaton
+        // (many short inputs, one long input); a launch needing more grows it
+        gfsm_tuning tu = a.tuning;
+        int64_t need = 0;
+        for (const auto &shape : {std::make_pair(s->cfg.max_batch_count, s->cfg.max_batch_bytes),
+                                  std::make_pair(int32_t(1), s->cfg.max_batch_bytes)}) {
+            tu.max_input_bytes = static_cast<int32_t>(
+                shape.first == 1 ? shape.second
+                                 : std::max<int64_t>(1, shape.second / shape.first));
+            need = std::max(need, workspace_need(a, shape.first, shape.second, tu));
+        }
+        for (Slot &sl : s->slots) {
+            if (need <= sl.ws_bytes) continue;
+            if (sl.d_ws) GFSM_HIP(hipFreeAsync(sl.d_ws, sl.stream));
+            sl.d_ws = nullptr;
+            sl.ws_bytes = 0;
+            if (hipMallocAsync(&sl.d_ws, static_cast<size_t>(need), sl.stream) != hipSuccess)
+                throw Error(GFSM_ENOMEM, "workspace of a serving slot");
+            sl.ws_bytes = need;
+            GFSM_HIP(hipStreamSynchronize(sl.stream));
+        }
+        s->resident_bytes += a.resident;
+        s->automata.push_back(a);
+        *out_index = static_cast<int32_t>(s->automata.size() - 1);
+    });
+}
+
+int gfsm_server_add_nfa(gfsm_server *s, gfsm_nfa *nfa, int32_t technique,
+                        const gfsm_tuning *tuning, int32_t *out_index) {
+    return nfa ? server_add(s, nfa, nullptr, technique, tuning, out_index)
+               : fail(GFSM_EINVAL, "null handle");
+}
+
+int gfsm_server_add_dfa(gfsm_server *s, gfsm_dfa *dfa, int32_t technique,
+                        const gfsm_tuning *tuning, int32_t *out_index) {
+    return dfa ? server_add(s, nullptr, dfa, technique, tuning, out_index)
+               : fail(GFSM_EINVAL, "null handle");
+}
+
+int gfsm_server_start(gfsm_server *s) {
+    return guarded([&] {
+        if (!s) throw Error(GFSM_EINVAL, "null server");
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->started) throw Error(GFSM_EINVAL, "already started");
+        if (s->automata.empty()) throw Error(GFSM_EINVAL, "no automata");
+        s->started = true;
+        s->thread = std::thread([s] { s->run(); });
+    });
+}
+
+int gfsm_server_submit(gfsm_server *s, int32_t automaton, const uint8_t *data, int64_t len,
+                       int64_t *out_id) {
+    return guarded([&] {
+        if (!s || !out_id || (len > 0 && !data)) throw Error(GFSM_EINVAL, "null argument");
+        if (len < 0 || len > (int64_t(1) << 31) - 1)
+            throw Error(GFSM_EINVAL, "request length must be in [0, 2^31 - 1]");
+        Req q;
+        q.automaton = automaton;
+        q.len = len;
+        q.data.reset(new uint8_t[static_cast<size_t>(len > 0 ? len : 1)]);
+        if (len) std::memcpy(q.data.get(), data, static_cast<size_t>(len));
+        q.arrival = now_s();
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (automaton < 0 || automaton >= static_cast<int32_t>(s->automata.size()))
+                throw Error(GFSM_EINVAL, "unknown automaton index " + std::to_string(automaton));
+            if (!s->started || s->stopping) throw Error(GFSM_EINVAL, "the server is not running");
+            if (s->failed) throw Error(GFSM_EDEVICE, s->error);
+            q.id = s->next_id++;
+            Record rec;
+            rec.r.status = kPending;
+            rec.r.automaton = automaton;
+            rec.r.arrival_s = q.arrival;
+            rec.r.bytes = len;
+            s->table.emplace(q.id, rec);
+            ++s->stats.submitted;
+            *out_id = q.id;
+            s->waiting.push_back(std::move(q));
+        }
+        s->cv_sched.notify_one();
+    });
+}
+
+int gfsm_server_wait(gfsm_server *s, const int64_t *ids, int64_t n, double timeout_s,
+                     gfsm_server_result *out) {
+    return guarded([&] {
+        if (!s || (n > 0 && (!ids || !out)) || n < 0) throw Error(GFSM_EINVAL, "null argument");
+        std::unique_lock<std::mutex> lk(s->mu);
+        for (int64_t k = 0; k < n; ++k)
+            if (s->table.find(ids[k]) == s->table.end())
+                throw Error(GFSM_EINVAL, "unknown or already collected request id " +
+                                             std::to_string(ids[k]));
+        // an id collected meanwhile by another wait counts as done (reported below)
+        auto all_done = [&] {
+            for (int64_t k = 0; k < n; ++k) {
+                auto it = s->table.find(ids[k]);
+                if (it != s->table.end() && it->second.r.status == kPending) return false;
+            }
+            return true;
+        };
+        if (timeout_s < 0)
+            s->cv_done.wait(lk, all_done);
+        else if (timeout_s > 0)
+            s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), all_done);
+        bool lost = false;
+        for (int64_t k = 0; k < n; ++k) {
+            auto it = s->table.find(ids[k]);
+            if (it == s->table.end()) {
+                std::memset(&out[k], 0, sizeof(out[k]));
+                out[k].status = GFSM_SERVE_ERROR;
+                lost = true;
+                continue;
+            }
+            out[k] = it->second.r;
+            if (it->second.r.status != kPending) s->table.erase(it);  // collected
+        }
+        if (lost) throw Error(GFSM_EINVAL, "a request was collected by another wait");
+    });
+}
+
+int gfsm_server_query(gfsm_server *s, gfsm_server_stats *out) {
+    return guarded([&] {
+        if (!s || !out) throw Error(GFSM_EINVAL, "null argument");
+        std::lock_guard<std::mutex> lk(s->mu);
+        *out = s->stats;
+        out->waiting = static_cast<int64_t>(s->waiting.size());
+        out->admitted = static_cast<int64_t>(s->admitted.size());
+        out->in_flight_bytes = s->in_flight_bytes;
+        out->resident_bytes = s->resident_bytes;
+        out->failed = s->failed ? 1 : 0;
+    });
+}
+
+int gfsm_server_error(gfsm_server *s, char *buf, int32_t buflen) {
+    return guarded([&] {
+        if (!s || !buf || buflen <= 0) throw Error(GFSM_EINVAL, "null argument");
+        std::lock_guard<std::mutex> lk(s->mu);
+        const size_t k = std::min(s->error.size(), static_cast<size_t>(buflen - 1));
+        std::memcpy(buf, s->error.data(), k);
+        buf[k] = '\0';
+    });
+}
+
+int gfsm_server_stop(gfsm_server *s, int32_t drain) {
+    return guarded([&] {
+        if (!s) throw Error(GFSM_EINVAL, "null server");
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (!s->started || s->stopping) return;
+            s->stopping = true;
+            s->drain = drain != 0;
+        }
+        s->cv_sched.notify_all();
+        if (s->thread.joinable()) s->thread.join();
+    });
+}
+
+int gfsm_server_destroy(gfsm_server *s) {
+    return guarded([&] {
+        if (!s) return;
+        (void)gfsm_server_stop(s, 0);
+        delete s;
+    });
+}
+
+}  // extern "C"

@@ -51,6 +51,16 @@ EXPORTS = (
     "gfsm_dfa_run_device_ws",
     "gfsm_dfa_resolve",
     "gfsm_dfa_run_device_static",
+    "gfsm_server_create",
+    "gfsm_server_add_nfa",
+    "gfsm_server_add_dfa",
+    "gfsm_server_start",
+    "gfsm_server_submit",
+    "gfsm_server_wait",
+    "gfsm_server_query",
+    "gfsm_server_error",
+    "gfsm_server_stop",
+    "gfsm_server_destroy",
 )
 
 TUNE_CHAIN_FIRST = 1
@@ -96,6 +106,45 @@ class DfaProps(ctypes.Structure):
         ("pad", ctypes.c_int32),
         ("device_bytes", ctypes.c_int64),
     ]
+
+
+class ServerConfig(ctypes.Structure):
+    """``gfsm_server_config``: zero fields mean the defaults."""
+
+    _fields_ = [
+        ("slots", ctypes.c_int32),
+        ("max_batch_count", ctypes.c_int32),
+        ("max_batch_bytes", ctypes.c_int64),
+        ("budget_bytes", ctypes.c_int64),
+        ("max_inflight_per_automaton", ctypes.c_int32),
+        ("poll_us", ctypes.c_int32),
+        ("queue_timeout_s", ctypes.c_double),
+    ]
+
+
+class ServerResult(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int32),
+        ("accepted", ctypes.c_int32),
+        ("match_len", ctypes.c_int32),
+        ("automaton", ctypes.c_int32),
+        ("bytes", ctypes.c_int64),
+        ("arrival_s", ctypes.c_double),
+        ("done_s", ctypes.c_double),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_int64) for name in (
+        "submitted", "completed", "rejected", "timed_out", "errors", "launches",
+        "credited_bytes", "waiting", "admitted", "in_flight_bytes", "resident_bytes")] + [
+        ("failed", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+# request status (GFSM_SERVE_*)
+SERVE_PENDING, SERVE_OK, SERVE_REJECTED, SERVE_TIMEOUT, SERVE_ERROR = 0, 1, 2, 3, 4
+SERVE_STATUS = {SERVE_PENDING: "queued", SERVE_OK: "ok", SERVE_REJECTED: "rejected",
+                SERVE_TIMEOUT: "timeout", SERVE_ERROR: "error"}
 
 
 class Tuning(ctypes.Structure):
@@ -146,6 +195,17 @@ _SIGNATURES = {
     "gfsm_dfa_resolve": (_c.c_int, [_vp, _i32, _c.POINTER(Tuning), _i32, _c.c_int64, _i32p]),
     "gfsm_dfa_run_device_static": (_c.c_int, [_vp, _i32, _c.POINTER(Tuning), _vp, _vp, _i32,
                                               _c.c_int64, _vp, _vp, _vp]),
+    "gfsm_server_create": (_c.c_int, [_c.POINTER(ServerConfig), _c.POINTER(_vp)]),
+    "gfsm_server_add_nfa": (_c.c_int, [_vp, _vp, _i32, _c.POINTER(Tuning), _i32p]),
+    "gfsm_server_add_dfa": (_c.c_int, [_vp, _vp, _i32, _c.POINTER(Tuning), _i32p]),
+    "gfsm_server_start": (_c.c_int, [_vp]),
+    "gfsm_server_submit": (_c.c_int, [_vp, _i32, _vp, _c.c_int64, _i64p]),
+    "gfsm_server_wait": (_c.c_int, [_vp, _i64p, _c.c_int64, _c.c_double,
+                                    _c.POINTER(ServerResult)]),
+    "gfsm_server_query": (_c.c_int, [_vp, _c.POINTER(ServerStats)]),
+    "gfsm_server_error": (_c.c_int, [_vp, _c.c_char_p, _i32]),
+    "gfsm_server_stop": (_c.c_int, [_vp, _i32]),
+    "gfsm_server_destroy": (_c.c_int, [_vp]),
 }
 
 STATIC_SCRATCH_BYTES = 32 << 20  # GFSM_STATIC_SCRATCH_BYTES

@@ -30,6 +30,14 @@ device. Automata stay resident for the life of the loop (:class:`~gpufsm.batch.R
 The device side sits behind a small slot interface (:class:`GpuSlots`), so the scheduling
 logic also runs against a host-side stand-in in the CPU test suite.
 
+:class:`NativeServingLoop` is the same loop run by the library itself (``gfsm_server_*``
+in ``csrc/server.hip``): a native scheduler thread does completion, admission, batching,
+packing and launching, so no request waits for the interpreter lock. Python only submits
+(one C call that copies the bytes) and collects verdicts. It has the same client surface
+(``submit`` / ``Request.wait`` / ``stats`` / ``start`` / ``stop``) and is what the
+benchmark serves with; :class:`ServingLoop` stays as the reference for the scheduling
+rules and for host-side tests.
+
 No upstream counterpart: upstream has no serving path.
 """
 
@@ -499,6 +507,202 @@ class ServingLoop:
         }
 
 
+class NativeRequest:
+    """One request of a :class:`NativeServingLoop`; :meth:`wait` collects its verdict."""
+
+    __slots__ = ("loop", "rid", "name", "nbytes", "arrival", "status", "accepted",
+                 "match_len", "done_at")
+
+    def __init__(self, loop: "NativeServingLoop", rid: int, name: str, nbytes: int) -> None:
+        self.loop = loop
+        self.rid = rid
+        self.name = name
+        self.nbytes = nbytes
+        self.arrival = 0.0
+        self.status = "queued"
+        self.accepted = False
+        self.match_len = 0
+        self.done_at = 0.0
+
+    @property
+    def latency(self) -> float:
+        return self.done_at - self.arrival
+
+    def wait(self, timeout: float | None = None) -> bool:
+        if self.status != "queued":
+            return True
+        return self.loop.collect([self], timeout)
+
+
+class NativeServingLoop:
+    """The serving loop run by the library's native scheduler (``gfsm_server_*``).
+
+    Same rules as :class:`ServingLoop` (HBM byte budget, per-automaton batching with an
+    in-flight cap, one launch per free slot, rejection and queue timeout), the same client
+    surface, and the same statistics. Automata are uploaded once (their handles live as
+    long as the loop). ``technique`` applies to every automaton ("auto": resolved by the
+    library per launch for the launch's shape).
+    """
+
+    def __init__(self, automata: dict, *, budget_bytes: int = 32 << 30, slots: int = 8,
+                 max_batch_bytes: int = 16 << 20, max_batch_count: int = 4096,
+                 queue_timeout_s: float | None = None, technique: str = "auto",
+                 max_inflight_per_automaton: int = 2, poll_us: int = 20,
+                 keep_completed: int = 1 << 20) -> None:
+        import ctypes
+
+        from .backends.hip import _lib
+        from .backends.hip.handles import DeviceDFA, DeviceNFA
+        from .core.dfa import DFA
+
+        self._ct = ctypes
+        self._lib = _lib
+        self._so = _lib.load()
+        self.automata = dict(automata)
+        self._handles: dict = {}
+        self._index: dict[str, int] = {}
+        self._srv = None
+        cfg = _lib.ServerConfig(int(slots), int(max_batch_count), int(max_batch_bytes),
+                                int(budget_bytes), int(max_inflight_per_automaton),
+                                int(poll_us), float(queue_timeout_s or 0.0))
+        srv = ctypes.c_void_p()
+        _lib.check(self._so.gfsm_server_create(ctypes.byref(cfg), ctypes.byref(srv)))
+        self._srv = srv
+        try:
+            for name, a in self.automata.items():
+                idx = ctypes.c_int32(0)
+                if isinstance(a, DFA):
+                    h = DeviceDFA(a)
+                    tech = _lib.DFA_TECHNIQUES[technique]
+                    fn = self._so.gfsm_server_add_dfa
+                else:
+                    h = DeviceNFA(a)
+                    tech = _lib.NFA_TECHNIQUES[technique]
+                    fn = self._so.gfsm_server_add_nfa
+                self._handles[name] = h
+                _lib.check(fn(self._srv, h.handle, tech, None, ctypes.byref(idx)))
+                self._index[name] = int(idx.value)
+        except BaseException:
+            self.close()
+            raise
+        self._pending: dict[int, NativeRequest] = {}
+        self.completed: collections.deque = collections.deque(maxlen=keep_completed)
+        self._lock = threading.Lock()
+        self._started = False
+
+    # -- lifecycle ------------------------------------------------------------------------
+    def start(self) -> "NativeServingLoop":
+        self._lib.check(self._so.gfsm_server_start(self._srv))
+        self._started = True
+        return self
+
+    def stop(self, drain: bool = True) -> None:
+        """Stop the scheduler: ``drain`` finishes every queued request first; without it the
+        requests not yet launched end with status "error"."""
+        if self._srv and self._started:
+            self._lib.check(self._so.gfsm_server_stop(self._srv, 1 if drain else 0))
+            self.collect(list(self._pending.values()), 0.0)
+
+    def close(self) -> None:
+        """Stop without draining and free the server, then the automata's handles."""
+        if self._srv:
+            self._so.gfsm_server_destroy(self._srv)
+            self._srv = None
+        for h in self._handles.values():
+            h.close()
+        self._handles = {}
+
+    def __enter__(self) -> "NativeServingLoop":
+        return self.start()
+
+    def __exit__(self, *exc) -> None:
+        self.stop()
+        self.close()
+
+    def __del__(self) -> None:
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- client side ----------------------------------------------------------------------
+    def submit(self, name: str, data: bytes) -> NativeRequest:
+        if name not in self._index:
+            raise KeyError(f"unknown automaton {name!r}; have {sorted(self.automata)}")
+        data = bytes(data)
+        rid = self._ct.c_int64(0)
+        self._lib.check(self._so.gfsm_server_submit(self._srv, self._index[name], data,
+                                                    len(data), self._ct.byref(rid)))
+        r = NativeRequest(self, int(rid.value), name, len(data))
+        with self._lock:
+            self._pending[r.rid] = r
+        return r
+
+    def collect(self, requests: list, timeout: float | None = None) -> bool:
+        """Wait up to ``timeout`` seconds (None: no limit, 0: poll) for ``requests`` and
+        record the verdicts of those that finished; True when all of them have."""
+        todo = [r for r in requests if r.status == "queued"]
+        if not todo:
+            return True
+        ids = np.fromiter((r.rid for r in todo), dtype=np.int64, count=len(todo))
+        out = (self._lib.ServerResult * len(todo))()
+        self._lib.check(self._so.gfsm_server_wait(
+            self._srv, ids.ctypes.data_as(self._ct.POINTER(self._ct.c_int64)), len(todo),
+            -1.0 if timeout is None else float(timeout), out))
+        done = True
+        with self._lock:
+            for r, res in zip(todo, out):
+                if res.status == self._lib.SERVE_PENDING:
+                    done = False
+                    continue
+                r.status = self._lib.SERVE_STATUS[res.status]
+                r.accepted = bool(res.accepted)
+                r.match_len = int(res.match_len)
+                r.arrival = float(res.arrival_s)
+                r.done_at = float(res.done_s)
+                self._pending.pop(r.rid, None)
+                self.completed.append(r)
+        return done
+
+    def error(self) -> str:
+        """The message that stopped the scheduler ("" while it runs)."""
+        buf = self._ct.create_string_buffer(1024)
+        self._lib.check(self._so.gfsm_server_error(self._srv, buf, len(buf)))
+        return buf.value.decode(errors="replace")
+
+    # -- reporting ------------------------------------------------------------------------
+    def query(self) -> dict:
+        """The scheduler's counters (``gfsm_server_stats``)."""
+        st = self._lib.ServerStats()
+        self._lib.check(self._so.gfsm_server_query(self._srv, self._ct.byref(st)))
+        return {name: getattr(st, name) for name, _ in st._fields_ if name != "pad"}
+
+    def stats(self, since: float = 0.0) -> dict:
+        """As :meth:`ServingLoop.stats`, over the requests collected so far (requests still
+        pending are polled first). Times are CLOCK_MONOTONIC, as ``time.perf_counter``."""
+        self.collect(list(self._pending.values()), 0.0)
+        with self._lock:
+            snapshot = list(self.completed)
+        done = [r for r in snapshot if r.arrival >= since]
+        ok = [r for r in done if r.status == "ok"]
+        lat = np.array([r.latency for r in ok]) if ok else np.zeros(0)
+        span = (max(r.done_at for r in ok) - min(r.arrival for r in ok)) if ok else 0.0
+        credited = sum(r.nbytes for r in ok)
+        examined = sum((r.match_len if r.accepted else r.nbytes) for r in ok)
+        return {
+            "requests": len(done),
+            "ok": len(ok),
+            "rejected": sum(r.status == "rejected" for r in done),
+            "timeout": sum(r.status == "timeout" for r in done),
+            "error": sum(r.status == "error" for r in done),
+            "p50_ms": float(np.percentile(lat, 50) * 1e3) if ok else math.nan,
+            "p99_ms": float(np.percentile(lat, 99) * 1e3) if ok else math.nan,
+            "credited_GBps": credited / span / 1e9 if span > 0 else math.nan,
+            "examined_GBps_upper": examined / span / 1e9 if span > 0 else math.nan,
+            "launches": int(self.query()["launches"]),
+        }
+
+
 def poisson_drive(loop: ServingLoop, requests: list[tuple[str, bytes]], qps: float,
                   seed: int = 0) -> list[Request]:
     """Submit ``requests`` open-loop at Poisson arrivals of rate ``qps``; wait for all."""
@@ -512,6 +716,8 @@ def poisson_drive(loop: ServingLoop, requests: list[tuple[str, bytes]], qps: flo
         if delay > 0:
             time.sleep(delay)
         out.append(loop.submit(name, data))
+    if isinstance(loop, NativeServingLoop):
+        loop.collect(out)  # one wait for all of them
     for r in out:
         r.wait()
     return out

@@ -507,13 +507,17 @@ def bench_plain_nfa(args, torch) -> dict:
 def bench_serving(args, torch) -> dict:
     """Serving loop: 8 resident automata, Poisson arrivals at a QPS sweep."""
     from gpufsm import workloads as W
-    from gpufsm.serve import ServingLoop, poisson_drive
+    from gpufsm.serve import NativeServingLoop, ServingLoop, poisson_drive
 
     from oracle import gate, native
 
     automata, unions = W.serving_automata()
-    loop = ServingLoop(automata, budget_bytes=32 << 30, slots=args.serve_slots,
-                       max_batch_bytes=16 << 20, max_batch_count=1024)
+    if args.serve_impl == "python":
+        loop = ServingLoop(automata, budget_bytes=32 << 30, slots=args.serve_slots,
+                           max_batch_bytes=16 << 20, max_batch_count=1024)
+    else:  # the library's native scheduler
+        loop = NativeServingLoop(automata, budget_bytes=32 << 30, slots=args.serve_slots,
+                                 max_batch_bytes=16 << 20, max_batch_count=1024)
     sweep = []
     with loop:
         warm = W.serving_requests(automata, unions, 64, seed=99)
@@ -550,10 +554,14 @@ def bench_serving(args, torch) -> dict:
                 gate.require_arrays(np.array([int(r.accepted)], np.int32),
                                     np.array([r.match_len], np.int32), wf, wn,
                                     f"served {name} request")
+    if hasattr(loop, "close"):
+        loop.close()
     return {"workload": "8 resident automata (DFAs 256/1K/4K/16K states, pattern unions "
                         "1K/4K/16K/64K states), Poisson arrivals, log-normal request "
                         f"lengths (median 16 KiB, clipped 4 MiB), HBM budget 32 GiB, "
                         f"{args.serve_slots} slots",
+            "loop": "native scheduler (gfsm_server_*)" if args.serve_impl != "python"
+                    else "Python scheduler (ServingLoop)",
             "slots": args.serve_slots, "sweep": sweep}
 
 
@@ -1078,6 +1086,8 @@ def main() -> int:
     ap.add_argument("--div-sample-inputs", type=int, default=1024)
     ap.add_argument("--div-sample-len", type=int, default=1 << 14)
     ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-impl", choices=("native", "python"), default="native",
+                    help="serving loop: the library's native scheduler or the Python one")
     ap.add_argument("--gen-count", type=int, default=1024)
     ap.add_argument("--gen-length", type=int, default=1 << 14)
     ap.add_argument("--gen-wl-count", type=int, default=64)

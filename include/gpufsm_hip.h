@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define GFSM_ABI_VERSION 3
+#define GFSM_ABI_VERSION 4
 
 /* status codes */
 #define GFSM_OK 0
@@ -232,6 +232,80 @@ int gfsm_dfa_run_device_static(gfsm_dfa *dfa, int32_t technique, const gfsm_tuni
                                const uint8_t *d_data, const int64_t *d_offsets,
                                int32_t num_strings, int64_t total_bytes, int32_t *d_flags,
                                int32_t *d_lens, void *stream);
+
+/* ---- Serving loop ----------------------------------------------------------------------
+ * Continuous batching of match requests over device-resident automata, run by a native
+ * scheduler thread (no interpreter lock on the request path). No upstream counterpart:
+ * upstream has no serving path; the north-star serving loop of this backend is this API.
+ *
+ *   gfsm_server_create      slots (HIP streams with pinned staging, device buffers and a
+ *                           workspace each) on the current device;
+ *   gfsm_server_add_nfa/dfa register resident handles (the caller keeps them alive until
+ *                           gfsm_server_destroy returns) before gfsm_server_start;
+ *   gfsm_server_submit      copies the request's bytes and queues it; returns its id;
+ *   gfsm_server_wait        waits for requests (timeout_s < 0: no limit, 0: poll) and
+ *                           collects the finished ones: each id is collected exactly once;
+ *   gfsm_server_stop        drain = 1 finishes every queued request first, 0 fails the ones
+ *                           not yet launched (status GFSM_SERVE_ERROR);
+ *   gfsm_server_destroy     stops without draining and frees the server (not the handles).
+ *
+ * Admission: a request is admitted while the resident automata plus the bytes of every
+ * admitted, unfinished request stay within budget_bytes; one longer than max_batch_bytes or
+ * than the budget left by the automata is rejected (GFSM_SERVE_REJECTED); one waiting
+ * longer than queue_timeout_s (> 0) ends with GFSM_SERVE_TIMEOUT. A launch takes the oldest
+ * admitted request whose automaton has fewer than max_inflight_per_automaton launches
+ * running, plus the later admitted requests of that automaton that fit one slot. A device
+ * error stops the loop: pending requests end with GFSM_SERVE_ERROR, later submits fail,
+ * and gfsm_server_error holds the message. Times are steady-clock seconds
+ * (CLOCK_MONOTONIC). Every entry point is thread-safe. */
+#define GFSM_SERVE_PENDING 0
+#define GFSM_SERVE_OK 1
+#define GFSM_SERVE_REJECTED 2
+#define GFSM_SERVE_TIMEOUT 3
+#define GFSM_SERVE_ERROR 4
+
+typedef struct gfsm_server gfsm_server;
+
+typedef struct {              /* a zero field means the default */
+    int32_t slots;            /* launch slots (streams), default 8, at most 64 */
+    int32_t max_batch_count;  /* requests per launch, default 4096 */
+    int64_t max_batch_bytes;  /* bytes per launch and longest request, default 16 MiB */
+    int64_t budget_bytes;     /* HBM admission budget, default 32 GiB */
+    int32_t max_inflight_per_automaton;  /* default 2 */
+    int32_t poll_us;          /* completion polling interval while launches run, default 20 */
+    double queue_timeout_s;   /* <= 0: no timeout */
+} gfsm_server_config;
+
+typedef struct {
+    int32_t status;           /* GFSM_SERVE_* */
+    int32_t accepted;
+    int32_t match_len;
+    int32_t automaton;
+    int64_t bytes;            /* request length */
+    double arrival_s, done_s; /* submit and completion times */
+} gfsm_server_result;
+
+typedef struct {
+    int64_t submitted, completed, rejected, timed_out, errors, launches;
+    int64_t credited_bytes;   /* bytes of the requests completed */
+    int64_t waiting, admitted, in_flight_bytes, resident_bytes;
+    int32_t failed, pad;
+} gfsm_server_stats;
+
+int gfsm_server_create(const gfsm_server_config *config, gfsm_server **out);
+int gfsm_server_add_nfa(gfsm_server *server, gfsm_nfa *nfa, int32_t technique,
+                        const gfsm_tuning *tuning, int32_t *out_index);
+int gfsm_server_add_dfa(gfsm_server *server, gfsm_dfa *dfa, int32_t technique,
+                        const gfsm_tuning *tuning, int32_t *out_index);
+int gfsm_server_start(gfsm_server *server);
+int gfsm_server_submit(gfsm_server *server, int32_t automaton, const uint8_t *data,
+                       int64_t len, int64_t *out_id);
+int gfsm_server_wait(gfsm_server *server, const int64_t *ids, int64_t n, double timeout_s,
+                     gfsm_server_result *out);
+int gfsm_server_query(gfsm_server *server, gfsm_server_stats *out);
+int gfsm_server_error(gfsm_server *server, char *buf, int32_t buflen);
+int gfsm_server_stop(gfsm_server *server, int32_t drain);
+int gfsm_server_destroy(gfsm_server *server);
 
 #ifdef __cplusplus
 }

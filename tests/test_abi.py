@@ -58,12 +58,17 @@ def test_header_constants_match_python():
     assert consts["GFSM_EUNSUPPORTED"] == _lib.EUNSUPPORTED
     assert consts["GFSM_TUNE_CHAIN_FIRST"] == _lib.TUNE_CHAIN_FIRST
     assert consts["GFSM_TUNE_CHAIN_WAVE64"] == _lib.TUNE_CHAIN_WAVE64
+    assert [consts[f"GFSM_SERVE_{k}"] for k in ("PENDING", "OK", "REJECTED", "TIMEOUT",
+                                                 "ERROR")] == [
+        _lib.SERVE_PENDING, _lib.SERVE_OK, _lib.SERVE_REJECTED, _lib.SERVE_TIMEOUT,
+        _lib.SERVE_ERROR]
 
 
 def test_props_struct_layout(tmp_path):
     """The ctypes mirrors have the C structs' size and field offsets (compiled with gcc)."""
     fields = {"gfsm_nfa_props": _lib.NfaProps, "gfsm_dfa_props": _lib.DfaProps,
-              "gfsm_tuning": _lib.Tuning}
+              "gfsm_tuning": _lib.Tuning, "gfsm_server_config": _lib.ServerConfig,
+              "gfsm_server_result": _lib.ServerResult, "gfsm_server_stats": _lib.ServerStats}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "gpufsm_hip.h"',
              "int main(void) {"]
     for cname, py in fields.items():

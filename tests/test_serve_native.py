@@ -1,0 +1,159 @@
+"""The native serving loop (``gfsm_server_*``, :class:`gpufsm.serve.NativeServingLoop`).
+
+Every served verdict is checked against the oracle and against the request's lone run;
+the admission, rejection, timeout and stop paths end every request (none is left waiting).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+
+import numpy as np
+import pytest
+
+from gpufsm import workloads as W
+from gpufsm.backends.hip import _lib
+from gpufsm.batch import BatchEngine
+from gpufsm.core.packing import pack_bytes
+from gpufsm.serve import NativeServingLoop, poisson_drive
+from oracle import native
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle(a, data, off):
+    fn = native.dfa_batch if hasattr(a, "trans") else native.nfa_batch
+    f, n, _ = fn(a, data, off)
+    return f, n
+
+
+def _check_against_oracle(automata, reqs, done):
+    by_name: dict = {}
+    for k, (name, _) in enumerate(reqs):
+        by_name.setdefault(name, []).append(k)
+    for name, ks in by_name.items():
+        data, off = pack_bytes([reqs[k][1] for k in ks])
+        wf, wn = _oracle(automata[name], data, off)
+        for j, k in enumerate(ks):
+            r = done[k]
+            assert r.status == "ok", (name, k, r.status)
+            assert (r.accepted, r.match_len) == (bool(wf[j]), int(wn[j])), (name, k)
+
+
+def test_native_configured_mix_equals_alone_and_oracle():
+    """The configured serving mix (4 random DFAs of 256 to 16,384 states, 4 pattern unions
+    of 1,024 to 65,536 states), Poisson arrivals, log-normal lengths plus requests of the
+    4 MiB cap: every served verdict equals the oracle and the request's lone run."""
+    automata, unions = W.serving_automata(seed=0)
+    reqs = W.serving_requests(automata, unions, 400, seed=13)
+    rng = np.random.default_rng(14)
+    for name in ("dfa16384", "nfa65536", "dfa4096", "nfa4096"):
+        n = 4 << 20
+        if name in unions:
+            d, _ = W.pattern_inputs(unions[name], np.array([n], dtype=np.int64), 0.70, seed=15)
+        else:
+            d = rng.integers(0, 256, size=n, dtype=np.uint8)
+        reqs.insert(int(rng.integers(0, len(reqs))), (name, d.tobytes()))
+    reqs.insert(7, ("dfa256", b""))  # an empty request
+    loop = NativeServingLoop(automata, budget_bytes=32 << 30, slots=8)
+    with loop:
+        done = poisson_drive(loop, reqs, qps=8000, seed=16)
+        st = loop.stats()
+    assert st["ok"] == len(reqs) and st["rejected"] == 0
+    assert st["launches"] < len(reqs)  # requests shared launches
+    _check_against_oracle(automata, reqs, done)
+    # lone runs of a sample, one request per launch
+    alone = BatchEngine(tuner=None)
+    try:
+        for k in range(0, len(reqs), 37):
+            name, data = reqs[k]
+            d = np.frombuffer(data, dtype=np.uint8)
+            f, n, _ = alone.run_packed(automata[name], d, np.array([0, d.size], np.int64))
+            assert (done[k].accepted, done[k].match_len) == (bool(f[0]), int(n[0])), (name, k)
+    finally:
+        alone.close()
+    assert all(r.latency >= 0 for r in done)
+
+
+def test_native_budget_serialises_and_rejects_oversized():
+    """A budget just above the resident automata and one request: requests are admitted
+    one at a time and all complete; a request longer than a slot is rejected at once."""
+    automata, unions = W.serving_automata(seed=1)
+    small = {k: automata[k] for k in ("dfa256", "nfa1024")}
+    probe = NativeServingLoop(small, slots=2)
+    resident = probe.query()["resident_bytes"]
+    probe.close()
+    reqs = W.serving_requests(small, {"nfa1024": unions["nfa1024"]}, 60, seed=21,
+                              max_len=64 << 10)
+    loop = NativeServingLoop(small, budget_bytes=resident + (64 << 10), slots=4,
+                             max_batch_bytes=1 << 20)
+    with loop:
+        big = loop.submit("dfa256", bytes(2 << 20))  # longer than a slot
+        done = [loop.submit(name, data) for name, data in reqs]
+        loop.collect(done)
+        assert big.wait(30.0)
+        assert big.status == "rejected"
+        st = loop.query()
+        assert st["rejected"] == 1 and st["completed"] == len(reqs)
+        assert st["in_flight_bytes"] == 0
+    _check_against_oracle(small, reqs, done)
+
+
+def test_native_queue_timeout_and_stop_without_drain():
+    """Requests waiting past the queue timeout end as timed out; a stop without drain ends
+    every request (ok or error), none is left pending."""
+    automata, unions = W.serving_automata(seed=2)
+    small = {"dfa1024": automata["dfa1024"]}
+    probe = NativeServingLoop(small, slots=1)
+    resident = probe.query()["resident_bytes"]
+    probe.close()
+    # budget for one 1 MiB request: the rest wait, and time out after 1 ms unless the
+    # first finishes sooner
+    loop = NativeServingLoop(small, budget_bytes=resident + (1 << 20), slots=1,
+                             queue_timeout_s=0.001)
+    loop.start()
+    reqs = [loop.submit("dfa1024", bytes(1 << 20)) for _ in range(200)]
+    loop.collect(reqs, 60.0)
+    statuses = {r.status for r in reqs}
+    assert statuses <= {"ok", "timeout"} and "timeout" in statuses
+    loop.stop()
+    loop.close()
+
+    loop = NativeServingLoop(small, slots=2)
+    loop.start()
+    reqs = [loop.submit("dfa1024", bytes(256 << 10)) for _ in range(400)]
+    loop.stop(drain=False)
+    assert all(r.status in ("ok", "error") for r in reqs)
+    assert any(r.status == "ok" for r in reqs)
+    with pytest.raises(_lib.HIPError):
+        loop.submit("dfa1024", b"abc")  # stopped
+    loop.close()
+
+
+def test_native_concurrent_submitters_and_bad_arguments():
+    """Several host threads submit at once; unknown names and indexes are errors."""
+    automata, unions = W.serving_automata(seed=3)
+    small = {k: automata[k] for k in ("dfa256", "nfa4096")}
+    reqs = W.serving_requests(small, {"nfa4096": unions["nfa4096"]}, 240, seed=31,
+                              max_len=256 << 10)
+    loop = NativeServingLoop(small, slots=4)
+    with loop:
+        with pytest.raises(KeyError):
+            loop.submit("nope", b"x")
+        rid = ctypes.c_int64(0)
+        rc = loop._so.gfsm_server_submit(loop._srv, 99, b"x", 1, ctypes.byref(rid))
+        assert rc == _lib.EINVAL and b"unknown automaton" in _lib.load().gfsm_last_error()
+        done: list = [None] * len(reqs)
+
+        def worker(lo):
+            for k in range(lo, len(reqs), 4):
+                done[k] = loop.submit(*reqs[k])
+
+        threads = [threading.Thread(target=worker, args=(lo,)) for lo in range(4)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        loop.collect(done)
+    _check_against_oracle(small, reqs, done)
