@@ -822,14 +822,18 @@ template <typename T>
 __global__ __launch_bounds__(kMapThreads) void dfa_split_map_kernel(const T *__restrict__ gtab,
                                                                     SplitArgs a) {
     extern __shared__ uint4 lds_raw[];
+    const unsigned int flagged = *a.map_count;
+    const unsigned int cnt = flagged < static_cast<unsigned int>(a.map_cap)
+                                 ? flagged : static_cast<unsigned int>(a.map_cap);
+    // a block without a part to map leaves before staging the table (the whole block takes
+    // the same branch): usually few heads need maps, and an idle block holding a CU's LDS
+    // and 1024 lanes keeps other work (the next batch's walk) off that CU
+    if (blockIdx.x >= cnt * kMapParts) return;
     const char *tab = stage_table<T, true>(gtab, a.table_bytes, lds_raw);
     const int n = a.n;  // at most 256: two entries per lane
     uint16_t *imap = reinterpret_cast<uint16_t *>(reinterpret_cast<char *>(lds_raw) + a.table_bytes);
     uint16_t *ihit = imap + kMapPartIntervals * n;
     const uint32_t col = a.col, acc = a.acc;
-    const unsigned int flagged = *a.map_count;
-    const unsigned int cnt = flagged < static_cast<unsigned int>(a.map_cap)
-                                 ? flagged : static_cast<unsigned int>(a.map_cap);
     const int64_t I = a.seg / kMapIntervals;  // a multiple of 16
     const int kk = threadIdx.x >> 7, sub = threadIdx.x & 127;
     for (unsigned int w = blockIdx.x; w < cnt * kMapParts; w += gridDim.x) {
