@@ -593,7 +593,8 @@ def bench_serving(args, torch) -> dict:
                            max_batch_bytes=16 << 20, max_batch_count=1024)
     else:  # the library's native scheduler
         loop = NativeServingLoop(automata, budget_bytes=32 << 30, slots=args.serve_slots,
-                                 max_batch_bytes=16 << 20, max_batch_count=1024)
+                                 max_batch_bytes=16 << 20, max_batch_count=1024,
+                                 max_inflight_per_automaton=args.serve_inflight)
     sweep = []
     with loop:
         warm = W.serving_requests(automata, unions, 64, seed=99)
@@ -1164,6 +1165,8 @@ def main() -> int:
     ap.add_argument("--div-sample-inputs", type=int, default=1024)
     ap.add_argument("--div-sample-len", type=int, default=1 << 14)
     ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-inflight", type=int, default=2,
+                    help="native serving loop: launches in flight per automaton")
     ap.add_argument("--serve-impl", choices=("native", "python"), default="native",
                     help="serving loop: the library's native scheduler or the Python one")
     ap.add_argument("--gen-count", type=int, default=1024)
