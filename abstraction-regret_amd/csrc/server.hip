@@ -418,7 +418,7 @@ int gfsm_server_create(const gfsm_server_config *config, gfsm_server **out) {
         *out = nullptr;
         gfsm_server_config c{};
         if (config) c = *config;
-        if (c.slots <= 0) c.slots = 8;
+        if (c.slots <= 0) c.slots = 4;  // the hardware queues a process gets by default
         if (c.max_batch_count <= 0) c.max_batch_count = 4096;
         if (c.max_batch_bytes <= 0) c.max_batch_bytes = int64_t(16) << 20;
         if (c.budget_bytes <= 0) c.budget_bytes = int64_t(32) << 30;

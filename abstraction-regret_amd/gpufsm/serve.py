@@ -541,10 +541,13 @@ class NativeServingLoop:
     in-flight cap, one launch per free slot, rejection and queue timeout), the same client
     surface, and the same statistics. Automata are uploaded once (their handles live as
     long as the loop). ``technique`` applies to every automaton ("auto": resolved by the
-    library per launch for the launch's shape).
+    library per launch for the launch's shape). ``slots`` defaults to 4, the hardware queues
+    a process gets: more slots share queues, and a long launch then delays the launches
+    queued behind it (on the configured mix at 16k QPS, p50 0.85 ms with 4 slots, 0.91 with
+    8, 1.8 with 16).
     """
 
-    def __init__(self, automata: dict, *, budget_bytes: int = 32 << 30, slots: int = 8,
+    def __init__(self, automata: dict, *, budget_bytes: int = 32 << 30, slots: int = 4,
                  max_batch_bytes: int = 16 << 20, max_batch_count: int = 4096,
                  queue_timeout_s: float | None = None, technique: str = "auto",
                  max_inflight_per_automaton: int = 2, poll_us: int = 20,

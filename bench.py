@@ -1157,7 +1157,7 @@ def main() -> int:
     ap.add_argument("--ragged-technique", default="auto",
                     help='"auto" (autotuned) or a fixed label such as "chain:chunk=2048"')
     ap.add_argument("--serve-qps", type=float, nargs="+", default=[1000.0, 4000.0, 16000.0])
-    ap.add_argument("--serve-slots", type=int, default=8,
+    ap.add_argument("--serve-slots", type=int, default=4,
                     help="serving loop launch slots (streams with their own batch buffers)")
     ap.add_argument("--div-count", type=int, default=1024)
     ap.add_argument("--div-length", type=int, default=1 << 20)

@@ -267,7 +267,7 @@ int gfsm_dfa_run_device_static(gfsm_dfa *dfa, int32_t technique, const gfsm_tuni
 typedef struct gfsm_server gfsm_server;
 
 typedef struct {              /* a zero field means the default */
-    int32_t slots;            /* launch slots (streams), default 8, at most 64 */
+    int32_t slots;            /* launch slots (streams), default 4, at most 64 */
     int32_t max_batch_count;  /* requests per launch, default 4096 */
     int64_t max_batch_bytes;  /* bytes per launch and longest request, default 16 MiB */
     int64_t budget_bytes;     /* HBM admission budget, default 32 GiB */
