@@ -219,6 +219,118 @@ __global__ __launch_bounds__(64 * kGenWaves) void nfa_gather_kernel(
     }
 }
 
+// The same step with kGenWaves waves of one block on one input: the block shares the
+// input's two bitsets and splits the byte's edge list between its 256 lanes, with three
+// barriers per byte (sets cleared; edges applied; accept and any-live reduced). One wave per
+// input leaves a batch of as many inputs as SIMDs at one wave per SIMD, each waiting on its
+// edge loads (61% of its cycles on the 65,536-state control); four waves per input keep four
+// of them in flight per SIMD, and a block holds one input's bitsets (16 KiB at 65,536 states)
+// instead of four.
+__global__ __launch_bounds__(64 * kGenWaves) void nfa_gather_block_kernel(
+    NfaGeneralDev d, const uint8_t *__restrict__ data, const int64_t *__restrict__ off,
+    int32_t count, int32_t *flags, int32_t *lens, int32_t staged) {
+    extern __shared__ uint64_t lds[];
+    __shared__ unsigned int red[2][2];  // per byte parity: any live, accept
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int32_t nthreads = 64 * kGenWaves;
+    const int32_t nwords = d.nwords;
+    const int32_t *boff = d.boff;
+    const uint32_t *bedge = d.bedge;
+    const uint64_t *accw = d.accw;
+    int64_t stage = 0;
+    if (staged) {
+        int32_t *sb = reinterpret_cast<int32_t *>(lds);
+        uint64_t *sa = lds + (258 + 1) / 2;
+        uint32_t *se = reinterpret_cast<uint32_t *>(sa + nwords);
+        for (int32_t j = tid; j < 258; j += nthreads) sb[j] = d.boff[j];
+        for (int32_t j = tid; j < nwords; j += nthreads) sa[j] = d.accw[j];
+        for (int32_t j = tid; j < d.nedges; j += nthreads) se[j] = d.bedge[j];
+        boff = sb;
+        accw = sa;
+        bedge = se;
+        stage = gather_stage_words(d);
+    }
+    uint64_t *const set0 = lds + stage;
+    const int64_t data_end = off[count];
+    __syncthreads();
+    const int32_t any0 = boff[kAnySymbol], any1 = boff[kAnySymbol + 1];
+
+    for (int32_t in = blockIdx.x; in < count; in += gridDim.x) {  // block-uniform
+        const int64_t base = off[in];
+        const int64_t len = off[in + 1] - base;
+        uint64_t *cur = set0, *nxt = set0 + nwords;
+        for (int32_t w = tid; w < nwords; w += nthreads) cur[w] = d.start_bits[w];
+        if (tid < 4) red[tid >> 1][tid & 1] = 0u;
+        int32_t res = 0;
+        Window win;
+        window_open(win, data, base, data_end, lane);
+        __syncthreads();
+        for (int64_t pos = 0; pos < len; ++pos) {
+            const int p = static_cast<int>(pos & 1);
+            const uint32_t c = window_at(win, data, base + pos, data_end, lane);
+            for (int32_t w = tid; w < nwords; w += nthreads) nxt[w] = 0;
+            __syncthreads();  // (a) nxt clear, red[p] clear
+            const int32_t e0 = boff[c];
+            const int32_t nc = boff[c + 1] - e0;
+            const int32_t total = nc + (any1 - any0);
+            int32_t j = tid;
+            for (; j + 7 * nthreads < total; j += 8 * nthreads) {
+                uint32_t v[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int32_t x = j + nthreads * q;
+                    v[q] = bedge[x < nc ? e0 + x : any0 + (x - nc)];
+                }
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const uint32_t s2 = v[q] >> 16, t = v[q] & 0xffffu;
+                    if ((cur[s2 >> 6] >> (s2 & 63)) & 1ull)
+                        atomicOr(reinterpret_cast<unsigned long long *>(&nxt[t >> 6]),
+                                 1ull << (t & 63));
+                }
+            }
+            for (; j < total; j += nthreads) {
+                const uint32_t v = bedge[j < nc ? e0 + j : any0 + (j - nc)];
+                const uint32_t s2 = v >> 16, t = v & 0xffffu;
+                if ((cur[s2 >> 6] >> (s2 & 63)) & 1ull)
+                    atomicOr(reinterpret_cast<unsigned long long *>(&nxt[t >> 6]),
+                             1ull << (t & 63));
+            }
+            __syncthreads();  // (b) nxt complete
+            uint64_t any = 0, acc = 0;
+            for (int32_t w = tid; w < nwords; w += nthreads) {
+                const uint64_t x = nxt[w];
+                any |= x;
+                acc |= x & accw[w];
+            }
+            const bool wany = __any(any != 0ull), wacc = __any(acc != 0ull);
+            if (lane == 0) {
+                if (wany) atomicOr(&red[p][0], 1u);
+                if (wacc) atomicOr(&red[p][1], 1u);
+            }
+            // the other parity was last read before barrier (a) of this byte: clear it for
+            // the next byte
+            if (tid < 2) red[p ^ 1][tid] = 0u;
+            __syncthreads();  // (c) red[p] complete
+            const bool live = red[p][0] != 0u, hit = red[p][1] != 0u;
+            uint64_t *t = cur;
+            cur = nxt;
+            nxt = t;
+            if (hit) {
+                res = static_cast<int32_t>(pos + 1);
+                break;
+            }
+            if (!live) break;
+        }
+        if (tid == 0) {
+            flags[in] = res > 0;
+            lens[in] = res;
+        }
+        __syncthreads();  // every thread is done with the sets before the next input
+    }
+}
+
 static constexpr size_t kGatherStageBudget = 96 * 1024;
 static constexpr size_t kGatherLdsMax = 160 * 1024;
 
@@ -237,6 +349,19 @@ void launch_nfa_gather(const NfaGeneralDev &d, const uint8_t *data, const int64_
     const size_t stage = static_cast<size_t>(gather_stage_words(d)) * 8;
     const bool staged = stage <= kGatherStageBudget && stage + sets <= kGatherLdsMax;
     const size_t bytes = sets + (staged ? stage : 0);
+    // One wave per input while the batch gives every SIMD four inputs of its own; below
+    // that, a block of four waves per input (a wave per input would leave SIMDs with one
+    // latency-bound wave, or none).
+    const int64_t simds = 4 * static_cast<int64_t>(device_cu_count());
+    if (static_cast<int64_t>(count) < 4 * simds || env_int("GFSM_GATHER_BLOCK", 0) == 1) {
+        const size_t bbytes = static_cast<size_t>(2) * d.nwords * 8 + (staged ? stage : 0);
+        ensure_dynamic_lds(reinterpret_cast<const void *>(&nfa_gather_block_kernel), bbytes);
+        const int32_t grid = count < 65536 ? count : 65536;  // blocks loop over the rest
+        nfa_gather_block_kernel<<<grid, 64 * kGenWaves, bbytes, stream>>>(
+            d, data, off, count, flags, lens, staged ? 1 : 0);
+        GFSM_HIP(hipGetLastError());
+        return;
+    }
     ensure_dynamic_lds(reinterpret_cast<const void *>(&nfa_gather_kernel), bytes);
     int32_t grid = (count + kGenWaves - 1) / kGenWaves;
     if (grid > 65536) grid = 65536;  // waves loop over the inputs beyond that

@@ -750,8 +750,9 @@ def bench_general(args, torch) -> dict:
     optional positions (epsilon edges s -> s + 1 on a quarter of the positions, so neither
     chain nor the tiled shift step applies) and no accepting state: with 95% pattern bytes
     thousands of states stay active on every byte and every byte is walked. Each technique
-    is gated against the C oracle on a prefix the oracle walks in seconds, then timed on its
-    own batch (the worklist on a smaller one: its work grows with the active set).
+    is gated against the C oracle on a prefix the oracle walks in seconds, then timed on a
+    batch of the same number of inputs (the worklist's inputs are shorter: its time per byte
+    grows with the active set, and rates are per byte).
     """
     from gpufsm import workloads as W
     from gpufsm.backends.hip import _lib
@@ -1171,8 +1172,8 @@ def main() -> int:
                     help="serving loop: the library's native scheduler or the Python one")
     ap.add_argument("--gen-count", type=int, default=1024)
     ap.add_argument("--gen-length", type=int, default=1 << 14)
-    ap.add_argument("--gen-wl-count", type=int, default=64)
-    ap.add_argument("--gen-wl-length", type=int, default=1 << 12)
+    ap.add_argument("--gen-wl-count", type=int, default=1024)
+    ap.add_argument("--gen-wl-length", type=int, default=1 << 10)
     ap.add_argument("--only", default="",
                     help="comma list of extra sections to run (ragged,plain,serving,"
                          "divergent,general); default all")

@@ -129,6 +129,18 @@ def test_gather_and_worklist_large_random(n):
         _check(nfa, data, off, tech)
 
 
+def test_gather_wave_per_input_on_large_batches():
+    """Batches with four or more inputs per SIMD take the one-wave-per-input bit-parallel
+    step (smaller batches take the four-waves-per-input block step, as every other case
+    here): 5000 ragged inputs, ε edges and wildcards."""
+    rng = random.Random(41)
+    nfa = random_nfa(rng, 3000, bytes(range(24)), max_sym=5, max_eps=1, p_accept=0.003,
+                     p_any=0.01)
+    data, off = _ragged(np.random.default_rng(41), 5000, 120, bytes(range(30)))
+    wf, _ = _check(nfa, data, off, "bitpacked")
+    assert wf.any() and not wf.all()
+
+
 def test_dense_active_non_chain_control():
     """A pattern union with optional positions (epsilon edges: not chain-structured), no
     accepting state, inputs of 95% pattern bytes: active sets in the hundreds on every byte,
