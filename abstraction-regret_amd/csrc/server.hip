@@ -565,10 +565,13 @@ int gfsm_server_wait(gfsm_server *s, const int64_t *ids, int64_t n, double timeo
             if (s->table.find(ids[k]) == s->table.end())
                 throw Error(GFSM_EINVAL, "unknown or already collected request id " +
                                              std::to_string(ids[k]));
-        // an id collected meanwhile by another wait counts as done (reported below)
+        // an id collected meanwhile by another wait counts as done (reported below). The
+        // scan resumes at the first id not yet seen done (ids finish roughly in order), so a
+        // wait on many ids costs about one pass over them, not one pass per wake-up.
+        int64_t first = 0;
         auto all_done = [&] {
-            for (int64_t k = 0; k < n; ++k) {
-                auto it = s->table.find(ids[k]);
+            for (; first < n; ++first) {
+                auto it = s->table.find(ids[first]);
                 if (it != s->table.end() && it->second.r.status == kPending) return false;
             }
             return true;

@@ -257,7 +257,8 @@ int gfsm_dfa_run_device_static(gfsm_dfa *dfa, int32_t technique, const gfsm_tuni
  * running, plus the later admitted requests of that automaton that fit one slot. A device
  * error stops the loop: pending requests end with GFSM_SERVE_ERROR, later submits fail,
  * and gfsm_server_error holds the message. Times are steady-clock seconds
- * (CLOCK_MONOTONIC). Every entry point is thread-safe. */
+ * (CLOCK_MONOTONIC). Every entry point is thread-safe, except that gfsm_server_destroy must
+ * not overlap any other call on the same server. */
 #define GFSM_SERVE_PENDING 0
 #define GFSM_SERVE_OK 1
 #define GFSM_SERVE_REJECTED 2
