@@ -13,8 +13,9 @@
 //               left by the automata) is rejected at once, one that waits longer than the
 //               queue timeout is rejected as timed out;
 //   batching    each free slot takes the oldest admitted request whose automaton has fewer
-//               than max_inflight_per_automaton launches running, plus every later admitted
-//               request of the same automaton that fits the slot (bytes and count), packs
+//               than max_inflight_per_automaton launches running, plus the admitted requests
+//               of the same automaton after it, in order, while they fit the slot (bytes and
+//               count; admitted requests wait in one queue per automaton), packs
 //               them (uint8 bytes, int64 offsets) into the slot's pinned staging buffers and
 //               enqueues on the slot's own stream: the host-to-device copy, the resident run
 //               (gfsm_*_run_device_ws, technique resolved per launch for its shape) and the
@@ -78,11 +79,12 @@ struct Record {
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    uint8_t *h_data = nullptr;
-    int64_t *h_off = nullptr;
-    int32_t *h_res = nullptr;  // [2 * count]: flags then lens
-    uint8_t *d_data = nullptr;
-    int64_t *d_off = nullptr;
+    // staging of one launch, pinned and on the device alike: the requests' bytes, then the
+    // int64 offsets at the next 256-byte boundary (one copy in); the verdicts: count flags
+    // then count lengths (one copy out)
+    uint8_t *h_buf = nullptr;
+    int32_t *h_res = nullptr;
+    uint8_t *d_buf = nullptr;
     int32_t *d_res = nullptr;
     void *d_ws = nullptr;
     int64_t ws_bytes = 0;
@@ -104,7 +106,9 @@ struct gfsm_server {
     std::mutex mu;
     std::condition_variable cv_sched;  // submit, stop
     std::condition_variable cv_done;   // completions
-    std::deque<Req> waiting, admitted;
+    std::deque<Req> waiting;
+    std::vector<std::deque<Req>> admitted;  // per automaton, in arrival order
+    int64_t admitted_count = 0;
     std::unordered_map<int64_t, Record> table;  // submitted and not yet collected
     int64_t next_id = 0;
     int64_t in_flight_bytes = 0, resident_bytes = 0;
@@ -164,11 +168,9 @@ void free_slot(Slot &s) {
     }
     if (s.done) (void)hipEventDestroy(s.done);
     if (s.stream) (void)hipStreamDestroy(s.stream);
-    if (s.h_data) (void)hipHostFree(s.h_data);
-    if (s.h_off) (void)hipHostFree(s.h_off);
+    if (s.h_buf) (void)hipHostFree(s.h_buf);
     if (s.h_res) (void)hipHostFree(s.h_res);
-    if (s.d_data) (void)hipFree(s.d_data);
-    if (s.d_off) (void)hipFree(s.d_off);
+    if (s.d_buf) (void)hipFree(s.d_buf);
     if (s.d_res) (void)hipFree(s.d_res);
     s = Slot();
 }
@@ -204,11 +206,13 @@ void gfsm_server::fail_all_locked(const std::string &msg) {
     error = msg;
     const double t = now_s();
     for (const Req &q : waiting) finish_locked(q, GFSM_SERVE_ERROR, 0, 0, t);
-    for (const Req &q : admitted) finish_locked(q, GFSM_SERVE_ERROR, 0, 0, t);
+    for (const auto &qa : admitted)
+        for (const Req &q : qa) finish_locked(q, GFSM_SERVE_ERROR, 0, 0, t);
     for (Slot &s : slots)
         for (const Req &q : s.reqs) finish_locked(q, GFSM_SERVE_ERROR, 0, 0, t);
     waiting.clear();
-    admitted.clear();
+    for (auto &qa : admitted) qa.clear();
+    admitted_count = 0;
     cv_done.notify_all();
 }
 
@@ -218,14 +222,18 @@ void gfsm_server::launch(Slot &s) {
     const Automaton &a = automata[static_cast<size_t>(s.automaton)];
     const int32_t count = static_cast<int32_t>(s.reqs.size());
     int64_t total = 0, longest = 0;
-    s.h_off[0] = 0;
-    for (int32_t i = 0; i < count; ++i) {
-        const Req &q = s.reqs[static_cast<size_t>(i)];
-        if (q.len) std::memcpy(s.h_data + total, q.data.get(), static_cast<size_t>(q.len));
+    for (const Req &q : s.reqs) {
+        if (q.len) std::memcpy(s.h_buf + total, q.data.get(), static_cast<size_t>(q.len));
         total += q.len;
         longest = std::max(longest, q.len);
-        s.h_off[i + 1] = total;
     }
+    const int64_t off_at = (total + 255) & ~int64_t(255);
+    int64_t *h_off = reinterpret_cast<int64_t *>(s.h_buf + off_at);
+    h_off[0] = 0;
+    for (int32_t i = 0; i < count; ++i)
+        h_off[i + 1] = h_off[i] + s.reqs[static_cast<size_t>(i)].len;
+    const uint8_t *d_data = s.d_buf;
+    const int64_t *d_off = reinterpret_cast<const int64_t *>(s.d_buf + off_at);
     gfsm_tuning tu = a.tuning;
     tu.max_input_bytes = static_cast<int32_t>(std::max<int64_t>(longest, 1));
     const int64_t need = workspace_need(a, count, total, tu);
@@ -238,24 +246,19 @@ void gfsm_server::launch(Slot &s) {
         GFSM_HIP(hipMallocAsync(&s.d_ws, static_cast<size_t>(need), s.stream));
         s.ws_bytes = need;
     }
-    if (total)
-        GFSM_HIP(hipMemcpyAsync(s.d_data, s.h_data, static_cast<size_t>(total),
-                                hipMemcpyHostToDevice, s.stream));
-    GFSM_HIP(hipMemcpyAsync(s.d_off, s.h_off, static_cast<size_t>(count + 1) * 8,
+    GFSM_HIP(hipMemcpyAsync(s.d_buf, s.h_buf, static_cast<size_t>(off_at + (count + 1) * 8),
                             hipMemcpyHostToDevice, s.stream));
-    int32_t *d_flags = s.d_res, *d_lens = s.d_res + cfg.max_batch_count;
+    int32_t *d_flags = s.d_res, *d_lens = s.d_res + count;
     if (a.nfa)
-        lib_check(gfsm_nfa_run_device_ws(a.nfa, a.technique, &tu, s.d_data, s.d_off, count,
-                                         total, d_flags, d_lens, s.d_ws, s.ws_bytes, s.stream),
+        lib_check(gfsm_nfa_run_device_ws(a.nfa, a.technique, &tu, d_data, d_off, count, total,
+                                         d_flags, d_lens, s.d_ws, s.ws_bytes, s.stream),
                   "gfsm_nfa_run_device_ws");
     else
-        lib_check(gfsm_dfa_run_device_ws(a.dfa, a.technique, &tu, s.d_data, s.d_off, count,
-                                         total, d_flags, d_lens, s.d_ws, s.ws_bytes, s.stream),
+        lib_check(gfsm_dfa_run_device_ws(a.dfa, a.technique, &tu, d_data, d_off, count, total,
+                                         d_flags, d_lens, s.d_ws, s.ws_bytes, s.stream),
                   "gfsm_dfa_run_device_ws");
-    GFSM_HIP(hipMemcpyAsync(s.h_res, d_flags, static_cast<size_t>(count) * 4,
+    GFSM_HIP(hipMemcpyAsync(s.h_res, s.d_res, static_cast<size_t>(count) * 8,
                             hipMemcpyDeviceToHost, s.stream));
-    GFSM_HIP(hipMemcpyAsync(s.h_res + cfg.max_batch_count, d_lens,
-                            static_cast<size_t>(count) * 4, hipMemcpyDeviceToHost, s.stream));
     GFSM_HIP(hipEventRecord(s.done, s.stream));
 }
 
@@ -272,7 +275,7 @@ void gfsm_server::run() {
         for (;;) {
             const bool any_busy = std::any_of(slots.begin(), slots.end(),
                                               [](const Slot &s) { return s.busy; });
-            if (stopping && (!drain || (!any_busy && waiting.empty() && admitted.empty())))
+            if (stopping && (!drain || (!any_busy && waiting.empty() && admitted_count == 0)))
                 break;
             bool progress = false;
             // completion (event queries outside the lock: they may take microseconds)
@@ -292,7 +295,8 @@ void gfsm_server::run() {
                 const double t = now_s();
                 for (size_t i : ready) {
                     Slot &s = slots[i];
-                    const int32_t *flags = s.h_res, *lens = s.h_res + cfg.max_batch_count;
+                    const int32_t *flags = s.h_res;
+                    const int32_t *lens = s.h_res + s.reqs.size();
                     for (size_t k = 0; k < s.reqs.size(); ++k) {
                         finish_locked(s.reqs[k], GFSM_SERVE_OK, flags[k] ? 1 : 0, lens[k], t);
                         in_flight_bytes -= s.reqs[k].len;
@@ -321,7 +325,8 @@ void gfsm_server::run() {
                 }
                 if (resident_bytes + in_flight_bytes + q.len <= cfg.budget_bytes) {
                     in_flight_bytes += q.len;
-                    admitted.push_back(std::move(q));
+                    admitted[static_cast<size_t>(q.automaton)].push_back(std::move(q));
+                    ++admitted_count;
                     waiting.pop_front();
                     progress = true;
                     continue;
@@ -337,29 +342,29 @@ void gfsm_server::run() {
             }
             // batching: one launch per free slot while admitted requests can go
             for (Slot &s : slots) {
-                if (s.busy || admitted.empty()) continue;
-                auto head = std::find_if(admitted.begin(), admitted.end(), [&](const Req &q) {
-                    return automata[static_cast<size_t>(q.automaton)].inflight <
-                           cfg.max_inflight_per_automaton;
-                });
-                if (head == admitted.end()) break;  // every automaton is at its cap
-                const int32_t ai = head->automaton;
+                if (s.busy || admitted_count == 0) continue;
+                // the automaton whose oldest admitted request is the oldest of all (ids are
+                // in submit order), among those below their in-flight cap
+                int32_t ai = -1;
+                for (size_t a = 0; a < admitted.size(); ++a) {
+                    if (admitted[a].empty() ||
+                        automata[a].inflight >= cfg.max_inflight_per_automaton)
+                        continue;
+                    if (ai < 0 || admitted[a].front().id <
+                                      admitted[static_cast<size_t>(ai)].front().id)
+                        ai = static_cast<int32_t>(a);
+                }
+                if (ai < 0) break;  // every automaton with requests is at its cap
+                std::deque<Req> &qa = admitted[static_cast<size_t>(ai)];
                 std::vector<Req> batch;
                 int64_t used = 0;
-                std::deque<Req> keep;
-                while (!admitted.empty()) {
-                    Req &q = admitted.front();
-                    if (q.automaton == ai &&
-                        static_cast<int32_t>(batch.size()) < cfg.max_batch_count &&
-                        (batch.empty() || used + q.len <= cfg.max_batch_bytes)) {
-                        used += q.len;
-                        batch.push_back(std::move(q));
-                    } else {
-                        keep.push_back(std::move(q));
-                    }
-                    admitted.pop_front();
+                while (!qa.empty() && static_cast<int32_t>(batch.size()) < cfg.max_batch_count &&
+                       (batch.empty() || used + qa.front().len <= cfg.max_batch_bytes)) {
+                    used += qa.front().len;
+                    batch.push_back(std::move(qa.front()));
+                    qa.pop_front();
                 }
-                admitted.swap(keep);
+                admitted_count -= static_cast<int64_t>(batch.size());
                 automata[static_cast<size_t>(ai)].inflight += 1;
                 stats.launches += 1;
                 // the slot owns its requests from here: a failing launch fails them too
@@ -389,9 +394,11 @@ void gfsm_server::run() {
     // a stop without drain: whatever is still queued will never run
     const double t = now_s();
     for (const Req &q : waiting) finish_locked(q, GFSM_SERVE_ERROR, 0, 0, t);
-    for (const Req &q : admitted) finish_locked(q, GFSM_SERVE_ERROR, 0, 0, t);
+    for (const auto &qa : admitted)
+        for (const Req &q : qa) finish_locked(q, GFSM_SERVE_ERROR, 0, 0, t);
     waiting.clear();
-    admitted.clear();
+    for (auto &qa : admitted) qa.clear();
+    admitted_count = 0;
     cv_done.notify_all();
     lk.unlock();
     // launches still in flight finish on the device before their slots are freed
@@ -400,7 +407,8 @@ void gfsm_server::run() {
     lk.lock();
     for (Slot &s : slots) {
         if (!s.busy) continue;
-        const int32_t *flags = s.h_res, *lens = s.h_res + cfg.max_batch_count;
+        const int32_t *flags = s.h_res;
+        const int32_t *lens = s.h_res + s.reqs.size();
         for (size_t k = 0; k < s.reqs.size(); ++k)
             finish_locked(s.reqs[k], failed ? GFSM_SERVE_ERROR : GFSM_SERVE_OK,
                           flags[k] ? 1 : 0, lens[k], now_s());
@@ -436,14 +444,12 @@ int gfsm_server_create(const gfsm_server_config *config, gfsm_server **out) {
         for (Slot &s : srv->slots) {
             GFSM_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
             GFSM_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
-            GFSM_HIP(hipHostMalloc(reinterpret_cast<void **>(&s.h_data), bytes + 256,
-                                   hipHostMallocDefault));
-            GFSM_HIP(hipHostMalloc(reinterpret_cast<void **>(&s.h_off), (cnt + 1) * 8,
+            const size_t staging = bytes + 256 + (cnt + 1) * 8;
+            GFSM_HIP(hipHostMalloc(reinterpret_cast<void **>(&s.h_buf), staging,
                                    hipHostMallocDefault));
             GFSM_HIP(hipHostMalloc(reinterpret_cast<void **>(&s.h_res), 2 * cnt * 4,
                                    hipHostMallocDefault));
-            if (hipMalloc(reinterpret_cast<void **>(&s.d_data), bytes + 256) != hipSuccess ||
-                hipMalloc(reinterpret_cast<void **>(&s.d_off), (cnt + 1) * 8) != hipSuccess ||
+            if (hipMalloc(reinterpret_cast<void **>(&s.d_buf), staging) != hipSuccess ||
                 hipMalloc(reinterpret_cast<void **>(&s.d_res), 2 * cnt * 4) != hipSuccess)
                 throw Error(GFSM_ENOMEM, "device buffers of a serving slot");
         }
@@ -518,6 +524,7 @@ int gfsm_server_start(gfsm_server *s) {
         std::lock_guard<std::mutex> lk(s->mu);
         if (s->started) throw Error(GFSM_EINVAL, "already started");
         if (s->automata.empty()) throw Error(GFSM_EINVAL, "no automata");
+        s->admitted.resize(s->automata.size());
         s->started = true;
         s->thread = std::thread([s] { s->run(); });
     });
@@ -602,7 +609,7 @@ int gfsm_server_query(gfsm_server *s, gfsm_server_stats *out) {
         std::lock_guard<std::mutex> lk(s->mu);
         *out = s->stats;
         out->waiting = static_cast<int64_t>(s->waiting.size());
-        out->admitted = static_cast<int64_t>(s->admitted.size());
+        out->admitted = s->admitted_count;
         out->in_flight_bytes = s->in_flight_bytes;
         out->resident_bytes = s->resident_bytes;
         out->failed = s->failed ? 1 : 0;

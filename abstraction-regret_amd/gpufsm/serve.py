@@ -539,7 +539,10 @@ class NativeServingLoop:
 
     Same rules as :class:`ServingLoop` (HBM byte budget, per-automaton batching with an
     in-flight cap, one launch per free slot, rejection and queue timeout), the same client
-    surface, and the same statistics. Automata are uploaded once (their handles live as
+    surface, and the same statistics; one difference: a launch takes its automaton's
+    admitted requests in arrival order and stops at the first that does not fit (each
+    automaton's requests wait in a queue of their own), where :class:`ServingLoop` also takes
+    later ones that fit. Automata are uploaded once (their handles live as
     long as the loop). ``technique`` applies to every automaton ("auto": resolved by the
     library per launch for the launch's shape). ``slots`` defaults to 4, the hardware queues
     a process gets: more slots share queues, and a long launch then delays the launches
