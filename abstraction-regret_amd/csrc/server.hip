@@ -67,10 +67,36 @@ struct Automaton {
 struct Req {
     int64_t id = 0;
     int32_t automaton = 0;
-    std::unique_ptr<uint8_t[]> data;
+    std::shared_ptr<uint8_t> block;  // the library's copy (shared by a bulk submit's
+                                     // requests; null for a caller buffer, GFSM_SERVE_NOCOPY)
+    const uint8_t *bytes = nullptr;  // this request's bytes
     int64_t len = 0;
     double arrival = 0.0;
 };
+
+// Copies n bytes with up to 8 threads, one per 2 MiB: one core's memcpy (about 5 GB/s into
+// pinned memory) falls far short of the host's memory bandwidth, and bulk submits and large
+// launches copy tens to hundreds of MiB while the GPU waits for them.
+void copy_parallel(uint8_t *dst, const uint8_t *src, size_t n) {
+    constexpr size_t kPerThread = size_t(2) << 20;
+    const size_t t = std::min<size_t>(8, n / kPerThread);
+    if (t <= 1) {
+        if (n) std::memcpy(dst, src, n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    const size_t part = (n + t - 1) / t;
+    for (size_t k = 1; k < t; ++k) {
+        const size_t lo = k * part, hi = std::min(n, lo + part);
+        if (lo < hi) pool.emplace_back([=] { std::memcpy(dst + lo, src + lo, hi - lo); });
+    }
+    std::memcpy(dst, src, std::min(n, part));
+    for (auto &th : pool) th.join();
+}
+
+std::shared_ptr<uint8_t> new_block(size_t n) {
+    return std::shared_ptr<uint8_t>(new uint8_t[n > 0 ? n : 1], std::default_delete<uint8_t[]>());
+}
 
 struct Record {
     gfsm_server_result r{};
@@ -222,11 +248,25 @@ void gfsm_server::launch(Slot &s) {
     const Automaton &a = automata[static_cast<size_t>(s.automaton)];
     const int32_t count = static_cast<int32_t>(s.reqs.size());
     int64_t total = 0, longest = 0;
-    for (const Req &q : s.reqs) {
-        if (q.len) std::memcpy(s.h_buf + total, q.data.get(), static_cast<size_t>(q.len));
+    // requests that sit back to back in one block (a bulk submit) are copied as one run
+    const uint8_t *run_src = nullptr;
+    int64_t run_at = 0, run_len = 0;
+    auto flush = [&] {
+        if (run_len) copy_parallel(s.h_buf + run_at, run_src, static_cast<size_t>(run_len));
+        run_len = 0;
+    };
+    for (size_t i = 0; i < s.reqs.size(); ++i) {
+        const Req &q = s.reqs[i];
+        if (!(run_len && q.bytes == run_src + run_len)) {
+            flush();
+            run_src = q.bytes;
+            run_at = total;
+        }
+        run_len += q.len;
         total += q.len;
         longest = std::max(longest, q.len);
     }
+    flush();
     const int64_t off_at = (total + 255) & ~int64_t(255);
     int64_t *h_off = reinterpret_cast<int64_t *>(s.h_buf + off_at);
     h_off[0] = 0;
@@ -539,8 +579,9 @@ int gfsm_server_submit(gfsm_server *s, int32_t automaton, const uint8_t *data, i
         Req q;
         q.automaton = automaton;
         q.len = len;
-        q.data.reset(new uint8_t[static_cast<size_t>(len > 0 ? len : 1)]);
-        if (len) std::memcpy(q.data.get(), data, static_cast<size_t>(len));
+        q.block = new_block(static_cast<size_t>(len));
+        q.bytes = q.block.get();
+        if (len) std::memcpy(q.block.get(), data, static_cast<size_t>(len));
         q.arrival = now_s();
         {
             std::lock_guard<std::mutex> lk(s->mu);
@@ -558,6 +599,62 @@ int gfsm_server_submit(gfsm_server *s, int32_t automaton, const uint8_t *data, i
             ++s->stats.submitted;
             *out_id = q.id;
             s->waiting.push_back(std::move(q));
+        }
+        s->cv_sched.notify_one();
+    });
+}
+
+int gfsm_server_submit_many(gfsm_server *s, int32_t automaton, const uint8_t *data,
+                            const int64_t *offsets, int32_t n, int32_t flags,
+                            int64_t *out_ids) {
+    return guarded([&] {
+        if (!s || n < 0 || (n > 0 && (!offsets || !out_ids)))
+            throw Error(GFSM_EINVAL, "null argument");
+        if (n == 0) return;
+        const int64_t first = offsets[0], total = offsets[n] - offsets[0];
+        if (first < 0) throw Error(GFSM_EINVAL, "offsets must start at 0 or above");
+        for (int32_t i = 0; i < n; ++i) {
+            const int64_t len = offsets[i + 1] - offsets[i];
+            if (len < 0 || len > (int64_t(1) << 31) - 1)
+                throw Error(GFSM_EINVAL, "offsets must not decrease and requests must be at "
+                                         "most 2^31 - 1 bytes");
+        }
+        if (total > 0 && !data) throw Error(GFSM_EINVAL, "null data");
+        if (flags & ~GFSM_SERVE_NOCOPY) throw Error(GFSM_EINVAL, "unknown submit flags");
+        // one copy of the whole run, shared by the requests; or none, the caller's buffer
+        // staying valid until every one of them has finished
+        std::shared_ptr<uint8_t> block;
+        const uint8_t *src = data + first;
+        if (!(flags & GFSM_SERVE_NOCOPY)) {
+            block = new_block(static_cast<size_t>(total));
+            copy_parallel(block.get(), src, static_cast<size_t>(total));
+            src = block.get();
+        }
+        const double t = now_s();
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (automaton < 0 || automaton >= static_cast<int32_t>(s->automata.size()))
+                throw Error(GFSM_EINVAL, "unknown automaton index " + std::to_string(automaton));
+            if (!s->started || s->stopping) throw Error(GFSM_EINVAL, "the server is not running");
+            if (s->failed) throw Error(GFSM_EDEVICE, s->error);
+            for (int32_t i = 0; i < n; ++i) {
+                Req q;
+                q.automaton = automaton;
+                q.len = offsets[i + 1] - offsets[i];
+                q.block = block;
+                q.bytes = src + (offsets[i] - first);
+                q.arrival = t;
+                q.id = s->next_id++;
+                Record rec;
+                rec.r.status = kPending;
+                rec.r.automaton = automaton;
+                rec.r.arrival_s = t;
+                rec.r.bytes = q.len;
+                s->table.emplace(q.id, rec);
+                out_ids[i] = q.id;
+                s->waiting.push_back(std::move(q));
+            }
+            s->stats.submitted += n;
         }
         s->cv_sched.notify_one();
     });

@@ -56,6 +56,7 @@ EXPORTS = (
     "gfsm_server_add_dfa",
     "gfsm_server_start",
     "gfsm_server_submit",
+    "gfsm_server_submit_many",
     "gfsm_server_wait",
     "gfsm_server_query",
     "gfsm_server_error",
@@ -143,6 +144,7 @@ class ServerStats(ctypes.Structure):
 
 # request status (GFSM_SERVE_*)
 SERVE_PENDING, SERVE_OK, SERVE_REJECTED, SERVE_TIMEOUT, SERVE_ERROR = 0, 1, 2, 3, 4
+SERVE_NOCOPY = 1  # submit_many flag: the library reads the caller's buffer in place
 SERVE_STATUS = {SERVE_PENDING: "queued", SERVE_OK: "ok", SERVE_REJECTED: "rejected",
                 SERVE_TIMEOUT: "timeout", SERVE_ERROR: "error"}
 
@@ -200,6 +202,7 @@ _SIGNATURES = {
     "gfsm_server_add_dfa": (_c.c_int, [_vp, _vp, _i32, _c.POINTER(Tuning), _i32p]),
     "gfsm_server_start": (_c.c_int, [_vp]),
     "gfsm_server_submit": (_c.c_int, [_vp, _i32, _vp, _c.c_int64, _i64p]),
+    "gfsm_server_submit_many": (_c.c_int, [_vp, _i32, _vp, _i64p, _i32, _i32, _i64p]),
     "gfsm_server_wait": (_c.c_int, [_vp, _i64p, _c.c_int64, _c.c_double,
                                     _c.POINTER(ServerResult)]),
     "gfsm_server_query": (_c.c_int, [_vp, _c.POINTER(ServerStats)]),

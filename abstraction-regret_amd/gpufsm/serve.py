@@ -534,6 +534,53 @@ class NativeRequest:
         return self.loop.collect([self], timeout)
 
 
+class NativeBatch:
+    """Requests submitted together by :meth:`NativeServingLoop.submit_many`.
+
+    After :meth:`wait`, ``status`` (the ``GFSM_SERVE_*`` codes), ``accepted``,
+    ``match_len`` and ``latency`` (seconds) are arrays in submit order.
+    """
+
+    def __init__(self, loop: "NativeServingLoop", name: str, ids: np.ndarray) -> None:
+        self.loop = loop
+        self.name = name
+        self.ids = ids                      # ids not collected yet
+        self._pos = np.arange(ids.size)     # their positions in submit order
+        self._keep = None                   # the caller's bytes, read in place
+        self.done = ids.size == 0
+        self.status = np.zeros(ids.size, dtype=np.int32)
+        self.accepted = np.zeros(ids.size, dtype=bool)
+        self.match_len = np.zeros(ids.size, dtype=np.int32)
+        self.latency = np.zeros(ids.size, dtype=np.float64)
+
+    def __len__(self) -> int:
+        return int(self.status.size)
+
+    def wait(self, timeout: float | None = None) -> bool:
+        """Wait up to ``timeout`` seconds (None: no limit) for every request; True when all
+        have finished (their verdicts are then in the arrays)."""
+        if self.done:
+            return True
+        lp = self.loop
+        out = (lp._lib.ServerResult * self.ids.size)()
+        lp._lib.check(lp._so.gfsm_server_wait(
+            lp._srv, self.ids.ctypes.data_as(lp._ct.POINTER(lp._ct.c_int64)), self.ids.size,
+            -1.0 if timeout is None else float(timeout), out))
+        res = np.ctypeslib.as_array(out)
+        fin = res["status"] != lp._lib.SERVE_PENDING  # these are collected now
+        at = self._pos[fin]
+        self.status[at] = res["status"][fin]
+        self.accepted[at] = res["accepted"][fin] != 0
+        self.match_len[at] = res["match_len"][fin]
+        self.latency[at] = res["done_s"][fin] - res["arrival_s"][fin]
+        self.ids = self.ids[~fin]
+        self._pos = self._pos[~fin]
+        self.done = self.ids.size == 0
+        if self.done:
+            self._keep = None
+        return self.done
+
+
 class NativeServingLoop:
     """The serving loop run by the library's native scheduler (``gfsm_server_*``).
 
@@ -543,8 +590,9 @@ class NativeServingLoop:
     admitted requests in arrival order and stops at the first that does not fit (each
     automaton's requests wait in a queue of their own), where :class:`ServingLoop` also takes
     later ones that fit. Automata are uploaded once (their handles live as
-    long as the loop). ``technique`` applies to every automaton ("auto": resolved by the
-    library per launch for the launch's shape). ``slots`` defaults to 4, the hardware queues
+    long as the loop). ``technique`` applies to every automaton: "auto" (resolved by the
+    library per launch for the launch's shape), a technique name, or a label with tuning
+    such as ``"chain:chunk=512:triple"`` (:meth:`gpufsm.tune.Choice.parse`). ``slots`` defaults to 4, the hardware queues
     a process gets: more slots share queues, and a long launch then delays the launches
     queued behind it (on the configured mix at 16k QPS, p50 0.85 ms with 4 slots, 0.91 with
     8, 1.8 with 16).
@@ -560,6 +608,7 @@ class NativeServingLoop:
         from .backends.hip import _lib
         from .backends.hip.handles import DeviceDFA, DeviceNFA
         from .core.dfa import DFA
+        from .tune import Choice
 
         self._ct = ctypes
         self._lib = _lib
@@ -574,19 +623,22 @@ class NativeServingLoop:
         srv = ctypes.c_void_p()
         _lib.check(self._so.gfsm_server_create(ctypes.byref(cfg), ctypes.byref(srv)))
         self._srv = srv
+        choice = Choice.parse(technique)
+        tuning = choice.tuning()  # the longest input is set per launch by the scheduler
+        tref = ctypes.byref(tuning) if tuning is not None else None
         try:
             for name, a in self.automata.items():
                 idx = ctypes.c_int32(0)
                 if isinstance(a, DFA):
                     h = DeviceDFA(a)
-                    tech = _lib.DFA_TECHNIQUES[technique]
+                    tech = _lib.DFA_TECHNIQUES[choice.technique]
                     fn = self._so.gfsm_server_add_dfa
                 else:
                     h = DeviceNFA(a)
-                    tech = _lib.NFA_TECHNIQUES[technique]
+                    tech = _lib.NFA_TECHNIQUES[choice.technique]
                     fn = self._so.gfsm_server_add_nfa
                 self._handles[name] = h
-                _lib.check(fn(self._srv, h.handle, tech, None, ctypes.byref(idx)))
+                _lib.check(fn(self._srv, h.handle, tech, tref, ctypes.byref(idx)))
                 self._index[name] = int(idx.value)
         except BaseException:
             self.close()
@@ -643,6 +695,32 @@ class NativeServingLoop:
         with self._lock:
             self._pending[r.rid] = r
         return r
+
+    def submit_many(self, name: str, data, offsets, copy: bool = True) -> "NativeBatch":
+        """Requests ``data[offsets[i]:offsets[i + 1]]`` for ``name`` in one call (``data``
+        uint8 and ``offsets`` int64 numpy arrays). Their verdicts come back as arrays through
+        the returned :class:`NativeBatch`, with no per-request Python object (they are not
+        part of :meth:`stats`). ``copy=False`` lets the library read ``data`` in place
+        instead of copying it first; the batch keeps a reference to it, and the caller must
+        not change it until the batch has finished."""
+        if name not in self._index:
+            raise KeyError(f"unknown automaton {name!r}; have {sorted(self.automata)}")
+        data = np.ascontiguousarray(data, dtype=np.uint8)
+        offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        n = int(offsets.size) - 1
+        ids = np.empty(max(n, 0), dtype=np.int64)
+        if n > 0:
+            if int(offsets[0]) < 0 or int(offsets[-1]) > data.size:
+                raise ValueError("offsets must lie inside the data")
+            i64p = self._ct.POINTER(self._ct.c_int64)
+            self._lib.check(self._so.gfsm_server_submit_many(
+                self._srv, self._index[name],
+                self._ct.c_void_p(data.ctypes.data if data.size else 0),
+                offsets.ctypes.data_as(i64p), n, 0 if copy else self._lib.SERVE_NOCOPY,
+                ids.ctypes.data_as(i64p)))
+        batch = NativeBatch(self, name, ids)
+        batch._keep = None if copy else data
+        return batch
 
     def collect(self, requests: list, timeout: float | None = None) -> bool:
         """Wait up to ``timeout`` seconds (None: no limit, 0: poll) for ``requests`` and

@@ -242,7 +242,8 @@ int gfsm_dfa_run_device_static(gfsm_dfa *dfa, int32_t technique, const gfsm_tuni
  *                           workspace each) on the current device;
  *   gfsm_server_add_nfa/dfa register resident handles (the caller keeps them alive until
  *                           gfsm_server_destroy returns) before gfsm_server_start;
- *   gfsm_server_submit      copies the request's bytes and queues it; returns its id;
+ *   gfsm_server_submit      copies the request's bytes and queues it; returns its id
+ *                           (gfsm_server_submit_many: many requests in one call);
  *   gfsm_server_wait        waits for requests (timeout_s < 0: no limit, 0: poll) and
  *                           collects the finished ones: each id is collected exactly once;
  *   gfsm_server_stop        drain = 1 finishes every queued request first, 0 fails the ones
@@ -301,6 +302,16 @@ int gfsm_server_add_dfa(gfsm_server *server, gfsm_dfa *dfa, int32_t technique,
 int gfsm_server_start(gfsm_server *server);
 int gfsm_server_submit(gfsm_server *server, int32_t automaton, const uint8_t *data,
                        int64_t len, int64_t *out_id);
+/* n requests for one automaton at once: request i is data[offsets[i], offsets[i + 1]);
+ * ids in order into out_ids[n]; the requests are queued together, in order, with one
+ * arrival time. The library copies the whole run once (with several threads when it is
+ * large), unless flags holds GFSM_SERVE_NOCOPY: then it reads the caller's buffer when it
+ * stages each launch, and the buffer must stay valid and unchanged until every one of the
+ * n requests has finished. */
+#define GFSM_SERVE_NOCOPY 1
+int gfsm_server_submit_many(gfsm_server *server, int32_t automaton, const uint8_t *data,
+                            const int64_t *offsets, int32_t n, int32_t flags,
+                            int64_t *out_ids);
 int gfsm_server_wait(gfsm_server *server, const int64_t *ids, int64_t n, double timeout_s,
                      gfsm_server_result *out);
 int gfsm_server_query(gfsm_server *server, gfsm_server_stats *out);

@@ -62,6 +62,7 @@ def test_header_constants_match_python():
                                                  "ERROR")] == [
         _lib.SERVE_PENDING, _lib.SERVE_OK, _lib.SERVE_REJECTED, _lib.SERVE_TIMEOUT,
         _lib.SERVE_ERROR]
+    assert consts["GFSM_SERVE_NOCOPY"] == _lib.SERVE_NOCOPY
 
 
 def test_props_struct_layout(tmp_path):

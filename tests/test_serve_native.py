@@ -157,3 +157,45 @@ def test_native_concurrent_submitters_and_bad_arguments():
             t.join()
         loop.collect(done)
     _check_against_oracle(small, reqs, done)
+
+
+def test_native_bulk_submit_matches_oracle_and_checks_offsets():
+    """submit_many: one call for a packed batch (a slice of a larger buffer, empty requests
+    included) next to single submits of another automaton; every verdict equals the
+    oracle. Offsets that decrease or leave the data are refused before anything is queued."""
+    automata, unions = W.serving_automata(seed=4)
+    small = {k: automata[k] for k in ("dfa4096", "nfa16384")}
+    pu = unions["nfa16384"]
+    lens = W.lognormal_lengths(700, 8192, 1.0, 0, 1 << 20, seed=41)
+    data, off = W.pattern_inputs(pu, lens, 0.7, seed=42)
+    loop = NativeServingLoop(small, slots=4, max_batch_bytes=32 << 20)
+    with loop:
+        singles = [("dfa4096", bytes(np.random.default_rng(k).integers(0, 256, 3000,
+                                                                       dtype=np.uint8)))
+                   for k in range(20)]
+        s_reqs = [loop.submit(name, x) for name, x in singles]
+        bulk = loop.submit_many("nfa16384", data, off[100:])  # offsets need not start at 0
+        in_place = loop.submit_many("nfa16384", data, off[:101], copy=False)
+        loop.collect(s_reqs)
+        assert bulk.wait(60.0) and in_place.wait(60.0)
+        with pytest.raises(ValueError):  # past the end of the data
+            loop.submit_many("nfa16384", data, np.array([0, 5, data.size + 1], np.int64))
+        with pytest.raises(_lib.HIPError):
+            ids = np.zeros(2, np.int64)
+            bad = np.array([0, 10, 4], np.int64)  # decreasing: refused by the library
+            rc = loop._so.gfsm_server_submit_many(
+                loop._srv, 0, ctypes.c_void_p(data.ctypes.data),
+                bad.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), 2, 0,
+                ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+            _lib.check(rc)
+        assert loop.query()["submitted"] == len(bulk) + len(in_place) + len(s_reqs)
+    sub_off = off[100:] - off[100]
+    wf, wn = _oracle(pu.nfa, data[off[100]:], sub_off)
+    assert len(bulk) == sub_off.size - 1 and (bulk.status == _lib.SERVE_OK).all()
+    np.testing.assert_array_equal(bulk.accepted, wf.astype(bool))
+    np.testing.assert_array_equal(bulk.match_len, wn)
+    assert (bulk.latency >= 0).all()
+    wf, wn = _oracle(pu.nfa, data, off[:101])
+    np.testing.assert_array_equal(in_place.accepted, wf.astype(bool))
+    np.testing.assert_array_equal(in_place.match_len, wn)
+    _check_against_oracle(small, singles, s_reqs)
