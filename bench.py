@@ -113,14 +113,16 @@ KERNELS = {
 
 
 def _measure_dfa(torch, batch, dfa, technique: str, data, offsets, steps: int, warmup: int,
-                 what: str, depth: int = 1) -> dict:
+                 what: str, depth: int = 1, coalesce: int = 1) -> dict:
     """Gate one DFA/technique on the full batch against the C oracle, then time it.
 
     Two timings: one batch at a time on one stream (``ms_per_step``), and, with ``depth``
     above 1, a stream of the same batch run back to back on ``depth`` streams, each with
     its own outputs and workspace, so that a batch's repair passes (latency-bound tails of
-    a few lanes) overlap the next batch's walk (``stream``). Every slot's verdicts are
-    gated against the oracle after the timed stream.
+    a few lanes) overlap the next batch's walk (``stream``). With ``coalesce`` above 1 each
+    launch of the stream covers that many consecutive batches (copies of the batch back to
+    back in HBM), as a stream engine coalesces batches that are queued together; times are
+    per batch. Every slot's verdicts are gated against the oracle after the timed stream.
     """
     from gpufsm.backends.hip import _lib
     from gpufsm.backends.hip.handles import DeviceDFA, make_tuning
@@ -144,17 +146,24 @@ def _measure_dfa(torch, batch, dfa, technique: str, data, offsets, steps: int, w
         piped = None
         if depth > 1 and count:
             tu = make_tuning(max_input_bytes=length_max)
-            need = dev.workspace_size(count, total, tech, tu)
+            ccount, ctotal = count * coalesce, total * coalesce
+            if coalesce > 1:
+                cdata = batch.data[:total].repeat(coalesce)
+                coff = torch.from_numpy(np.concatenate(
+                    [offsets[:-1] + k * total for k in range(coalesce)] + [[ctotal]])).to("cuda")
+            else:
+                cdata, coff = batch.data, batch.offsets
+            need = dev.workspace_size(ccount, ctotal, tech, tu)
             slots = [(torch.cuda.Stream(),
-                      torch.empty(count, dtype=torch.int32, device="cuda"),
-                      torch.empty(count, dtype=torch.int32, device="cuda"),
+                      torch.empty(ccount, dtype=torch.int32, device="cuda"),
+                      torch.empty(ccount, dtype=torch.int32, device="cuda"),
                       torch.empty(max(need, 256), dtype=torch.uint8, device="cuda"))
                      for _ in range(depth)]
             cur = torch.cuda.current_stream()
 
             def run(k: int) -> None:
                 s, fl, ln, ws = slots[k % depth]
-                dev.run_device_ws(batch.data.data_ptr(), batch.offsets.data_ptr(), count, total,
+                dev.run_device_ws(cdata.data_ptr(), coff.data_ptr(), ccount, ctotal,
                                   fl.data_ptr(), ln.data_ptr(), ws.data_ptr(), ws.numel(),
                                   s.cuda_stream, tech, tu)
 
@@ -166,21 +175,27 @@ def _measure_dfa(torch, batch, dfa, technique: str, data, offsets, steps: int, w
                 for s, *_ in slots:
                     cur.wait_stream(s)
 
+            # at least three launches per slot, so the stream reaches its steady state
+            launches = max(3 * depth, -(-steps // coalesce))
             stream_of(0, max(warmup, 1) * depth)
             torch.cuda.synchronize()
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
             t0 = time.perf_counter()
             e0.record(cur)
-            stream_of(0, steps)
+            stream_of(0, launches)
             e1.record(cur)
             torch.cuda.synchronize()
             pwall = time.perf_counter() - t0
             for _, fl, ln, _ in slots:
-                gate.require_arrays(fl.cpu().numpy(), ln.cpu().numpy(), wf, wm,
+                gate.require_arrays(fl.cpu().numpy(), ln.cpu().numpy(), np.tile(wf, coalesce),
+                                    np.tile(wm, coalesce),
                                     f"hip dfa {technique} ({what}, stream of {depth})")
-            piped = {"depth": depth, "ms_per_step": 1e3 * pwall / steps,
-                     "event_ms_per_step": e0.elapsed_time(e1) / steps}
+            nb = launches * coalesce
+            piped = {"depth": depth, "batches_per_launch": coalesce,
+                     "ms_per_step": 1e3 * pwall / nb,
+                     "event_ms_per_step": e0.elapsed_time(e1) / nb}
+            del cdata, coff, slots
     examined = int(wex.sum())
     credited = int(offsets[-1])
     step_s = wall / steps
@@ -203,7 +218,8 @@ def _measure_dfa(torch, batch, dfa, technique: str, data, offsets, steps: int, w
         piped["examined_GBps"] = examined / (piped["ms_per_step"] * 1e-3) / 1e9
         piped["credited_GBps"] = credited / (piped["ms_per_step"] * 1e-3) / 1e9
         out["stream"] = piped
-        msg += f"; stream of {depth}: {piped['examined_GBps']:.1f} GB/s, " \
+        per = f", {coalesce} batches per launch" if coalesce > 1 else ""
+        msg += f"; stream of {depth}{per}: {piped['examined_GBps']:.1f} GB/s, " \
                f"{piped['ms_per_step']:.3f} ms/batch"
     _log(msg)
     return out
@@ -223,7 +239,8 @@ def bench_dfa(args, torch) -> dict:
     # headline: non-latching control of the chosen table size
     dfa = W.dense_dfa(n, 0.0, seed=0)
     head = _measure_dfa(torch, batch, dfa, args.technique, data, offsets, args.steps,
-                        args.warmup, f"dfa{n} non-latching", depth=args.pipeline)
+                        args.warmup, f"dfa{n} non-latching", depth=args.pipeline,
+                        coalesce=args.coalesce)
     if args.profile:
         # a plain device copy of the same input bytes: the reference point that scales the
         # FETCH_SIZE / WRITE_SIZE counters of a profiled run into HBM bytes
@@ -241,7 +258,8 @@ def bench_dfa(args, torch) -> dict:
                 extras["variants"].append(_measure_dfa(
                     torch, batch, d, t, data, offsets, max(2, args.steps // 2), 1,
                     f"dfa{other_n} {'latching 10%' if latching else 'non-latching'}",
-                    depth=max(args.pipeline, 4) if (t == "auto" and not latching) else 1))
+                    depth=args.pipeline if (t == "auto" and not latching) else 1,
+                    coalesce=2 if (t == "auto" and not latching and other_n > 256) else 1))
     _log("oracle gate passed on every variant (full batch)")
 
     # CPU baseline: the C oracle over all host cores, full batch, headline automaton
@@ -290,11 +308,14 @@ def bench_dfa(args, torch) -> dict:
         # a stream of batches: the step time is the steady-state time per batch
         value, step_ms, kernel_ms = piped["examined_GBps"], piped["ms_per_step"], \
             piped["event_ms_per_step"]
-        kernel_name = (f"{kernel_name}, {piped['depth']} batches in flight on "
-                       f"{piped['depth']} streams (time per batch in steady state)")
-        mode = (f"batches run back to back, {piped['depth']} in flight on their own streams "
-                f"and workspaces (a batch's repair passes overlap the next batch's walk); "
-                f"every slot's verdicts gated; one batch alone is under single_batch")
+        per = (f", {piped['batches_per_launch']} batches per launch"
+               if piped["batches_per_launch"] > 1 else "")
+        kernel_name = (f"{kernel_name}, {piped['depth']} launches in flight on "
+                       f"{piped['depth']} streams{per} (time per batch in steady state)")
+        mode = (f"batches run back to back, {piped['depth']} launches in flight on their own "
+                f"streams and workspaces{per} (a batch's repair passes overlap the next "
+                f"batch's walk); every slot's verdicts gated; one batch alone is under "
+                f"single_batch")
     else:
         value, step_ms, kernel_ms = head["examined_GBps"], head["ms_per_step"], \
             head["kernel_ms"]
@@ -316,6 +337,7 @@ def bench_dfa(args, torch) -> dict:
             "num_states": n, "num_inputs": count, "input_bytes": length,
             "technique": args.technique, "resolved_technique": tech,
             "batches_in_flight": piped["depth"] if piped else 1,
+            "batches_per_launch": piped["batches_per_launch"] if piped else 1,
         },
         "single_batch": single,
         "roofline": {
@@ -1195,6 +1217,8 @@ def main() -> int:
     ap.add_argument("--technique", default="auto")
     ap.add_argument("--pipeline", type=int, default=3,
                     help="DFA batches in flight (streams) for the headline; 1 = one at a time")
+    ap.add_argument("--coalesce", type=int, default=1,
+                    help="DFA headline stream: batches per launch (copies back to back)")
     ap.add_argument("--ragged-count", type=int, default=16384)
     ap.add_argument("--ragged-technique", default="auto",
                     help='"auto" (autotuned) or a fixed label such as "chain:chunk=2048"')
