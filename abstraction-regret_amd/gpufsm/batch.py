@@ -168,9 +168,14 @@ class DeviceBatch:
             except Exception:  # interpreter shutdown or a failed device: nothing to wait on
                 pass
 
-    def load(self, data: np.ndarray, offsets: np.ndarray, stream=None) -> None:
-        """Stage a packed batch (asynchronously on ``stream``) into the resident buffers."""
+    def load(self, data, offsets: np.ndarray, stream=None) -> None:
+        """Stage a packed batch (asynchronously on ``stream``) into the resident buffers.
+
+        ``data`` may also be a list of byte arrays that lie back to back in the batch (the
+        offsets index their concatenation); each is copied straight into the staging buffer.
+        """
         torch = self._torch
+        parts = data if isinstance(data, (list, tuple)) else [data]
         offsets = np.asarray(offsets, dtype=np.int64)
         count = int(offsets.size) - 1
         if count < 0:
@@ -182,8 +187,9 @@ class DeviceBatch:
             raise ValueError("offsets must start at 0 and never decrease")
         if count and int(lengths.max()) > np.iinfo(np.int32).max:
             raise ValueError("an input is longer than 2^31 - 1 bytes (match_len is int32)")
-        if len(data) < total:
-            raise ValueError(f"offsets end at {total}, past the {len(data)}-byte data")
+        size = sum(len(p) for p in parts)
+        if size < total:
+            raise ValueError(f"offsets end at {total}, past the {size}-byte data")
         if count > self.cap_count or total > self.cap_bytes:
             raise ValueError(f"batch of {count} inputs / {total} bytes exceeds the capacity "
                              f"{self.cap_count} / {self.cap_bytes}")
@@ -191,7 +197,12 @@ class DeviceBatch:
         # the previous copy from or into the pinned buffers may still be in flight
         if self._staged is not None:
             self._staged.synchronize()
-        _copy_into(self._h_data[:total].numpy(), np.asarray(data[:total], dtype=np.uint8))
+        staging, at = self._h_data.numpy(), 0
+        for p in parts:
+            n = min(len(p), total - at)
+            if n > 0:
+                _copy_into(staging[at:at + n], np.asarray(p[:n], dtype=np.uint8))
+                at += n
         self._h_off[: count + 1].numpy()[:] = offsets
         # asynchronous copies on the stream's raw handle (no stream context switch)
         s = stream.cuda_stream
@@ -442,27 +453,64 @@ class BatchEngine:
         return flags, lens, e0.elapsed_time(e1)
 
     def run_stream(self, automaton, batches, technique: str = "auto", depth: int = 2,
-                   graph: bool = True, capacity_bytes: int = 0, capacity_count: int = 0):
+                   graph: bool = True, capacity_bytes: int = 0, capacity_count: int = 0,
+                   coalesce: int = 1):
         """Verdicts ``(flags, lens)`` of a stream of host batches ``(data, offsets)``, in order.
 
         ``depth`` slots run at once, each with its own stream, resident batch buffers and
         captured graph: while the GPU walks batch k, the host copies batch k + 1 into the
         next slot's pinned staging and its host-to-device copy is enqueued, so the staging
         cost of a host-resident stream overlaps the kernels instead of adding to them. A
-        slot is reused once its previous verdicts have been taken. Batches must fit the
-        first batch's size unless ``capacity_bytes`` / ``capacity_count`` say more.
+        slot is reused once its previous verdicts have been taken. With ``coalesce`` above
+        1, up to that many consecutive batches share one slot and one launch (their bytes
+        back to back): a launch whose passes are latency-bound (a DFA table read from L2)
+        then walks several batches' chains at once. Slots must hold the first launch's
+        bytes and inputs unless ``capacity_bytes`` / ``capacity_count`` say more.
         """
         torch = self._torch
         if depth < 1:
             raise ValueError("depth must be >= 1")
+        if coalesce < 1:
+            raise ValueError("coalesce must be >= 1")
         slots: list[tuple[DeviceBatch, object]] = []
-        pending: collections.deque = collections.deque()  # (slot, event) in batch order
-        for k, (data, offsets) in enumerate(batches):
+        pending: collections.deque = collections.deque()  # (slot, event, counts) in order
+
+        def groups():
+            group: list = []
+            for b in batches:
+                group.append(b)
+                if len(group) == coalesce:
+                    yield group
+                    group = []
+            if group:
+                yield group
+
+        def results(slot: int, counts: list[int]):
+            f, n = slots[slot][0].results()
+            at = 0
+            for c in counts:
+                yield f[at:at + c], n[at:at + c]
+                at += c
+
+        for k, group in enumerate(groups()):
+            counts = [int(np.asarray(o).size) - 1 for _, o in group]
+            if len(group) == 1:
+                data, offsets = group[0]
+            else:  # one launch over the group: bytes back to back, offsets shifted
+                data, offs, base = [], [np.zeros(1, np.int64)], 0
+                for d, o in group:
+                    o = np.asarray(o, dtype=np.int64)
+                    if o.size < 1 or o[0] != 0:
+                        raise ValueError("offsets must start at 0 and never decrease")
+                    data.append(d[: int(o[-1])])
+                    offs.append(o[1:] + base)
+                    base += int(o[-1])
+                offsets = np.concatenate(offs)
             j = k % depth
             while pending and any(p[0] == j for p in pending):
-                slot, ev = pending.popleft()
+                slot, ev, cts = pending.popleft()
                 ev.synchronize()
-                yield slots[slot][0].results()
+                yield from results(slot, cts)
             if j == len(slots):
                 st = torch.cuda.Stream()  # default priority
                 slots.append((DeviceBatch(data, offsets, capacity_bytes=capacity_bytes,
@@ -471,11 +519,11 @@ class BatchEngine:
                 slots[j][0].load(data, offsets, stream=slots[j][1])
             batch, st = slots[j]
             self.enqueue(automaton, batch, technique, stream=st, graph=graph)
-            pending.append((j, batch.download(st)))
+            pending.append((j, batch.download(st), counts))
         while pending:
-            slot, ev = pending.popleft()
+            slot, ev, cts = pending.popleft()
             ev.synchronize()
-            yield slots[slot][0].results()
+            yield from results(slot, cts)
 
     def run_batch(self, automaton, inputs: list[bytes], technique: str = "auto") -> list[Result]:
         """Drop-in ``run_batch`` over the resident path (same results, same Result layout)."""

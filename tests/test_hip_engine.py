@@ -374,3 +374,41 @@ def test_run_stream_matches_oracle_in_order(depth):
             np.testing.assert_array_equal(n, wn, err_msg=f"batch {k}")
     finally:
         eng.close()
+
+
+@pytest.mark.parametrize("coalesce,depth", [(2, 3), (3, 2)])
+def test_run_stream_coalesced_matches_oracle(coalesce, depth):
+    """Several host batches per launch (bytes back to back, offsets shifted): each batch's
+    verdicts come back as its own slice, in order, equal to the oracle, for a DFA and an
+    NFA, with a short last group, an empty batch and a batch of empty inputs."""
+    eng = BatchEngine(tuner=None)
+    try:
+        rng = np.random.default_rng(80 + coalesce)
+        dfa = W.dense_dfa(4096, 0.3, seed=81)
+        pu = W.pattern_union(200, (4, 16), (1, 20), seed=82)
+        for a in (dfa, pu.nfa):
+            batches = []
+            for k in range(8):
+                count = (0, 1, 250, 37, 250, 250, 5, 120)[k]
+                lens = W.lognormal_lengths(count, 3000, 1.0, 0, 30000, seed=83 + k)
+                if k == 6:
+                    lens = np.zeros(count, np.int64)
+                if a is pu.nfa:
+                    d, o = W.pattern_inputs(pu, lens, 0.7, seed=90 + k)
+                else:
+                    o = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+                    d = rng.integers(0, 256, size=int(o[-1]) + 7, dtype=np.uint8)  # slack
+                batches.append((d, o))
+            cap_b = sum(sorted(int(o[-1]) for _, o in batches)[-coalesce:])
+            got = list(eng.run_stream(a, batches, "auto", depth=depth, coalesce=coalesce,
+                                      capacity_bytes=cap_b, capacity_count=250 * coalesce))
+            assert len(got) == len(batches)
+            for k, ((d, o), (f, n)) in enumerate(zip(batches, got)):
+                wf, wn = _oracle(a, d, o)
+                assert f.shape == wf.shape, k
+                np.testing.assert_array_equal(f, wf, err_msg=f"batch {k}")
+                np.testing.assert_array_equal(n, wn, err_msg=f"batch {k}")
+        with pytest.raises(ValueError):
+            list(eng.run_stream(dfa, [], coalesce=0))
+    finally:
+        eng.close()
