@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <atomic>
 #include <climits>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -177,6 +178,15 @@ bool use_wl2(const gfsm_nfa *h) { return h->general_ok && h->n <= 65536; }
 
 // The bit-parallel step on the general layout (automata that are not chain-structured).
 bool gather_ok(const gfsm_nfa *h) { return h->general_ok && nfa_gather_fits(h->gen); }
+
+// GFSM_GATHER_DIAG: 0 keeps every folded edge in the gather lists, 2 forces the first and
+// diagonal masks wherever they exist (tests), anything else lets the layout decide.
+DiagMode gather_diag_mode() {
+    const char *v = std::getenv("GFSM_GATHER_DIAG");
+    if (v && v[0] == '0' && v[1] == '\0') return DiagMode::off;
+    if (v && v[0] == '2' && v[1] == '\0') return DiagMode::force;
+    return DiagMode::automatic;
+}
 
 int32_t resolve_nfa(const gfsm_nfa *h, int32_t technique, int32_t count) {
     switch (technique) {
@@ -593,6 +603,11 @@ void pack_nfa(gfsm_nfa *h, int32_t num_states, const WorklistHost &w, const Tile
             pk.add(&h->gen.boff, g.boff);
             pk.add(&h->gen.bedge, g.bedge);
             h->gen.nedges = static_cast<int32_t>(g.bedge.size());
+            if (!g.fmask.empty()) pk.add(&h->gen.fmask, g.fmask);
+            if (g.ndiag) pk.add(&h->gen.dmask, g.dmask);
+            h->gen.ndiag = g.ndiag;
+            static_assert(sizeof(h->gen.doff) == sizeof(g.doff), "diagonal offsets");
+            for (int k = 0; k < kMaxDiag; ++k) h->gen.doff[k] = g.doff[k];
         }
         h->gen.n = num_states;
         h->gen.nwords = r.nwords;
@@ -667,7 +682,7 @@ int gfsm_nfa_create(int32_t num_states, int32_t start_state, const uint8_t *acce
         WorklistHost w = build_worklist(a);
         TileHost t = build_tiles(a);
         GeneralHost g;
-        if (wants_general(num_states, t)) g = build_general(a);
+        if (wants_general(num_states, t)) g = build_general(a, gather_diag_mode());
         else g.why = "not built for chain-structured automata above 512 states";
         auto *h = new gfsm_nfa();
         try {
@@ -742,7 +757,7 @@ int gfsm_nfa_analyze(int32_t num_states, int32_t start_state, const uint8_t *acc
         WorklistHost w = build_worklist(a);
         TileHost t = build_tiles(a);
         GeneralHost g;
-        if (wants_general(num_states, t)) g = build_general(a);
+        if (wants_general(num_states, t)) g = build_general(a, gather_diag_mode());
         std::memset(out, 0, sizeof(*out));
         out->num_states = num_states;
         out->start_state = start_state;

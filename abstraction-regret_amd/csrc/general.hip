@@ -40,6 +40,7 @@ namespace gfsm {
 namespace {
 
 constexpr int kGenWaves = 4;  // waves per block (gather, worklist)
+constexpr int kMaxDiag = 4;   // diagonal masks of the gather layout (NfaGeneralDev::doff)
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -116,14 +117,122 @@ __device__ __forceinline__ void for_edges(const NfaGeneralDev &d, int32_t s, uin
     }
 }
 
+// Word w of the state set shifted by dd positions (dd in [-63, 63], not 0): bit s of the
+// set lands on bit s + dd.
+__device__ __forceinline__ uint64_t set_shifted(const uint64_t *set, int32_t w, int32_t nwords,
+                                                int32_t dd) {
+    if (dd > 0) {
+        uint64_t x = set[w] << dd;
+        if (w > 0) x |= set[w - 1] >> (64 - dd);
+        return x;
+    }
+    uint64_t x = set[w] >> -dd;
+    if (w + 1 < nwords) x |= set[w + 1] << (64 + dd);
+    return x;
+}
+
+// The masks' part of the next set on byte c, for the words w = first, first + stride, ...:
+// the first masks (successors of an always-active start) and, for each diagonal offset d,
+// (set shifted by d) & mask (the masks are indexed by target). Written with plain stores
+// (each word by its owner), so the gather lists' atomics come after a barrier. Four words
+// per round: their mask loads are all issued before the first is used, one round trip per
+// four words.
+__device__ __forceinline__ void gather_masks_into(const NfaGeneralDev &d, const uint64_t *cur,
+                                                  uint64_t *nxt, uint32_t c, int32_t first,
+                                                  int32_t stride) {
+    const int32_t nwords = d.nwords;
+    if (!d.fmask && d.ndiag == 0) {
+        for (int32_t w = first; w < nwords; w += stride) nxt[w] = 0;
+        return;
+    }
+    constexpr int kW = 4;
+    const uint64_t *fm = d.fmask ? d.fmask + static_cast<size_t>(c) * nwords : nullptr;
+    const uint64_t *dm = d.dmask + static_cast<size_t>(c) * d.ndiag * nwords;
+    for (int32_t w0 = first; w0 < nwords; w0 += kW * stride) {
+        uint64_t f[kW], m[kW][kMaxDiag];
+#pragma unroll
+        for (int i = 0; i < kW; ++i) {
+            const int32_t w = w0 + i * stride;
+            const bool in = w < nwords;
+            f[i] = fm && in ? fm[w] : 0ull;
+#pragma unroll
+            for (int k = 0; k < kMaxDiag; ++k)
+                m[i][k] = in && k < d.ndiag ? dm[static_cast<size_t>(k) * nwords + w] : 0ull;
+        }
+#pragma unroll
+        for (int i = 0; i < kW; ++i) {
+            const int32_t w = w0 + i * stride;
+            if (w >= nwords) break;
+            uint64_t x = f[i];
+#pragma unroll
+            for (int k = 0; k < kMaxDiag; ++k) {
+                if (k >= d.ndiag) break;
+                x |= set_shifted(cur, w, nwords, d.doff[k]) & m[i][k];
+            }
+            nxt[w] = x;
+        }
+    }
+}
+
+// Edges of the byte's gather list (c, then ANY) applied to the next set: an edge whose
+// source bit is set ORs its target bit in (ds_or_b64, no return).
+__device__ __forceinline__ void gather_apply(const uint64_t *cur, uint64_t *nxt, uint32_t v) {
+    const uint32_t s = v >> 16, t = v & 0xffffu;
+    if ((cur[s >> 6] >> (s & 63)) & 1ull)
+        atomicOr(reinterpret_cast<unsigned long long *>(&nxt[t >> 6]), 1ull << (t & 63));
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- gather (bit-parallel) --
 
-// LDS of a gather block: [staged: boff (258 int32, padded to 8-byte words) | accw | bedge]
-// then two bitsets of nwords per wave.
-__host__ __device__ static int64_t gather_stage_words(const NfaGeneralDev &d) {
-    return (258 + 1) / 2 + d.nwords + (static_cast<int64_t>(d.nedges) + 1) / 2;
+// LDS of a gather block: [staged: boff (258 int32, padded to 8-byte words) | accw | bedge
+// (level 2 only)] then the bitsets. Staging level 1 holds the list offsets and accept words,
+// level 2 also the lists.
+__host__ __device__ static int64_t gather_stage_words(const NfaGeneralDev &d, int32_t level) {
+    if (level <= 0) return 0;
+    return (258 + 1) / 2 + d.nwords + (level >= 2 ? (static_cast<int64_t>(d.nedges) + 1) / 2 : 0);
+}
+
+// Copies the staged part of the layout into LDS (no barrier) and points the views at it.
+__device__ __forceinline__ int64_t gather_stage(const NfaGeneralDev &d, int32_t level,
+                                                uint64_t *lds, int32_t tid, int32_t nthreads,
+                                                const int32_t *&boff, const uint64_t *&accw,
+                                                const uint32_t *&bedge) {
+    if (level <= 0) return 0;
+    int32_t *sb = reinterpret_cast<int32_t *>(lds);
+    uint64_t *sa = lds + (258 + 1) / 2;
+    for (int32_t j = tid; j < 258; j += nthreads) sb[j] = d.boff[j];
+    for (int32_t j = tid; j < d.nwords; j += nthreads) sa[j] = d.accw[j];
+    boff = sb;
+    accw = sa;
+    if (level >= 2) {
+        uint32_t *se = reinterpret_cast<uint32_t *>(sa + d.nwords);
+        for (int32_t j = tid; j < d.nedges; j += nthreads) se[j] = d.bedge[j];
+        bedge = se;
+    }
+    return gather_stage_words(d, level);
+}
+
+// Eight edges of the byte's list per thread, at j, j + stride, ..., loads issued together
+// (the lists of a dense automaton are read from L2 or the MALL, one round trip per round);
+// entries past the list's end read as 0 and are skipped when applied.
+__device__ __forceinline__ void gather_load_round(const uint32_t *bedge, int32_t e0, int32_t nc,
+                                                  int32_t any0, int32_t total, int32_t j,
+                                                  int32_t stride, uint32_t (&v)[8]) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int32_t x = j + stride * q;
+        v[q] = x < total ? bedge[x < nc ? e0 + x : any0 + (x - nc)] : 0u;
+    }
+}
+
+__device__ __forceinline__ void gather_apply_round(const uint64_t *cur, uint64_t *nxt,
+                                                   int32_t total, int32_t j, int32_t stride,
+                                                   const uint32_t (&v)[8]) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        if (j + stride * q < total) gather_apply(cur, nxt, v[q]);
 }
 
 __global__ __launch_bounds__(64 * kGenWaves) void nfa_gather_kernel(
@@ -136,20 +245,8 @@ __global__ __launch_bounds__(64 * kGenWaves) void nfa_gather_kernel(
     const int32_t *boff = d.boff;
     const uint32_t *bedge = d.bedge;
     const uint64_t *accw = d.accw;
-    int64_t stage = 0;
-    if (staged) {
-        int32_t *sb = reinterpret_cast<int32_t *>(lds);
-        uint64_t *sa = lds + (258 + 1) / 2;
-        uint32_t *se = reinterpret_cast<uint32_t *>(sa + nwords);
-        for (int32_t j = threadIdx.x; j < 258; j += blockDim.x) sb[j] = d.boff[j];
-        for (int32_t j = threadIdx.x; j < nwords; j += blockDim.x) sa[j] = d.accw[j];
-        for (int32_t j = threadIdx.x; j < d.nedges; j += blockDim.x) se[j] = d.bedge[j];
-        __syncthreads();
-        boff = sb;
-        accw = sa;
-        bedge = se;
-        stage = gather_stage_words(d);
-    }
+    const int64_t stage = gather_stage(d, staged, lds, threadIdx.x, blockDim.x, boff, accw, bedge);
+    __syncthreads();
     uint64_t *cur = lds + stage + static_cast<int64_t>(wave) * 2 * nwords;
     uint64_t *nxt = cur + nwords;
     const int64_t data_end = off[count];
@@ -165,35 +262,18 @@ __global__ __launch_bounds__(64 * kGenWaves) void nfa_gather_kernel(
         window_open(win, data, base, data_end, lane);
         for (int64_t pos = 0; pos < len; ++pos) {
             const uint32_t c = window_at(win, data, base + pos, data_end, lane);
-            for (int32_t w = lane; w < nwords; w += 64) nxt[w] = 0;
-            wave_sync();
             const int32_t e0 = boff[c];
             const int32_t nc = boff[c + 1] - e0;
             const int32_t total = nc + (any1 - any0);
-            // eight edges per lane per round, their loads issued together (the edge lists of
-            // a dense automaton are read from L2 or the MALL, one round trip per round)
-            int32_t j = lane;
-            for (; j + 448 < total; j += 512) {
-                uint32_t v[8];
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const int32_t x = j + 64 * q;
-                    v[q] = bedge[x < nc ? e0 + x : any0 + (x - nc)];
-                }
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const uint32_t s = v[q] >> 16, t = v[q] & 0xffffu;
-                    if ((cur[s >> 6] >> (s & 63)) & 1ull)
-                        atomicOr(reinterpret_cast<unsigned long long *>(&nxt[t >> 6]),
-                                 1ull << (t & 63));
-                }
-            }
-            for (; j < total; j += 64) {
-                const uint32_t v = bedge[j < nc ? e0 + j : any0 + (j - nc)];
-                const uint32_t s = v >> 16, t = v & 0xffffu;
-                if ((cur[s >> 6] >> (s & 63)) & 1ull)
-                    atomicOr(reinterpret_cast<unsigned long long *>(&nxt[t >> 6]),
-                             1ull << (t & 63));
+            // the list's first round is loaded with the masks, one round trip for both
+            uint32_t v[8];
+            gather_load_round(bedge, e0, nc, any0, total, lane, 64, v);
+            gather_masks_into(d, cur, nxt, c, lane, 64);
+            wave_sync();
+            gather_apply_round(cur, nxt, total, lane, 64, v);
+            for (int32_t j = lane + 512; j < total; j += 512) {
+                gather_load_round(bedge, e0, nc, any0, total, j, 64, v);
+                gather_apply_round(cur, nxt, total, j, 64, v);
             }
             wave_sync();
             uint64_t any = 0, acc = 0;
@@ -238,19 +318,7 @@ __global__ __launch_bounds__(64 * kGenWaves) void nfa_gather_block_kernel(
     const int32_t *boff = d.boff;
     const uint32_t *bedge = d.bedge;
     const uint64_t *accw = d.accw;
-    int64_t stage = 0;
-    if (staged) {
-        int32_t *sb = reinterpret_cast<int32_t *>(lds);
-        uint64_t *sa = lds + (258 + 1) / 2;
-        uint32_t *se = reinterpret_cast<uint32_t *>(sa + nwords);
-        for (int32_t j = tid; j < 258; j += nthreads) sb[j] = d.boff[j];
-        for (int32_t j = tid; j < nwords; j += nthreads) sa[j] = d.accw[j];
-        for (int32_t j = tid; j < d.nedges; j += nthreads) se[j] = d.bedge[j];
-        boff = sb;
-        accw = sa;
-        bedge = se;
-        stage = gather_stage_words(d);
-    }
+    const int64_t stage = gather_stage(d, staged, lds, tid, nthreads, boff, accw, bedge);
     uint64_t *const set0 = lds + stage;
     const int64_t data_end = off[count];
     __syncthreads();
@@ -269,33 +337,18 @@ __global__ __launch_bounds__(64 * kGenWaves) void nfa_gather_block_kernel(
         for (int64_t pos = 0; pos < len; ++pos) {
             const int p = static_cast<int>(pos & 1);
             const uint32_t c = window_at(win, data, base + pos, data_end, lane);
-            for (int32_t w = tid; w < nwords; w += nthreads) nxt[w] = 0;
-            __syncthreads();  // (a) nxt clear, red[p] clear
             const int32_t e0 = boff[c];
             const int32_t nc = boff[c + 1] - e0;
             const int32_t total = nc + (any1 - any0);
-            int32_t j = tid;
-            for (; j + 7 * nthreads < total; j += 8 * nthreads) {
-                uint32_t v[8];
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const int32_t x = j + nthreads * q;
-                    v[q] = bedge[x < nc ? e0 + x : any0 + (x - nc)];
-                }
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const uint32_t s2 = v[q] >> 16, t = v[q] & 0xffffu;
-                    if ((cur[s2 >> 6] >> (s2 & 63)) & 1ull)
-                        atomicOr(reinterpret_cast<unsigned long long *>(&nxt[t >> 6]),
-                                 1ull << (t & 63));
-                }
-            }
-            for (; j < total; j += nthreads) {
-                const uint32_t v = bedge[j < nc ? e0 + j : any0 + (j - nc)];
-                const uint32_t s2 = v >> 16, t = v & 0xffffu;
-                if ((cur[s2 >> 6] >> (s2 & 63)) & 1ull)
-                    atomicOr(reinterpret_cast<unsigned long long *>(&nxt[t >> 6]),
-                             1ull << (t & 63));
+            // the list's first round is loaded with the masks, one round trip for both
+            uint32_t v[8];
+            gather_load_round(bedge, e0, nc, any0, total, tid, nthreads, v);
+            gather_masks_into(d, cur, nxt, c, tid, nthreads);
+            __syncthreads();  // (a) nxt holds the masks' part, red[p] clear
+            gather_apply_round(cur, nxt, total, tid, nthreads, v);
+            for (int32_t j = tid + 8 * nthreads; j < total; j += 8 * nthreads) {
+                gather_load_round(bedge, e0, nc, any0, total, j, nthreads, v);
+                gather_apply_round(cur, nxt, total, j, nthreads, v);
             }
             __syncthreads();  // (b) nxt complete
             uint64_t any = 0, acc = 0;
@@ -346,19 +399,23 @@ void launch_nfa_gather(const NfaGeneralDev &d, const uint8_t *data, const int64_
         throw Error(3, "bitpacked (general): the bitsets of " + std::to_string(d.n) +
                            " states do not fit LDS");
     const size_t sets = static_cast<size_t>(kGenWaves) * 2 * d.nwords * 8;
-    const size_t stage = static_cast<size_t>(gather_stage_words(d)) * 8;
-    const bool staged = stage <= kGatherStageBudget && stage + sets <= kGatherLdsMax;
-    const size_t bytes = sets + (staged ? stage : 0);
+    // stage the lists too when they are small, else the offsets and accept words only
+    const size_t all = static_cast<size_t>(gather_stage_words(d, 2)) * 8;
+    const size_t idx = static_cast<size_t>(gather_stage_words(d, 1)) * 8;
+    const int32_t staged = all <= kGatherStageBudget && all + sets <= kGatherLdsMax ? 2
+                           : (idx + sets <= kGatherLdsMax ? 1 : 0);
+    const size_t stage = static_cast<size_t>(gather_stage_words(d, staged)) * 8;
+    const size_t bytes = sets + stage;
     // One wave per input while the batch gives every SIMD four inputs of its own; below
     // that, a block of four waves per input (a wave per input would leave SIMDs with one
     // latency-bound wave, or none).
     const int64_t simds = 4 * static_cast<int64_t>(device_cu_count());
     if (static_cast<int64_t>(count) < 4 * simds || env_int("GFSM_GATHER_BLOCK", 0) == 1) {
-        const size_t bbytes = static_cast<size_t>(2) * d.nwords * 8 + (staged ? stage : 0);
+        const size_t bbytes = static_cast<size_t>(2) * d.nwords * 8 + stage;
         ensure_dynamic_lds(reinterpret_cast<const void *>(&nfa_gather_block_kernel), bbytes);
         const int32_t grid = count < 65536 ? count : 65536;  // blocks loop over the rest
         nfa_gather_block_kernel<<<grid, 64 * kGenWaves, bbytes, stream>>>(
-            d, data, off, count, flags, lens, staged ? 1 : 0);
+            d, data, off, count, flags, lens, staged);
         GFSM_HIP(hipGetLastError());
         return;
     }
@@ -366,7 +423,7 @@ void launch_nfa_gather(const NfaGeneralDev &d, const uint8_t *data, const int64_
     int32_t grid = (count + kGenWaves - 1) / kGenWaves;
     if (grid > 65536) grid = 65536;  // waves loop over the inputs beyond that
     nfa_gather_kernel<<<grid, 64 * kGenWaves, bytes, stream>>>(d, data, off, count, flags, lens,
-                                                               staged ? 1 : 0);
+                                                               staged);
     GFSM_HIP(hipGetLastError());
 }
 

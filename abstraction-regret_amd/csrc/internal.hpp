@@ -75,7 +75,11 @@ struct NfaGeneralDev {
     const int32_t *boff;       // [258] or null: edges labelled c at [boff[c], boff[c + 1]),
                                // ANY edges at [boff[256], boff[257])
     const uint32_t *bedge;     // [nedges] source << 16 | target
+    const uint64_t *fmask;     // [256][nwords] or null: successors of an always-active start
+    const uint64_t *dmask;     // [256][ndiag][nwords]: targets of edges t - doff[k] -> t
     int32_t n, nwords, k0, nedges;
+    int32_t ndiag;             // at most 4 (kMaxDiag)
+    int32_t doff[4];
 };
 constexpr int kRegMaxWords = 8;  // register technique: at most 512 states
 

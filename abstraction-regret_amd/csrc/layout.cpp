@@ -511,7 +511,89 @@ FoldedHost fold_epsilon(const NfaInput &a, int64_t cap) {
     return h;
 }
 
-GeneralHost build_general(const NfaInput &a) {
+namespace {
+
+// Whether the masks of g carry the folded edge s -> t (whatever its label).
+bool mask_covers(const GeneralHost &g, int32_t start, int32_t s, int32_t t) {
+    if (!g.fmask.empty() && s == start) return true;
+    for (int32_t k = 0; k < g.ndiag; ++k)
+        if (t - s == g.doff[k]) return true;
+    return false;
+}
+
+// First masks and diagonal masks of the gather layout (see GeneralHost).
+void build_gather_masks(GeneralHost &g, const FoldedHost &f, int32_t start, int32_t n,
+                        DiagMode mode) {
+    if (mode == DiagMode::off) return;
+    const int32_t nwords = (n + 63) / 64;
+    const size_t nw = static_cast<size_t>(nwords);
+    // list edges a mask would replace, summed over the 256 bytes (an ANY edge is read on
+    // every byte); a mask costs nwords words per byte
+    const double keep = mode == DiagMode::force ? 0.5 : 256.0 * nwords;
+    auto weight = [](int32_t lab) { return lab == kAnySymbol ? 256.0 : 1.0; };
+    // the start stays active when it loops on every byte
+    bool loop_any = false;
+    std::vector<uint8_t> loops(kAlphabet, 0);
+    double start_w = 0;
+    for (int32_t k = f.srp[start]; k < f.srp[start + 1]; ++k) {
+        start_w += weight(f.ss[k]);
+        if (f.st[k] != start) continue;
+        if (f.ss[k] == kAnySymbol) loop_any = true;
+        else loops[static_cast<size_t>(f.ss[k])] = 1;
+    }
+    const bool always = loop_any || std::all_of(loops.begin(), loops.end(),
+                                                [](uint8_t x) { return x != 0; });
+    if (always && start_w >= keep) {
+        g.fmask.assign(static_cast<size_t>(kAlphabet) * nw, 0ull);
+        for (int32_t k = f.srp[start]; k < f.srp[start + 1]; ++k) {
+            const int32_t t = f.st[k];
+            const uint64_t bit = 1ull << (t & 63);
+            const int c0 = f.ss[k] == kAnySymbol ? 0 : f.ss[k];
+            const int c1 = f.ss[k] == kAnySymbol ? kAlphabet : f.ss[k] + 1;
+            for (int c = c0; c < c1; ++c) g.fmask[static_cast<size_t>(c) * nw + (t >> 6)] |= bit;
+        }
+    }
+    // the most used offsets among the edges the first masks leave
+    std::vector<double> by_off(127, 0.0);  // offset d at d + 63
+    for (int32_t s = 0; s < n; ++s) {
+        if (s == start && !g.fmask.empty()) continue;
+        for (int32_t k = f.srp[s]; k < f.srp[s + 1]; ++k) {
+            const int32_t d = f.st[k] - s;
+            if (d != 0 && d >= -63 && d <= 63) by_off[static_cast<size_t>(d + 63)] += weight(f.ss[k]);
+        }
+    }
+    std::vector<int32_t> order;
+    for (int32_t d = -63; d <= 63; ++d)
+        if (d != 0 && by_off[static_cast<size_t>(d + 63)] >= keep) order.push_back(d);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+        return by_off[static_cast<size_t>(x + 63)] > by_off[static_cast<size_t>(y + 63)];
+    });
+    g.ndiag = static_cast<int32_t>(std::min<size_t>(order.size(), kMaxDiag));
+    for (int32_t k = 0; k < g.ndiag; ++k) g.doff[k] = order[static_cast<size_t>(k)];
+    if (!g.ndiag) return;
+    const size_t nd = static_cast<size_t>(g.ndiag);
+    g.dmask.assign(static_cast<size_t>(kAlphabet) * nd * nw, 0ull);
+    for (int32_t s = 0; s < n; ++s) {
+        if (s == start && !g.fmask.empty()) continue;
+        for (int32_t k = f.srp[s]; k < f.srp[s + 1]; ++k) {
+            const int32_t d = f.st[k] - s;
+            for (int32_t q = 0; q < g.ndiag; ++q) {
+                if (g.doff[q] != d) continue;
+                const int32_t t = f.st[k];  // indexed by target
+                const uint64_t bit = 1ull << (t & 63);
+                const int c0 = f.ss[k] == kAnySymbol ? 0 : f.ss[k];
+                const int c1 = f.ss[k] == kAnySymbol ? kAlphabet : f.ss[k] + 1;
+                for (int c = c0; c < c1; ++c)
+                    g.dmask[(static_cast<size_t>(c) * nd + static_cast<size_t>(q)) * nw +
+                            static_cast<size_t>(t >> 6)] |= bit;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+GeneralHost build_general(const NfaInput &a, DiagMode mode) {
     GeneralHost g;
     FoldedHost f = fold_epsilon(a);
     if (!f.ok) {
@@ -528,17 +610,21 @@ GeneralHost build_general(const NfaInput &a) {
     g.start_bits.assign(static_cast<size_t>(g.rows.nwords), 0ull);
     for (int32_t s : f.start_set) g.start_bits[s >> 6] |= 1ull << (s & 63);
     if (n <= 65536) {
-        // counting sort of the folded edges by label (ANY last)
-        const int32_t m = static_cast<int32_t>(f.st.size());
+        build_gather_masks(g, f, a.start, n, mode);
+        // counting sort of the folded edges left to the lists by label (ANY last)
+        auto listed = [&](int32_t s, int32_t k) { return !mask_covers(g, a.start, s, f.st[k]); };
         g.boff.assign(kAnySymbol + 2, 0);
-        for (int32_t k = 0; k < m; ++k) g.boff[f.ss[k] + 1]++;
-        for (int c = 0; c <= kAnySymbol; ++c) g.boff[c + 1] += g.boff[c];
-        std::vector<int32_t> cur(g.boff.begin(), g.boff.end() - 1);
-        g.bedge.resize(static_cast<size_t>(m));
         for (int32_t s = 0; s < n; ++s)
             for (int32_t k = f.srp[s]; k < f.srp[s + 1]; ++k)
-                g.bedge[cur[f.ss[k]]++] = (static_cast<uint32_t>(s) << 16) |
-                                          static_cast<uint32_t>(f.st[k]);
+                if (listed(s, k)) g.boff[f.ss[k] + 1]++;
+        for (int c = 0; c <= kAnySymbol; ++c) g.boff[c + 1] += g.boff[c];
+        std::vector<int32_t> cur(g.boff.begin(), g.boff.end() - 1);
+        g.bedge.resize(static_cast<size_t>(g.boff[kAnySymbol + 1]));
+        for (int32_t s = 0; s < n; ++s)
+            for (int32_t k = f.srp[s]; k < f.srp[s + 1]; ++k)
+                if (listed(s, k))
+                    g.bedge[cur[f.ss[k]]++] = (static_cast<uint32_t>(s) << 16) |
+                                              static_cast<uint32_t>(f.st[k]);
         g.gather_ok = true;
     }
     g.ok = true;

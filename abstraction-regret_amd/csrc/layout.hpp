@@ -107,6 +107,16 @@ FoldedHost fold_epsilon(const NfaInput &a, int64_t cap = 0);
 //    (source << 16 | target) at [boff[c], boff[c + 1]), the ANY edges at
 //    [boff[256], boff[257]);
 //  * start_bits: closure(start) as a bitset.
+//  * masks that take edges out of the gather lists (n <= 65536, see DiagMode):
+//    - first masks fmask[c][nwords], when the start state stays active on every byte (a
+//      folded self-loop on each byte or ANY): the folded successors of the start on c or
+//      ANY, OR'ed into every step; the start's edges leave the lists;
+//    - diagonal masks dmask[c][k][nwords] for up to kMaxDiag offsets doff[k] (1 <= |d| <= 63):
+//      bit t set when t - d -c-> t or t - d -ANY-> t (indexed by target); a step adds
+//      (set shifted by d across words) & mask, and those edges leave the lists.
+//    The lists keep every other edge, so lists and masks together are the folded relation.
+constexpr int kMaxDiag = 4;  // also the size of NfaGeneralDev::doff
+enum class DiagMode { off, automatic, force };
 struct GeneralHost {
     bool ok = false;                    // folded (and the rows built)
     std::string why;
@@ -115,8 +125,15 @@ struct GeneralHost {
     std::vector<int32_t> boff;          // [258]
     std::vector<uint32_t> bedge;
     std::vector<uint64_t> start_bits;   // [nwords]
+    std::vector<uint64_t> fmask;        // [256][nwords] or empty
+    std::vector<uint64_t> dmask;        // [256][ndiag][nwords] or empty
+    int32_t ndiag = 0;
+    int32_t doff[kMaxDiag] = {0, 0, 0, 0};
 };
-GeneralHost build_general(const NfaInput &a);
+// mode automatic: a mask is kept when it replaces at least one list edge per 64 states per
+// byte on average (one 8-byte mask word read instead of >= 1 four-byte edge per word);
+// force: every start loop and the kMaxDiag most used offsets (tests).
+GeneralHost build_general(const NfaInput &a, DiagMode mode = DiagMode::automatic);
 
 // DFA device layout. States are renumbered so that the accepting ones come last; the
 // table is stored by byte (column c holds the successor of every state on byte c).

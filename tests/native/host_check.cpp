@@ -5,7 +5,8 @@
 // checks, on random automata:
 //   * the epsilon fold: the folded automaton, walked through the layouts the kernels read
 //     (label-sorted rows with the per-label index; byte-major gather lists with the start
-//     bitset), gives the verdict of an independent set simulation of reference.py:36-65
+//     bitset, first masks and diagonal masks, which together must rebuild the folded
+//     relation byte by byte), gives the verdict of an independent set simulation of reference.py:36-65
 //     (closure after every step) and of the C oracle, byte for byte;
 //   * the fold cap (a closure too large to fold is refused, not truncated);
 //   * the DFA table (renumbered, byte-major, pre-scaled entries) walked as the kernels walk
@@ -103,6 +104,51 @@ Nfa random_nfa(Rng &r, int32_t n, int alphabet, int max_sym, int max_eps, double
     return a;
 }
 
+// A pattern union with optional positions: a start state that loops on every byte (ANY, or
+// one edge per byte), chains of class-labelled states with epsilon edges skipping optional
+// positions, the last state of each chain accepting. reversed: each chain's states are
+// numbered downwards, so successor edges run to s - 1 (negative diagonals).
+Nfa union_nfa(Rng &r, int patterns, int min_len, int max_len, int width, int alphabet,
+              double p_opt, bool any_loop, bool reversed) {
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> sym(1);  // (label, target) per state
+    std::vector<std::vector<int32_t>> eps(1);
+    std::vector<uint8_t> acc(1, 0);
+    if (any_loop) sym[0].push_back({kAnySymbol, 0});
+    else
+        for (int c = 0; c < kAlphabet; ++c) sym[0].push_back({c, 0});
+    for (int p = 0; p < patterns; ++p) {
+        const int len = min_len + r.below(max_len - min_len + 1);
+        const int32_t base = static_cast<int32_t>(sym.size());
+        auto id = [&](int j) { return reversed ? base + len - 1 - j : base + j; };
+        sym.resize(sym.size() + static_cast<size_t>(len));
+        eps.resize(sym.size());
+        acc.resize(sym.size(), 0);
+        for (int j = 0; j < len; ++j) {
+            const int w = 1 + r.below(width);
+            const int32_t from = j == 0 ? 0 : id(j - 1);
+            for (int q = 0; q < w; ++q) sym[static_cast<size_t>(from)].push_back({r.below(alphabet), id(j)});
+            if (j + 1 < len && r.unit() < p_opt) eps[static_cast<size_t>(id(j))].push_back(id(j + 1));
+        }
+        acc[static_cast<size_t>(id(len - 1))] = 1;
+    }
+    Nfa a;
+    a.n = static_cast<int32_t>(sym.size());
+    a.start = 0;
+    a.acc = acc;
+    a.srp.push_back(0);
+    a.erp.push_back(0);
+    for (int32_t s = 0; s < a.n; ++s) {
+        for (auto [lab, t] : sym[static_cast<size_t>(s)]) {
+            a.ss.push_back(lab);
+            a.st.push_back(t);
+        }
+        a.srp.push_back(static_cast<int32_t>(a.st.size()));
+        for (int32_t t : eps[static_cast<size_t>(s)]) a.et.push_back(t);
+        a.erp.push_back(static_cast<int32_t>(a.et.size()));
+    }
+    return a;
+}
+
 // reference.py:36-65, restated independently of the oracle: sets, closure after each step.
 std::set<int32_t> closure(const Nfa &a, std::set<int32_t> s) {
     std::vector<int32_t> stack(s.begin(), s.end());
@@ -172,15 +218,39 @@ std::pair<int, int> sim_rows(const WorklistHost &r, const uint8_t *acc, const ui
     return {0, 0};
 }
 
-// The byte-major gather lists and start bitset as the bit-parallel kernel reads them.
+// The masks' part of the next set on byte c (gather_masks_into in general.hip).
+void mask_step(const GeneralHost &g, const std::vector<uint64_t> &cur, std::vector<uint64_t> &nxt,
+               int c) {
+    const size_t nw = cur.size();
+    const size_t nd = static_cast<size_t>(g.ndiag);
+    for (size_t w = 0; w < nw; ++w) {
+        uint64_t x = g.fmask.empty() ? 0ull : g.fmask[static_cast<size_t>(c) * nw + w];
+        for (size_t k = 0; k < nd; ++k) {
+            const uint64_t *m = g.dmask.data() + (static_cast<size_t>(c) * nd + k) * nw;
+            const int d = g.doff[k];
+            uint64_t sh;  // word w of the set shifted by d
+            if (d > 0) {
+                sh = cur[w] << d;
+                if (w > 0) sh |= cur[w - 1] >> (64 - d);
+            } else {
+                sh = cur[w] >> -d;
+                if (w + 1 < nw) sh |= cur[w + 1] << (64 + d);
+            }
+            x |= sh & m[w];
+        }
+        nxt[w] = x;
+    }
+}
+
+// The byte-major gather lists, masks and start bitset as the bit-parallel kernel reads them.
 std::pair<int, int> sim_gather(const GeneralHost &g, const uint8_t *p, int len) {
     const WorklistHost &r = g.rows;
     std::vector<uint64_t> cur(g.start_bits), nxt(cur.size());
     for (size_t w = 0; w < cur.size(); ++w)
         if (cur[w] & r.accw[w]) return {1, 0};
     for (int i = 0; i < len; ++i) {
-        std::fill(nxt.begin(), nxt.end(), 0ull);
         const int c = p[i];
+        mask_step(g, cur, nxt, c);
         for (int pass = 0; pass < 2; ++pass) {
             const int lab = pass ? kAnySymbol : c;
             for (int32_t e = g.boff[lab]; e < g.boff[lab + 1]; ++e) {
@@ -200,9 +270,62 @@ std::pair<int, int> sim_gather(const GeneralHost &g, const uint8_t *p, int len) 
     return {0, 0};
 }
 
-void check_nfa(const Nfa &a, Rng &r, int inputs, int max_len, int alphabet) {
+// Lists and masks together hold exactly the folded relation on each byte: the pairs
+// (s, t) with s -c-> t or s -ANY-> t in the folded rows (so masks hold no pair the rows
+// lack), and an always-active start (first masks) does loop on every byte.
+void check_gather_partition(const GeneralHost &g, int32_t start, int32_t n) {
+    const size_t nw = static_cast<size_t>(g.rows.nwords);
+    const size_t nd = static_cast<size_t>(g.ndiag);
+    if (g.dmask.size() != nd * nw * kAlphabet) fail("diagonal mask size", n, g.ndiag);
+    if (!g.fmask.empty() && g.fmask.size() != nw * kAlphabet) fail("first mask size", n, 0);
+    for (size_t k = 0; k < nd; ++k) {
+        if (g.doff[k] == 0 || g.doff[k] < -63 || g.doff[k] > 63) fail("diagonal offset", n, g.doff[k]);
+        for (size_t q = 0; q < k; ++q)
+            if (g.doff[q] == g.doff[k]) fail("repeated diagonal", n, g.doff[k]);
+    }
+    for (int c = 0; c < kAlphabet; ++c) {
+        std::set<std::pair<int32_t, int32_t>> want, got;
+        for (int32_t s = 0; s < n; ++s)
+            for (int32_t e = g.rows.row[s]; e < g.rows.row[s + 1]; ++e) {
+                const uint32_t lab = g.rows.edge[e] & 511u;
+                if (lab == static_cast<uint32_t>(c) || lab == kAnySymbol)
+                    want.insert({s, static_cast<int32_t>(g.rows.edge[e] >> 9)});
+            }
+        for (int pass = 0; pass < 2; ++pass) {
+            const int lab = pass ? kAnySymbol : c;
+            for (int32_t e = g.boff[lab]; e < g.boff[lab + 1]; ++e) {
+                const auto pr = std::make_pair(static_cast<int32_t>(g.bedge[e] >> 16),
+                                               static_cast<int32_t>(g.bedge[e] & 0xffffu));
+                got.insert(pr);  // a pair may sit in both the c and the ANY list
+            }
+        }
+        for (size_t w = 0; w < nw; ++w) {
+            const uint64_t fw = g.fmask.empty() ? 0ull : g.fmask[static_cast<size_t>(c) * nw + w];
+            for (int b = 0; b < 64; ++b)
+                if ((fw >> b) & 1ull) {
+                    const int32_t t = static_cast<int32_t>(w * 64) + b;
+                    got.insert({start, t});
+                }
+            for (size_t k = 0; k < nd; ++k) {
+                const uint64_t mw = g.dmask[(static_cast<size_t>(c) * nd + k) * nw + w];
+                for (int b = 0; b < 64; ++b)
+                    if ((mw >> b) & 1ull) {
+                        const int32_t t = static_cast<int32_t>(w * 64) + b;
+                        got.insert({t - g.doff[k], t});
+                    }
+            }
+        }
+        if (got != want) fail("lists and masks differ from the folded relation", n, c);
+        if (!g.fmask.empty() && !want.count({start, start})) fail("first masks without a loop", n, c);
+    }
+}
+
+void check_nfa(const Nfa &a, Rng &r, int inputs, int max_len, int alphabet,
+               DiagMode mode = DiagMode::automatic) {
     validate_nfa(a.in());
-    GeneralHost g = build_general(a.in());
+    GeneralHost g = build_general(a.in(), mode);
+    if (g.ok && g.gather_ok && a.n <= 5000) check_gather_partition(g, a.start, a.n);
+    if (mode == DiagMode::off && (g.ndiag || !g.fmask.empty())) fail("masks built when off", a.n, 0);
     if (!g.ok) fail("fold refused a small automaton", a.n, 0);
     if (static_cast<int32_t>(g.rows.row.size()) != a.n + 1) fail("folded rows size", a.n, 0);
     if (g.gather_ok != (a.n <= 65536)) fail("gather lists presence", a.n, 0);
@@ -589,6 +712,28 @@ int main() {
     {
         Nfa a = random_nfa(r, 70000, 8, 3, 1, 0.01, 0.001);  // above 65536: no gather lists
         check_nfa(a, r, 8, 30, 8);
+    }
+    // first and diagonal masks: pattern unions with optional positions (chains forward and
+    // reversed, start looping on ANY or on each byte), masks chosen by the layout, forced
+    // and switched off; random automata with forced masks (any offsets, both signs)
+    for (int trial = 0; trial < 24; ++trial) {
+        const bool any_loop = trial % 2 == 0, reversed = (trial / 2) % 2 == 1;
+        const int patterns = trial < 12 ? 3 + r.below(20) : 150 + r.below(200);
+        Nfa a = union_nfa(r, patterns, 2, 9, 4, 8, 0.3, any_loop, reversed);
+        const DiagMode mode = trial % 3 == 0 ? DiagMode::automatic
+                                             : (trial % 3 == 1 ? DiagMode::force : DiagMode::off);
+        check_nfa(a, r, 20, 80, 8, mode);
+    }
+    {
+        // about 200 edges per byte on offset 1 and 48 from the start, against 41 words
+        Nfa a = union_nfa(r, 400, 4, 9, 60, 256, 0.25, true, false);
+        GeneralHost g = build_general(a.in(), DiagMode::automatic);
+        if (g.fmask.empty() || g.ndiag == 0 || g.doff[0] != 1) fail("union: masks not chosen", a.n, g.ndiag);
+    }
+    for (int trial = 0; trial < 60; ++trial) {
+        const int32_t n = 1 + r.below(trial < 40 ? 130 : 700);
+        Nfa a = random_nfa(r, n, 6, 3, 2, 0.05, 0.1);
+        check_nfa(a, r, 16, 40, 6, DiagMode::force);
     }
     check_fold_cap();
     for (int32_t n : {1, 2, 255, 256, 300, 4096, 33000}) check_dfa(r, n, n < 300 ? 0.1 : 0.01);

@@ -5,11 +5,15 @@ Every verdict is compared with the C oracle (itself pinned against the upstream 
 corpus in the CPU suite). The cases cover the epsilon fold (cycles, chains, closures of the
 start state), wildcards, one to 313 words of state set, LDS-staged and global edge lists,
 global worklist scratch, ragged and unaligned batches, dense active sets, and automata
-whose closure is too large to fold (they keep the unfolded worklist).
+whose closure is too large to fold (they keep the unfolded worklist), and the bit-parallel
+step's first and diagonal masks (chosen by the layout, forced, switched off; forward and
+reversed chains).
 """
 
 from __future__ import annotations
 
+import dataclasses
+import os
 import random
 
 import numpy as np
@@ -196,3 +200,91 @@ def test_auto_resolution_for_general_automata():
         assert dev.resolve(1, 1, _lib.NFA_BITPACKED) == _lib.NFA_BITPACKED
     with DeviceNFA(W.pattern_union(50, seed=1).nfa) as dev:
         assert dev.resolve(4096, 1 << 20) == _lib.NFA_CHAIN
+
+
+def _renumber_reversed(nfa):
+    """The same automaton with every state but the start renumbered s -> n - s (start 0
+    stays 0): successor edges s -> s + 1 of a pattern chain become s -> s - 1."""
+    n = nfa.num_states
+    assert nfa.start_state == 0
+    new = np.arange(n)
+    new[1:] = n - np.arange(1, n)
+    inv = np.argsort(new)
+    b = NFABuilder()
+    for s in range(n):
+        b.add_state(accept=bool(nfa.accept[inv[s]]))
+    b.set_start(0)
+    rp, sy, tg = nfa.sym_row_ptr, nfa.sym_symbols, nfa.sym_targets
+    erp, et = nfa.eps_row_ptr, nfa.eps_targets
+    for old in range(n):
+        for k in range(rp[old], rp[old + 1]):
+            b.add_transition(int(new[old]), int(sy[k]), int(new[tg[k]]))
+        for k in range(erp[old], erp[old + 1]):
+            b.add_epsilon(int(new[old]), int(new[et[k]]))
+    return b.build()
+
+
+@pytest.fixture
+def diag_mode():
+    """Sets GFSM_GATHER_DIAG for the handles created inside the test (read when a handle's
+    layout is built), restoring the previous value afterwards."""
+    old = os.environ.get("GFSM_GATHER_DIAG")
+
+    def set_mode(v):
+        os.environ["GFSM_GATHER_DIAG"] = v
+
+    yield set_mode
+    if old is None:
+        os.environ.pop("GFSM_GATHER_DIAG", None)
+    else:
+        os.environ["GFSM_GATHER_DIAG"] = old
+
+
+@pytest.mark.parametrize("reverse", [False, True])
+@pytest.mark.parametrize("count", [48, 4200])
+def test_gather_masks_match_oracle_every_mode(diag_mode, reverse, count):
+    """Pattern unions with optional positions through the bit-parallel step with the masks
+    the layout chooses (first masks for the looping start, diagonals +1/+2 or -1/-2), forced
+    masks and no masks: equal verdicts, equal to the oracle. 48 inputs take the block step,
+    4200 the wave-per-input step; reversed chains give negative diagonals."""
+    pu = W.pattern_union(300, (4, 12), (8, 64), seed=51)
+    nfa = W.optional_positions(pu, 0.3, seed=52)
+    # 9 accepting pattern ends of 300: first matches hundreds of bytes in, sets stay dense
+    acc = np.asarray(nfa.accept).copy()
+    ends = np.flatnonzero(acc)
+    acc[ends[np.random.default_rng(55).random(ends.size) >= 0.03]] = False
+    nfa = dataclasses.replace(nfa, accept=acc)
+    if reverse:
+        nfa = _renumber_reversed(nfa)
+    lens = W.lognormal_lengths(count, 200 if count > 100 else 1500, 1.0, 0, 4000, seed=53)
+    data, off = W.pattern_inputs(pu, lens, 0.9, seed=54)
+    wf, wn, _ = native.nfa_batch(nfa, data, off)
+    assert wf.any() and not wf.all()
+    sizes = {}
+    for mode in ("0", "1", "2"):
+        diag_mode(mode)
+        with DeviceNFA(nfa) as dev:
+            sizes[mode] = dev.props.device_bytes
+            f, n, _ = dev.run_host(data, off, _lib.NFA_BITPACKED)
+        np.testing.assert_array_equal(f, wf, err_msg=f"mode {mode}")
+        np.testing.assert_array_equal(n, wn, err_msg=f"mode {mode}")
+    # masks were built (here smaller than the list edges they replace)
+    assert sizes["1"] != sizes["0"] and sizes["2"] != sizes["0"]
+
+
+def test_gather_forced_masks_on_random_automata(diag_mode):
+    """Forced masks on random automata with epsilon edges and wildcards (offsets of either
+    sign, starts without a loop), one to 313 words; the dense non-chain control prefix with
+    the layout's masks."""
+    diag_mode("2")
+    for n in (3, 64, 65, 200, 1000, 20000):
+        rng = random.Random(n + 100)
+        nfa = random_nfa(rng, n, b"abcdefgh", max_sym=4, max_eps=2, p_accept=0.01,
+                         p_any=0.05)
+        data, off = _ragged(np.random.default_rng(n), 64, 400, b"abcdefghij")
+        _check(nfa, data, off, "bitpacked")
+    diag_mode("1")
+    pu = W.pattern_union(8000, (5, 12), (16, 64), seed=0, total_states=65536, accepting=False)
+    nfa = W.optional_positions(pu, 0.25, seed=1)
+    data, off = W.pattern_inputs(pu, np.full(8, 2048, dtype=np.int64), 0.95, seed=3)
+    _check(nfa, data, off, "bitpacked")
