@@ -193,11 +193,15 @@ int32_t resolve_nfa(const gfsm_nfa *h, int32_t technique, int32_t count) {
         case GFSM_NFA_AUTO:
             // chain-structured automata: the sparse chain walk (its own tiers fall back to
             // the tiled pass on inputs whose active sets outgrow the lists); small general
-            // automata over batches that fill the lanes: one lane per input; otherwise the
-            // wave-cooperative worklist
+            // automata over batches that fill the lanes: one lane per input; general
+            // automata whose start loops on every byte with first masks (at least one
+            // successor per 64 states per byte, so every set holds that many states and
+            // never dies): the masked bit-parallel step, whose cost per byte does not grow
+            // with the set; otherwise the wave-cooperative worklist
             if (h->chain_ok) return GFSM_NFA_CHAIN;
             if (h->general_ok && h->gen.nwords <= kRegMaxWords && count >= 1024)
                 return GFSM_NFA_REGISTER;
+            if (gather_ok(h) && h->gen.fmask) return GFSM_NFA_BITPACKED;
             return GFSM_NFA_WORKLIST;
         case GFSM_NFA_WORKLIST:
             return technique;

@@ -187,8 +187,9 @@ def test_start_closure_accepts_gives_true_zero():
 
 
 def test_auto_resolution_for_general_automata():
-    """auto: register for small general automata over large batches, worklist otherwise,
-    chain for chain-structured ones."""
+    """auto: register for small general automata over large batches, the masked
+    bit-parallel step for general automata whose start loops on every byte with many
+    successors, worklist otherwise, chain for chain-structured ones."""
     rng = random.Random(3)
     small = random_nfa(rng, 100, b"ab", max_sym=2, max_eps=1)
     big = random_nfa(rng, 2000, b"ab", max_sym=2, max_eps=1)
@@ -200,6 +201,12 @@ def test_auto_resolution_for_general_automata():
         assert dev.resolve(1, 1, _lib.NFA_BITPACKED) == _lib.NFA_BITPACKED
     with DeviceNFA(W.pattern_union(50, seed=1).nfa) as dev:
         assert dev.resolve(4096, 1 << 20) == _lib.NFA_CHAIN
+    # an unanchored union with optional positions: the start loops on every byte with
+    # first masks, so the masked bit-parallel step
+    pu = W.pattern_union(300, (4, 12), (8, 64), seed=51)
+    with DeviceNFA(W.optional_positions(pu, 0.3, seed=52)) as dev:
+        assert dev.resolve(16, 1 << 20) == _lib.NFA_BITPACKED
+        assert dev.resolve(4096, 1 << 20) == _lib.NFA_BITPACKED
 
 
 def _renumber_reversed(nfa):
