@@ -1199,7 +1199,6 @@ static void split_launch_passes(const DfaDev &d, SplitArgs &a, hipStream_t strea
         allow_big_lds(&dfa_split_walk_kernel<T, kLds, kChains>, lds);
         allow_big_lds(&dfa_split_fix_kernel<T, kLds, kChains, 64>, lds);
         allow_big_lds(&dfa_split_fix_kernel<T, kLds, kChains, 32>, lds);
-        allow_big_lds(&dfa_split_finish_kernel<T, kLds>, lds);
     }
     // spread the segments over every CU, kChains per lane
     struct Shape {
@@ -1258,7 +1257,11 @@ static void split_launch_passes(const DfaDev &d, SplitArgs &a, hipStream_t strea
                 GFSM_HIP(hipGetLastError());
             }
         }
-        dfa_split_finish_kernel<T, kLds><<<(a.count + 255) / 256, 256, lds, stream>>>(tab, a);
+        // The finish pass reads the table from L2: it walks only the rare heads that have
+        // neither a consistent fix nor a map, and a launch that reserved the table's LDS
+        // would wait for a CU with that much free (behind other batches' walk and fix
+        // blocks, in a stream of batches)
+        dfa_split_finish_kernel<T, false><<<(a.count + 255) / 256, 256, 0, stream>>>(tab, a);
         GFSM_HIP(hipGetLastError());
     }
 }
