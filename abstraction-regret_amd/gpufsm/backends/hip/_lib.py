@@ -301,6 +301,40 @@ def memcpy_async(dst: int, src: int, nbytes: int, kind: int, stream: int) -> Non
         raise HIPError(rc, f"hipMemcpyAsync of {nbytes} bytes failed with HIP error {rc}")
 
 
+_graph_fns = None
+
+
+def _graph_api():
+    global _graph_fns
+    if _graph_fns is None:
+        lib = load()  # the HIP runtime this library is bound to (see memcpy_async)
+        launch = lib.hipGraphLaunch
+        launch.restype = ctypes.c_int
+        launch.argtypes = [_vp, _vp]
+        nodes = lib.hipGraphGetNodes
+        nodes.restype = ctypes.c_int
+        nodes.argtypes = [_vp, _vp, ctypes.POINTER(ctypes.c_size_t)]
+        _graph_fns = (launch, nodes)
+    return _graph_fns
+
+
+def graph_launch(graph_exec: int, stream: int) -> None:
+    """``hipGraphLaunch`` of an instantiated graph on a raw stream handle (no stream context
+    switch, which costs more host time than the launch)."""
+    rc = _graph_api()[0](graph_exec, stream)
+    if rc != 0:
+        raise HIPError(rc, f"hipGraphLaunch failed with HIP error {rc}")
+
+
+def graph_node_count(graph: int) -> int:
+    """Number of nodes of a captured (kept) graph: ``hipGraphGetNodes`` with no node array."""
+    n = ctypes.c_size_t(0)
+    rc = _graph_api()[1](graph, None, ctypes.byref(n))
+    if rc != 0:
+        raise HIPError(rc, f"hipGraphGetNodes failed with HIP error {rc}")
+    return int(n.value)
+
+
 def device_arch(ordinal: int = 0) -> str:
     buf = ctypes.create_string_buffer(64)
     check(load().gfsm_device_arch(ordinal, buf, 64))

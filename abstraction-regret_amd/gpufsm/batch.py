@@ -287,18 +287,23 @@ class BatchEngine:
     """Resident automata + resident batches + autotuned, optionally graph-captured launches."""
 
     def __init__(self, *, budget_bytes: int | None = None, tuner: Autotuner | None | bool = True,
-                 arch: str | None = None) -> None:
+                 arch: str | None = None, graph_min_kernels: int = 3) -> None:
         self._torch = _torch()
         self.cache = ResidentCache(budget_bytes, on_evict=self._drop_graphs_of)
         if tuner is True:
             tuner = Autotuner()
         self.tuner = tuner or None
         self.arch = arch or _lib.device_arch(0)
-        # (id(handle), batch serial, choice, count, total, longest input) -> graph; the cache holds every
-        # handle it names, and a batch's graphs are dropped when the batch is collected
-        self._graphs: dict[tuple, object] = {}
+        # (id(handle), batch serial, choice, count, total, longest input) -> (graph, kernels);
+        # the cache holds every handle it names, and a batch's graphs are dropped when the
+        # batch is collected
+        self._graphs: dict[tuple, tuple] = {}
         self._graphs_lock = threading.Lock()
         self.captures = 0
+        # hipGraphLaunch costs about 17 us of host time on ROCm, three times a kernel launch:
+        # a launch sequence of fewer kernels than this is launched directly, not replayed
+        self.graph_min_kernels = int(graph_min_kernels)
+        self.replays = 0
 
     def _drop_graphs_of(self, handle) -> None:
         """Forget the graphs captured on ``handle``: its device memory is about to be freed."""
@@ -402,7 +407,10 @@ class BatchEngine:
         """Enqueue one run of ``automaton`` over ``batch`` on ``stream``; returns the choice.
 
         With ``graph=True`` the launch sequence is captured into a HIP graph on first use
-        (per automaton, batch, choice and shape) and replayed afterwards.
+        (per automaton, batch, choice and shape) and replayed afterwards with
+        ``hipGraphLaunch``; a sequence of fewer than ``graph_min_kernels`` kernels (one-kernel
+        techniques) is launched directly instead, since a graph launch costs more host time
+        than it saves there.
         """
         torch = self._torch
         stream = stream or torch.cuda.current_stream()
@@ -416,26 +424,34 @@ class BatchEngine:
             return choice
         key = (id(dev), batch.serial, choice, batch.count, batch.total, batch.max_len)
         with self._graphs_lock:
-            g = self._graphs.get(key)
-        if g is None:
+            entry = self._graphs.get(key)
+        if entry is None:
             # one eager run first: it sizes the workspace and sets kernel attributes, so the
             # capture records kernel launches only
             self._launch(dev, kind, batch, choice, stream)
             stream.synchronize()
-            g = torch.cuda.CUDAGraph()
+            g = torch.cuda.CUDAGraph(keep_graph=True)
             side = torch.cuda.Stream()
             side.wait_stream(stream)
             with torch.cuda.graph(g, stream=side):
                 self._launch(dev, kind, batch, choice, torch.cuda.current_stream())
+            kernels = _lib.graph_node_count(g.raw_cuda_graph())  # kernel nodes only
+            if kernels >= self.graph_min_kernels:
+                g.instantiate()
+            entry = (g, kernels)
             with self._graphs_lock:
                 first = not any(k[1] == batch.serial for k in self._graphs)
-                self._graphs[key] = g
+                self._graphs[key] = entry
             if first:
                 weakref.finalize(batch, _drop_batch_graphs, weakref.ref(self), batch.serial)
             self.captures += 1
             stream.wait_stream(side)
-        with torch.cuda.stream(stream):
-            g.replay()
+        g, kernels = entry
+        if kernels < self.graph_min_kernels:
+            self._launch(dev, kind, batch, choice, stream)
+        else:
+            _lib.graph_launch(g.raw_cuda_graph_exec(), stream.cuda_stream)
+            self.replays += 1
         return choice
 
     def run_packed(self, automaton, data: np.ndarray, offsets: np.ndarray,

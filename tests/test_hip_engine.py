@@ -103,6 +103,39 @@ def test_graph_replays_match_eager(engine, technique):
         obatch.fetch(s)
 
 
+def test_graph_replays_and_direct_launches_by_kernel_count():
+    """graph=True replays a captured multi-kernel sequence (split: six kernels) with
+    hipGraphLaunch and launches a one-kernel technique (gather) directly; with
+    graph_min_kernels=1 the one-kernel graph is replayed too. Every run equals the oracle,
+    with eager work on another batch between runs."""
+    import torch
+
+    a = W.dense_dfa(256, 0.0005, seed=143)
+    d, o = W.uniform_batch(32, 1 << 14, seed=144)
+    wf, wn = _oracle(a, d, o)
+    s = torch.cuda.current_stream()
+    for min_kernels in (3, 1):
+        eng = BatchEngine(tuner=None, graph_min_kernels=min_kernels)
+        try:
+            batch = DeviceBatch(d, o, stream=s)
+            other = DeviceBatch(*W.uniform_batch(8, 2048, seed=145), stream=s)
+            for technique, replayed in (("split:segment=1024", True),
+                                        ("gather", min_kernels == 1)):
+                before = eng.replays
+                for rep in range(3):
+                    batch.flags.fill_(-1)
+                    batch.lens.fill_(-1)
+                    eng.enqueue(a, batch, technique, stream=s, graph=True)
+                    f, n = batch.fetch(s)
+                    np.testing.assert_array_equal(f, wf, err_msg=f"{technique} run {rep}")
+                    np.testing.assert_array_equal(n, wn, err_msg=f"{technique} run {rep}")
+                    eng.enqueue(a, other, "lds", stream=s)
+                    other.fetch(s)
+                assert (eng.replays - before == 3) == replayed, (technique, min_kernels)
+        finally:
+            eng.close()
+
+
 @pytest.mark.parametrize("segment", [256, 512, 1024, 8192])
 def test_split_segment_sizes(engine, segment):
     """Split at fixed segment lengths: up to 64 checkpoint intervals per segment the fix
