@@ -16,7 +16,9 @@
 //               than max_inflight_per_automaton launches running, plus the admitted requests
 //               of the same automaton after it, in order, while they fit the slot (bytes and
 //               count; admitted requests wait in one queue per automaton), packs
-//               them (uint8 bytes, int64 offsets) into the slot's pinned staging buffers and
+//               them (uint8 bytes, int64 offsets) into the slot's pinned staging buffers
+//               (bytes a caller submitted in place from pinned memory are copied to the
+//               device from there) and
 //               enqueues on the slot's own stream: the host-to-device copy, the resident run
 //               (gfsm_*_run_device_ws, technique resolved per launch for its shape) and the
 //               copy back of the verdicts, then records the slot's event.
@@ -72,7 +74,23 @@ struct Req {
     const uint8_t *bytes = nullptr;  // this request's bytes
     int64_t len = 0;
     double arrival = 0.0;
+    bool pinned = false;  // the bytes lie in pinned host memory (a caller buffer): the
+                          // launch copies them to the device directly, without staging
 };
+
+// Whether [p, p + n) lies in pinned (page-locked) host memory, judged by its two ends. A
+// pageable pointer makes hipPointerGetAttributes fail; that error is cleared so that no
+// later hipGetLastError reports it.
+bool pinned_range(const uint8_t *p, int64_t n) {
+    if (n <= 0) return false;
+    for (const uint8_t *q : {p, p + (n - 1)}) {
+        hipPointerAttribute_t attr{};
+        const hipError_t rc = hipPointerGetAttributes(&attr, q);
+        (void)hipGetLastError();
+        if (rc != hipSuccess || attr.type != hipMemoryTypeHost) return false;
+    }
+    return true;
+}
 
 // Copies n bytes with up to 8 threads, one per 2 MiB: one core's memcpy (about 5 GB/s into
 // pinned memory) falls far short of the host's memory bandwidth, and bulk submits and large
@@ -248,19 +266,27 @@ void gfsm_server::launch(Slot &s) {
     const Automaton &a = automata[static_cast<size_t>(s.automaton)];
     const int32_t count = static_cast<int32_t>(s.reqs.size());
     int64_t total = 0, longest = 0;
-    // requests that sit back to back in one block (a bulk submit) are copied as one run
+    // Requests that sit back to back in one block (a bulk submit) are copied as one run:
+    // into the pinned staging buffer, or, for a caller's pinned buffer, straight to the
+    // device (`direct`: launch offset, source, length).
+    std::vector<std::pair<int64_t, std::pair<const uint8_t *, int64_t>>> direct;
     const uint8_t *run_src = nullptr;
     int64_t run_at = 0, run_len = 0;
+    bool run_pinned = false;
     auto flush = [&] {
-        if (run_len) copy_parallel(s.h_buf + run_at, run_src, static_cast<size_t>(run_len));
+        if (run_len) {
+            if (run_pinned) direct.push_back({run_at, {run_src, run_len}});
+            else copy_parallel(s.h_buf + run_at, run_src, static_cast<size_t>(run_len));
+        }
         run_len = 0;
     };
     for (size_t i = 0; i < s.reqs.size(); ++i) {
         const Req &q = s.reqs[i];
-        if (!(run_len && q.bytes == run_src + run_len)) {
+        if (!(run_len && q.bytes == run_src + run_len && q.pinned == run_pinned)) {
             flush();
             run_src = q.bytes;
             run_at = total;
+            run_pinned = q.pinned;
         }
         run_len += q.len;
         total += q.len;
@@ -286,8 +312,30 @@ void gfsm_server::launch(Slot &s) {
         GFSM_HIP(hipMallocAsync(&s.d_ws, static_cast<size_t>(need), s.stream));
         s.ws_bytes = need;
     }
-    GFSM_HIP(hipMemcpyAsync(s.d_buf, s.h_buf, static_cast<size_t>(off_at + (count + 1) * 8),
-                            hipMemcpyHostToDevice, s.stream));
+    if (direct.empty()) {  // one copy in: the staged bytes and the offsets after them
+        GFSM_HIP(hipMemcpyAsync(s.d_buf, s.h_buf, static_cast<size_t>(off_at + (count + 1) * 8),
+                                hipMemcpyHostToDevice, s.stream));
+    } else {
+        // the pinned runs from their own memory, the staged runs (the gaps between them)
+        // from the staging buffer, then the offsets
+        int64_t at = 0;
+        auto staged_upto = [&](int64_t end) {
+            if (end > at)
+                GFSM_HIP(hipMemcpyAsync(s.d_buf + at, s.h_buf + at, static_cast<size_t>(end - at),
+                                        hipMemcpyHostToDevice, s.stream));
+        };
+        for (const auto &d : direct) {
+            staged_upto(d.first);
+            GFSM_HIP(hipMemcpyAsync(s.d_buf + d.first, d.second.first,
+                                    static_cast<size_t>(d.second.second),
+                                    hipMemcpyHostToDevice, s.stream));
+            at = d.first + d.second.second;
+        }
+        staged_upto(total);
+        GFSM_HIP(hipMemcpyAsync(s.d_buf + off_at, s.h_buf + off_at,
+                                static_cast<size_t>((count + 1) * 8), hipMemcpyHostToDevice,
+                                s.stream));
+    }
     int32_t *d_flags = s.d_res, *d_lens = s.d_res + count;
     if (a.nfa)
         lib_check(gfsm_nfa_run_device_ws(a.nfa, a.technique, &tu, d_data, d_off, count, total,
@@ -625,10 +673,13 @@ int gfsm_server_submit_many(gfsm_server *s, int32_t automaton, const uint8_t *da
         // staying valid until every one of them has finished
         std::shared_ptr<uint8_t> block;
         const uint8_t *src = data + first;
+        bool pinned = false;
         if (!(flags & GFSM_SERVE_NOCOPY)) {
             block = new_block(static_cast<size_t>(total));
             copy_parallel(block.get(), src, static_cast<size_t>(total));
             src = block.get();
+        } else {
+            pinned = pinned_range(src, total);
         }
         const double t = now_s();
         {
@@ -644,6 +695,7 @@ int gfsm_server_submit_many(gfsm_server *s, int32_t automaton, const uint8_t *da
                 q.block = block;
                 q.bytes = src + (offsets[i] - first);
                 q.arrival = t;
+                q.pinned = pinned;
                 q.id = s->next_id++;
                 Record rec;
                 rec.r.status = kPending;

@@ -306,7 +306,8 @@ int gfsm_server_submit(gfsm_server *server, int32_t automaton, const uint8_t *da
  * ids in order into out_ids[n]; the requests are queued together, in order, with one
  * arrival time. The library copies the whole run once (with several threads when it is
  * large), unless flags holds GFSM_SERVE_NOCOPY: then it reads the caller's buffer when it
- * stages each launch, and the buffer must stay valid and unchanged until every one of the
+ * stages each launch (a buffer in pinned host memory is copied to the device directly,
+ * without staging), and the buffer must stay valid and unchanged until every one of the
  * n requests has finished. */
 #define GFSM_SERVE_NOCOPY 1
 int gfsm_server_submit_many(gfsm_server *server, int32_t automaton, const uint8_t *data,

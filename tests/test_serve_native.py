@@ -199,3 +199,40 @@ def test_native_bulk_submit_matches_oracle_and_checks_offsets():
     np.testing.assert_array_equal(in_place.accepted, wf.astype(bool))
     np.testing.assert_array_equal(in_place.match_len, wn)
     _check_against_oracle(small, singles, s_reqs)
+
+
+def test_native_bulk_submit_from_pinned_memory():
+    """submit_many in place from a caller buffer in pinned host memory: the scheduler copies
+    those runs to the device directly. Launches that mix them with staged requests of the
+    same automaton (single submits and a pageable bulk submit between them) keep every
+    verdict equal to the oracle."""
+    import torch
+
+    automata, unions = W.serving_automata(seed=5)
+    small = {"nfa16384": automata["nfa16384"]}
+    pu = unions["nfa16384"]
+    lens = W.lognormal_lengths(900, 6000, 1.0, 0, 1 << 20, seed=51)
+    data, off = W.pattern_inputs(pu, lens, 0.7, seed=52)
+    pinned_t = torch.empty(int(off[-1]), dtype=torch.uint8, pin_memory=True)
+    pinned = pinned_t.numpy()
+    pinned[:] = data[: int(off[-1])]
+    wf, wn = _oracle(pu.nfa, data, off)
+    loop = NativeServingLoop(small, slots=3, max_batch_bytes=8 << 20, max_batch_count=256)
+    with loop:
+        a = loop.submit_many("nfa16384", pinned, off[:401], copy=False)  # inputs 0..399
+        singles = [loop.submit("nfa16384", data[off[k]:off[k + 1]].tobytes())
+                   for k in range(400, 450)]
+        b = loop.submit_many("nfa16384", data, off[450:701], copy=False)  # pageable
+        c = loop.submit_many("nfa16384", pinned, off[700:], copy=False)  # not from 0
+        assert a.wait(60.0) and b.wait(60.0) and c.wait(60.0)
+        loop.collect(singles)
+        assert loop.query()["launches"] >= 3  # launches of mixed pinned and staged runs
+    del pinned_t
+    np.testing.assert_array_equal(a.accepted, wf[:400].astype(bool))
+    np.testing.assert_array_equal(a.match_len, wn[:400])
+    np.testing.assert_array_equal([r.accepted for r in singles], wf[400:450].astype(bool))
+    np.testing.assert_array_equal([r.match_len for r in singles], wn[400:450])
+    np.testing.assert_array_equal(b.accepted, wf[450:700].astype(bool))
+    np.testing.assert_array_equal(b.match_len, wn[450:700])
+    np.testing.assert_array_equal(c.accepted, wf[700:].astype(bool))
+    np.testing.assert_array_equal(c.match_len, wn[700:])
