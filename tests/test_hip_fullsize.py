@@ -4,7 +4,8 @@ The oracle gate in ``bench.py`` compares full batches bit for bit; these tests c
 same sizes, what must hold whatever the verdicts are:
 
 * techniques agree with each other (split, gather and lds on 256 MiB; chain and bitpacked on
-  the 109 MB ragged mix and on the 1 GiB high-divergence batch);
+  the 109 MB ragged mix and on the 1 GiB high-divergence batch; the masked bit-parallel step
+  and the same step over gather lists only on the general 65,536-state control);
 * verdicts do not depend on batch order (split segments and chain chunks then fall on other
   bytes of every input), nor on the chain chunk length;
 * latch-first-match: ``match_len <= len``, a rejected input reports 0, an accepted input's
@@ -135,3 +136,46 @@ def test_divergent_full_batch_chain_agrees_with_tiles():
         np.testing.assert_array_equal(bf, f)
         np.testing.assert_array_equal(bn, n)
         _check_latch_invariants(lambda d, o: _run_chain(dev, d, o, 4096), data, off, f, n)
+
+
+def test_general_control_full_batch_masks_agree_with_lists():
+    """The general (non-chain) 65,536-state control at its bench size, 1024 inputs of
+    16 KiB, with 7 accepting pattern ends so that half the inputs match thousands of bytes
+    in: the masked bit-parallel step (first and diagonal masks) equals the same step with
+    every edge in the gather lists, on the whole batch; latch-first-match holds; and 8
+    inputs equal the oracle."""
+    import dataclasses
+    import os
+
+    from oracle import native
+
+    pu = W.pattern_union(8000, (5, 12), (16, 64), seed=0, total_states=65536)
+    nfa = W.optional_positions(pu, 0.25, seed=1)
+    acc = np.asarray(nfa.accept).copy()
+    ends = np.flatnonzero(acc)
+    acc[:] = False
+    acc[ends[np.random.default_rng(7).random(ends.size) < 0.002]] = True
+    nfa = dataclasses.replace(nfa, accept=acc)
+    data, off = W.pattern_inputs(pu, np.full(1024, 16384, dtype=np.int64), 0.95, seed=2)
+    old = os.environ.get("GFSM_GATHER_DIAG")
+    try:
+        os.environ["GFSM_GATHER_DIAG"] = "0"
+        with DeviceNFA(nfa) as dev:
+            lf, ln = dev.run_host(data, off, _lib.NFA_BITPACKED)[:2]
+        os.environ["GFSM_GATHER_DIAG"] = "1"
+        with DeviceNFA(nfa) as dev:
+            assert dev.resolve(1024, int(off[-1])) == _lib.NFA_BITPACKED  # first masks
+            f, n = dev.run_host(data, off, _lib.NFA_BITPACKED)[:2]
+            np.testing.assert_array_equal(f, lf)
+            np.testing.assert_array_equal(n, ln)
+            _check_latch_invariants(lambda d, o: dev.run_host(d, o, _lib.NFA_BITPACKED)[:2],
+                                    data, off, f, n)
+    finally:
+        if old is None:
+            os.environ.pop("GFSM_GATHER_DIAG", None)
+        else:
+            os.environ["GFSM_GATHER_DIAG"] = old
+    assert 0.2 < f.mean() < 0.8 and np.median(n[f > 0]) > 1000
+    wf, wn, _ = native.nfa_batch(nfa, data, off[:9])
+    np.testing.assert_array_equal(f[:8], wf)
+    np.testing.assert_array_equal(n[:8], wn)
