@@ -654,12 +654,19 @@ template <typename T, bool kLds, int K, int kFixRound>
 __global__ __launch_bounds__(fix_threads_for(kFixRound) / K) void dfa_split_fix_kernel(
     const T *__restrict__ gtab, SplitArgs a, int32_t repair) {
     extern __shared__ uint4 lds_raw[];
+    // Heads are handed out from a counter per block over the block's contiguous range, so a
+    // lane that finishes a short merge walk takes the next head: the pass then lasts about
+    // as long as its longest head, not as long as the unluckiest lane's fixed share.
+    __shared__ unsigned int s_next;
+    if (threadIdx.x == 0) s_next = 0u;
+    __syncthreads();
     const char *tab = stage_table<T, kLds>(gtab, a.table_bytes, lds_raw);
     const uint32_t col = a.col, acc = a.acc;
     const int64_t rmask = (1ll << a.rshift) - 1;
     const typename Ck<T>::type *ckpt = static_cast<const typename Ck<T>::type *>(a.ckpt);
-    const int64_t lanes = static_cast<int64_t>(gridDim.x) * blockDim.x;
-    int64_t next_g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t per_block = (a.segments + gridDim.x - 1) / gridDim.x;
+    const int64_t g_lo = static_cast<int64_t>(blockIdx.x) * per_block;
+    const int64_t g_hi = (g_lo + per_block) < a.segments ? (g_lo + per_block) : a.segments;
     const uint4 zero = make_uint4(0, 0, 0, 0);
 
     int64_t g[K], q[K], hend[K];
@@ -668,13 +675,15 @@ __global__ __launch_bounds__(fix_threads_for(kFixRound) / K) void dfa_split_fix_
     constexpr int kFixVec = kFixRound / 16;
     uint4 nv[K][kFixVec];
 
+    // one LDS atomic per head taken, per lane (no wave-wide step inside): the loop ends when
+    // the block's range is used up, and a lane that finds it used up stays idle
     auto pick = [&](int c) {
         act[c] = false;
         have[c] = false;
         e[c] = a.start;
-        while (next_g < a.segments) {
-            const int64_t gg = next_g;
-            next_g += lanes;
+        for (;;) {
+            const int64_t gg = g_lo + static_cast<int64_t>(atomicAdd(&s_next, 1u));
+            if (gg >= g_hi) return;
             if (gg == 0) continue;
             const int64_t s0 = gg * a.seg;
             const int32_t i = a.seg_input[gg];
@@ -1224,7 +1233,19 @@ static void split_launch_passes(const DfaDev &d, SplitArgs &a, hipStream_t strea
         }
         return sh;
     };
-    const Shape walk = shape(kWalkThreads), fix = shape(kFixThreads);
+    const Shape walk = shape(kWalkThreads);
+    Shape fix = shape(kFixThreads);
+    if (kLds && fix.threads > fix_threads_for(64) / kChains) {
+        // more than 512 merge walks per CU: two blocks of half the size per CU (two tables
+        // fit the LDS), so the fix pass keeps its 64-byte rounds (a 1024-thread block would
+        // have to use 32-byte rounds)
+        const int32_t half = fix.threads / 2;
+        const int64_t need = (a.segments + static_cast<int64_t>(half) * kChains - 1) /
+                             (static_cast<int64_t>(half) * kChains);
+        const int64_t cap = 2 * static_cast<int64_t>(cus);
+        fix.threads = half;
+        fix.blocks = static_cast<int32_t>(need < cap ? need : cap);
+    }
     dfa_split_walk_kernel<T, kLds, kChains><<<walk.blocks, walk.threads, lds, stream>>>(tab, a);
     GFSM_HIP(hipGetLastError());
     if (a.segments > 1) {

@@ -94,11 +94,46 @@ class ResidentCache:
 
 _batch_serials = itertools.count(1)
 
+_stream_pool: list = []
+_stream_pool_lock = threading.Lock()
+
+
+def slot_streams(n: int) -> list:
+    """``n`` streams from a process-wide pool, created once (default priority) and reused.
+
+    The HIP runtime gives each new stream the next of the process's hardware queues in turn
+    (4 by default, one of them the default stream's), so streams created afresh call after
+    call wrap around onto queues that other streams hold, and launches on two streams that
+    share a queue run one after the other. Slots that take their streams from this pool
+    keep the same queues across calls: up to three slots on three distinct queues, as long
+    as the pool's streams are the process's first.
+    """
+    torch = _torch()
+    with _stream_pool_lock:
+        while len(_stream_pool) < n:
+            _stream_pool.append(torch.cuda.Stream())
+        return _stream_pool[:n]
+
+
+_capture_stream = None
+
+
+def capture_stream():
+    """The one stream graph captures run on (created after three pooled slot streams, so
+    that it takes no queue before them; captured work never runs on it)."""
+    global _capture_stream
+    if _capture_stream is None:
+        slot_streams(3)
+        with _stream_pool_lock:
+            if _capture_stream is None:
+                _capture_stream = _torch().cuda.Stream()
+    return _capture_stream
+
 # Host copies into pinned staging are split over a few threads above this size: one thread
 # copies at a fraction of what the host's memory and the PCIe link carry, and numpy drops
 # the GIL inside a contiguous copy.
 _COPY_SPLIT_BYTES = 16 << 20
-_COPY_THREADS = 4
+_COPY_THREADS = 8
 _copy_pool: concurrent.futures.ThreadPoolExecutor | None = None
 _copy_pool_lock = threading.Lock()
 
@@ -431,7 +466,7 @@ class BatchEngine:
             self._launch(dev, kind, batch, choice, stream)
             stream.synchronize()
             g = torch.cuda.CUDAGraph(keep_graph=True)
-            side = torch.cuda.Stream()
+            side = capture_stream()
             side.wait_stream(stream)
             with torch.cuda.graph(g, stream=side):
                 self._launch(dev, kind, batch, choice, torch.cuda.current_stream())
@@ -469,12 +504,16 @@ class BatchEngine:
         return flags, lens, e0.elapsed_time(e1)
 
     def run_stream(self, automaton, batches, technique: str = "auto", depth: int = 2,
-                   graph: bool = True, capacity_bytes: int = 0, capacity_count: int = 0,
+                   graph: bool = False, capacity_bytes: int = 0, capacity_count: int = 0,
                    coalesce: int = 1):
         """Verdicts ``(flags, lens)`` of a stream of host batches ``(data, offsets)``, in order.
 
-        ``depth`` slots run at once, each with its own stream, resident batch buffers and
-        captured graph: while the GPU walks batch k, the host copies batch k + 1 into the
+        ``depth`` slots run at once, each with its own stream (from the process-wide pool)
+        and resident batch buffers, launched directly: graph replays on several streams at
+        once overlapped less on ROCm (2.9 ms per batch of the ragged mix against 2.25 ms,
+        three slots), and a graph saves microseconds of launch work against milliseconds of
+        kernels; ``graph=True`` replays a captured graph per slot instead. While the GPU
+        walks batch k, the host copies batch k + 1 into the
         next slot's pinned staging and its host-to-device copy is enqueued, so the staging
         cost of a host-resident stream overlaps the kernels instead of adding to them. A
         slot is reused once its previous verdicts have been taken. With ``coalesce`` above
@@ -528,7 +567,7 @@ class BatchEngine:
                 ev.synchronize()
                 yield from results(slot, cts)
             if j == len(slots):
-                st = torch.cuda.Stream()  # default priority
+                st = slot_streams(depth)[j]  # pooled: the same hardware queues every call
                 slots.append((DeviceBatch(data, offsets, capacity_bytes=capacity_bytes,
                                           capacity_count=capacity_count, stream=st), st))
             else:

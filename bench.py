@@ -154,11 +154,15 @@ def _measure_dfa(torch, batch, dfa, technique: str, data, offsets, steps: int, w
             else:
                 cdata, coff = batch.data, batch.offsets
             need = dev.workspace_size(ccount, ctotal, tech, tu)
-            slots = [(torch.cuda.Stream(),
+            from gpufsm.batch import slot_streams
+
+            # streams from the process-wide pool: each measurement's slots keep distinct
+            # hardware queues (fresh streams would wrap onto queues other slots hold)
+            slots = [(st,
                       torch.empty(ccount, dtype=torch.int32, device="cuda"),
                       torch.empty(ccount, dtype=torch.int32, device="cuda"),
                       torch.empty(max(need, 256), dtype=torch.uint8, device="cuda"))
-                     for _ in range(depth)]
+                     for st in slot_streams(depth)]
             cur = torch.cuda.current_stream()
 
             def run(k: int) -> None:
@@ -193,8 +197,11 @@ def _measure_dfa(torch, batch, dfa, technique: str, data, offsets, steps: int, w
                                     f"hip dfa {technique} ({what}, stream of {depth})")
             nb = launches * coalesce
             piped = {"depth": depth, "batches_per_launch": coalesce,
+                     "technique": names[dev.resolve(ccount, ctotal, tech, tu)],
+                     "launch_inputs": ccount, "launch_bytes": ctotal,
                      "ms_per_step": 1e3 * pwall / nb,
-                     "event_ms_per_step": e0.elapsed_time(e1) / nb}
+                     "event_ms_per_step": e0.elapsed_time(e1) / nb,
+                     "ms_per_launch": e0.elapsed_time(e1) / launches * depth}
             del cdata, coff, slots
     examined = int(wex.sum())
     credited = int(offsets[-1])
@@ -258,7 +265,8 @@ def bench_dfa(args, torch) -> dict:
                 extras["variants"].append(_measure_dfa(
                     torch, batch, d, t, data, offsets, max(2, args.steps // 2), 1,
                     f"dfa{other_n} {'latching 10%' if latching else 'non-latching'}",
-                    depth=args.pipeline if (t == "auto" and not latching) else 1,
+                    # the 4096-state table's best stream: 3 launches of 2 batches in flight
+                    depth=3 if (t == "auto" and not latching) else 1,
                     coalesce=2 if (t == "auto" and not latching and other_n > 256) else 1))
     _log("oracle gate passed on every variant (full batch)")
 
@@ -308,14 +316,18 @@ def bench_dfa(args, torch) -> dict:
         # a stream of batches: the step time is the steady-state time per batch
         value, step_ms, kernel_ms = piped["examined_GBps"], piped["ms_per_step"], \
             piped["event_ms_per_step"]
+        tech = piped["technique"]  # what `auto` picked for the launch shape
+        kernel_name = KERNELS.get(tech, tech)
         per = (f", {piped['batches_per_launch']} batches per launch"
                if piped["batches_per_launch"] > 1 else "")
         kernel_name = (f"{kernel_name}, {piped['depth']} launches in flight on "
                        f"{piped['depth']} streams{per} (time per batch in steady state)")
+        how = ("one lane walks each input of the launch's batches (the per-input walk: "
+               f"{piped['launch_inputs']} inputs per launch)" if tech in ("lds", "gather")
+               else "a batch's repair passes overlap the next batch's walk")
         mode = (f"batches run back to back, {piped['depth']} launches in flight on their own "
-                f"streams and workspaces{per} (a batch's repair passes overlap the next "
-                f"batch's walk); every slot's verdicts gated; one batch alone is under "
-                f"single_batch")
+                f"streams and workspaces{per} ({how}; {piped['ms_per_launch']:.2f} ms per "
+                f"launch); every slot's verdicts gated; one batch alone is under single_batch")
     else:
         value, step_ms, kernel_ms = head["examined_GBps"], head["ms_per_step"], \
             head["kernel_ms"]
@@ -485,7 +497,7 @@ def bench_ragged(args, torch) -> dict:
         batch.fetch(s)
     staged_s = (time.perf_counter() - t0) / reps
     # the same host-resident batches as a pipelined stream: while the GPU walks batch k, the
-    # host stages batch k + 1 into another slot (pinned copy on 4 threads, H2D on that slot's
+    # host stages batch k + 1 into another slot (pinned copy on 8 threads, H2D on that slot's
     # stream), so staging overlaps the kernels. Timed in steady state: from the result of
     # batch `depth` to the last one; every result is gated.
     depth, n_stream = 3, max(8, steps) + 3
@@ -1233,9 +1245,10 @@ def main() -> int:
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--length", type=int, default=65536)
     ap.add_argument("--technique", default="auto")
-    ap.add_argument("--pipeline", type=int, default=3,
-                    help="DFA batches in flight (streams) for the headline; 1 = one at a time")
-    ap.add_argument("--coalesce", type=int, default=1,
+    ap.add_argument("--pipeline", type=int, default=2,
+                    help="DFA launches in flight (streams) for the headline; 1 = one batch at "
+                         "a time")
+    ap.add_argument("--coalesce", type=int, default=64,
                     help="DFA headline stream: batches per launch (copies back to back)")
     ap.add_argument("--ragged-count", type=int, default=16384)
     ap.add_argument("--ragged-technique", default="auto",

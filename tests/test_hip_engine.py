@@ -384,8 +384,8 @@ def test_serving_configured_mix_equals_alone_and_oracle():
         alone.close()
 
 
-@pytest.mark.parametrize("depth", [1, 2, 3])
-def test_run_stream_matches_oracle_in_order(depth):
+@pytest.mark.parametrize("depth,graph", [(1, False), (2, False), (3, False), (3, True)])
+def test_run_stream_matches_oracle_in_order(depth, graph):
     """A stream of host batches through several slots (their own streams, buffers and
     graphs): every batch's verdicts come back in order and match the oracle, also when a
     later batch is smaller than the first."""
@@ -398,8 +398,8 @@ def test_run_stream_matches_oracle_in_order(depth):
             lens = W.lognormal_lengths(count, 2000, 1.0, 0, 40000, seed=62 + k)
             batches.append(W.pattern_inputs(pu, lens, 0.7, seed=70 + k))
         cap_b = max(int(o[-1]) for _, o in batches)
-        got = list(eng.run_stream(pu.nfa, batches, "auto", depth=depth, capacity_bytes=cap_b,
-                                  capacity_count=400))
+        got = list(eng.run_stream(pu.nfa, batches, "auto", depth=depth, graph=graph,
+                                  capacity_bytes=cap_b, capacity_count=400))
         assert len(got) == len(batches)
         for k, ((d, o), (f, n)) in enumerate(zip(batches, got)):
             wf, wn = _oracle(pu.nfa, d, o)
