@@ -261,7 +261,10 @@ int32_t resolve_dfa(const gfsm_dfa *h, int32_t technique, int32_t count, int64_t
             // Segment the batch when inputs span several segments: one lane per input
             // would serialise each of them into one long dependent chain. With 256 inputs
             // per CU or more, one lane per input already keeps the LDS busy and needs no
-            // repair passes (65,536 x 16 KiB, 256 states: 1.86 TB/s against 1.69 split).
+            // repair passes (65,536 x 16 KiB, 256 states: 1.86 TB/s against 1.69 split),
+            // and for a table in L2 it keeps more lookups in flight than the segments do
+            // (65,536 x 64 KiB, 4096 states: gather 261 GB/s against 186 split; at 32,768
+            // inputs split still leads, 186 against 179).
             // A known longest input that is long against both the segment and the batch
             // is segmented whatever the other inputs are: one lane per input would walk it
             // alone. The count rule needs the longest input for the same reason.
@@ -270,8 +273,8 @@ int32_t resolve_dfa(const gfsm_dfa *h, int32_t technique, int32_t count, int64_t
             if (tu.max_len > 0) {
                 const int64_t bound = std::max<int64_t>(4 * plan.seg_bytes, total / kLaneWalkSkew);
                 if (tu.max_len > bound) return GFSM_DFA_SPLIT;
-                if (h->lds_ok && count >= 256 * static_cast<int64_t>(device_cu_count()))
-                    return GFSM_DFA_LDS;
+                if (count >= 256 * static_cast<int64_t>(device_cu_count()))
+                    return h->lds_ok ? GFSM_DFA_LDS : GFSM_DFA_GATHER;
             }
             if (avg >= 4 * plan.seg_bytes) return GFSM_DFA_SPLIT;
             return h->lds_ok ? GFSM_DFA_LDS : GFSM_DFA_GATHER;

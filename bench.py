@@ -265,9 +265,10 @@ def bench_dfa(args, torch) -> dict:
                 extras["variants"].append(_measure_dfa(
                     torch, batch, d, t, data, offsets, max(2, args.steps // 2), 1,
                     f"dfa{other_n} {'latching 10%' if latching else 'non-latching'}",
-                    # the 4096-state table's best stream: 3 launches of 2 batches in flight
-                    depth=3 if (t == "auto" and not latching) else 1,
-                    coalesce=2 if (t == "auto" and not latching and other_n > 256) else 1))
+                    # the same stream as the headline: 2 launches of 64 batches in flight
+                    # (for the 4096-state table auto then takes the per-input gather walk)
+                    depth=2 if (t == "auto" and not latching) else 1,
+                    coalesce=64 if (t == "auto" and not latching) else 1))
     _log("oracle gate passed on every variant (full batch)")
 
     # CPU baseline: the C oracle over all host cores, full batch, headline automaton

@@ -217,9 +217,13 @@ def test_dfa_auto_resolution_by_batch_shape():
         wf, wn, _ = native.dfa_batch(dev.dfa, data, off)
         np.testing.assert_array_equal(f, wf)
         np.testing.assert_array_equal(n, wn)
+    gather = _lib.DFA_TECHNIQUES["gather"]
     with DeviceDFA(random_dfa(4096, accept_prob=0.0, seed=5)) as dev:  # table in L2
-        assert dev.resolve(1 << 20, (1 << 20) << 14, tuning=longest(1 << 14)) == \
-            _lib.DFA_TECHNIQUES["gather"]
+        assert dev.resolve(1 << 20, (1 << 20) << 14, tuning=longest(1 << 14)) == gather
+        assert dev.resolve(4096, 4096 << 16) == split        # one batch of 64 KiB inputs
+        # 64 such batches in one launch: one lane per input once the longest is known
+        assert dev.resolve(262144, 262144 << 16, tuning=longest(1 << 16)) == gather
+        assert dev.resolve(262144, 262144 << 16) == split  # longest not known
 
 
 @pytest.mark.parametrize("count", [131072, 262144])
