@@ -359,10 +359,12 @@ def bench_dfa(args, torch) -> dict:
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": _traffic(tech, workload_name),
+            "traffic": _traffic(tech, workload_name),  # HBM bytes per step (one batch)
             "kernel": kernel_name,
-            "kernel_ms": kernel_ms,
-            "algorithmic_bytes_per_launch": algorithmic,
+            "kernel_ms": kernel_ms,  # per step: a launch's batches, launches overlapping
+            "algorithmic_bytes_per_step": algorithmic,
+            "algorithmic_bytes_per_launch": algorithmic * (piped["batches_per_launch"]
+                                                           if piped else 1),
         },
         "cpu_baseline": {
             "value": examined / cpu_s / 1e9,
