@@ -503,9 +503,19 @@ class BatchEngine:
         flags, lens = batch.fetch(stream)
         return flags, lens, e0.elapsed_time(e1)
 
+    def auto_coalesce(self, offsets, max_bytes: int = 4 << 30) -> int:
+        """Batches per launch for a stream of batches shaped like ``offsets``: enough for 256
+        inputs per CU, at most ``max_bytes`` of input per launch (at least one batch)."""
+        offsets = np.asarray(offsets, dtype=np.int64)
+        count = max(int(offsets.size) - 1, 1)
+        total = max(int(offsets[-1]) if offsets.size else 0, 1)
+        cus = self._torch.cuda.get_device_properties(0).multi_processor_count
+        want = -(-256 * cus // count)
+        return max(1, min(want, int(max_bytes) // total))
+
     def run_stream(self, automaton, batches, technique: str = "auto", depth: int = 2,
                    graph: bool = False, capacity_bytes: int = 0, capacity_count: int = 0,
-                   coalesce: int = 1):
+                   coalesce: int | str = 1, coalesce_bytes: int = 4 << 30):
         """Verdicts ``(flags, lens)`` of a stream of host batches ``(data, offsets)``, in order.
 
         ``depth`` slots run at once, each with its own stream (from the process-wide pool)
@@ -519,14 +529,25 @@ class BatchEngine:
         slot is reused once its previous verdicts have been taken. With ``coalesce`` above
         1, up to that many consecutive batches share one slot and one launch (their bytes
         back to back): a launch whose passes are latency-bound (a DFA table read from L2)
-        then walks several batches' chains at once. Slots must hold the first launch's
-        bytes and inputs unless ``capacity_bytes`` / ``capacity_count`` say more.
+        then walks several batches' chains at once. ``coalesce="auto"`` sizes the groups
+        from the first batch: enough batches for 256 inputs per CU (where the DFA's `auto`
+        walks one lane per input, with no repair passes), at most ``coalesce_bytes`` of
+        input per launch; throughput for latency, since a batch's verdicts come with its
+        group's launch. Slots must hold the first launch's bytes and inputs unless
+        ``capacity_bytes`` / ``capacity_count`` say more.
         """
         torch = self._torch
         if depth < 1:
             raise ValueError("depth must be >= 1")
-        if coalesce < 1:
-            raise ValueError("coalesce must be >= 1")
+        batches = iter(batches)
+        if coalesce == "auto":
+            first = next(batches, None)
+            if first is None:
+                return
+            coalesce = self.auto_coalesce(first[1], coalesce_bytes)
+            batches = itertools.chain([first], batches)
+        if not isinstance(coalesce, int) or coalesce < 1:
+            raise ValueError("coalesce must be an int >= 1 or 'auto'")
         slots: list[tuple[DeviceBatch, object]] = []
         pending: collections.deque = collections.deque()  # (slot, event, counts) in order
 

@@ -103,6 +103,39 @@ def test_graph_replays_match_eager(engine, technique):
         obatch.fetch(s)
 
 
+def test_run_stream_auto_coalesce_groups_small_batches():
+    """coalesce="auto": batches of 2000 inputs are grouped until a launch gives every CU 256
+    inputs (capped by bytes per launch); each batch's verdicts still equal the oracle."""
+    import torch
+
+    eng = BatchEngine(tuner=None)
+    try:
+        cus = torch.cuda.get_device_properties(0).multi_processor_count
+        dfa = W.dense_dfa(256, 0.0005, seed=151)
+        rng = np.random.default_rng(152)
+        batches = []
+        for k in range(5):
+            lens = rng.integers(0, 600, size=2000)
+            o = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+            batches.append((rng.integers(0, 256, size=int(o[-1]), dtype=np.uint8), o))
+        k = eng.auto_coalesce(batches[0][1])
+        assert k == -(-256 * cus // 2000)
+        assert eng.auto_coalesce(batches[0][1], max_bytes=int(batches[0][1][-1]) * 3) == 3
+        cap_b = sum(int(o[-1]) for _, o in batches)
+        got = list(eng.run_stream(dfa, batches, "auto", depth=2, coalesce="auto",
+                                  capacity_bytes=cap_b, capacity_count=2000 * 5))
+        assert len(got) == len(batches)
+        for j, ((d, o), (f, n)) in enumerate(zip(batches, got)):
+            wf, wn = _oracle(dfa, d, o)
+            np.testing.assert_array_equal(f, wf, err_msg=f"batch {j}")
+            np.testing.assert_array_equal(n, wn, err_msg=f"batch {j}")
+        assert list(eng.run_stream(dfa, [], coalesce="auto")) == []
+        with pytest.raises(ValueError):
+            list(eng.run_stream(dfa, batches, coalesce="many"))
+    finally:
+        eng.close()
+
+
 def test_graph_replays_and_direct_launches_by_kernel_count():
     """graph=True replays a captured multi-kernel sequence (split: six kernels) with
     hipGraphLaunch and launches a one-kernel technique (gather) directly; with
