@@ -118,11 +118,13 @@ def _measure_dfa(torch, batch, dfa, technique: str, data, offsets, steps: int, w
 
     Two timings: one batch at a time on one stream (``ms_per_step``), and, with ``depth``
     above 1, a stream of the same batch run back to back on ``depth`` streams, each with
-    its own outputs and workspace, so that a batch's repair passes (latency-bound tails of
-    a few lanes) overlap the next batch's walk (``stream``). With ``coalesce`` above 1 each
-    launch of the stream covers that many consecutive batches (copies of the batch back to
-    back in HBM), as a stream engine coalesces batches that are queued together; times are
-    per batch. Every slot's verdicts are gated against the oracle after the timed stream.
+    its own outputs and workspace, so that a launch's tail overlaps the next launch
+    (``stream``). With ``coalesce`` above 1 each launch of the stream covers that many
+    consecutive batches (copies of the batch back to back in HBM), as a stream engine
+    coalesces batches that are queued together. A step of the stream is one launch:
+    ``warmup`` untimed launches, then exactly ``steps`` timed ones; ``ms_per_step`` is per
+    launch and ``ms_per_batch`` per batch. Every slot's verdicts are gated against the
+    oracle after the timed stream.
     """
     from gpufsm.backends.hip import _lib
     from gpufsm.backends.hip.handles import DeviceDFA, make_tuning
@@ -179,9 +181,11 @@ def _measure_dfa(torch, batch, dfa, technique: str, data, offsets, steps: int, w
                 for s, *_ in slots:
                     cur.wait_stream(s)
 
-            # at least three launches per slot, so the stream reaches its steady state
-            launches = max(3 * depth, -(-steps // coalesce))
-            stream_of(0, max(warmup, 1) * depth)
+            # A step of the stream is one launch (``coalesce`` batches): exactly ``warmup``
+            # untimed launches, then exactly ``steps`` timed ones, round-robin over the slots.
+            launches = steps
+            if warmup > 0:
+                stream_of(0, warmup)
             torch.cuda.synchronize()
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
@@ -191,7 +195,7 @@ def _measure_dfa(torch, batch, dfa, technique: str, data, offsets, steps: int, w
             e1.record(cur)
             torch.cuda.synchronize()
             pwall = time.perf_counter() - t0
-            for _, fl, ln, _ in slots:
+            for _, fl, ln, _ in slots[:min(depth, launches)]:  # every slot that ran
                 gate.require_arrays(fl.cpu().numpy(), ln.cpu().numpy(), np.tile(wf, coalesce),
                                     np.tile(wm, coalesce),
                                     f"hip dfa {technique} ({what}, stream of {depth})")
@@ -199,9 +203,13 @@ def _measure_dfa(torch, batch, dfa, technique: str, data, offsets, steps: int, w
             piped = {"depth": depth, "batches_per_launch": coalesce,
                      "technique": names[dev.resolve(ccount, ctotal, tech, tu)],
                      "launch_inputs": ccount, "launch_bytes": ctotal,
-                     "ms_per_step": 1e3 * pwall / nb,
-                     "event_ms_per_step": e0.elapsed_time(e1) / nb,
-                     "ms_per_launch": e0.elapsed_time(e1) / launches * depth}
+                     "steps": launches, "warmup": warmup,
+                     "ms_per_step": 1e3 * pwall / launches,  # a step: one launch
+                     "event_ms_per_step": e0.elapsed_time(e1) / launches,
+                     "ms_per_batch": 1e3 * pwall / nb,
+                     "event_ms_per_batch": e0.elapsed_time(e1) / nb,
+                     # one launch's duration, launches overlapping
+                     "ms_per_launch": e0.elapsed_time(e1) / launches * min(depth, launches)}
             del cdata, coff, slots
     examined = int(wex.sum())
     credited = int(offsets[-1])
@@ -222,12 +230,12 @@ def _measure_dfa(torch, batch, dfa, technique: str, data, offsets, steps: int, w
     msg = f"{what} [{technique}]: {out['examined_GBps']:.1f} GB/s examined, " \
           f"{out['ms_per_step']:.3f} ms/step"
     if piped:
-        piped["examined_GBps"] = examined / (piped["ms_per_step"] * 1e-3) / 1e9
-        piped["credited_GBps"] = credited / (piped["ms_per_step"] * 1e-3) / 1e9
+        piped["examined_GBps"] = examined / (piped["ms_per_batch"] * 1e-3) / 1e9
+        piped["credited_GBps"] = credited / (piped["ms_per_batch"] * 1e-3) / 1e9
         out["stream"] = piped
         per = f", {coalesce} batches per launch" if coalesce > 1 else ""
         msg += f"; stream of {depth}{per}: {piped['examined_GBps']:.1f} GB/s, " \
-               f"{piped['ms_per_step']:.3f} ms/batch"
+               f"{piped['ms_per_batch']:.3f} ms/batch"
     _log(msg)
     return out
 
@@ -313,27 +321,33 @@ def bench_dfa(args, torch) -> dict:
     single = {"examined_GBps": head["examined_GBps"], "ms_per_step": head["ms_per_step"],
               "kernel_ms": head["kernel_ms"],
               "roofline_frac": algorithmic / (head["kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS}
+    per_step = 1  # configured batches per step
     if piped:
-        # a stream of batches: the step time is the steady-state time per batch
+        # a stream of batches: a step is one launch of the stream (its batches coalesced),
+        # timed in steady state with the next launches overlapping
         value, step_ms, kernel_ms = piped["examined_GBps"], piped["ms_per_step"], \
             piped["event_ms_per_step"]
+        per_step = piped["batches_per_launch"]
         tech = piped["technique"]  # what `auto` picked for the launch shape
         kernel_name = KERNELS.get(tech, tech)
         per = (f", {piped['batches_per_launch']} batches per launch"
                if piped["batches_per_launch"] > 1 else "")
         kernel_name = (f"{kernel_name}, {piped['depth']} launches in flight on "
-                       f"{piped['depth']} streams{per} (time per batch in steady state)")
+                       f"{piped['depth']} streams{per} (time per step, one launch, in steady "
+                       f"state)")
         how = ("one lane walks each input of the launch's batches (the per-input walk: "
                f"{piped['launch_inputs']} inputs per launch)" if tech in ("lds", "gather")
                else "a batch's repair passes overlap the next batch's walk")
         mode = (f"batches run back to back, {piped['depth']} launches in flight on their own "
                 f"streams and workspaces{per} ({how}; {piped['ms_per_launch']:.2f} ms per "
-                f"launch); every slot's verdicts gated; one batch alone is under single_batch")
+                f"launch); a step is one launch ({per_step} batch"
+                f"{'es' if per_step > 1 else ''}, {piped['ms_per_batch']:.4f} ms per batch); "
+                f"every slot's verdicts gated; one batch alone is under single_batch")
     else:
         value, step_ms, kernel_ms = head["examined_GBps"], head["ms_per_step"], \
             head["kernel_ms"]
         mode = "one batch at a time"
-    achieved = algorithmic / (kernel_ms * 1e-3) / 1e9
+    achieved = algorithmic * per_step / (kernel_ms * 1e-3) / 1e9
     return {
         "metric": "input GB/s batched vs plain run_batch on MI355X; p50/p99 latency at fixed QPS",
         "value": value,
@@ -351,6 +365,9 @@ def bench_dfa(args, torch) -> dict:
             "technique": args.technique, "resolved_technique": tech,
             "batches_in_flight": piped["depth"] if piped else 1,
             "batches_per_launch": piped["batches_per_launch"] if piped else 1,
+            "batches_per_step": per_step,
+            "global_batch": count * per_step,  # inputs per step
+            "ms_per_batch": piped["ms_per_batch"] if piped else head["ms_per_step"],
         },
         "single_batch": single,
         "roofline": {
@@ -359,12 +376,13 @@ def bench_dfa(args, torch) -> dict:
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": _traffic(tech, workload_name),  # HBM bytes per step (one batch)
+            # HBM bytes per step: the profile's per-batch figure times the step's batches
+            "traffic": (lambda t: t * per_step if t is not None else None)(
+                _traffic(tech, workload_name)),
             "kernel": kernel_name,
-            "kernel_ms": kernel_ms,  # per step: a launch's batches, launches overlapping
-            "algorithmic_bytes_per_step": algorithmic,
-            "algorithmic_bytes_per_launch": algorithmic * (piped["batches_per_launch"]
-                                                           if piped else 1),
+            "kernel_ms": kernel_ms,  # per step, launches overlapping
+            "algorithmic_bytes_per_step": algorithmic * per_step,
+            "algorithmic_bytes_per_batch": algorithmic,
         },
         "cpu_baseline": {
             "value": examined / cpu_s / 1e9,
