@@ -92,12 +92,21 @@ bool pinned_range(const uint8_t *p, int64_t n) {
     return true;
 }
 
-// Copies n bytes with up to 8 threads, one per 2 MiB: one core's memcpy (about 5 GB/s into
-// pinned memory) falls far short of the host's memory bandwidth, and bulk submits and large
-// launches copy tens to hundreds of MiB while the GPU waits for them.
+// Threads of a staging copy: GFSM_COPY_THREADS, else 8.
+size_t copy_threads() {
+    static const size_t t = [] {
+        const int64_t v = env_int("GFSM_COPY_THREADS", 8);
+        return static_cast<size_t>(v < 1 ? 1 : (v > 64 ? 64 : v));
+    }();
+    return t;
+}
+
+// Copies n bytes with up to copy_threads() threads, one per 2 MiB: one core's memcpy (about
+// 5 GB/s into pinned memory) falls far short of the host's memory bandwidth, and bulk
+// submits and large launches copy tens to hundreds of MiB while the GPU waits for them.
 void copy_parallel(uint8_t *dst, const uint8_t *src, size_t n) {
     constexpr size_t kPerThread = size_t(2) << 20;
-    const size_t t = std::min<size_t>(8, n / kPerThread);
+    const size_t t = std::min<size_t>(copy_threads(), n / kPerThread);
     if (t <= 1) {
         if (n) std::memcpy(dst, src, n);
         return;
