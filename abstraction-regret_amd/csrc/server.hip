@@ -41,7 +41,6 @@
 #include <mutex>
 #include <string>
 #include <thread>
-#include <unordered_map>
 #include <utility>
 #include <vector>
 
@@ -127,6 +126,7 @@ std::shared_ptr<uint8_t> new_block(size_t n) {
 
 struct Record {
     gfsm_server_result r{};
+    bool collected = false;  // returned by a wait (a tombstone until the front passes it)
 };
 
 struct Slot {
@@ -162,7 +162,17 @@ struct gfsm_server {
     std::deque<Req> waiting;
     std::vector<std::deque<Req>> admitted;  // per automaton, in arrival order
     int64_t admitted_count = 0;
-    std::unordered_map<int64_t, Record> table;  // submitted and not yet collected
+    // every request's record by id: ids are handed out in order, so record id lives at
+    // records[id - first_id]; collected records at the front are dropped as they go (a
+    // record never collected keeps the ones after it, as tombstones, until it is)
+    std::deque<Record> records;
+    int64_t first_id = 0;
+    Record *record(int64_t id) {
+        if (id < first_id || id - first_id >= static_cast<int64_t>(records.size()))
+            return nullptr;
+        Record &r = records[static_cast<size_t>(id - first_id)];
+        return r.collected ? nullptr : &r;
+    }
     int64_t next_id = 0;
     int64_t in_flight_bytes = 0, resident_bytes = 0;
     gfsm_server_stats stats{};
@@ -236,9 +246,9 @@ gfsm_server::~gfsm_server() {
 
 void gfsm_server::finish_locked(const Req &q, int32_t status, int32_t accepted, int32_t len,
                                 double t) {
-    auto it = table.find(q.id);
-    if (it == table.end() || it->second.r.status != kPending) return;
-    gfsm_server_result &r = it->second.r;
+    Record *rec = record(q.id);
+    if (!rec || rec->r.status != kPending) return;
+    gfsm_server_result &r = rec->r;
     r.status = status;
     r.accepted = accepted;
     r.match_len = len;
@@ -652,7 +662,7 @@ int gfsm_server_submit(gfsm_server *s, int32_t automaton, const uint8_t *data, i
             rec.r.automaton = automaton;
             rec.r.arrival_s = q.arrival;
             rec.r.bytes = len;
-            s->table.emplace(q.id, rec);
+            s->records.push_back(rec);
             ++s->stats.submitted;
             *out_id = q.id;
             s->waiting.push_back(std::move(q));
@@ -711,7 +721,7 @@ int gfsm_server_submit_many(gfsm_server *s, int32_t automaton, const uint8_t *da
                 rec.r.automaton = automaton;
                 rec.r.arrival_s = t;
                 rec.r.bytes = q.len;
-                s->table.emplace(q.id, rec);
+                s->records.push_back(rec);
                 out_ids[i] = q.id;
                 s->waiting.push_back(std::move(q));
             }
@@ -727,7 +737,7 @@ int gfsm_server_wait(gfsm_server *s, const int64_t *ids, int64_t n, double timeo
         if (!s || (n > 0 && (!ids || !out)) || n < 0) throw Error(GFSM_EINVAL, "null argument");
         std::unique_lock<std::mutex> lk(s->mu);
         for (int64_t k = 0; k < n; ++k)
-            if (s->table.find(ids[k]) == s->table.end())
+            if (!s->record(ids[k]))
                 throw Error(GFSM_EINVAL, "unknown or already collected request id " +
                                              std::to_string(ids[k]));
         // an id collected meanwhile by another wait counts as done (reported below). The
@@ -736,8 +746,8 @@ int gfsm_server_wait(gfsm_server *s, const int64_t *ids, int64_t n, double timeo
         int64_t first = 0;
         auto all_done = [&] {
             for (; first < n; ++first) {
-                auto it = s->table.find(ids[first]);
-                if (it != s->table.end() && it->second.r.status == kPending) return false;
+                const Record *r = s->record(ids[first]);
+                if (r && r->r.status == kPending) return false;
             }
             return true;
         };
@@ -747,15 +757,20 @@ int gfsm_server_wait(gfsm_server *s, const int64_t *ids, int64_t n, double timeo
             s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), all_done);
         bool lost = false;
         for (int64_t k = 0; k < n; ++k) {
-            auto it = s->table.find(ids[k]);
-            if (it == s->table.end()) {
+            Record *r = s->record(ids[k]);
+            if (!r) {
                 std::memset(&out[k], 0, sizeof(out[k]));
                 out[k].status = GFSM_SERVE_ERROR;
                 lost = true;
                 continue;
             }
-            out[k] = it->second.r;
-            if (it->second.r.status != kPending) s->table.erase(it);  // collected
+            out[k] = r->r;
+            if (r->r.status != kPending) r->collected = true;
+        }
+        // drop the collected records at the front (ids below first_id are unknown from now)
+        while (!s->records.empty() && s->records.front().collected) {
+            s->records.pop_front();
+            ++s->first_id;
         }
         if (lost) throw Error(GFSM_EINVAL, "a request was collected by another wait");
     });

@@ -544,7 +544,7 @@ def bench_ragged(args, torch) -> dict:
     plain_s = float(np.median(plain))
     engine.close()
     # served: every input a request to the native serving loop (the library's scheduler
-    # batches an automaton's admitted requests into launches of up to 64 MiB / 8192
+    # batches an automaton's admitted requests into launches of up to 128 MiB / 16384
     # requests, stages them through pinned memory and launches on its slot streams), all
     # submitted at once; from the first submit to the last verdict, every verdict gated.
     # Twice: from the caller's ordinary (pageable) buffer, and from a caller buffer in
@@ -555,13 +555,13 @@ def bench_ragged(args, torch) -> dict:
     pinned_t = torch.empty(int(off[-1]), dtype=torch.uint8, pin_memory=True)
     pinned_buf = pinned_t.numpy()
     pinned_buf[:] = data[: int(off[-1])]
-    loop = NativeServingLoop({"ragged": pu.nfa}, slots=4, max_batch_bytes=64 << 20,
-                             max_batch_count=8192, max_inflight_per_automaton=4,
+    loop = NativeServingLoop({"ragged": pu.nfa}, slots=4, max_batch_bytes=128 << 20,
+                             max_batch_count=16384, max_inflight_per_automaton=4,
                              technique=choice.label())
     with loop:
         loop.collect([loop.submit("ragged", x) for x in inputs[:256]])  # builds the lists
         for buf, times in ((data, served), (pinned_buf, served_pinned)):
-            for _ in range(2):
+            for _ in range(3):
                 t0 = time.perf_counter()
                 bulk = loop.submit_many("ragged", buf, off, copy=False)  # one call, no copy
                 bulk.wait()
@@ -618,7 +618,7 @@ def bench_ragged(args, torch) -> dict:
                    "launches": int(launches),
                    "note": f"the {count} inputs submitted as requests to the native serving "
                            f"loop in one bulk call, read in place (4 slots, launches of up to "
-                           f"64 MiB / 8192 requests, technique {choice.label()}); first "
+                           f"128 MiB / 16384 requests, technique {choice.label()}); first "
                            f"submit to last verdict, host copies included; also one submit "
                            f"call per request"},
         "served_pinned": {"ms_per_batch": 1e3 * served_pinned_s,
