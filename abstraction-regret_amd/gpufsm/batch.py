@@ -509,7 +509,8 @@ class BatchEngine:
         offsets = np.asarray(offsets, dtype=np.int64)
         count = max(int(offsets.size) - 1, 1)
         total = max(int(offsets[-1]) if offsets.size else 0, 1)
-        cus = self._torch.cuda.get_device_properties(0).multi_processor_count
+        torch = self._torch
+        cus = torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
         want = -(-256 * cus // count)
         return max(1, min(want, int(max_bytes) // total))
 
