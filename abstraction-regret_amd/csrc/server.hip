@@ -33,6 +33,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -45,6 +46,7 @@
 #include <vector>
 
 #include "../../include/gpufsm_hip.h"
+#include "copy_pool.hpp"
 #include "internal.hpp"
 
 using namespace gfsm;
@@ -132,8 +134,9 @@ struct Record {
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
+    hipEvent_t copied = nullptr;  // recorded after the launch's copies in
     // staging of one launch, pinned and on the device alike: the requests' bytes, then the
-    // int64 offsets at the next 256-byte boundary (one copy in); the verdicts: count flags
+    // int64 offsets at the next 256-byte boundary; the verdicts: count flags
     // then count lengths (one copy out)
     uint8_t *h_buf = nullptr;
     int32_t *h_res = nullptr;
@@ -179,6 +182,8 @@ struct gfsm_server {
     bool started = false, stopping = false, drain = true, failed = false;
     std::string error;
     std::thread thread;
+    std::unique_ptr<CopyPool> copies;  // the scheduler's staging copies
+    hipEvent_t last_copied = nullptr;  // `copied` of the latest launch (its slot's event)
 
     ~gfsm_server();
     void run();
@@ -230,6 +235,7 @@ void free_slot(Slot &s) {
         s.d_ws = nullptr;
     }
     if (s.done) (void)hipEventDestroy(s.done);
+    if (s.copied) (void)hipEventDestroy(s.copied);
     if (s.stream) (void)hipStreamDestroy(s.stream);
     if (s.h_buf) (void)hipHostFree(s.h_buf);
     if (s.h_res) (void)hipHostFree(s.h_res);
@@ -285,33 +291,24 @@ void gfsm_server::launch(Slot &s) {
     const Automaton &a = automata[static_cast<size_t>(s.automaton)];
     const int32_t count = static_cast<int32_t>(s.reqs.size());
     int64_t total = 0, longest = 0;
-    // Requests that sit back to back in one block (a bulk submit) are copied as one run:
-    // into the pinned staging buffer, or, for a caller's pinned buffer, straight to the
-    // device (`direct`: launch offset, source, length).
-    std::vector<std::pair<int64_t, std::pair<const uint8_t *, int64_t>>> direct;
-    const uint8_t *run_src = nullptr;
-    int64_t run_at = 0, run_len = 0;
-    bool run_pinned = false;
-    auto flush = [&] {
-        if (run_len) {
-            if (run_pinned) direct.push_back({run_at, {run_src, run_len}});
-            else copy_parallel(s.h_buf + run_at, run_src, static_cast<size_t>(run_len));
-        }
-        run_len = 0;
+    // Requests that sit back to back in one block (a bulk submit) form one run: staged
+    // through the pinned buffer, or, from a caller's pinned buffer, copied straight to the
+    // device.
+    struct Run {
+        int64_t at, len;  // place in the launch's bytes
+        const uint8_t *src;
+        bool pinned;
     };
+    std::vector<Run> runs;
     for (size_t i = 0; i < s.reqs.size(); ++i) {
         const Req &q = s.reqs[i];
-        if (!(run_len && q.bytes == run_src + run_len && q.pinned == run_pinned)) {
-            flush();
-            run_src = q.bytes;
-            run_at = total;
-            run_pinned = q.pinned;
-        }
-        run_len += q.len;
+        if (runs.empty() || q.bytes != runs.back().src + runs.back().len ||
+            q.pinned != runs.back().pinned)
+            runs.push_back({total, 0, q.bytes, q.pinned});
+        runs.back().len += q.len;
         total += q.len;
         longest = std::max(longest, q.len);
     }
-    flush();
     const int64_t off_at = (total + 255) & ~int64_t(255);
     int64_t *h_off = reinterpret_cast<int64_t *>(s.h_buf + off_at);
     h_off[0] = 0;
@@ -331,30 +328,43 @@ void gfsm_server::launch(Slot &s) {
         GFSM_HIP(hipMallocAsync(&s.d_ws, static_cast<size_t>(need), s.stream));
         s.ws_bytes = need;
     }
-    if (direct.empty()) {  // one copy in: the staged bytes and the offsets after them
-        GFSM_HIP(hipMemcpyAsync(s.d_buf, s.h_buf, static_cast<size_t>(off_at + (count + 1) * 8),
-                                hipMemcpyHostToDevice, s.stream));
-    } else {
-        // the pinned runs from their own memory, the staged runs (the gaps between them)
-        // from the staging buffer, then the offsets
-        int64_t at = 0;
-        auto staged_upto = [&](int64_t end) {
-            if (end > at)
-                GFSM_HIP(hipMemcpyAsync(s.d_buf + at, s.h_buf + at, static_cast<size_t>(end - at),
-                                        hipMemcpyHostToDevice, s.stream));
-        };
-        for (const auto &d : direct) {
-            staged_upto(d.first);
-            GFSM_HIP(hipMemcpyAsync(s.d_buf + d.first, d.second.first,
-                                    static_cast<size_t>(d.second.second),
+    // Copies in: the offsets first (a small copy after a large one starts tens of
+    // microseconds after it, delaying the kernels), then the bytes in launch order. Staged
+    // bytes go to the device in chunks of kStageChunk as soon as they are staged, so the DMA
+    // of one chunk overlaps the staging of the next; pinned runs are copied from the
+    // caller's memory. A launch's copies start after the previous launch's copies have
+    // ended: launches copied side by side share the host link, so neither's kernels could
+    // start before both copies were done; in turn, the previous launch's kernels run while
+    // this launch's bytes arrive.
+    if (last_copied) GFSM_HIP(hipStreamWaitEvent(s.stream, last_copied, 0));
+    GFSM_HIP(hipMemcpyAsync(s.d_buf + off_at, s.h_buf + off_at,
+                            static_cast<size_t>((count + 1) * 8), hipMemcpyHostToDevice,
+                            s.stream));
+    constexpr int64_t kStageChunk = int64_t(8) << 20;
+    int64_t sent = 0;  // launch bytes [0, sent) enqueued for the device
+    auto send_staged = [&](int64_t end) {
+        if (end > sent)
+            GFSM_HIP(hipMemcpyAsync(s.d_buf + sent, s.h_buf + sent, static_cast<size_t>(end - sent),
                                     hipMemcpyHostToDevice, s.stream));
-            at = d.first + d.second.second;
+        sent = std::max(sent, end);
+    };
+    for (const Run &r : runs) {
+        if (r.pinned) {
+            send_staged(r.at);
+            GFSM_HIP(hipMemcpyAsync(s.d_buf + r.at, r.src, static_cast<size_t>(r.len),
+                                    hipMemcpyHostToDevice, s.stream));
+            sent = r.at + r.len;
+            continue;
         }
-        staged_upto(total);
-        GFSM_HIP(hipMemcpyAsync(s.d_buf + off_at, s.h_buf + off_at,
-                                static_cast<size_t>((count + 1) * 8), hipMemcpyHostToDevice,
-                                s.stream));
+        for (int64_t lo = 0; lo < r.len; lo += kStageChunk) {
+            const int64_t m = std::min(kStageChunk, r.len - lo);
+            copies->copy(s.h_buf + r.at + lo, r.src + lo, static_cast<size_t>(m));
+            if (r.at + lo + m - sent >= kStageChunk) send_staged(r.at + lo + m);
+        }
     }
+    send_staged(total);
+    GFSM_HIP(hipEventRecord(s.copied, s.stream));
+    last_copied = s.copied;
     int32_t *d_flags = s.d_res, *d_lens = s.d_res + count;
     if (a.nfa)
         lib_check(gfsm_nfa_run_device_ws(a.nfa, a.technique, &tu, d_data, d_off, count, total,
@@ -544,6 +554,7 @@ int gfsm_server_create(const gfsm_server_config *config, gfsm_server **out) {
             throw Error(GFSM_EINVAL, "max_batch_bytes must be below 2 GiB");
         auto srv = std::make_unique<gfsm_server>();
         srv->cfg = c;
+        srv->copies = std::make_unique<CopyPool>(copy_threads() - 1);
         GFSM_HIP(hipGetDevice(&srv->device));
         srv->slots.resize(static_cast<size_t>(c.slots));
         const size_t cnt = static_cast<size_t>(c.max_batch_count);
@@ -551,6 +562,7 @@ int gfsm_server_create(const gfsm_server_config *config, gfsm_server **out) {
         for (Slot &s : srv->slots) {
             GFSM_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
             GFSM_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+            GFSM_HIP(hipEventCreateWithFlags(&s.copied, hipEventDisableTiming));
             const size_t staging = bytes + 256 + (cnt + 1) * 8;
             GFSM_HIP(hipHostMalloc(reinterpret_cast<void **>(&s.h_buf), staging,
                                    hipHostMallocDefault));

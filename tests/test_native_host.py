@@ -1,13 +1,16 @@
 """Host code under AddressSanitizer and UndefinedBehaviorSanitizer (CPU suite, no GPU).
 
-Builds two checks with g++/gcc ``-fsanitize=address,undefined`` and runs them:
+Builds checks with g++/gcc ``-fsanitize=address,undefined`` (and one with
+``-fsanitize=thread``) and runs them:
 
 * ``tests/native/host_check.cpp`` with ``csrc/layout.cpp`` (the device-layout builders,
   which parse untrusted CSR input) and ``oracle/fsm_oracle.c`` (the C oracle): the epsilon
   fold, rows, gather lists, DFA tables and chain tiles checked against an independent set
   simulation and the oracle on random automata, plus malformed-input refusals;
 * ``tests/native/capi_check.cpp`` with ``csrc/capi.hip`` compiled as host C++ and the
-  library's kernel objects (linked, never launched): the C ABI's host-only paths.
+  library's kernel objects (linked, never launched): the C ABI's host-only paths;
+* ``tests/native/pool_check.cpp`` with ``csrc/copy_pool.hpp`` under ThreadSanitizer: the
+  serving scheduler's staging copy pool.
 
 Any sanitizer report aborts the program, which fails the test.
 """
@@ -91,3 +94,15 @@ def test_capi_host_paths_under_sanitizers(objs):
           "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib", "-pthread", "-o", str(exe)])
     res = _run([str(exe)], env=_env(), timeout=300)
     assert "capi_check ok" in res.stdout
+
+
+def test_copy_pool_under_thread_sanitizer(tmp_path):
+    """The serving scheduler's staging copy pool (``csrc/copy_pool.hpp``) under
+    ThreadSanitizer: successive copies into one guarded buffer, pools of 0 to 7 helpers."""
+    exe = tmp_path / "pool_check"
+    _run(["g++", "-std=c++17", "-O1", "-g", "-fsanitize=thread", f"-I{CSRC}",
+          str(ROOT / "tests/native/pool_check.cpp"), "-pthread", "-o", str(exe)])
+    env = dict(os.environ)
+    env["TSAN_OPTIONS"] = "halt_on_error=1"
+    res = _run([str(exe)], env=env, timeout=300)
+    assert "pool_check ok" in res.stdout
