@@ -340,7 +340,9 @@ void gfsm_server::launch(Slot &s) {
     GFSM_HIP(hipMemcpyAsync(s.d_buf + off_at, s.h_buf + off_at,
                             static_cast<size_t>((count + 1) * 8), hipMemcpyHostToDevice,
                             s.stream));
-    constexpr int64_t kStageChunk = int64_t(8) << 20;
+    // 16 MiB: with 8 MiB chunks (twice the copies queued) a copy call of the second of two
+    // 64 MiB launches in flight was seen to hold the scheduler for 5 ms
+    constexpr int64_t kStageChunk = int64_t(16) << 20;
     int64_t sent = 0;  // launch bytes [0, sent) enqueued for the device
     auto send_staged = [&](int64_t end) {
         if (end > sent)

@@ -544,9 +544,10 @@ def bench_ragged(args, torch) -> dict:
     plain_s = float(np.median(plain))
     engine.close()
     # served: every input a request to the native serving loop (the library's scheduler
-    # batches an automaton's admitted requests into launches of up to 128 MiB / 16384
-    # requests, stages them through pinned memory and launches on its slot streams), all
-    # submitted at once; from the first submit to the last verdict, every verdict gated.
+    # batches an automaton's admitted requests into launches of up to 64 MiB / 16384
+    # requests, stages them through pinned memory and launches on its slot streams: the
+    # second launch's bytes arrive while the first one's kernels run), all submitted at
+    # once; from the first submit to the last verdict, every verdict gated.
     # Twice: from the caller's ordinary (pageable) buffer, and from a caller buffer in
     # pinned memory, which the scheduler copies to the device without staging.
     from gpufsm.serve import NativeServingLoop
@@ -555,7 +556,7 @@ def bench_ragged(args, torch) -> dict:
     pinned_t = torch.empty(int(off[-1]), dtype=torch.uint8, pin_memory=True)
     pinned_buf = pinned_t.numpy()
     pinned_buf[:] = data[: int(off[-1])]
-    loop = NativeServingLoop({"ragged": pu.nfa}, slots=4, max_batch_bytes=128 << 20,
+    loop = NativeServingLoop({"ragged": pu.nfa}, slots=4, max_batch_bytes=64 << 20,
                              max_batch_count=16384, max_inflight_per_automaton=4,
                              technique=choice.label())
     with loop:
@@ -618,7 +619,7 @@ def bench_ragged(args, torch) -> dict:
                    "launches": int(launches),
                    "note": f"the {count} inputs submitted as requests to the native serving "
                            f"loop in one bulk call, read in place (4 slots, launches of up to "
-                           f"128 MiB / 16384 requests, technique {choice.label()}); first "
+                           f"64 MiB / 16384 requests, technique {choice.label()}); first "
                            f"submit to last verdict, host copies included; also one submit "
                            f"call per request"},
         "served_pinned": {"ms_per_batch": 1e3 * served_pinned_s,
