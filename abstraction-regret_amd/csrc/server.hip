@@ -328,21 +328,33 @@ void gfsm_server::launch(Slot &s) {
         GFSM_HIP(hipMallocAsync(&s.d_ws, static_cast<size_t>(need), s.stream));
         s.ws_bytes = need;
     }
-    // Copies in: the offsets first (a small copy after a large one starts tens of
-    // microseconds after it, delaying the kernels), then the bytes in launch order. Staged
-    // bytes go to the device in chunks of kStageChunk as soon as they are staged, so the DMA
-    // of one chunk overlaps the staging of the next; pinned runs are copied from the
-    // caller's memory. A launch's copies start after the previous launch's copies have
-    // ended: launches copied side by side share the host link, so neither's kernels could
-    // start before both copies were done; in turn, the previous launch's kernels run while
-    // this launch's bytes arrive.
-    if (last_copied) GFSM_HIP(hipStreamWaitEvent(s.stream, last_copied, 0));
-    GFSM_HIP(hipMemcpyAsync(s.d_buf + off_at, s.h_buf + off_at,
-                            static_cast<size_t>((count + 1) * 8), hipMemcpyHostToDevice,
-                            s.stream));
-    // 16 MiB: with 8 MiB chunks (twice the copies queued) a copy call of the second of two
-    // 64 MiB launches in flight was seen to hold the scheduler for 5 ms
+    // Copies in, in launch order; a launch's copies start after the previous launch's
+    // copies have ended: launches copied side by side share the host link, so neither's
+    // kernels could start before both copies were done; in turn, the previous launch's
+    // kernels run while this launch's bytes arrive. Pinned runs are copied from the caller's
+    // memory. When nothing is being copied, staged bytes go to the device in chunks of
+    // kStageChunk as soon as they are staged, so the DMA of one chunk overlaps the staging
+    // of the next (the offsets go first: a small copy after a large one starts tens of
+    // microseconds after it, delaying the kernels). Behind another launch's copies the
+    // launch is staged whole and copied at once: the link is busy anyway, and copies queued
+    // behind the wait for the previous launch were seen to hold the scheduler in the copy
+    // call for 5-8 ms (several 8 or 16 MiB chunks queued; one copy per launch never did).
+    bool behind = false;
+    if (last_copied) {
+        behind = hipEventQuery(last_copied) == hipErrorNotReady;
+        (void)hipGetLastError();  // a pending event's status is not an error
+        GFSM_HIP(hipStreamWaitEvent(s.stream, last_copied, 0));
+    }
     constexpr int64_t kStageChunk = int64_t(16) << 20;
+    const int64_t chunk = behind ? std::max<int64_t>(total, 1) : kStageChunk;
+    const size_t off_bytes = static_cast<size_t>((count + 1) * 8);
+    const bool any_pinned =
+        std::any_of(runs.begin(), runs.end(), [](const Run &r) { return r.pinned; });
+    // the offsets with the bytes when the launch is one staged copy, else first
+    const bool offsets_last = behind && !any_pinned;
+    if (!offsets_last)
+        GFSM_HIP(hipMemcpyAsync(s.d_buf + off_at, s.h_buf + off_at, off_bytes,
+                                hipMemcpyHostToDevice, s.stream));
     int64_t sent = 0;  // launch bytes [0, sent) enqueued for the device
     auto send_staged = [&](int64_t end) {
         if (end > sent)
@@ -358,11 +370,16 @@ void gfsm_server::launch(Slot &s) {
             sent = r.at + r.len;
             continue;
         }
-        for (int64_t lo = 0; lo < r.len; lo += kStageChunk) {
-            const int64_t m = std::min(kStageChunk, r.len - lo);
+        for (int64_t lo = 0; lo < r.len; lo += chunk) {
+            const int64_t m = std::min(chunk, r.len - lo);
             copies->copy(s.h_buf + r.at + lo, r.src + lo, static_cast<size_t>(m));
-            if (r.at + lo + m - sent >= kStageChunk) send_staged(r.at + lo + m);
+            if (!offsets_last && r.at + lo + m - sent >= chunk) send_staged(r.at + lo + m);
         }
+    }
+    if (offsets_last) {  // one copy: the staged bytes, the padding and the offsets
+        GFSM_HIP(hipMemcpyAsync(s.d_buf, s.h_buf, static_cast<size_t>(off_at) + off_bytes,
+                                hipMemcpyHostToDevice, s.stream));
+        sent = total;
     }
     send_staged(total);
     GFSM_HIP(hipEventRecord(s.copied, s.stream));

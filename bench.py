@@ -434,6 +434,50 @@ def _examined_upper(flags: np.ndarray, lens: np.ndarray, offsets: np.ndarray) ->
     return int(np.where(flags != 0, lens, np.diff(offsets)).sum())
 
 
+def _serve_bulk(torch, pu, data, off, inputs, wf, wn, technique: str) -> dict:
+    """The ragged mix served: every input a request to the native serving loop (the
+    library's scheduler batches an automaton's admitted requests into launches of up to
+    64 MiB / 16384 requests, stages them through pinned memory and launches on its slot
+    streams: the second launch's bytes arrive while the first one's kernels run), all
+    submitted at once; from the first submit to the last verdict, every verdict gated.
+    Twice: from the caller's ordinary (pageable) buffer, and from a caller buffer in pinned
+    memory, which the scheduler copies to the device without staging. Then one submit call
+    per request. Times are the best of 3 bulk submits."""
+    from gpufsm.serve import NativeServingLoop
+
+    from oracle import gate
+
+    count = len(inputs)
+    served, served_pinned = [], []
+    pinned_t = torch.empty(int(off[-1]), dtype=torch.uint8, pin_memory=True)
+    pinned_buf = pinned_t.numpy()
+    pinned_buf[:] = data[: int(off[-1])]
+    loop = NativeServingLoop({"ragged": pu.nfa}, slots=4, max_batch_bytes=64 << 20,
+                             max_batch_count=16384, max_inflight_per_automaton=4,
+                             technique=technique)
+    with loop:
+        loop.collect([loop.submit("ragged", x) for x in inputs[:256]])  # builds the lists
+        for buf, times in ((data, served), (pinned_buf, served_pinned)):
+            for _ in range(3):
+                t0 = time.perf_counter()
+                bulk = loop.submit_many("ragged", buf, off, copy=False)  # one call, no copy
+                bulk.wait()
+                times.append(time.perf_counter() - t0)
+                gate.require_arrays(bulk.accepted.astype(np.int32), bulk.match_len, wf, wn,
+                                    "hip nfa served in bulk (ragged)")
+        one_by_one = time.perf_counter()
+        single = [loop.submit("ragged", x) for x in inputs]
+        loop.collect(single)
+        one_by_one = time.perf_counter() - one_by_one
+        launches = loop.query()["launches"]
+    gate.require_arrays(np.fromiter((r.accepted for r in single), np.int32, count),
+                        np.fromiter((r.match_len for r in single), np.int32, count), wf, wn,
+                        "hip nfa served one request per call (ragged)")
+    del pinned_t
+    return {"served_s": min(served), "served_pinned_s": min(served_pinned),
+            "one_by_one_s": one_by_one, "launches": int(launches)}
+
+
 def bench_ragged(args, torch) -> dict:
     """Ragged NFA mix: plain per-call ``gpufsm.run_batch`` vs the resident engine.
 
@@ -543,43 +587,9 @@ def bench_ragged(args, torch) -> dict:
         plain.append(time.perf_counter() - t0)
     plain_s = float(np.median(plain))
     engine.close()
-    # served: every input a request to the native serving loop (the library's scheduler
-    # batches an automaton's admitted requests into launches of up to 64 MiB / 16384
-    # requests, stages them through pinned memory and launches on its slot streams: the
-    # second launch's bytes arrive while the first one's kernels run), all submitted at
-    # once; from the first submit to the last verdict, every verdict gated.
-    # Twice: from the caller's ordinary (pageable) buffer, and from a caller buffer in
-    # pinned memory, which the scheduler copies to the device without staging.
-    from gpufsm.serve import NativeServingLoop
-
-    served, served_pinned = [], []
-    pinned_t = torch.empty(int(off[-1]), dtype=torch.uint8, pin_memory=True)
-    pinned_buf = pinned_t.numpy()
-    pinned_buf[:] = data[: int(off[-1])]
-    loop = NativeServingLoop({"ragged": pu.nfa}, slots=4, max_batch_bytes=64 << 20,
-                             max_batch_count=16384, max_inflight_per_automaton=4,
-                             technique=choice.label())
-    with loop:
-        loop.collect([loop.submit("ragged", x) for x in inputs[:256]])  # builds the lists
-        for buf, times in ((data, served), (pinned_buf, served_pinned)):
-            for _ in range(3):
-                t0 = time.perf_counter()
-                bulk = loop.submit_many("ragged", buf, off, copy=False)  # one call, no copy
-                bulk.wait()
-                times.append(time.perf_counter() - t0)
-                gate.require_arrays(bulk.accepted.astype(np.int32), bulk.match_len, wf, wn,
-                                    "hip nfa served in bulk (ragged)")
-        one_by_one = time.perf_counter()
-        single = [loop.submit("ragged", x) for x in inputs]
-        loop.collect(single)
-        one_by_one = time.perf_counter() - one_by_one
-        launches = loop.query()["launches"]
-    gate.require_arrays(np.fromiter((r.accepted for r in single), np.int32, count),
-                        np.fromiter((r.match_len for r in single), np.int32, count), wf, wn,
-                        "hip nfa served one request per call (ragged)")
-    served_s = min(served)
-    served_pinned_s = min(served_pinned)
-    del pinned_t
+    sv = _serve_bulk(torch, pu, data, off, inputs, wf, wn, choice.label())
+    served_s, served_pinned_s = sv["served_s"], sv["served_pinned_s"]
+    one_by_one, launches = sv["one_by_one_s"], sv["launches"]
     out = {
         "workload": f"ragged NFA mix: {pu.nfa.num_states}-state union of 2000 patterns "
                     f"(lengths 5-40, class widths 1-32, wildcard start), {count} inputs, "
@@ -995,6 +1005,63 @@ def ragged_line(args, torch) -> dict:
     }
 
 
+def served_line(args, torch) -> dict:
+    """The ragged mix served through the native serving loop only (for profiling that path):
+    the mix, the oracle, the autotuner's technique for the resident batch, then the bulk
+    submits of :func:`_serve_bulk`; value = examined bytes per second of the pageable-buffer
+    submit, first submit to last verdict."""
+    from gpufsm import workloads as W
+    from gpufsm.batch import BatchEngine, DeviceBatch
+    from gpufsm.tune import Autotuner
+
+    from oracle import native
+
+    count = args.ragged_count
+    pu, (data, off) = W.workload_ragged(count=count)
+    threads = _threads()
+    t0 = time.perf_counter()
+    wf, wn, wex = native.nfa_batch(pu.nfa, data, off, threads=threads)
+    cpu_s = time.perf_counter() - t0
+    examined, credited = int(wex.sum()), int(off[-1])
+    s = torch.cuda.current_stream()
+    engine = BatchEngine(tuner=Autotuner(persist=False))
+    batch = DeviceBatch(data, off, stream=s)
+    choice = engine.enqueue(pu.nfa, batch, args.ragged_technique, stream=s)
+    batch.fetch(s)
+    engine.close()
+    del batch
+    inputs = [data[off[i]:off[i + 1]].tobytes() for i in range(count)]
+    sv = _serve_bulk(torch, pu, data, off, inputs, wf, wn, choice.label())
+    _log(f"served: {examined / sv['served_s'] / 1e9:.2f} GB/s from a pageable buffer, "
+         f"{examined / sv['served_pinned_s'] / 1e9:.2f} GB/s pinned ({choice.label()})")
+    return {
+        "metric": "input GB/s batched vs plain run_batch on MI355X; p50/p99 latency at fixed QPS",
+        "value": examined / sv["served_s"] / 1e9,
+        "unit": "GB/s",
+        "steps": 3,   # bulk submits from the pageable buffer (the best one is the value)
+        "warmup": 1,  # one submit of 256 requests
+        "ms_per_step": 1e3 * sv["served_s"],
+        "dtype": "u8",
+        "config": {
+            "workload": f"ragged NFA mix served: {count} requests submitted in one bulk call "
+                        f"to the native serving loop, read in place from a pageable buffer "
+                        f"(value), launches of up to 64 MiB; {pu.nfa.num_states}-state union "
+                        f"of 2000 patterns; examined input GB/s, host copies included",
+            "num_states": pu.nfa.num_states, "num_inputs": count,
+            "credited_bytes": credited, "examined_bytes": examined,
+            "technique": choice.label(),
+        },
+        "roofline": None,
+        "cpu_baseline": {
+            "value": examined / cpu_s / 1e9, "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"full batch ({count} inputs), C oracle over {threads} threads, one run",
+        },
+        "extras": {"served_pinned_GBps": examined / sv["served_pinned_s"] / 1e9,
+                   "one_request_per_call_ms": 1e3 * sv["one_by_one_s"],
+                   "launches": sv["launches"]},
+    }
+
+
 def divergent_line(args, torch) -> dict:
     """The high-divergence NFA as the line: the as-configured latching batch, each technique
     gated and timed; ``value`` is the fastest technique's examined input GB/s."""
@@ -1259,9 +1326,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="dfa",
-                    choices=["dfa", "ragged", "divergent", "general"],
+                    choices=["dfa", "ragged", "served", "divergent", "general"],
                     help="dfa: the DFA memory-bound face (default, with every extra); "
-                         "ragged: the ragged NFA mix as the line; divergent: the "
+                         "ragged: the ragged NFA mix as the line; served: the ragged mix "
+                         "served through the native loop only; divergent: the "
                          "high-divergence NFA as the line")
     ap.add_argument("--states", type=int, default=256)
     ap.add_argument("--count", type=int, default=4096)
@@ -1325,6 +1393,8 @@ def main() -> int:
         return 2
     if args.workload == "ragged":
         out = ragged_line(args, torch)
+    elif args.workload == "served":
+        out = served_line(args, torch)
     elif args.workload == "divergent":
         out = divergent_line(args, torch)
     elif args.workload == "general":
@@ -1349,8 +1419,8 @@ def main() -> int:
         "value": out["value"],
         "unit": out["unit"],
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": out.get("steps", args.steps),
+        "warmup": out.get("warmup", args.warmup),
         "ms_per_step": out["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
