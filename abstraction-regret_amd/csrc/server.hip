@@ -134,7 +134,7 @@ struct Record {
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    hipEvent_t copied = nullptr;  // recorded after the launch's copies in
+    hipEvent_t copied = nullptr;  // recorded on the copy stream after the launch's copies in
     // staging of one launch, pinned and on the device alike: the requests' bytes, then the
     // int64 offsets at the next 256-byte boundary; the verdicts: count flags
     // then count lengths (one copy out)
@@ -183,7 +183,7 @@ struct gfsm_server {
     std::string error;
     std::thread thread;
     std::unique_ptr<CopyPool> copies;  // the scheduler's staging copies
-    hipEvent_t last_copied = nullptr;  // `copied` of the latest launch (its slot's event)
+    hipStream_t copy_stream = nullptr;  // every launch's host-to-device copies, in order
 
     ~gfsm_server();
     void run();
@@ -247,7 +247,9 @@ void free_slot(Slot &s) {
 }  // namespace
 
 gfsm_server::~gfsm_server() {
+    if (copy_stream) (void)hipStreamSynchronize(copy_stream);
     for (Slot &s : slots) free_slot(s);
+    if (copy_stream) (void)hipStreamDestroy(copy_stream);
 }
 
 void gfsm_server::finish_locked(const Req &q, int32_t status, int32_t accepted, int32_t len,
@@ -328,62 +330,44 @@ void gfsm_server::launch(Slot &s) {
         GFSM_HIP(hipMallocAsync(&s.d_ws, static_cast<size_t>(need), s.stream));
         s.ws_bytes = need;
     }
-    // Copies in, in launch order; a launch's copies start after the previous launch's
-    // copies have ended: launches copied side by side share the host link, so neither's
-    // kernels could start before both copies were done; in turn, the previous launch's
-    // kernels run while this launch's bytes arrive. Pinned runs are copied from the caller's
-    // memory. When nothing is being copied, staged bytes go to the device in chunks of
-    // kStageChunk as soon as they are staged, so the DMA of one chunk overlaps the staging
-    // of the next (the offsets go first: a small copy after a large one starts tens of
-    // microseconds after it, delaying the kernels). Behind another launch's copies the
-    // launch is staged whole and copied at once: the link is busy anyway, and copies queued
-    // behind the wait for the previous launch were seen to hold the scheduler in the copy
-    // call for 5-8 ms (several 8 or 16 MiB chunks queued; one copy per launch never did).
-    bool behind = false;
-    if (last_copied) {
-        behind = hipEventQuery(last_copied) == hipErrorNotReady;
-        (void)hipGetLastError();  // a pending event's status is not an error
-        GFSM_HIP(hipStreamWaitEvent(s.stream, last_copied, 0));
-    }
+    // Copies in, on the server's one copy stream in launch order, so a launch's copies
+    // start after the previous launch's: launches copied side by side share the host link,
+    // so neither's kernels could start before both copies were done; in turn, the previous
+    // launch's kernels run while this launch's bytes arrive. The slot's stream waits for
+    // its copies before the kernels. (Copies on the slot's own stream behind a wait for the
+    // previous launch's copies were seen to hold the scheduler in a copy call for 5-8 ms.)
+    // Staged bytes go to the device in chunks of kStageChunk as soon as they are staged, so
+    // the DMA of one chunk overlaps the staging of the next; pinned runs are copied from the
+    // caller's memory; the offsets go first (a small copy after a large one starts tens of
+    // microseconds after it, delaying the kernels).
     constexpr int64_t kStageChunk = int64_t(16) << 20;
-    const int64_t chunk = behind ? std::max<int64_t>(total, 1) : kStageChunk;
-    const size_t off_bytes = static_cast<size_t>((count + 1) * 8);
-    const bool any_pinned =
-        std::any_of(runs.begin(), runs.end(), [](const Run &r) { return r.pinned; });
-    // the offsets with the bytes when the launch is one staged copy, else first
-    const bool offsets_last = behind && !any_pinned;
-    if (!offsets_last)
-        GFSM_HIP(hipMemcpyAsync(s.d_buf + off_at, s.h_buf + off_at, off_bytes,
-                                hipMemcpyHostToDevice, s.stream));
+    hipStream_t cs = copy_stream;
+    GFSM_HIP(hipMemcpyAsync(s.d_buf + off_at, s.h_buf + off_at,
+                            static_cast<size_t>((count + 1) * 8), hipMemcpyHostToDevice, cs));
     int64_t sent = 0;  // launch bytes [0, sent) enqueued for the device
     auto send_staged = [&](int64_t end) {
         if (end > sent)
             GFSM_HIP(hipMemcpyAsync(s.d_buf + sent, s.h_buf + sent, static_cast<size_t>(end - sent),
-                                    hipMemcpyHostToDevice, s.stream));
+                                    hipMemcpyHostToDevice, cs));
         sent = std::max(sent, end);
     };
     for (const Run &r : runs) {
         if (r.pinned) {
             send_staged(r.at);
             GFSM_HIP(hipMemcpyAsync(s.d_buf + r.at, r.src, static_cast<size_t>(r.len),
-                                    hipMemcpyHostToDevice, s.stream));
+                                    hipMemcpyHostToDevice, cs));
             sent = r.at + r.len;
             continue;
         }
-        for (int64_t lo = 0; lo < r.len; lo += chunk) {
-            const int64_t m = std::min(chunk, r.len - lo);
+        for (int64_t lo = 0; lo < r.len; lo += kStageChunk) {
+            const int64_t m = std::min(kStageChunk, r.len - lo);
             copies->copy(s.h_buf + r.at + lo, r.src + lo, static_cast<size_t>(m));
-            if (!offsets_last && r.at + lo + m - sent >= chunk) send_staged(r.at + lo + m);
+            if (r.at + lo + m - sent >= kStageChunk) send_staged(r.at + lo + m);
         }
     }
-    if (offsets_last) {  // one copy: the staged bytes, the padding and the offsets
-        GFSM_HIP(hipMemcpyAsync(s.d_buf, s.h_buf, static_cast<size_t>(off_at) + off_bytes,
-                                hipMemcpyHostToDevice, s.stream));
-        sent = total;
-    }
     send_staged(total);
-    GFSM_HIP(hipEventRecord(s.copied, s.stream));
-    last_copied = s.copied;
+    GFSM_HIP(hipEventRecord(s.copied, cs));
+    GFSM_HIP(hipStreamWaitEvent(s.stream, s.copied, 0));
     int32_t *d_flags = s.d_res, *d_lens = s.d_res + count;
     if (a.nfa)
         lib_check(gfsm_nfa_run_device_ws(a.nfa, a.technique, &tu, d_data, d_off, count, total,
@@ -575,6 +559,7 @@ int gfsm_server_create(const gfsm_server_config *config, gfsm_server **out) {
         srv->cfg = c;
         srv->copies = std::make_unique<CopyPool>(copy_threads() - 1);
         GFSM_HIP(hipGetDevice(&srv->device));
+        GFSM_HIP(hipStreamCreateWithFlags(&srv->copy_stream, hipStreamNonBlocking));
         srv->slots.resize(static_cast<size_t>(c.slots));
         const size_t cnt = static_cast<size_t>(c.max_batch_count);
         const size_t bytes = static_cast<size_t>(c.max_batch_bytes);
