@@ -236,3 +236,44 @@ def test_native_bulk_submit_from_pinned_memory():
     np.testing.assert_array_equal(b.match_len, wn[450:700])
     np.testing.assert_array_equal(c.accepted, wf[700:].astype(bool))
     np.testing.assert_array_equal(c.match_len, wn[700:])
+
+
+def test_native_chunked_staging_across_launches():
+    """A pageable bulk submit of about 100 MiB of ragged requests in launches of up to
+    40 MiB: each launch is staged in 16 MiB chunks (requests straddle chunk boundaries) and
+    the launches' copies follow each other on the copy stream while earlier launches run;
+    single submits between two bulk ones break the staged runs. Every verdict equals the
+    oracle's."""
+    dfa = W.dense_dfa(256, 0.0, seed=41)
+    trans = np.asarray(dfa.trans).reshape(256, 256)
+    trans[trans == 5] = 6
+    trans[90, 2] = 5  # one (state, byte) pair enters the accepting state: matches are rare
+    dfa.trans[:] = trans.reshape(-1)
+    dfa.accept[5] = True
+    rng = np.random.default_rng(42)
+    lens = np.minimum(rng.lognormal(np.log(48 << 10), 1.2, size=1600).astype(np.int64), 6 << 20)
+    off = np.zeros(lens.size + 1, dtype=np.int64)
+    np.cumsum(lens, out=off[1:])
+    data = rng.integers(0, 256, size=int(off[-1]), dtype=np.uint8)
+    assert off[-1] > (80 << 20)
+    wf, wn = _oracle(dfa, data, off)
+    assert wf.any() and not wf.all()
+    loop = NativeServingLoop({"d": dfa}, slots=4, max_batch_bytes=40 << 20,
+                             max_batch_count=4096, max_inflight_per_automaton=4)
+    with loop:
+        half = lens.size // 2
+        first = loop.submit_many("d", data, off[:half + 1], copy=False)
+        singles = [loop.submit("d", data[off[i]:off[i + 1]].tobytes())
+                   for i in range(half, half + 8)]
+        second = loop.submit_many("d", data, off[half + 8:], copy=False)
+        assert first.wait(60.0) and second.wait(60.0)
+        loop.collect(singles, 60.0)
+        launches = loop.stats()["launches"]
+    assert launches >= 3
+    np.testing.assert_array_equal(first.accepted, wf[:half].astype(bool))
+    np.testing.assert_array_equal(first.match_len, wn[:half])
+    np.testing.assert_array_equal(second.accepted, wf[half + 8:].astype(bool))
+    np.testing.assert_array_equal(second.match_len, wn[half + 8:])
+    for k, r in enumerate(singles):
+        assert r.status == "ok"
+        assert (r.accepted, r.match_len) == (bool(wf[half + k]), int(wn[half + k]))
