@@ -134,7 +134,6 @@ struct Record {
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    hipEvent_t copied = nullptr;  // recorded on the copy stream after the launch's copies in
     // staging of one launch, pinned and on the device alike: the requests' bytes, then the
     // int64 offsets at the next 256-byte boundary; the verdicts: count flags
     // then count lengths (one copy out)
@@ -183,7 +182,6 @@ struct gfsm_server {
     std::string error;
     std::thread thread;
     std::unique_ptr<CopyPool> copies;  // the scheduler's staging copies
-    hipStream_t copy_stream = nullptr;  // every launch's host-to-device copies, in order
 
     ~gfsm_server();
     void run();
@@ -235,7 +233,6 @@ void free_slot(Slot &s) {
         s.d_ws = nullptr;
     }
     if (s.done) (void)hipEventDestroy(s.done);
-    if (s.copied) (void)hipEventDestroy(s.copied);
     if (s.stream) (void)hipStreamDestroy(s.stream);
     if (s.h_buf) (void)hipHostFree(s.h_buf);
     if (s.h_res) (void)hipHostFree(s.h_res);
@@ -247,9 +244,7 @@ void free_slot(Slot &s) {
 }  // namespace
 
 gfsm_server::~gfsm_server() {
-    if (copy_stream) (void)hipStreamSynchronize(copy_stream);
     for (Slot &s : slots) free_slot(s);
-    if (copy_stream) (void)hipStreamDestroy(copy_stream);
 }
 
 void gfsm_server::finish_locked(const Req &q, int32_t status, int32_t accepted, int32_t len,
@@ -330,18 +325,16 @@ void gfsm_server::launch(Slot &s) {
         GFSM_HIP(hipMallocAsync(&s.d_ws, static_cast<size_t>(need), s.stream));
         s.ws_bytes = need;
     }
-    // Copies in, on the server's one copy stream in launch order, so a launch's copies
-    // start after the previous launch's: launches copied side by side share the host link,
-    // so neither's kernels could start before both copies were done; in turn, the previous
-    // launch's kernels run while this launch's bytes arrive. The slot's stream waits for
-    // its copies before the kernels. (Copies on the slot's own stream behind a wait for the
-    // previous launch's copies were seen to hold the scheduler in a copy call for 5-8 ms.)
-    // Staged bytes go to the device in chunks of kStageChunk as soon as they are staged, so
-    // the DMA of one chunk overlaps the staging of the next; pinned runs are copied from the
-    // caller's memory; the offsets go first (a small copy after a large one starts tens of
-    // microseconds after it, delaying the kernels).
+    // Copies in, on the slot's own stream, whose earlier work (the slot's previous launch)
+    // has finished: staged bytes go to the device in chunks of kStageChunk as soon as they
+    // are staged, so the DMA of one chunk overlaps the staging of the next; pinned runs are
+    // copied from the caller's memory; the offsets go first (a small copy after a large one
+    // starts tens of microseconds after it, delaying the kernels). Launches in flight copy
+    // side by side. (Ordering one launch's copies after another's, by an event wait on the
+    // device, made some copy calls hold the scheduler for 5-8 ms, and ordering them on the
+    // host measured no faster.)
     constexpr int64_t kStageChunk = int64_t(16) << 20;
-    hipStream_t cs = copy_stream;
+    hipStream_t cs = s.stream;
     GFSM_HIP(hipMemcpyAsync(s.d_buf + off_at, s.h_buf + off_at,
                             static_cast<size_t>((count + 1) * 8), hipMemcpyHostToDevice, cs));
     int64_t sent = 0;  // launch bytes [0, sent) enqueued for the device
@@ -366,8 +359,6 @@ void gfsm_server::launch(Slot &s) {
         }
     }
     send_staged(total);
-    GFSM_HIP(hipEventRecord(s.copied, cs));
-    GFSM_HIP(hipStreamWaitEvent(s.stream, s.copied, 0));
     int32_t *d_flags = s.d_res, *d_lens = s.d_res + count;
     if (a.nfa)
         lib_check(gfsm_nfa_run_device_ws(a.nfa, a.technique, &tu, d_data, d_off, count, total,
@@ -559,14 +550,12 @@ int gfsm_server_create(const gfsm_server_config *config, gfsm_server **out) {
         srv->cfg = c;
         srv->copies = std::make_unique<CopyPool>(copy_threads() - 1);
         GFSM_HIP(hipGetDevice(&srv->device));
-        GFSM_HIP(hipStreamCreateWithFlags(&srv->copy_stream, hipStreamNonBlocking));
         srv->slots.resize(static_cast<size_t>(c.slots));
         const size_t cnt = static_cast<size_t>(c.max_batch_count);
         const size_t bytes = static_cast<size_t>(c.max_batch_bytes);
         for (Slot &s : srv->slots) {
             GFSM_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
             GFSM_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
-            GFSM_HIP(hipEventCreateWithFlags(&s.copied, hipEventDisableTiming));
             const size_t staging = bytes + 256 + (cnt + 1) * 8;
             GFSM_HIP(hipHostMalloc(reinterpret_cast<void **>(&s.h_buf), staging,
                                    hipHostMallocDefault));
