@@ -437,9 +437,9 @@ def _examined_upper(flags: np.ndarray, lens: np.ndarray, offsets: np.ndarray) ->
 def _serve_bulk(torch, pu, data, off, inputs, wf, wn, technique: str) -> dict:
     """The ragged mix served: every input a request to the native serving loop (the
     library's scheduler batches an automaton's admitted requests into launches of up to
-    64 MiB / 16384 requests, stages them through pinned memory and launches on its slot
-    streams: the second launch's bytes arrive while the first one's kernels run), all
-    submitted at once; from the first submit to the last verdict, every verdict gated.
+    64 MiB / 16384 requests, stages them through pinned memory in chunks whose copies
+    overlap the staging, and launches on its slot streams), all submitted at once; from the
+    first submit to the last verdict, every verdict gated.
     Twice: from the caller's ordinary (pageable) buffer, and from a caller buffer in pinned
     memory, which the scheduler copies to the device without staging. Then one submit call
     per request. Times are the best of 3 bulk submits."""
