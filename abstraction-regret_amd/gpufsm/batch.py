@@ -193,6 +193,10 @@ class DeviceBatch:
         # before a pinned buffer is rewritten and before the buffers are released
         self._staged = None
         self._event = None   # one event, re-recorded after each staging copy (see _mark)
+        # the buffers were allocated, and zero-filled, on the current stream; a batch staged
+        # on another stream (non-blocking, as PyTorch's streams are) must not copy into them
+        # before that stream has got there, or a late fill overwrites the staged bytes
+        _order_after_current(torch, stream)
         self.load(data, offsets, stream=stream)
 
     def __del__(self) -> None:
@@ -297,13 +301,17 @@ class DeviceBatch:
         c = self.count
         return out[:c].copy(), out[self.cap_count:self.cap_count + c].copy()
 
-    def ensure_workspace(self, nbytes: int) -> int:
-        """Device pointer of a workspace of at least ``nbytes`` (256-byte aligned)."""
+    def ensure_workspace(self, nbytes: int, stream=None) -> int:
+        """Device pointer of a workspace of at least ``nbytes`` (256-byte aligned), to be used
+        on ``stream`` (the current stream if None)."""
         torch = self._torch
         if self.workspace is None or self.workspace.numel() < nbytes:
             if self.workspace is not None:
                 self._retired.append(self.workspace)
             self.workspace = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device="cuda")
+            # allocated on the current stream: its memory may have served work still queued
+            # there, which another stream must not overtake
+            _order_after_current(torch, stream)
         return self.workspace.data_ptr()
 
     def fetch(self, stream=None) -> tuple[np.ndarray, np.ndarray]:
@@ -312,6 +320,17 @@ class DeviceBatch:
         self.download(stream)
         stream.synchronize()
         return self.results()
+
+
+def _order_after_current(torch, stream) -> None:
+    """Make ``stream`` wait for the work queued so far on the current stream (a no-op when
+    they are the same stream): device memory allocated, or filled, on the current stream is
+    then safe to use on ``stream``."""
+    if stream is None:
+        return
+    cur = torch.cuda.current_stream()
+    if stream != cur:
+        stream.wait_stream(cur)
 
 
 def _techniques(kind: Kind) -> dict[str, int]:
@@ -410,7 +429,7 @@ class BatchEngine:
         # the longest input of a prefix is at most the batch's
         tuning = choice.tuning(max(batch.max_len, 1))
         need = dev.workspace_size(c, total, tech, tuning)
-        ws = batch.ensure_workspace(need)
+        ws = batch.ensure_workspace(need, stream)
         dev.run_device_ws(batch.data.data_ptr(), batch.offsets.data_ptr(), c, total,
                           batch.flags.data_ptr(), batch.lens.data_ptr(), ws,
                           batch.workspace.numel(), stream.cuda_stream, tech, tuning)
@@ -431,7 +450,7 @@ class BatchEngine:
         tech = _techniques(kind)[choice.technique]
         tuning = choice.tuning(max(int(max_len), 1))
         need = dev.workspace_size(count, total, tech, tuning)
-        ws = batch.ensure_workspace(need)
+        ws = batch.ensure_workspace(need, stream)
         dev.run_device_ws(batch.data.data_ptr() + byte_start, batch.offsets.data_ptr() + 8 * off_start,
                           count, total, batch.flags.data_ptr() + 4 * in_start,
                           batch.lens.data_ptr() + 4 * in_start, ws, batch.workspace.numel(),

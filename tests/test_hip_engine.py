@@ -478,3 +478,39 @@ def test_run_stream_coalesced_matches_oracle(coalesce, depth):
             list(eng.run_stream(dfa, [], coalesce=0))
     finally:
         eng.close()
+
+
+def test_batch_staged_on_another_stream_waits_for_its_buffers():
+    """A DeviceBatch staged on a slot stream allocates and zero-fills its buffers on the
+    current stream; the slot stream must wait for that stream before copying into them (a
+    late fill would zero the staged offsets). With the current stream held by a bounded GPU
+    sleep, the slot stream's first copy is still waiting a few milliseconds later, and the
+    verdicts equal the oracle's."""
+    import time
+
+    import torch
+
+    from gpufsm.batch import slot_streams
+
+    pu = W.pattern_union(200, (4, 16), (1, 24), seed=81)
+    lens = W.lognormal_lengths(300, 1500, 1.0, 0, 20000, seed=82)
+    d, o = W.pattern_inputs(pu, lens, 0.7, seed=83)
+    eng = BatchEngine(tuner=None)
+    try:
+        st = slot_streams(1)[0]
+        torch.cuda.synchronize()
+        torch.cuda._sleep(20_000_000)  # the current stream busy for milliseconds (bounded)
+        batch = DeviceBatch(d, o, stream=st)
+        staged = torch.cuda.Event()
+        staged.record(st)
+        time.sleep(0.002)
+        waiting = not staged.query()
+        eng.enqueue(pu.nfa, batch, "auto", stream=st)
+        f, n = batch.fetch(st)
+        torch.cuda.synchronize()
+        assert waiting, "the slot stream did not wait for the current stream"
+        wf, wn = _oracle(pu.nfa, d, o)
+        np.testing.assert_array_equal(f, wf)
+        np.testing.assert_array_equal(n, wn)
+    finally:
+        eng.close()
