@@ -108,7 +108,7 @@ def _run_chain(dev, data, off, chunk):
 
 
 def test_ragged_mix_full_batch_chain_agrees_with_tiles():
-    """configs[2] at full size: 16,384 inputs, 109 MB."""
+    """The ragged NFA mix at full size: 16,384 inputs, 109 MB."""
     pu, (data, off) = W.workload_ragged(count=16384)
     with DeviceNFA(pu.nfa) as dev:
         f, n = _run_chain(dev, data, off, 2048)
@@ -128,7 +128,7 @@ def test_ragged_mix_full_batch_chain_agrees_with_tiles():
 
 
 def test_divergent_full_batch_chain_agrees_with_tiles():
-    """configs[4] at full size: 65,536 states, 1024 inputs of 1 MiB."""
+    """The high-divergence NFA at full size: 65,536 states, 1024 inputs of 1 MiB."""
     pu, (data, off) = W.workload_divergent(count=1024, length=1 << 20, seed=0)
     with DeviceNFA(pu.nfa) as dev:
         f, n = _run_chain(dev, data, off, 4096)
