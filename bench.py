@@ -1444,7 +1444,8 @@ def main() -> int:
     if rag:
         # the metric's comparison: batched (resident, and host-staged) over plain per-call
         line["batched_vs_plain"] = {
-            "workload": "ragged NFA mix (45,912-state union, 16,384 inputs), examined bytes",
+            "workload": f"ragged NFA mix ({rag['states']}-state union, {rag['count']} inputs), "
+                        f"examined bytes",
             "plain_GBps": rag["plain_per_call"]["examined_GBps"],
             "batched_resident_GBps": rag["batched"]["examined_GBps"],
             "batched_host_pipelined_GBps": rag["batched_host_pipelined"]["examined_GBps"],
